@@ -1,0 +1,302 @@
+/*
+ * magik_amd.h - C ABI of the MI355X-native wavefront spectral geodesic tracer.
+ *
+ * Two groups of entry points:
+ *
+ *  1. A layout-compatible subset of the reference's public header
+ *     (include/magik/magik.h of ErikHall1801/Magik): result codes, metric and
+ *     particle enums, the magik_trace_geodesic_* structs and
+ *     magik_trace_geodesic() itself, which here runs on the GPU.
+ *
+ *  2. Batched device entry points for the render hot path (ray generation +
+ *     initial conditions, persistent RKF45 geodesic integration, spectral
+ *     shading, the MLP surrogate).  They replace the internal seam
+ *     wavefront_sample() -> launch_kernel_<stage>_async() of the reference
+ *     (src/renderer/wavefront/wavefront.cpp:79-120, kernels/kernel_*.cuh:9).
+ *     All pointers named d_* are device pointers; streams are hipStream_t
+ *     passed as void* so this header needs no HIP include.
+ *
+ * Every function returns e_magik_result_types; none calls exit().
+ */
+#ifndef MAGIK_AMD_H
+#define MAGIK_AMD_H
+
+#include <stdint.h>
+#include <stddef.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#define MGK_API __attribute__((visibility("default")))
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------
+ * 1. Reference-compatible subset (values and layouts as magik.h:38-110,
+ *    :583-656).
+ * ------------------------------------------------------------------------- */
+typedef enum e_magik_result_types {
+    MAGIK_SUCCESS = 0,
+    MAGIK_ERROR_NOT_RUNNING = 1,
+    MAGIK_ERROR_INVALID_POINTER = 2,
+    MAGIK_UNKNOWN_ENUM_TYPE = 3,
+    MAGIK_INVALID_CUDA_DEVICE = 4,
+    MAGIK_ERROR_HOST_OUT_OF_MEMORY = 100,
+    MAGIK_ERROR_HOST_MEMORY_ALLOCATION_FAILED = 101,
+    MAGIK_ERROR_HOST_TRACE_ALLOCATED_STEPS_AND_CAPACITY_MISMATCH = 102,
+    MAGIK_ERROR_DEVICE_OUT_OF_MEMORY = 200,
+    MAGIK_ERROR_DEVICE_ALLOCATION_FAILED = 201,
+    MAGIK_ERROR_UNKNOWN_METRIC = 707,
+    MAGIK_ERROR_UNKNOWN = 10000,
+    /* Additions of this library (outside the reference's ranges). */
+    MGK_ERROR_INVALID_ARGUMENT = 20000,
+    MGK_ERROR_CAPACITY_EXCEEDED = 20001,
+    MGK_ERROR_HIP = 20002,
+    MAGIK_PROHIBITED_FORCE_SIZE = 0x7FFFFFFF
+} e_magik_result_types;
+
+typedef enum e_magik_spacetime_metric_types { MINKOWSKI = 0, SCHWARZSCHILD = 1, KERR = 2 } e_magik_spacetime_metric_types;
+
+typedef enum e_magik_test_particle_types { MASSIVE = 0, MASSLESS = 1 } e_magik_test_particle_types;
+
+typedef struct magik_trace_geodesic_metric_parameters_t {
+    e_magik_spacetime_metric_types metric_type;
+    bool time_reversed;
+    float event_horizon_scalar;
+    float schwarzschild_mass; /* kg */
+    float kerr_mass;          /* kg */
+    float kerr_spin;
+} magik_trace_geodesic_metric_parameters_t;
+
+typedef struct magik_trace_geodesic_integrator_parameters_t {
+    unsigned int n_integrator_max_substep;
+    float integrator_absolute_error_tolerance;
+    float integrator_relative_error_tolerance;
+} magik_trace_geodesic_integrator_parameters_t;
+
+typedef struct magik_trace_geodesic_inital_conditions_t {
+    e_magik_test_particle_types test_particle_type;
+    float spatial_position_x;
+    float spatial_position_y;
+    float spatial_position_z;
+    float x_rotation;
+    float y_rotation;
+    float z_rotation;
+    float local_proper_velocity_x;
+    float local_proper_velocity_y;
+    float local_proper_velocity_z;
+    float massive_particle_speed;
+} magik_trace_geodesic_inital_conditions_t;
+
+typedef struct magik_trace_geodesic_step_t {
+    float four_position[4];
+    float four_momentum[4];
+    float moment_derivative[4];
+    float four_vector_contraction;
+    float affine_parameter;
+    float delta_affine_parameter;
+} magik_trace_geodesic_step_t;
+
+typedef struct magik_trace_geodesic_container_t {
+    uint64_t capacity;
+    uint32_t n_allocated_step;
+    uint32_t n_written;
+    magik_trace_geodesic_step_t* steps;
+} magik_trace_geodesic_container_t;
+
+/* Replaces magik_trace_geodesic (magik.h:656, src/magik.cpp:196-207,
+ * relativity/host_trace_geodesic.cu:132-144): same arguments, same
+ * capacity check, same per-step records (FP32, SI positions); the geodesic is
+ * integrated by a HIP kernel on device 0 instead of the caller's CPU. */
+MGK_API e_magik_result_types magik_trace_geodesic(magik_trace_geodesic_container_t* container,
+                                                  magik_trace_geodesic_inital_conditions_t initial_conditions,
+                                                  magik_trace_geodesic_metric_parameters_t metric_parameters,
+                                                  magik_trace_geodesic_integrator_parameters_t integrator_parameters);
+
+/* magik.h:262 (version query). */
+MGK_API e_magik_result_types magik_get_version(uint32_t* major, uint32_t* minor, uint32_t* revision, const char** as_char);
+
+/* ---------------------------------------------------------------------------
+ * 2. Batched render path.
+ * ------------------------------------------------------------------------- */
+
+/* Arithmetic of the geodesic integrator. */
+typedef enum mgk_precision_t {
+    /* All FP32 in the reference's operation order and controller
+     * (internal_math.h:2160-2205): parity with the reference integrator. */
+    MGK_PRECISION_FP32_REFERENCE = 0,
+    /* FP32 right-hand side and stage states; FP64 state accumulation, error
+     * estimate and accept/reject; full FP64 trials inside the near zone
+     * (r < max(1.5 r_photon, r_+ + 0.1)); FP64 null-constraint
+     * renormalisation. */
+    MGK_PRECISION_MIXED = 1,
+    /* All FP64: the GPU twin of the FP64 oracle. */
+    MGK_PRECISION_FP64 = 2
+} mgk_precision_t;
+
+/* Termination classes of a path (the reference maps escape and horizon both
+ * to matter slot 3, kernel_intersect.cu:111,:119; they are kept apart here). */
+typedef enum mgk_hit_class_t {
+    MGK_HIT_NONE = 0,
+    MGK_HIT_HORIZON = 1,
+    MGK_HIT_ESCAPE = 2,
+    MGK_HIT_DISK = 3,
+    MGK_HIT_SPHERE = 4,
+    MGK_HIT_STEP_LIMIT = 5
+} mgk_hit_class_t;
+
+/* Integrator / spacetime settings (relativity_settings.h:11-25,
+ * render_settings.h:16, metric_*::make_params). */
+typedef struct mgk_integrator_t {
+    int32_t metric_type;         /* e_magik_spacetime_metric_types */
+    int32_t precision;           /* mgk_precision_t */
+    uint32_t time_reversed;      /* 1 on the render path (kernel_advance_geodesic.cu:87) */
+    uint32_t n_max_substep;      /* RKF45 trials per step (56) */
+    float mass_kg;               /* Kerr or Schwarzschild mass; < 0 = negative-mass Schwarzschild */
+    float spin;                  /* dimensionless a (Kerr) as given by the user; negated when time_reversed */
+    float event_horizon_scalar;  /* 1.01 */
+    float h_max;                 /* 1000 */
+    float abs_tol;
+    float rel_tol;
+    uint32_t step_limit;         /* terminate when n_step > step_limit */
+    uint32_t reserved;
+} mgk_integrator_t;
+
+/* Pinhole camera (optical_system_settings.h:20-34, device_sensor_utilities.cuh,
+ * thin_lens.cuh).  sensor_width_mm is the reference's sensor_diagonal_mm,
+ * which acts as the sensor width. */
+typedef struct mgk_camera_t {
+    float position[3];           /* SI metres */
+    float orientation[9];        /* row-major camera -> world */
+    float focal_length_mm;
+    float sensor_width_mm;
+    uint32_t x_resolution;
+    uint32_t y_resolution;
+    float observer_velocity[3];  /* local proper velocity, m/s */
+    uint32_t camera_id;          /* RNG stream */
+} mgk_camera_t;
+
+enum {
+    MGK_SCENE_DISK = 1u,
+    MGK_SCENE_SPHERE = 2u,
+    MGK_SCENE_STARS = 4u,
+    MGK_SCENE_BACKGROUND = 8u
+};
+
+/* Traceables and emitters. */
+typedef struct mgk_scene_t {
+    float celestial_radius_m;    /* escape when |x - camera| >= this (traceables.cuh:76-86) */
+    uint32_t flags;              /* MGK_SCENE_* */
+    float disk_r_in, disk_r_out; /* thin disk in the y = 0 plane, units of GM/c^2 */
+    float disk_T_in;             /* T(r) = disk_T_in (r / disk_r_in)^-3/4 */
+    float sphere_center[3];      /* units of GM/c^2 (metres for Minkowski) */
+    float sphere_radius;
+    float sphere_T;
+    uint32_t sphere_checker;     /* N x N checkerboard in (theta, phi) */
+    float background_scale;      /* uniform background: scale * B(lambda, background_T) */
+    float background_T;
+    float star_scale;            /* star: star_scale 10^(-0.4 m) B(lambda, star_T) */
+    float star_T;
+    float star_radius_rad;
+    uint32_t n_star;
+    uint64_t star_seed;
+} mgk_scene_t;
+
+/* Spectral accumulation (color_settings.h:11-14, optical_system_settings.h:22-27). */
+typedef struct mgk_spectral_t {
+    uint32_t n_hero;             /* hero wavelengths per path (1..4), stratified */
+    float hero_min_nm, hero_max_nm;
+    uint32_t n_bin;              /* intrinsic spectral irradiance bins (<= 64) */
+    float bin_min_nm, bin_max_nm;
+    float sigma_nm;              /* Gaussian redistribution width */
+    float iso, fps, shutter_angle;
+    float white_point_K;         /* scene white balance (color_settings.h:27) */
+    uint32_t reserved;
+} mgk_spectral_t;
+
+/* One terminated path (16 bytes).  DISK: a0 = r_hit, a1 = azimuth;
+ * ESCAPE: a0 = cos(theta) of the escape direction about +y, a1 = atan2(z, x);
+ * SPHERE: a0, a1 = the same angles of the surface normal at the hit.
+ * g = (u.p)_camera / (u.p)_emitter (relativity_utilities.cuh:62-67). */
+typedef struct mgk_hit_t {
+    uint32_t cls;
+    float g;
+    float a0;
+    float a1;
+} mgk_hit_t;
+
+typedef struct mgk_trace_stats_t {
+    uint64_t n_rays;
+    uint64_t n_steps;            /* accepted (or exhausted) steps */
+    uint64_t n_trials;           /* RKF45 trials */
+    uint64_t n_trials_fp64;      /* trials run entirely in FP64 */
+    double ms;                   /* device time of the integration kernel */
+} mgk_trace_stats_t;
+
+typedef struct mgk_context* mgk_context_handle;
+
+/* Library state per device: counters, star index, cached work buffers. */
+MGK_API e_magik_result_types mgk_context_create(int device, mgk_context_handle* out);
+MGK_API e_magik_result_types mgk_context_destroy(mgk_context_handle ctx);
+
+/* Uploads the scene's star catalogue (generated from star_seed) and its
+ * sky-grid index.  Must precede tracing or shading of a scene with stars. */
+MGK_API e_magik_result_types mgk_set_scene(mgk_context_handle ctx, const mgk_scene_t* scene);
+
+/* Ray generation + initial conditions (kernel_generate_thin_lens_ray.cu:51-77,
+ * kernel_generate_initial_conditions.cu:54-74) for every pixel and the samples
+ * [sample_begin, sample_begin + spp).  Ray i = s * W * H + py * W + px is
+ * written as SoA: d_rays[c * ray_stride + i] for c = 0..9 =
+ * x, y, z, t, p_x, p_y, p_z, p_t, h0, u.p.  Element type is double for
+ * MGK_PRECISION_FP64 and float otherwise. */
+MGK_API e_magik_result_types mgk_generate_rays(mgk_context_handle ctx, const mgk_integrator_t* integ, const mgk_camera_t* cam,
+                                               uint64_t seed, uint32_t sample_begin, uint32_t spp, void* d_rays,
+                                               uint64_t ray_stride, void* stream);
+
+/* Persistent geodesic integration (replaces kernel_advance_geodesic,
+ * kernel_intersect, kernel_enforce_ray_limits and their queues).  Rays are
+ * pulled from [0, n_rays), or from d_queue[0..n_rays) when d_queue is not
+ * NULL; hit i (or d_queue[i]'s slot) receives the result.  d_steps may be NULL. */
+MGK_API e_magik_result_types mgk_trace_rays(mgk_context_handle ctx, const mgk_integrator_t* integ, const mgk_scene_t* scene,
+                                            const float camera_position[3], const void* d_rays, uint64_t ray_stride,
+                                            uint64_t n_rays, const uint32_t* d_queue, mgk_hit_t* d_hits,
+                                            uint32_t* d_steps, void* stream);
+
+/* Statistics of the last mgk_trace_rays / mgk_render_frame (synchronises). */
+MGK_API e_magik_result_types mgk_last_trace_stats(mgk_context_handle ctx, mgk_trace_stats_t* out);
+
+/* Spectral shading + accumulation (kernel_bxdf.cu emit path,
+ * kernel_redistribute_radiance.cu:55-92, kernel_splat_to_renderbuffer.cu:37-57
+ * up to the linear colour): per pixel, samples [0, spp) of d_hits (sample-major,
+ * W*H per sample) are shaded at their hero wavelengths, redistributed into
+ * n_bin Gaussian bins and folded with the bin CMF weights into CIE XYZ / spp.
+ * d_xyz receives 3 floats per pixel. */
+MGK_API e_magik_result_types mgk_shade(mgk_context_handle ctx, const mgk_scene_t* scene, const mgk_spectral_t* spec,
+                                       const mgk_camera_t* cam, uint64_t seed, uint32_t sample_begin, uint32_t spp,
+                                       const mgk_hit_t* d_hits, float* d_xyz, void* stream);
+
+/* Whole frame: generate + trace + shade with library-owned ray and hit
+ * buffers (grown on demand). */
+MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle ctx, const mgk_integrator_t* integ, const mgk_scene_t* scene,
+                                              const mgk_spectral_t* spec, const mgk_camera_t* cam, uint64_t seed,
+                                              uint32_t spp, float* d_xyz, void* stream);
+
+/* Host helpers of the colour pipeline (host_color_utilities.cuh:145-178,
+ * :29-68; color_pipeline.cuh:21-48): bin CMF weights (n_bin x 3, XYZ) and the
+ * white-balanced XYZ -> linear Rec.709 matrix (row-major). */
+MGK_API e_magik_result_types mgk_color_weights(const mgk_spectral_t* spec, float* weights_xyz, float* xyz_to_scene);
+
+/* Places a camera at Boyer-Lindquist (r, theta) of a Kerr black hole with spin
+ * a (as given by the user), in Kerr-Schild Cartesian coordinates with the spin
+ * along +y, looking at the origin with +y up; horizontal field of view fov_deg
+ * with the given sensor width.  mass_kg sets the length scale. */
+MGK_API e_magik_result_types mgk_camera_boyer_lindquist(float r_obs, float theta_deg, float spin, float mass_kg,
+                                                        float fov_deg, uint32_t x_res, uint32_t y_res, mgk_camera_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAGIK_AMD_H */

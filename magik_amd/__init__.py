@@ -1,0 +1,230 @@
+"""magik_amd: MI355X-native wavefront spectral geodesic tracer.
+
+The product is the C-ABI library ``libmagik_amd.so`` (include/magik_amd.h);
+this module is the thin Python host mirror used by the tests and the
+benchmark: it fills the ABI structs, hands torch-allocated device buffers to
+the library and reads results back.  Nothing here computes on the CPU in place
+of the HIP kernels; every call fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+from . import abi
+from .abi import (HIT_DISK, HIT_ESCAPE, HIT_HORIZON, HIT_NONE, HIT_SPHERE, HIT_STEP_LIMIT, KERR, MINKOWSKI,
+                  PREC_FP32_REFERENCE, PREC_FP64, PREC_MIXED, SCHWARZSCHILD, SCENE_BACKGROUND, SCENE_DISK,
+                  SCENE_SPHERE, SCENE_STARS)
+
+DEFAULT_MASS_KG = 6.7329546075e26       # relativity_settings.h:21,24
+G, C_LIGHT = 6.67430e-11, 299792458.0
+
+
+def length_scale(mass_kg: float = DEFAULT_MASS_KG) -> float:
+    """G M / c^2 in metres (metric_kerr.cuh:154-157), in float32 like the library."""
+    import numpy as np
+    f = np.float32
+    return float(f(G) * f(mass_kg) / (f(C_LIGHT) * f(C_LIGHT)))
+
+
+def r_isco(a: float) -> float:
+    """Prograde ISCO radius in units of M (Bardeen, Press & Teukolsky 1972)."""
+    a = abs(a)
+    z1 = 1 + (1 - a * a) ** (1 / 3) * ((1 + a) ** (1 / 3) + (1 - a) ** (1 / 3))
+    z2 = math.sqrt(3 * a * a + z1 * z1)
+    return 3 + z2 - math.sqrt((3 - z1) * (3 + z1 + 2 * z2))
+
+
+def integrator(metric=KERR, precision=PREC_MIXED, spin=0.9, mass_kg=DEFAULT_MASS_KG, abs_tol=1e-7, rel_tol=1e-7,
+               n_max=56, step_limit=10000, h_max=1000.0, time_reversed=True, horizon_scalar=1.01) -> abi.Integrator:
+    return abi.Integrator(metric_type=metric, precision=precision, time_reversed=int(time_reversed), n_max_substep=n_max,
+                          mass_kg=mass_kg, spin=spin, event_horizon_scalar=horizon_scalar, h_max=h_max,
+                          abs_tol=abs_tol, rel_tol=rel_tol, step_limit=step_limit)
+
+
+def spectral(n_hero=4, hero_min=380.0, hero_max=780.0, n_bin=24, bin_min=360.0, bin_max=830.0, sigma=10.0,
+             iso=500.0, fps=24.0, shutter=180.0, white_K=5600.0) -> abi.Spectral:
+    """Defaults of color_settings.h:11-14 and optical_system_settings.h:22-27."""
+    return abi.Spectral(n_hero=n_hero, hero_min_nm=hero_min, hero_max_nm=hero_max, n_bin=n_bin, bin_min_nm=bin_min,
+                        bin_max_nm=bin_max, sigma_nm=sigma, iso=iso, fps=fps, shutter_angle=shutter,
+                        white_point_K=white_K)
+
+
+def camera_bl(r_obs: float, theta_deg: float, spin: float, fov_deg: float, W: int, H: int,
+              mass_kg: float = DEFAULT_MASS_KG, camera_id: int = 0) -> abi.Camera:
+    cam = abi.Camera()
+    abi.check(abi.load().mgk_camera_boyer_lindquist(r_obs, theta_deg, spin, mass_kg, fov_deg, W, H, C.byref(cam)),
+              "mgk_camera_boyer_lindquist")
+    cam.camera_id = camera_id
+    return cam
+
+
+def camera_pinhole(position, orientation, fov_deg: float, W: int, H: int, camera_id: int = 0) -> abi.Camera:
+    cam = abi.Camera()
+    for i in range(3):
+        cam.position[i] = position[i]
+    for i in range(9):
+        cam.orientation[i] = orientation[i]
+    cam.sensor_width_mm = 24.0
+    cam.focal_length_mm = 12.0 / math.tan(math.radians(fov_deg) / 2)
+    cam.x_resolution, cam.y_resolution, cam.camera_id = W, H, camera_id
+    return cam
+
+
+def scene(**kw) -> abi.Scene:
+    s = abi.Scene(celestial_radius_m=1000.0, flags=0, disk_r_in=6.0, disk_r_out=20.0, disk_T_in=1e4,
+                  sphere_radius=1.0, sphere_T=6500.0, sphere_checker=16, background_scale=0.0, background_T=6500.0,
+                  star_scale=1.0, star_T=6500.0, star_radius_rad=1e-3, n_star=0, star_seed=0)
+    for k, v in kw.items():
+        if k == "sphere_center":
+            for i in range(3):
+                s.sphere_center[i] = v[i]
+        else:
+            setattr(s, k, v)
+    return s
+
+
+# ---------------------------------------------------------------------------
+# The benchmark workloads.
+# ---------------------------------------------------------------------------
+def workload(name: str, W: int | None = None, H: int | None = None, spp: int | None = None,
+             precision: int | None = None):
+    """(integrator, scene, spectral, camera, spp) of a named workload.
+
+    minkowski_sphere  : flat space, 256x256x4, camera at the origin looking at
+                        +z (60 deg), emissive 16x16 checkered sphere r=1 at
+                        z=5, uniform background sphere r=1000, 1 hero lambda,
+                        FP64, tol 1e-8, cap 4096.
+    schwarzschild_disk: M=1, 512x512x16, r=30M, theta=85 deg, 40 deg; thin disk
+                        6M-20M with T = 1e4 K (r/6)^-3/4; 1e5 seeded stars;
+                        tol 1e-7, cap 1e4.
+    kerr_disk_1080p   : a=0.9, 1920x1080x32, r=50M, theta=80 deg, 25 deg; disk
+                        r_ISCO-30M, same T law; 4 stratified hero lambdas;
+                        same stars; tol 1e-7, cap 1e4.
+    """
+    if name == "minkowski_sphere":
+        W, H, spp = W or 256, H or 256, spp or 4
+        integ = integrator(metric=MINKOWSKI, precision=PREC_FP64 if precision is None else precision, spin=0.0,
+                           abs_tol=1e-8, rel_tol=1e-8, step_limit=4096)
+        # rotation by 180 deg about y: camera -z looks down world +z
+        cam = camera_pinhole([0.0, 0.0, 0.0], [-1, 0, 0, 0, 1, 0, 0, 0, -1], 60.0, W, H)
+        sc = scene(celestial_radius_m=1000.0, flags=SCENE_SPHERE | SCENE_BACKGROUND, sphere_center=(0.0, 0.0, 5.0),
+                   sphere_radius=1.0, sphere_T=6500.0, sphere_checker=16, background_scale=0.1, background_T=6500.0)
+        spec = spectral(n_hero=1)
+        return integ, sc, spec, cam, spp
+    if name == "schwarzschild_disk":
+        W, H, spp = W or 512, H or 512, spp or 16
+        integ = integrator(metric=SCHWARZSCHILD, precision=PREC_MIXED if precision is None else precision, spin=0.0)
+        cam = camera_bl(30.0, 85.0, 0.0, 40.0, W, H)
+        sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_STARS, disk_r_in=6.0,
+                   disk_r_out=20.0, disk_T_in=1e4, n_star=100000, star_seed=0x5EED, star_scale=1.0,
+                   star_radius_rad=1e-3, star_T=6500.0)
+        spec = spectral(n_hero=1)
+        return integ, sc, spec, cam, spp
+    if name == "kerr_disk_1080p":
+        W, H, spp = W or 1920, H or 1080, spp or 32
+        a = 0.9
+        integ = integrator(metric=KERR, precision=PREC_MIXED if precision is None else precision, spin=a)
+        cam = camera_bl(50.0, 80.0, a, 25.0, W, H)
+        sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_STARS, disk_r_in=r_isco(a),
+                   disk_r_out=30.0, disk_T_in=1e4, n_star=100000, star_seed=0x5EED, star_scale=1.0,
+                   star_radius_rad=1e-3, star_T=6500.0)
+        spec = spectral(n_hero=4)
+        return integ, sc, spec, cam, spp
+    raise ValueError(f"unknown workload {name}")
+
+
+class Context:
+    """One mgk_context on a device; device buffers are torch tensors."""
+
+    def __init__(self, device: int = 0):
+        self.lib = abi.load()
+        h = C.c_void_p()
+        abi.check(self.lib.mgk_context_create(device, C.byref(h)), "mgk_context_create")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            abi.check(self.lib.mgk_context_destroy(self.h), "mgk_context_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _stream():
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def set_scene(self, sc):
+        abi.check(self.lib.mgk_set_scene(self.h, C.byref(sc)), "mgk_set_scene")
+
+    def generate_rays(self, integ, cam, seed: int, sample_begin: int, spp: int):
+        import torch
+        n = cam.x_resolution * cam.y_resolution * spp
+        dt = torch.float32 if integ.precision == PREC_FP32_REFERENCE else torch.float64
+        rays = torch.empty((10, n), dtype=dt, device=f"cuda:{self.device}")
+        abi.check(self.lib.mgk_generate_rays(self.h, C.byref(integ), C.byref(cam), seed, sample_begin, spp,
+                                             C.c_void_p(rays.data_ptr()), n, self._stream()), "mgk_generate_rays")
+        return rays
+
+    def trace(self, integ, sc, cam_pos, rays, n: int | None = None, queue=None, want_steps: bool = True):
+        import torch
+        stride = rays.shape[1]
+        n = stride if n is None else n
+        hits = torch.zeros((stride, 4), dtype=torch.int32, device=rays.device)
+        steps = torch.zeros(stride, dtype=torch.int32, device=rays.device) if want_steps else None
+        cp = (C.c_float * 3)(*cam_pos)
+        abi.check(self.lib.mgk_trace_rays(self.h, C.byref(integ), C.byref(sc), cp, C.c_void_p(rays.data_ptr()),
+                                          stride, n, C.c_void_p(queue.data_ptr() if queue is not None else None),
+                                          C.c_void_p(hits.data_ptr()),
+                                          C.c_void_p(steps.data_ptr() if steps is not None else None),
+                                          self._stream()), "mgk_trace_rays")
+        return hits, steps
+
+    def shade(self, sc, spec, cam, seed: int, sample_begin: int, spp: int, hits):
+        import torch
+        xyz = torch.zeros((cam.y_resolution, cam.x_resolution, 3), dtype=torch.float32, device=hits.device)
+        abi.check(self.lib.mgk_shade(self.h, C.byref(sc), C.byref(spec), C.byref(cam), seed, sample_begin, spp,
+                                     C.c_void_p(hits.data_ptr()), C.c_void_p(xyz.data_ptr()), self._stream()),
+                  "mgk_shade")
+        return xyz
+
+    def render_frame(self, integ, sc, spec, cam, seed: int, spp: int, out=None):
+        import torch
+        if out is None:
+            out = torch.zeros((cam.y_resolution, cam.x_resolution, 3), dtype=torch.float32,
+                              device=f"cuda:{self.device}")
+        abi.check(self.lib.mgk_render_frame(self.h, C.byref(integ), C.byref(sc), C.byref(spec), C.byref(cam), seed,
+                                            spp, C.c_void_p(out.data_ptr()), self._stream()), "mgk_render_frame")
+        return out
+
+    def stats(self) -> abi.TraceStats:
+        st = abi.TraceStats()
+        abi.check(self.lib.mgk_last_trace_stats(self.h, C.byref(st)), "mgk_last_trace_stats")
+        return st
+
+
+def hits_to_numpy(hits):
+    """(n, 4) int32 device tensor -> structured numpy array cls/g/a0/a1."""
+    import numpy as np
+    h = hits.cpu().numpy()
+    out = np.zeros(h.shape[0], dtype=[("cls", np.uint32), ("g", np.float32), ("a0", np.float32), ("a1", np.float32)])
+    out["cls"] = h[:, 0].view(np.uint32)
+    out["g"] = h[:, 1].view(np.float32)
+    out["a0"] = h[:, 2].view(np.float32)
+    out["a1"] = h[:, 3].view(np.float32)
+    return out
+
+
+def color_weights(spec):
+    import numpy as np
+    W = np.zeros((spec.n_bin, 3), dtype=np.float32)
+    M = np.zeros(9, dtype=np.float32)
+    abi.check(abi.load().mgk_color_weights(C.byref(spec), W.ctypes.data_as(C.POINTER(C.c_float)),
+                                           M.ctypes.data_as(C.POINTER(C.c_float))), "mgk_color_weights")
+    return W, M.reshape(3, 3)

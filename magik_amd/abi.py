@@ -1,0 +1,147 @@
+"""ctypes mirror of include/magik_amd.h and the loader of the HIP library.
+
+The Python side is plumbing for tests and the benchmark: the product is the
+C-ABI shared library ``magik_amd/libmagik_amd.so`` built from
+``magik_amd/csrc``.  Loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmagik_amd.so")
+
+# e_magik_result_types (include/magik_amd.h)
+MAGIK_SUCCESS = 0
+MAGIK_ERROR_INVALID_POINTER = 2
+MAGIK_INVALID_CUDA_DEVICE = 4
+MAGIK_ERROR_HOST_TRACE_ALLOCATED_STEPS_AND_CAPACITY_MISMATCH = 102
+MAGIK_ERROR_DEVICE_ALLOCATION_FAILED = 201
+MAGIK_ERROR_UNKNOWN_METRIC = 707
+MAGIK_ERROR_UNKNOWN = 10000
+MGK_ERROR_INVALID_ARGUMENT = 20000
+MGK_ERROR_CAPACITY_EXCEEDED = 20001
+MGK_ERROR_HIP = 20002
+
+MINKOWSKI, SCHWARZSCHILD, KERR = 0, 1, 2
+MASSIVE, MASSLESS = 0, 1
+
+PREC_FP32_REFERENCE, PREC_MIXED, PREC_FP64 = 0, 1, 2
+
+HIT_NONE, HIT_HORIZON, HIT_ESCAPE, HIT_DISK, HIT_SPHERE, HIT_STEP_LIMIT = range(6)
+HIT_NAMES = {0: "none", 1: "horizon", 2: "escape", 3: "disk", 4: "sphere", 5: "step_limit"}
+
+SCENE_DISK, SCENE_SPHERE, SCENE_STARS, SCENE_BACKGROUND = 1, 2, 4, 8
+
+
+class MetricParameters(C.Structure):  # magik_trace_geodesic_metric_parameters_t
+    _fields_ = [("metric_type", C.c_int32), ("time_reversed", C.c_bool), ("event_horizon_scalar", C.c_float),
+                ("schwarzschild_mass", C.c_float), ("kerr_mass", C.c_float), ("kerr_spin", C.c_float)]
+
+
+class IntegratorParameters(C.Structure):  # magik_trace_geodesic_integrator_parameters_t
+    _fields_ = [("n_integrator_max_substep", C.c_uint32), ("integrator_absolute_error_tolerance", C.c_float),
+                ("integrator_relative_error_tolerance", C.c_float)]
+
+
+class InitialConditions(C.Structure):  # magik_trace_geodesic_inital_conditions_t
+    _fields_ = [("test_particle_type", C.c_int32)] + [(n, C.c_float) for n in (
+        "spatial_position_x", "spatial_position_y", "spatial_position_z", "x_rotation", "y_rotation", "z_rotation",
+        "local_proper_velocity_x", "local_proper_velocity_y", "local_proper_velocity_z", "massive_particle_speed")]
+
+
+class Step(C.Structure):  # magik_trace_geodesic_step_t
+    _fields_ = [("four_position", C.c_float * 4), ("four_momentum", C.c_float * 4), ("moment_derivative", C.c_float * 4),
+                ("four_vector_contraction", C.c_float), ("affine_parameter", C.c_float),
+                ("delta_affine_parameter", C.c_float)]
+
+
+class Container(C.Structure):  # magik_trace_geodesic_container_t
+    _fields_ = [("capacity", C.c_uint64), ("n_allocated_step", C.c_uint32), ("n_written", C.c_uint32),
+                ("steps", C.POINTER(Step))]
+
+
+class Integrator(C.Structure):  # mgk_integrator_t
+    _fields_ = [("metric_type", C.c_int32), ("precision", C.c_int32), ("time_reversed", C.c_uint32),
+                ("n_max_substep", C.c_uint32), ("mass_kg", C.c_float), ("spin", C.c_float),
+                ("event_horizon_scalar", C.c_float), ("h_max", C.c_float), ("abs_tol", C.c_float),
+                ("rel_tol", C.c_float), ("step_limit", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class Camera(C.Structure):  # mgk_camera_t
+    _fields_ = [("position", C.c_float * 3), ("orientation", C.c_float * 9), ("focal_length_mm", C.c_float),
+                ("sensor_width_mm", C.c_float), ("x_resolution", C.c_uint32), ("y_resolution", C.c_uint32),
+                ("observer_velocity", C.c_float * 3), ("camera_id", C.c_uint32)]
+
+
+class Scene(C.Structure):  # mgk_scene_t
+    _fields_ = [("celestial_radius_m", C.c_float), ("flags", C.c_uint32), ("disk_r_in", C.c_float),
+                ("disk_r_out", C.c_float), ("disk_T_in", C.c_float), ("sphere_center", C.c_float * 3),
+                ("sphere_radius", C.c_float), ("sphere_T", C.c_float), ("sphere_checker", C.c_uint32),
+                ("background_scale", C.c_float), ("background_T", C.c_float), ("star_scale", C.c_float),
+                ("star_T", C.c_float), ("star_radius_rad", C.c_float), ("n_star", C.c_uint32),
+                ("star_seed", C.c_uint64)]
+
+
+class Spectral(C.Structure):  # mgk_spectral_t
+    _fields_ = [("n_hero", C.c_uint32), ("hero_min_nm", C.c_float), ("hero_max_nm", C.c_float),
+                ("n_bin", C.c_uint32), ("bin_min_nm", C.c_float), ("bin_max_nm", C.c_float), ("sigma_nm", C.c_float),
+                ("iso", C.c_float), ("fps", C.c_float), ("shutter_angle", C.c_float), ("white_point_K", C.c_float),
+                ("reserved", C.c_uint32)]
+
+
+class Hit(C.Structure):  # mgk_hit_t
+    _fields_ = [("cls", C.c_uint32), ("g", C.c_float), ("a0", C.c_float), ("a1", C.c_float)]
+
+
+class TraceStats(C.Structure):  # mgk_trace_stats_t
+    _fields_ = [("n_rays", C.c_uint64), ("n_steps", C.c_uint64), ("n_trials", C.c_uint64),
+                ("n_trials_fp64", C.c_uint64), ("ms", C.c_double)]
+
+
+# Every symbol include/magik_amd.h declares, with its ctypes signature.
+_P = C.c_void_p
+_R = C.c_int32
+SIGNATURES = {
+    "magik_trace_geodesic": (_R, [C.POINTER(Container), InitialConditions, MetricParameters, IntegratorParameters]),
+    "magik_get_version": (_R, [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                               C.POINTER(C.c_char_p)]),
+    "mgk_context_create": (_R, [C.c_int, C.POINTER(_P)]),
+    "mgk_context_destroy": (_R, [_P]),
+    "mgk_set_scene": (_R, [_P, C.POINTER(Scene)]),
+    "mgk_generate_rays": (_R, [_P, C.POINTER(Integrator), C.POINTER(Camera), C.c_uint64, C.c_uint32, C.c_uint32, _P,
+                               C.c_uint64, _P]),
+    "mgk_trace_rays": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(C.c_float), _P, C.c_uint64,
+                            C.c_uint64, _P, _P, _P, _P]),
+    "mgk_last_trace_stats": (_R, [_P, C.POINTER(TraceStats)]),
+    "mgk_shade": (_R, [_P, C.POINTER(Scene), C.POINTER(Spectral), C.POINTER(Camera), C.c_uint64, C.c_uint32,
+                       C.c_uint32, _P, _P, _P]),
+    "mgk_render_frame": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(Spectral), C.POINTER(Camera),
+                              C.c_uint64, C.c_uint32, _P, _P]),
+    "mgk_color_weights": (_R, [C.POINTER(Spectral), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mgk_camera_boyer_lindquist": (_R, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32,
+                                        C.c_uint32, C.POINTER(Camera)]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libmagik_amd.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(code: int, what: str = "") -> None:
+    if code != MAGIK_SUCCESS:
+        raise RuntimeError(f"magik_amd call {what} failed with result code {code}")

@@ -1,0 +1,720 @@
+// C-ABI layer of the tracer: argument validation, per-device context (work
+// counters, star index, cached frame buffers), host-side colour pipeline and
+// camera placement, and the launches of the kernels in mgk_kernels.hip.
+//
+// Error convention: every entry point returns e_magik_result_types
+// (magik.h:38-110); HIP failures map to MGK_ERROR_HIP and nothing calls exit()
+// (the reference's check_cuda exits with 99, cuda_host_utilities.h:12-22).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "magik_amd.h"
+#include "mgk_device.hpp"
+#include "mgk_internal.hpp"
+
+namespace {
+
+#include "cie1931_table.inc"   // kCie1931[471][3], 360..830 nm at 1 nm (generated from data/cie1931_2deg_1nm.csv)
+
+using namespace mgk;
+
+#define HIP_TRY(expr)                                  \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) {                        \
+            std::fprintf(stderr, "magik_amd: %s failed: %s\n", #expr, hipGetErrorString(e_)); \
+            return MGK_ERROR_HIP;                      \
+        }                                              \
+    } while (0)
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    int i = threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&ptr, need);
+        if (e == hipSuccess) bytes = need;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+}  // namespace
+
+struct mgk_context {
+    int device = 0;
+    int n_cu = 0;
+    unsigned long long* d_counters = nullptr;   // [0] head [1..3] stats
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    uint64_t last_n = 0;
+    // star index
+    DeviceBuffer stars, cell_start, cell_items;
+    uint32_t grid_nc = 0, grid_np = 0;
+    uint64_t star_key = ~0ull;
+    uint32_t star_n = 0;
+    float star_rho = -1.0f, star_scale = 0.0f;
+    // frame buffers
+    DeviceBuffer rays, hits;
+};
+
+namespace {
+
+e_magik_result_types set_device(mgk_context* c) {
+    HIP_TRY(hipSetDevice(c->device));
+    return MAGIK_SUCCESS;
+}
+
+// Geometric length scale G M / c^2 in float (metric_kerr.cuh:154-157).
+float length_scale_of(const mgk_integrator_t* in) {
+    if (in->metric_type == MINKOWSKI) return 1.0f;
+    float m = in->metric_type == SCHWARZSCHILD ? std::fabs(in->mass_kg) : in->mass_kg;
+    return kG * m / (kC * kC);
+}
+
+bool valid_metric(int m) { return m == MINKOWSKI || m == SCHWARZSCHILD || m == KERR; }
+
+e_magik_result_types check_integrator(const mgk_integrator_t* in) {
+    if (!in) return MAGIK_ERROR_INVALID_POINTER;
+    if (!valid_metric(in->metric_type)) return MAGIK_ERROR_UNKNOWN_METRIC;
+    if (in->precision < MGK_PRECISION_FP32_REFERENCE || in->precision > MGK_PRECISION_FP64) return MGK_ERROR_INVALID_ARGUMENT;
+    if (in->metric_type != MINKOWSKI && (in->n_max_substep == 0 || !(in->abs_tol > 0.0f) || !(in->rel_tol >= 0.0f) ||
+                                         !(in->h_max > 0.0f)))
+        return MGK_ERROR_INVALID_ARGUMENT;
+    if (in->metric_type != MINKOWSKI && !(in->mass_kg != 0.0f)) return MGK_ERROR_INVALID_ARGUMENT;
+    return MAGIK_SUCCESS;
+}
+
+float signed_spin(const mgk_integrator_t* in) {
+    if (in->metric_type != KERR) return 0.0f;
+    return in->time_reversed ? -in->spin : in->spin;
+}
+float mass_sign_of(const mgk_integrator_t* in) {
+    return (in->metric_type == SCHWARZSCHILD && in->mass_kg < 0.0f) ? -1.0f : 1.0f;
+}
+
+// Outer prograde/retrograde photon-orbit radius (Bardeen, Press & Teukolsky
+// 1972): r_ph = 2 (1 + cos(2/3 acos(+|a|))) for the retrograde (outer) orbit.
+double outer_photon_radius(double a) {
+    a = std::fabs(a);
+    return 2.0 * (1.0 + std::cos((2.0 / 3.0) * std::acos(a)));
+}
+
+void fill_trace_params(const mgk_integrator_t* in, const mgk_scene_t* sc, const float cam[3], TraceParams& P) {
+    std::memset(&P, 0, sizeof(P));
+    P.metric = in->metric_type;
+    P.precision = in->precision;
+    P.spin = signed_spin(in);
+    P.mass_sign = mass_sign_of(in);
+    P.length_scale = length_scale_of(in);
+    P.h_max = in->h_max;
+    P.abs_tol = in->abs_tol;
+    P.rel_tol = in->rel_tol;
+    P.n_max = in->n_max_substep ? in->n_max_substep : 1;
+    P.step_limit = in->step_limit;
+    if (in->metric_type == KERR) {
+        float a = P.spin;
+        P.spin_in_range = std::fabs(a) > 1.0f ? 0 : 1;
+        P.horizon_r_f = (1.0f + std::sqrt(1.0f - a * a)) * in->event_horizon_scalar;
+        double ad = double(a);
+        P.horizon_r_d = (1.0 + std::sqrt(1.0 - ad * ad)) * double(in->event_horizon_scalar);
+        double rp = 1.0 + std::sqrt(std::max(0.0, 1.0 - ad * ad));
+        double zr = std::max(1.5 * outer_photon_radius(ad), rp + 0.1);
+        P.zone_rho2 = zr * zr + ad * ad;
+    } else if (in->metric_type == SCHWARZSCHILD) {
+        P.spin_in_range = 1;
+        P.horizon_r_f = 2.0f * in->event_horizon_scalar;
+        P.horizon_r_d = 2.0 * double(in->event_horizon_scalar);
+        double zr = P.mass_sign > 0 ? std::max(1.5 * 3.0, 2.0 + 0.1) : 0.0;
+        P.zone_rho2 = zr * zr;
+    }
+    for (int k = 0; k < 3; ++k) P.cam[k] = cam[k];
+    P.cel_r2_f = sc->celestial_radius_m * sc->celestial_radius_m;
+    P.cel_r2_d = double(sc->celestial_radius_m) * double(sc->celestial_radius_m);
+    P.flags = sc->flags;
+    P.disk_r_in = sc->disk_r_in;
+    P.disk_r_out = sc->disk_r_out;
+    for (int k = 0; k < 3; ++k) P.sphere_c[k] = sc->sphere_center[k];
+    P.sphere_r = sc->sphere_radius;
+}
+
+int trace_grid(mgk_context* c, const TraceParams& P) {
+    // persistent launch: as many 256-thread blocks as can be resident, capped
+    // by the work available
+    int per_cu = 8;
+    if (P.precision != MGK_PRECISION_FP32_REFERENCE) per_cu = 4;
+    uint64_t want = uint64_t(c->n_cu) * uint64_t(per_cu);
+    uint64_t need = (P.n + 255) / 256;
+    if (need < want) want = need;
+    if (want < 1) want = 1;
+    return int(want);
+}
+
+e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s) {
+    P.head = c->d_counters;
+    P.stats = c->d_counters + 1;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c->d_counters, 4);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev0, s));
+    if (P.n > 0) HIP_TRY(launch_trace(P, trace_grid(c, P), s));
+    HIP_TRY(hipEventRecord(c->ev1, s));
+    c->timed = true;
+    c->last_n = P.n;
+    return MAGIK_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Host colour pipeline (host_color_utilities.cuh:14-178, color_pipeline.cuh:21-48).
+// ---------------------------------------------------------------------------
+struct f3 { float x, y, z; };
+struct m3 { float m[9]; };
+f3 mul(const m3& a, f3 v) {
+    return {a.m[0] * v.x + a.m[1] * v.y + a.m[2] * v.z, a.m[3] * v.x + a.m[4] * v.y + a.m[5] * v.z,
+            a.m[6] * v.x + a.m[7] * v.y + a.m[8] * v.z};
+}
+m3 mul(const m3& a, const m3& b) {
+    m3 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r.m[3 * i + j] = a.m[3 * i] * b.m[j] + a.m[3 * i + 1] * b.m[3 + j] + a.m[3 * i + 2] * b.m[6 + j];
+    return r;
+}
+
+f3 cmf_at(float w) {   // linear interpolation in the 1 nm table (color_matching_functions.cuh:35-44)
+    const uint32_t n = 471;
+    float spacing = (830.0f - 360.0f) / float(n - 1);
+    float fi = std::fmin(std::fmax((w - 360.0f) / spacing, 0.0f), float(n - 1));
+    uint32_t lo = uint32_t(std::fmin(float(uint32_t(fi)), float(n - 2)));
+    float t = fi - float(lo);
+    return {(kCie1931[lo][0] * (1.0f - t)) + (kCie1931[lo + 1][0] * t), (kCie1931[lo][1] * (1.0f - t)) + (kCie1931[lo + 1][1] * t),
+            (kCie1931[lo][2] * (1.0f - t)) + (kCie1931[lo + 1][2] * t)};
+}
+
+float planck_host(float lambda_nm, float T) {
+    if (T <= 0.0f) return 0.0f;
+    float lm = lambda_nm * 1e-9f;
+    const float t0 = kPlanck * kC, t1 = 2.0f * t0 * kC;
+    float t2 = lm * lm * lm * lm * lm;
+    float t3 = std::exp(t0 / (lm * kBoltzmann * T)) - 1.0f;
+    return (t1 / t2) * (1.0f / t3);
+}
+
+void bin_weights(const mgk_spectral_t* sp, float* W) {
+    float sum_y = 0.0f;
+    float bin_range = (sp->bin_max_nm - sp->bin_min_nm) / float(sp->n_bin);
+    float tab_range = (830.0f - 360.0f) / float(471 - 1);
+    uint32_t sub = uint32_t(std::ceil(bin_range / tab_range));
+    float sub_w = bin_range / float(sub);
+    for (uint32_t i = 0; i < sp->n_bin; ++i) {
+        f3 w{0, 0, 0};
+        float x0 = sp->bin_min_nm + float(i) * bin_range;
+        for (uint32_t j = 0; j < sub; ++j) {
+            f3 c = cmf_at(x0 + (float(j) + 0.5f) * sub_w);
+            w = {w.x + c.x * sub_w, w.y + c.y * sub_w, w.z + c.z * sub_w};
+        }
+        W[3 * i] = w.x; W[3 * i + 1] = w.y; W[3 * i + 2] = w.z;
+        sum_y += w.y;
+    }
+    float k = 1.0f / sum_y, inv_bw = 1.0f / bin_range;
+    for (uint32_t i = 0; i < 3 * sp->n_bin; ++i) W[i] = W[i] * (k * inv_bw);
+}
+
+f3 xy_to_XYZ(float x, float y) { return {x / y, 1.0f, (1.0f - x - y) / y}; }
+
+// Blackbody white point shifted by tint along the locus normal (host_color_utilities.cuh:29-57).
+f3 white_point_xyz(float T, float tint) {
+    f3 a{0, 0, 0}, b{0, 0, 0};
+    float w = 360.0f, dw = (830.0f - 360.0f) / float(471 - 1);
+    for (uint32_t i = 0; i < 471; ++i) {
+        f3 c = cmf_at(w);
+        float p0 = planck_host(w, T), p1 = planck_host(w, T + 1);
+        a = {a.x + c.x * p0, a.y + c.y * p0, a.z + c.z * p0};
+        b = {b.x + c.x * p1, b.y + c.y * p1, b.z + c.z * p1};
+        w += dw;
+    }
+    a = {a.x / a.y, a.y / a.y, a.z / a.y};
+    b = {b.x / b.y, b.y / b.y, b.z / b.y};
+    auto uv = [](f3 v) {
+        float d = v.x + 15.0f * v.y + 3.0f * v.z;
+        return std::pair<float, float>{4.0f * v.x / d, 6.0f * v.y / d};
+    };
+    auto u0 = uv(a), u1 = uv(b);
+    float nx = u1.first - u0.first, ny = u1.second - u0.second;
+    float nl = std::sqrt(nx * nx + ny * ny);
+    nx /= nl; ny /= nl;
+    float tx = -ny, ty = nx;
+    if (ty < 0.0f) { tx = -tx; ty = -ty; }
+    float up = u0.first + tx * tint, vp = u0.second + ty * tint;
+    float den = 2.0f * up - 8.0f * vp + 4.0f;
+    return xy_to_XYZ(3.0f * up / den, 2.0f * vp / den);
+}
+
+// Bradford chromatic adaptation (host_color_utilities.cuh:59-68).
+m3 bradford(f3 src, f3 dst) {
+    const m3 M{{0.8951f, 0.2664f, -0.1614f, -0.7502f, 1.7135f, 0.0367f, 0.0389f, -0.0685f, 1.0296f}};
+    const m3 Mi{{0.9869929f, -0.1470543f, 0.1599627f, 0.4323053f, 0.5183603f, 0.0492912f, -0.0085287f, 0.0400428f, 0.9684867f}};
+    f3 ls = mul(M, src), ld = mul(M, dst);
+    m3 D{{ld.x / ls.x, 0, 0, 0, ld.y / ls.y, 0, 0, 0, ld.z / ls.z}};
+    return mul(mul(Mi, D), M);
+}
+
+// XYZ -> white-balanced linear Rec.709 (IEC 61966-2-1 / ITU-R BT.709 primaries, D65).
+m3 xyz_to_scene(float white_K) {
+    const m3 to709{{3.24100326f, -1.53739899f, -0.49861587f, -0.96922426f, 1.87592999f, 0.04155422f, 0.05563942f, -0.2040112f,
+                    1.05714897f}};
+    f3 src = white_point_xyz(white_K, 0.0f);
+    f3 d65 = xy_to_XYZ(0.31272f, 0.32903f);
+    return mul(to709, bradford(src, d65));
+}
+
+// ---------------------------------------------------------------------------
+// Star catalogue + equal-area sky grid (cos(theta) about +y x azimuth).
+// ---------------------------------------------------------------------------
+void star_catalogue(uint64_t seed, uint32_t n, float scale, std::vector<float4>& out) {
+    out.resize(n);
+    const uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32) ^ 0x53544152u;
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t c[4] = {i, 0u, 0u, 0x73746172u};
+        philox4x32_10(c, k0, k1);
+        float cz = 2.0f * u01(c[0]) - 1.0f;
+        float ph = 2.0f * kPi * u01(c[1]) - kPi;
+        float m = 8.0f * u01(c[2]);
+        float sn = std::sqrt(std::fmax(0.0f, 1.0f - cz * cz));
+        out[i] = make_float4(sn * std::cos(ph), cz, sn * std::sin(ph), scale * std::pow(10.0f, -0.4f * m));
+    }
+}
+
+void star_grid(const std::vector<float4>& st, float rho, uint32_t nc, uint32_t np, std::vector<uint32_t>& start,
+               std::vector<uint32_t>& items) {
+    std::vector<std::vector<uint32_t>> cells(size_t(nc) * np);
+    const double pi = 3.14159265358979323846;
+    const double half_c = 1.0 / nc, half_p = pi / np;   // half a cell of margin
+    for (uint32_t i = 0; i < st.size(); ++i) {
+        double c = st[i].y, th = std::acos(std::max(-1.0, std::min(1.0, c)));
+        double ph = std::atan2(double(st[i].z), double(st[i].x));
+        double t0 = th - rho, t1 = th + rho;
+        double c_hi = t0 <= 0 ? 1.0 : std::cos(t0), c_lo = t1 >= pi ? -1.0 : std::cos(t1);
+        c_hi = std::min(1.0, c_hi + half_c);
+        c_lo = std::max(-1.0, c_lo - half_c);
+        bool all_phi = t0 <= 0 || t1 >= pi;
+        double dph = pi;
+        if (!all_phi) {
+            double smin = std::min(std::sin(t0), std::sin(t1));
+            if (std::sin(rho) >= smin) all_phi = true;
+            else dph = std::asin(std::sin(rho) / smin) + half_p;
+        }
+        int ic0 = std::max(0, int(std::floor((c_lo + 1.0) * 0.5 * nc)));
+        int ic1 = std::min(int(nc) - 1, int(std::floor((c_hi + 1.0) * 0.5 * nc)));
+        int ip0, ip1;
+        if (all_phi || dph >= pi) { ip0 = 0; ip1 = int(np) - 1; }
+        else {
+            ip0 = int(std::floor((ph - dph + pi) / (2 * pi) * np));
+            ip1 = int(std::floor((ph + dph + pi) / (2 * pi) * np));
+        }
+        for (int a = ic0; a <= ic1; ++a)
+            for (int b = ip0; b <= ip1; ++b) {
+                int bb = ((b % int(np)) + int(np)) % int(np);
+                cells[size_t(a) * np + bb].push_back(i);
+            }
+    }
+    start.assign(size_t(nc) * np + 1, 0);
+    items.clear();
+    for (size_t k = 0; k < cells.size(); ++k) {
+        start[k] = uint32_t(items.size());
+        // a star can land twice in one cell when its azimuth window wraps
+        std::sort(cells[k].begin(), cells[k].end());
+        cells[k].erase(std::unique(cells[k].begin(), cells[k].end()), cells[k].end());
+        items.insert(items.end(), cells[k].begin(), cells[k].end());
+    }
+    start.back() = uint32_t(items.size());
+}
+
+e_magik_result_types ensure_stars(mgk_context* c, const mgk_scene_t* sc) {
+    if (!(sc->flags & MGK_SCENE_STARS) || sc->n_star == 0) return MAGIK_SUCCESS;
+    if (c->star_key == sc->star_seed && c->star_n == sc->n_star && c->star_rho == sc->star_radius_rad &&
+        c->star_scale == sc->star_scale)
+        return MAGIK_SUCCESS;
+    std::vector<float4> st;
+    star_catalogue(sc->star_seed, sc->n_star, sc->star_scale, st);
+    std::vector<uint32_t> start, items;
+    const uint32_t nc = 256, np = 512;
+    star_grid(st, sc->star_radius_rad, nc, np, start, items);
+    HIP_TRY(c->stars.reserve(st.size() * sizeof(float4)));
+    HIP_TRY(c->cell_start.reserve(start.size() * sizeof(uint32_t)));
+    HIP_TRY(c->cell_items.reserve(std::max<size_t>(1, items.size()) * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(c->stars.ptr, st.data(), st.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->cell_start.ptr, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!items.empty()) HIP_TRY(hipMemcpy(c->cell_items.ptr, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->grid_nc = nc;
+    c->grid_np = np;
+    c->star_key = sc->star_seed;
+    c->star_n = sc->n_star;
+    c->star_rho = sc->star_radius_rad;
+    c->star_scale = sc->star_scale;
+    return MAGIK_SUCCESS;
+}
+
+e_magik_result_types fill_shade_params(mgk_context* c, const mgk_scene_t* sc, const mgk_spectral_t* sp, const mgk_camera_t* cam,
+                                       uint64_t seed, uint32_t sample_begin, uint32_t spp, const mgk_hit_t* hits, float* xyz,
+                                       ShadeParams& P) {
+    if (sp->n_bin == 0 || sp->n_bin > uint32_t(kMaxBins) || sp->n_hero == 0 || sp->n_hero > 4 || !(sp->sigma_nm > 0.0f))
+        return MGK_ERROR_INVALID_ARGUMENT;
+    e_magik_result_types r = ensure_stars(c, sc);
+    if (r != MAGIK_SUCCESS) return r;
+    std::memset(&P, 0, sizeof(P));
+    P.W = cam->x_resolution;
+    P.H = cam->y_resolution;
+    P.spp = spp;
+    P.sample_begin = sample_begin;
+    P.camera_id = cam->camera_id;
+    P.seed = seed;
+    P.hits = hits;
+    P.xyz = xyz;
+    P.flags = sc->flags;
+    P.disk_r_in = sc->disk_r_in;
+    P.disk_T_in = sc->disk_T_in;
+    P.sphere_T = sc->sphere_T;
+    P.sphere_checker = sc->sphere_checker ? sc->sphere_checker : 1;
+    P.background_scale = sc->background_scale;
+    P.background_T = sc->background_T;
+    P.star_T = sc->star_T;
+    P.cos_star_r = std::cos(sc->star_radius_rad);
+    if ((sc->flags & MGK_SCENE_STARS) && sc->n_star > 0) {
+        P.stars = reinterpret_cast<const float4*>(c->stars.ptr);
+        P.cell_start = reinterpret_cast<const uint32_t*>(c->cell_start.ptr);
+        P.cell_items = reinterpret_cast<const uint32_t*>(c->cell_items.ptr);
+        P.grid_nc = c->grid_nc;
+        P.grid_np = c->grid_np;
+    } else {
+        P.flags &= ~uint32_t(MGK_SCENE_STARS);
+    }
+    P.n_hero = sp->n_hero;
+    P.hero_min = sp->hero_min_nm;
+    P.hero_span = sp->hero_max_nm - sp->hero_min_nm;
+    P.n_bin = sp->n_bin;
+    P.bin_min = sp->bin_min_nm;
+    P.bin_width = (sp->bin_max_nm - sp->bin_min_nm) / float(sp->n_bin);
+    P.sigma = sp->sigma_nm;
+    // radiance_to_irradiance (device_color_utilities.cuh:22-29) for throughput span / n_hero
+    float inv_b = cam->sensor_width_mm / std::sqrt(float(cam->x_resolution * cam->x_resolution + cam->y_resolution * cam->y_resolution));
+    float sw = float(cam->x_resolution) * inv_b, sh = float(cam->y_resolution) * inv_b;
+    float area = (sw * sh) / 1e6f;
+    float npx = float(cam->x_resolution * cam->y_resolution);
+    float thr = P.hero_span / float(sp->n_hero);
+    P.irr_scale = thr * (area / npx) * sp->iso * 2.0f * kPi / ((360.0f * sp->fps) / sp->shutter_angle);
+    bin_weights(sp, P.weights);
+    return MAGIK_SUCCESS;
+}
+
+e_magik_result_types fill_raygen(const mgk_integrator_t* in, const mgk_camera_t* cam, uint64_t seed, uint32_t sample_begin,
+                                 uint32_t spp, void* rays, uint64_t stride, RayGenParams& P) {
+    std::memset(&P, 0, sizeof(P));
+    P.metric = in->metric_type;
+    P.precision = in->precision;
+    P.time_reversed = in->time_reversed;
+    P.spin = signed_spin(in);
+    P.mass_sign = mass_sign_of(in);
+    P.length_scale = length_scale_of(in);
+    for (int k = 0; k < 3; ++k) P.position[k] = cam->position[k];
+    for (int k = 0; k < 9; ++k) P.orientation[k] = cam->orientation[k];
+    P.focal_mm = cam->focal_length_mm;
+    P.sensor_width_mm = cam->sensor_width_mm;
+    P.W = cam->x_resolution;
+    P.H = cam->y_resolution;
+    P.camera_id = cam->camera_id;
+    for (int k = 0; k < 3; ++k) P.observer_velocity[k] = cam->observer_velocity[k];
+    P.seed = seed;
+    P.sample_begin = sample_begin;
+    P.spp = spp;
+    P.rays = rays;
+    P.stride = stride;
+    if (P.W == 0 || P.H == 0) return MGK_ERROR_INVALID_ARGUMENT;
+    if (uint64_t(P.W) * P.H * spp > stride) return MGK_ERROR_CAPACITY_EXCEEDED;
+    return MAGIK_SUCCESS;
+}
+
+size_t ray_elem(const mgk_integrator_t* in) { return in->precision == MGK_PRECISION_FP32_REFERENCE ? 4 : 8; }
+
+}  // namespace
+
+extern "C" {
+
+MGK_API e_magik_result_types magik_get_version(uint32_t* major, uint32_t* minor, uint32_t* revision, const char** as_char) {
+    if (major) *major = 0;
+    if (minor) *minor = 7;
+    if (revision) *revision = 0;
+    if (as_char) *as_char = "magik_amd 0.7.0 - gfx950";
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_context_create(int device, mgk_context_handle* out) {
+    if (!out) return MAGIK_ERROR_INVALID_POINTER;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MAGIK_INVALID_CUDA_DEVICE;
+    mgk_context* c = new (std::nothrow) mgk_context();
+    if (!c) return MAGIK_ERROR_HOST_OUT_OF_MEMORY;
+    c->device = device;
+    auto fail = [&](e_magik_result_types r) {
+        mgk_context_destroy(c);
+        return r;
+    };
+    if (set_device(c) != MAGIK_SUCCESS) return fail(MGK_ERROR_HIP);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MGK_ERROR_HIP);
+    c->n_cu = prop.multiProcessorCount;
+    if (hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess) return fail(MAGIK_ERROR_DEVICE_ALLOCATION_FAILED);
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(MGK_ERROR_HIP);
+    *out = c;
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_context_destroy(mgk_context_handle c) {
+    if (!c) return MAGIK_ERROR_INVALID_POINTER;
+    (void)hipSetDevice(c->device);
+    (void)hipDeviceSynchronize();
+    if (c->d_counters) (void)hipFree(c->d_counters);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    c->stars.release();
+    c->cell_start.release();
+    c->cell_items.release();
+    c->rays.release();
+    c->hits.release();
+    delete c;
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_set_scene(mgk_context_handle c, const mgk_scene_t* sc) {
+    if (!c || !sc) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = set_device(c);
+    if (r != MAGIK_SUCCESS) return r;
+    return ensure_stars(c, sc);
+}
+
+MGK_API e_magik_result_types mgk_generate_rays(mgk_context_handle c, const mgk_integrator_t* in, const mgk_camera_t* cam, uint64_t seed,
+                                               uint32_t sample_begin, uint32_t spp, void* d_rays, uint64_t stride, void* stream) {
+    if (!c || !cam || !d_rays) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = check_integrator(in);
+    if (r != MAGIK_SUCCESS) return r;
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    RayGenParams P;
+    if ((r = fill_raygen(in, cam, seed, sample_begin, spp, d_rays, stride, P)) != MAGIK_SUCCESS) return r;
+    HIP_TRY(launch_generate_rays(P, hipStream_t(stream)));
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_trace_rays(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc, const float cam_pos[3],
+                                            const void* d_rays, uint64_t stride, uint64_t n, const uint32_t* d_queue, mgk_hit_t* d_hits,
+                                            uint32_t* d_steps, void* stream) {
+    if (!c || !sc || !cam_pos || !d_hits || (n > 0 && !d_rays)) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = check_integrator(in);
+    if (r != MAGIK_SUCCESS) return r;
+    if (!d_queue && n > stride) return MGK_ERROR_CAPACITY_EXCEEDED;
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    TraceParams P;
+    fill_trace_params(in, sc, cam_pos, P);
+    P.rays = d_rays;
+    P.stride = stride;
+    P.n = n;
+    P.queue = d_queue;
+    P.hits = d_hits;
+    P.steps_out = d_steps;
+    return do_trace(c, P, hipStream_t(stream));
+}
+
+MGK_API e_magik_result_types mgk_last_trace_stats(mgk_context_handle c, mgk_trace_stats_t* out) {
+    if (!c || !out) return MAGIK_ERROR_INVALID_POINTER;
+    std::memset(out, 0, sizeof(*out));
+    if (!c->timed) return MAGIK_SUCCESS;
+    e_magik_result_types r = set_device(c);
+    if (r != MAGIK_SUCCESS) return r;
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    unsigned long long h[4];
+    HIP_TRY(hipMemcpy(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost));
+    out->n_rays = c->last_n;
+    out->n_steps = h[1];
+    out->n_trials = h[2];
+    out->n_trials_fp64 = h[3];
+    out->ms = double(ms);
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_shade(mgk_context_handle c, const mgk_scene_t* sc, const mgk_spectral_t* sp, const mgk_camera_t* cam,
+                                       uint64_t seed, uint32_t sample_begin, uint32_t spp, const mgk_hit_t* d_hits, float* d_xyz,
+                                       void* stream) {
+    if (!c || !sc || !sp || !cam || !d_hits || !d_xyz) return MAGIK_ERROR_INVALID_POINTER;
+    if (spp == 0) return MGK_ERROR_INVALID_ARGUMENT;
+    e_magik_result_types r = set_device(c);
+    if (r != MAGIK_SUCCESS) return r;
+    ShadeParams P;
+    if ((r = fill_shade_params(c, sc, sp, cam, seed, sample_begin, spp, d_hits, d_xyz, P)) != MAGIK_SUCCESS) return r;
+    HIP_TRY(launch_shade(P, hipStream_t(stream)));
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc,
+                                              const mgk_spectral_t* sp, const mgk_camera_t* cam, uint64_t seed, uint32_t spp,
+                                              float* d_xyz, void* stream) {
+    if (!c || !sc || !sp || !cam || !d_xyz) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = check_integrator(in);
+    if (r != MAGIK_SUCCESS) return r;
+    if (spp == 0) return MGK_ERROR_INVALID_ARGUMENT;
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    const uint64_t n = uint64_t(cam->x_resolution) * cam->y_resolution * spp;
+    HIP_TRY(c->rays.reserve(n * 10 * ray_elem(in)));
+    HIP_TRY(c->hits.reserve(n * sizeof(mgk_hit_t)));
+    hipStream_t s = hipStream_t(stream);
+    RayGenParams G;
+    if ((r = fill_raygen(in, cam, seed, 0, spp, c->rays.ptr, n, G)) != MAGIK_SUCCESS) return r;
+    HIP_TRY(launch_generate_rays(G, s));
+    TraceParams P;
+    fill_trace_params(in, sc, cam->position, P);
+    P.rays = c->rays.ptr;
+    P.stride = n;
+    P.n = n;
+    P.hits = reinterpret_cast<mgk_hit_t*>(c->hits.ptr);
+    if ((r = do_trace(c, P, s)) != MAGIK_SUCCESS) return r;
+    ShadeParams S;
+    if ((r = fill_shade_params(c, sc, sp, cam, seed, 0, spp, reinterpret_cast<mgk_hit_t*>(c->hits.ptr), d_xyz, S)) != MAGIK_SUCCESS)
+        return r;
+    HIP_TRY(launch_shade(S, s));
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_color_weights(const mgk_spectral_t* sp, float* weights_xyz, float* xyz_to_scene_out) {
+    if (!sp) return MAGIK_ERROR_INVALID_POINTER;
+    if (sp->n_bin == 0 || sp->n_bin > uint32_t(kMaxBins)) return MGK_ERROR_INVALID_ARGUMENT;
+    if (weights_xyz) bin_weights(sp, weights_xyz);
+    if (xyz_to_scene_out) {
+        m3 m = xyz_to_scene(sp->white_point_K > 0.0f ? sp->white_point_K : 5600.0f);
+        std::memcpy(xyz_to_scene_out, m.m, sizeof(m.m));
+    }
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_camera_boyer_lindquist(float r_obs, float theta_deg, float spin, float mass_kg, float fov_deg,
+                                                        uint32_t x_res, uint32_t y_res, mgk_camera_t* out) {
+    if (!out) return MAGIK_ERROR_INVALID_POINTER;
+    if (!(r_obs > 0.0f) || !(fov_deg > 0.0f && fov_deg < 180.0f) || x_res == 0 || y_res == 0) return MGK_ERROR_INVALID_ARGUMENT;
+    std::memset(out, 0, sizeof(*out));
+    const double d2r = 3.14159265358979323846 / 180.0;
+    double th = double(theta_deg) * d2r, r = r_obs, a = spin;
+    // KS Cartesian point of BL (r, theta) at the azimuth where x = 0:
+    // y = r cos(theta), z = sqrt(r^2 + a^2) sin(theta)
+    double P[3] = {0.0, r * std::cos(th), std::sqrt(r * r + a * a) * std::sin(th)};
+    double L = mass_kg == 0.0f ? 1.0 : double(kG) * double(mass_kg) / (double(kC) * double(kC));
+    double n = std::sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
+    double zc[3] = {P[0] / n, P[1] / n, P[2] / n};   // camera looks down -z_cam, i.e. at the origin
+    double up[3] = {0.0, 1.0, 0.0};
+    double d = up[0] * zc[0] + up[1] * zc[1] + up[2] * zc[2];
+    double yc[3] = {up[0] - d * zc[0], up[1] - d * zc[1], up[2] - d * zc[2]};
+    double yl = std::sqrt(yc[0] * yc[0] + yc[1] * yc[1] + yc[2] * yc[2]);
+    if (yl < 1e-9) return MGK_ERROR_INVALID_ARGUMENT;   // on the spin axis
+    for (int k = 0; k < 3; ++k) yc[k] /= yl;
+    double xc[3] = {yc[1] * zc[2] - yc[2] * zc[1], yc[2] * zc[0] - yc[0] * zc[2], yc[0] * zc[1] - yc[1] * zc[0]};
+    for (int k = 0; k < 3; ++k) out->position[k] = float(P[k] * L);
+    for (int rrow = 0; rrow < 3; ++rrow) {
+        out->orientation[3 * rrow + 0] = float(xc[rrow]);
+        out->orientation[3 * rrow + 1] = float(yc[rrow]);
+        out->orientation[3 * rrow + 2] = float(zc[rrow]);
+    }
+    out->sensor_width_mm = 24.0f;
+    out->focal_length_mm = float(12.0 / std::tan(0.5 * double(fov_deg) * d2r));
+    out->x_resolution = x_res;
+    out->y_resolution = y_res;
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types magik_trace_geodesic(magik_trace_geodesic_container_t* container,
+                                                  magik_trace_geodesic_inital_conditions_t ic,
+                                                  magik_trace_geodesic_metric_parameters_t mp,
+                                                  magik_trace_geodesic_integrator_parameters_t ip) {
+    if (!container || !container->steps) return MAGIK_ERROR_INVALID_POINTER;
+    if (container->capacity != sizeof(magik_trace_geodesic_step_t) * container->n_allocated_step)
+        return MAGIK_ERROR_HOST_TRACE_ALLOCATED_STEPS_AND_CAPACITY_MISMATCH;
+    if (!valid_metric(mp.metric_type)) return MAGIK_ERROR_UNKNOWN;
+    container->n_written = 0;
+    if (container->n_allocated_step == 0) return MAGIK_SUCCESS;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MAGIK_INVALID_CUDA_DEVICE;
+    static std::mutex mu;   // one in-flight host-API trace at a time
+    std::lock_guard<std::mutex> lock(mu);
+    HIP_TRY(hipSetDevice(0));
+    GeodesicStepParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.metric = mp.metric_type;
+    P.time_reversed = mp.time_reversed ? 1u : 0u;
+    P.n_alloc = container->n_allocated_step;
+    float mass = mp.metric_type == KERR ? mp.kerr_mass : mp.schwarzschild_mass;
+    P.spin = mp.metric_type == KERR ? (mp.time_reversed ? -mp.kerr_spin : mp.kerr_spin) : 0.0f;
+    P.mass_sign = (mp.metric_type == SCHWARZSCHILD && mass <= 0.0f) ? -1.0f : 1.0f;
+    P.length_scale = mp.metric_type == MINKOWSKI ? 1.0f : kG * (mp.metric_type == SCHWARZSCHILD ? std::fabs(mass) : mass) / (kC * kC);
+    P.spin_in_range = std::fabs(P.spin) > 1.0f ? 0 : 1;
+    P.horizon_r = mp.metric_type == KERR ? (1.0f + std::sqrt(1.0f - P.spin * P.spin)) * mp.event_horizon_scalar
+                                         : 2.0f * mp.event_horizon_scalar;
+    P.h_max = 1000.0f;
+    P.abs_tol = ip.integrator_absolute_error_tolerance;
+    P.rel_tol = ip.integrator_relative_error_tolerance;
+    P.n_max = ip.n_integrator_max_substep;
+    P.position[0] = ic.spatial_position_x;
+    P.position[1] = ic.spatial_position_y;
+    P.position[2] = ic.spatial_position_z;
+    // rotate(x_rot, y_rot, z_rot) = m_x(z_rot) m_y(y_rot) m_z(x_rot) (internal_math.h:1784-1810)
+    {
+        float ca = std::cos(ic.x_rotation), sa = std::sin(ic.x_rotation), cb = std::cos(ic.y_rotation), sb = std::sin(ic.y_rotation);
+        float cg = std::cos(ic.z_rotation), sg = std::sin(ic.z_rotation);
+        m3 mz{{ca, -sa, 0, sa, ca, 0, 0, 0, 1}}, my{{cb, 0, sb, 0, 1, 0, -sb, 0, cb}}, mx{{1, 0, 0, 0, cg, -sg, 0, sg, cg}};
+        m3 m = mul(mul(mx, my), mz);
+        std::memcpy(P.rot, m.m, sizeof(P.rot));
+    }
+    P.lpv[0] = ic.local_proper_velocity_x;
+    P.lpv[1] = ic.local_proper_velocity_y;
+    P.lpv[2] = ic.local_proper_velocity_z;
+    P.massive = ic.test_particle_type == MASSIVE ? 1 : 0;
+    P.speed = ic.massive_particle_speed;
+    magik_trace_geodesic_step_t* d_steps = nullptr;
+    uint32_t* d_n = nullptr;
+    size_t bytes = sizeof(magik_trace_geodesic_step_t) * container->n_allocated_step;
+    if (hipMalloc(&d_steps, bytes) != hipSuccess) return MAGIK_ERROR_DEVICE_ALLOCATION_FAILED;
+    if (hipMalloc(&d_n, sizeof(uint32_t)) != hipSuccess) {
+        (void)hipFree(d_steps);
+        return MAGIK_ERROR_DEVICE_ALLOCATION_FAILED;
+    }
+    P.steps = d_steps;
+    P.n_written = d_n;
+    e_magik_result_types res = MAGIK_SUCCESS;
+    uint32_t n = 0;
+    if (launch_geodesic_steps(P, nullptr) != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(&n, d_n, sizeof(n), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(container->steps, d_steps, sizeof(magik_trace_geodesic_step_t) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        res = MGK_ERROR_HIP;
+    else
+        container->n_written = n;
+    (void)hipFree(d_steps);
+    (void)hipFree(d_n);
+    return res;
+}
+
+}  // extern "C"

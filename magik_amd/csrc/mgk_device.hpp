@@ -1,0 +1,359 @@
+// Device-side arithmetic of the geodesic tracer for gfx950 (CDNA4, wave64).
+//
+// Everything here is __device__ code written for one ray per lane: SoA loads,
+// registers only, no local memory.  The spacetime is a compile-time template
+// parameter (the reference's metric<T> plugin seam,
+// relativity_utilities.cuh:97-120); the real type S selects FP32 or FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "magik_amd.h"
+
+namespace mgk {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kC = 299792458.0f;
+constexpr float kPlanck = 6.62607015E-34f;
+constexpr float kBoltzmann = 1.380649E-23f;
+constexpr float kG = 6.67430E-11f;
+
+// ---------------------------------------------------------------------------
+// Reciprocal / square root.  FAST selects the single-instruction gfx950
+// v_rcp_f32 / v_sqrt_f32 (1 ulp) for the bulk FP32 right-hand side; otherwise
+// IEEE-rounded division and square root (the reference-parity and FP64 paths).
+// ---------------------------------------------------------------------------
+template <bool FAST> __device__ __forceinline__ float rcp(float x) {
+    if constexpr (FAST) return __builtin_amdgcn_rcpf(x);
+    else return 1.0f / x;
+}
+template <bool FAST> __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+template <bool FAST> __device__ __forceinline__ float sqrt_(float x) {
+    if constexpr (FAST) return __builtin_amdgcn_sqrtf(x);
+    else return __fsqrt_rn(x);
+}
+template <bool FAST> __device__ __forceinline__ double sqrt_(double x) { return __dsqrt_rn(x); }
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Counter (pixel, sample, block, 0),
+// key derived from (seed, camera): the per-path random stream.
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ uint32_t mulhi32(uint32_t a, uint32_t b) {
+    return uint32_t((uint64_t(a) * uint64_t(b)) >> 32);
+}
+__host__ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c[0], hi0 = mulhi32(0xD2511F53u, c[0]);
+        uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = mulhi32(0xCD9E8D57u, c[2]);
+        uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+__host__ __device__ __forceinline__ float u01(uint32_t v) { return float(v >> 8) * (1.0f / 16777216.0f); }
+__device__ __forceinline__ void path_uniforms(uint64_t seed, uint32_t cam, uint32_t pixel, uint32_t sample, uint32_t block,
+                                              float u[4]) {
+    uint32_t c[4] = {pixel, sample, block, 0u};
+    philox4x32_10(c, uint32_t(seed) ^ (cam * 0x9E3779B9u), uint32_t(seed >> 32) + cam);
+    u[0] = u01(c[0]); u[1] = u01(c[1]); u[2] = u01(c[2]); u[3] = u01(c[3]);
+}
+
+// ---------------------------------------------------------------------------
+// Spacetime parameters shared by every kernel of one launch.
+// ---------------------------------------------------------------------------
+struct MetricArgs {
+    float spin;          // signed spin of the equations (negated when time reversed)
+    float mass_sign;     // Schwarzschild
+    float horizon_scalar;
+    float length_scale;  // G M / c^2 in metres (1 for Minkowski)
+    int time_reversed;
+};
+
+// Geodesic state x^mu (q) and covariant p_mu, t last.
+template <class S> struct St {
+    S qx, qy, qz, qt, px, py, pz, pt;
+};
+
+// Right-hand side split as dq = P(p) - c, dp = (dpx, dpy, dpz, 0), with
+// P(p) = (p_x, p_y, p_z, -p_t) and c = f (k.p - p_t) (k_x, k_y, k_z, -1): the
+// Kerr-Schild Hamiltonian equations of metric_kerr.cuh:38-105 (Schwarzschild:
+// metric_schwarzschild.cuh:8-72; Minkowski: c = 0, dp = 0).
+template <class S> struct Rhs {
+    S cx, cy, cz, ct, dpx, dpy, dpz;
+};
+
+template <int METRIC, class S, bool FAST>
+__device__ __forceinline__ Rhs<S> rhs(S x, S y, S z, S px, S py, S pz, S pt, S a, S mass_sign) {
+    Rhs<S> o;
+    if constexpr (METRIC == MINKOWSKI) {
+        o.cx = o.cy = o.cz = o.ct = o.dpx = o.dpy = o.dpz = S(0);
+        return o;
+    } else {
+        S a_sq, w_sq, sigma;
+        if constexpr (METRIC == KERR) {
+            a_sq = a * a;
+            w_sq = (x * x + y * y + z * z) - a_sq;
+            sigma = sqrt_<FAST>(S(4) * a_sq * (y * y) + w_sq * w_sq);
+        } else {
+            a_sq = S(0);
+            w_sq = (x * x + y * y + z * z);
+            sigma = w_sq;
+        }
+        S r = sqrt_<FAST>(S(0.5) * (w_sq + sigma));
+        S r_sq = r * r, r_cu = r_sq * r;
+        S inv_r = rcp<FAST>(r);
+        S inv_rsq = inv_r * inv_r;
+        S inv_sigma = rcp<FAST>(sigma);
+        S scale_xz = S(0.5) * inv_r * (S(1) + w_sq * inv_sigma);
+        S scale_y = S(0.5) * inv_r * (S(1) + (w_sq + S(2) * a_sq) * inv_sigma);
+        S drx = x * scale_xz, dry = y * scale_y, drz = z * scale_xz;
+        S d = r_sq + a_sq;
+        S inv_d = rcp<FAST>(d);
+        S inv_dsq = inv_d * inv_d;
+        S kx_num = r * x - a * z;
+        S kz_num = r * z + a * x;
+        S kx = kx_num * inv_d, kz = kz_num * inv_d, ky = y * inv_r;
+        S f_den = r_sq * r_sq + a_sq * (y * y);
+        S inv_f_den = rcp<FAST>(f_den);
+        S inv_f_den_sq = inv_f_den * inv_f_den;
+        S f, dfs, dfy_extra;
+        if constexpr (METRIC == KERR) {
+            f = S(2) * r_cu * inv_f_den;
+            dfs = S(2) * (S(3) * a_sq * r_sq * (y * y) - r_cu * r_cu) * inv_f_den_sq;
+            dfy_extra = S(4) * a_sq * r_cu * y * inv_f_den_sq;
+        } else {
+            f = mass_sign * S(2) * r_cu * inv_f_den;
+            dfs = mass_sign * S(2) * -(r_cu * r_cu) * inv_f_den_sq;
+            dfy_extra = S(0);
+        }
+        S dfx = dfs * drx, dfy = dfs * dry - dfy_extra, dfz = dfs * drz;
+        S two_r = S(2) * r;
+        S dkx_dx = ((x * drx + r) * d - two_r * drx * kx_num) * inv_dsq;
+        S dkx_dy = ((x * dry) * d - two_r * dry * kx_num) * inv_dsq;
+        S dkx_dz = ((x * drz - a) * d - two_r * drz * kx_num) * inv_dsq;
+        S dky_dx = -y * drx * inv_rsq;
+        S dky_dy = (r - y * dry) * inv_rsq;
+        S dky_dz = -y * drz * inv_rsq;
+        S dkz_dx = ((a + z * drx) * d - two_r * drx * kz_num) * inv_dsq;
+        S dkz_dy = ((z * dry) * d - two_r * dry * kz_num) * inv_dsq;
+        S dkz_dz = ((r + z * drz) * d - two_r * drz * kz_num) * inv_dsq;
+        S kTp = px * kx + py * ky + pz * kz - pt;
+        S f_kTp = f * kTp;
+        S half_kTp_sq = S(0.5) * kTp * kTp;
+        S bx = px * dkx_dx + py * dky_dx + pz * dkz_dx;
+        S by = px * dkx_dy + py * dky_dy + pz * dkz_dy;
+        S bz = px * dkx_dz + py * dky_dz + pz * dkz_dz;
+        o.cx = f_kTp * kx;
+        o.cy = f_kTp * ky;
+        o.cz = f_kTp * kz;
+        o.ct = -f_kTp;
+        o.dpx = half_kTp_sq * dfx + f_kTp * bx;
+        o.dpy = half_kTp_sq * dfy + f_kTp * by;
+        o.dpz = half_kTp_sq * dfz + f_kTp * bz;
+        return o;
+    }
+}
+
+// The same equations in the exact operation order of the reference
+// (metric_kerr.cuh:38-105), used by the FP32 reference-parity path so its
+// rounding follows the reference integrator's.
+template <int METRIC, class S>
+__device__ __forceinline__ St<S> rhs_ref(const St<S>& s, S a, S mass_sign) {
+#pragma clang fp contract(off)
+    St<S> o;
+    if constexpr (METRIC == MINKOWSKI) {
+        o.qx = s.px; o.qy = s.py; o.qz = s.pz; o.qt = -s.pt;
+        o.px = o.py = o.pz = o.pt = S(0);
+        return o;
+    } else {
+        S x = s.qx, y = s.qy, z = s.qz, pt = s.pt, px = s.px, py = s.py, pz = s.pz;
+        S a_sq, y_sq = y * y, w_sq, sigma;
+        if constexpr (METRIC == KERR) {
+            a_sq = a * a;
+            w_sq = (x * x + y * y + z * z) - a_sq;
+            sigma = sqrt_<false>(S(4) * a_sq * y_sq + w_sq * w_sq);
+        } else {
+            a_sq = S(0);
+            a = S(0);
+            w_sq = (x * x + y * y + z * z);
+            sigma = w_sq;
+        }
+        S r = sqrt_<false>(S(0.5) * (w_sq + sigma));
+        S r_sq = r * r, r_cu = r_sq * r, r_quad = r_sq * r_sq, r_six = r_cu * r_cu;
+        S inv_r = S(1) / r;
+        S inv_rsq = inv_r * inv_r;
+        S inv_sigma = S(1) / sigma;
+        S scale_xz = S(0.5) * inv_r * (S(1) + w_sq * inv_sigma);
+        S scale_y = METRIC == KERR ? S(0.5) * inv_r * (S(1) + (w_sq + S(2) * a_sq) * inv_sigma)
+                                   : S(0.5) * inv_r * (S(1) + w_sq * inv_sigma);
+        S drx = x * scale_xz, dry = y * scale_y, drz = z * scale_xz;
+        S d = r_sq + a_sq;
+        S inv_d = S(1) / d;
+        S inv_dsq = inv_d * inv_d;
+        S kx_num = METRIC == KERR ? r * x - a * z : r * x;
+        S kz_num = METRIC == KERR ? r * z + a * x : r * z;
+        S kx = kx_num * inv_d, kz = kz_num * inv_d, ky = y * inv_r;
+        S f_den = METRIC == KERR ? r_quad + a_sq * y_sq : r_quad;
+        S inv_f_den_sq = S(1) / (f_den * f_den);
+        S f, dfs, dfx, dfy, dfz;
+        if constexpr (METRIC == KERR) {
+            f = S(2) * r_cu / f_den;
+            dfs = S(2) * (S(3) * a_sq * r_sq * y_sq - r_six) * inv_f_den_sq;
+            dfx = dfs * drx;
+            dfy = dfs * dry - S(4) * a_sq * r_cu * y * inv_f_den_sq;
+            dfz = dfs * drz;
+        } else {
+            f = mass_sign * S(2) * r_cu / f_den;
+            dfs = mass_sign * S(2) * -r_six * inv_f_den_sq;
+            dfx = dfs * drx;
+            dfy = dfs * dry;
+            dfz = dfs * drz;
+        }
+        S dkx_dx = ((x * drx + r) * d - S(2) * r * drx * kx_num) * inv_dsq;
+        S dkx_dy = ((x * dry) * d - S(2) * r * dry * kx_num) * inv_dsq;
+        S dkx_dz = (METRIC == KERR ? (x * drz - a) * d : (x * drz) * d) - S(2) * r * drz * kx_num;
+        dkx_dz = dkx_dz * inv_dsq;
+        S dky_dx = -y * drx * inv_rsq;
+        S dky_dy = (r - y * dry) * inv_rsq;
+        S dky_dz = -y * drz * inv_rsq;
+        S dkz_dx = (METRIC == KERR ? (a + z * drx) * d : (z * drx) * d) - S(2) * r * drx * kz_num;
+        dkz_dx = dkz_dx * inv_dsq;
+        S dkz_dy = ((z * dry) * d - S(2) * r * dry * kz_num) * inv_dsq;
+        S dkz_dz = ((r + z * drz) * d - S(2) * r * drz * kz_num) * inv_dsq;
+        S kTp = px * kx + py * ky + pz * kz - pt;
+        S f_kTp = f * kTp;
+        S half_kTp_sq = S(0.5) * kTp * kTp;
+        S bx = px * dkx_dx + py * dky_dx + pz * dkz_dx;
+        S by = px * dkx_dy + py * dky_dy + pz * dkz_dy;
+        S bz = px * dkx_dz + py * dky_dz + pz * dkz_dz;
+        o.qx = px - f_kTp * kx; o.qy = py - f_kTp * ky; o.qz = pz - f_kTp * kz; o.qt = -pt + f_kTp;
+        o.px = half_kTp_sq * dfx + f_kTp * bx;
+        o.py = half_kTp_sq * dfy + f_kTp * by;
+        o.pz = half_kTp_sq * dfz + f_kTp * bz;
+        o.pt = S(0);
+        return o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Metric tensor pieces: f and k of g = eta + f l l (metric_kerr.cuh:8-33,
+// :177-205; metric_schwarzschild.cuh:143-165).
+// ---------------------------------------------------------------------------
+template <class S> struct Sym4 { S tt, tx, ty, tz, xx, xy, xz, yy, yz, zz; };
+
+template <class S> __device__ __forceinline__ S kerr_schild_r(S x, S y, S z, S a) {
+    S R = (x * x + y * y + z * z - a * a);
+    S r2 = S(0.5) * R + S(0.5) * sqrt_<false>(R * R + S(4) * a * a * y * y);
+    return sqrt_<false>(r2);
+}
+
+template <int METRIC, class S> __device__ __forceinline__ Sym4<S> metric_g(S x, S y, S z, S a, S mass_sign, bool inverse) {
+    if constexpr (METRIC == MINKOWSKI) {
+        return {S(-1), S(0), S(0), S(0), S(1), S(0), S(0), S(1), S(0), S(1)};
+    } else {
+        S f, kx, ky, kz;
+        if constexpr (METRIC == KERR) {
+            S r = kerr_schild_r(x, y, z, a);
+            f = S(2) * r * r * r / (r * r * r * r + a * a * y * y);
+            kx = (r * x - a * z) / (r * r + a * a);
+            ky = y / r;
+            kz = (r * z + a * x) / (r * r + a * a);
+        } else {
+            S r = sqrt_<false>(x * x + y * y + z * z);
+            f = S(2) * mass_sign / r;
+            kx = x / r; ky = y / r; kz = z / r;
+        }
+        if (!inverse)
+            return {S(-1) + f, f * kx, f * ky, f * kz, S(1) + f * kx * kx, f * kx * ky, f * kx * kz, S(1) + f * ky * ky,
+                    f * ky * kz, S(1) + f * kz * kz};
+        return {S(-1) - f, f * kx, f * ky, f * kz, S(1) - f * kx * kx, -f * kx * ky, -f * kx * kz, S(1) - f * ky * ky,
+                -f * ky * kz, S(1) - f * kz * kz};
+    }
+}
+
+template <class S> struct V4 { S x, y, z, t; };
+
+template <class S> __device__ __forceinline__ S inner(const Sym4<S>& g, V4<S> u, V4<S> v) {
+    return g.tt * u.t * v.t + g.tx * (u.t * v.x + u.x * v.t) + g.ty * (u.t * v.y + u.y * v.t) + g.tz * (u.t * v.z + u.z * v.t) +
+           g.xx * u.x * v.x + g.xy * (u.x * v.y + u.y * v.x) + g.xz * (u.x * v.z + u.z * v.x) + g.yy * u.y * v.y +
+           g.yz * (u.y * v.z + u.z * v.y) + g.zz * u.z * v.z;
+}
+template <class S> __device__ __forceinline__ V4<S> lower(const Sym4<S>& g, V4<S> u) {
+    return {g.tx * u.t + g.xx * u.x + g.xy * u.y + g.xz * u.z, g.ty * u.t + g.xy * u.x + g.yy * u.y + g.yz * u.z,
+            g.tz * u.t + g.xz * u.x + g.yz * u.y + g.zz * u.z, g.tt * u.t + g.tx * u.x + g.ty * u.y + g.tz * u.z};
+}
+template <class S> __device__ __forceinline__ V4<S> axpy(S s, V4<S> a, V4<S> b) {   // b + s a
+    return {b.x + s * a.x, b.y + s * a.y, b.z + s * a.z, b.t + s * a.t};
+}
+template <class S> __device__ __forceinline__ V4<S> scale(V4<S> a, S s) { return {a.x * s, a.y * s, a.z * s, a.t * s}; }
+template <class S> __device__ __forceinline__ V4<S> divs(V4<S> a, S s) { return {a.x / s, a.y / s, a.z / s, a.t / s}; }
+template <class S> __device__ __forceinline__ S dot4(V4<S> a, V4<S> b) { return ((a.x * b.x) + (a.y * b.y) + (a.z * b.z)) + (a.t * b.t); }
+
+// Reference observer (metric_kerr.cuh:207-226; metric_schwarzschild.cuh:167-171).
+template <int METRIC, class S> __device__ __forceinline__ V4<S> reference_observer(S x, S y, S z, S a, S mass_sign) {
+    if constexpr (METRIC == KERR) {
+        S r = kerr_schild_r(x, y, z, a);
+        S r_sq = r * r, a_sq = a * a;
+        S ct = y / r;
+        S s2 = S(1) - ct * ct;
+        S sigma = r_sq + a_sq * ct * ct;
+        S delta = r_sq - S(2) * r + a_sq;
+        S Lam = (r_sq + a_sq) * (r_sq + a_sq) - a_sq * delta * s2;
+        S ut = sqrt_<false>(Lam / (delta * sigma));
+        S uph = S(2) * a * r / sqrt_<false>(Lam * delta * sigma);
+        return {z * uph, S(0), -x * uph, ut};
+    } else if constexpr (METRIC == SCHWARZSCHILD) {
+        Sym4<S> g = metric_g<METRIC, S>(x, y, z, a, mass_sign, false);
+        return {S(0), S(0), S(0), S(1) / sqrt_<false>(-g.tt)};
+    } else {
+        return {S(0), S(0), S(0), S(1)};
+    }
+}
+
+// Gram-Schmidt tetrad with e_t = u in the order z, x, y (relativity_utilities.cuh:141-156);
+// returns the global vector of local components (l.x, l.y, l.z, l.t) in that frame.
+template <class S>
+__device__ __forceinline__ V4<S> tetrad_apply(const Sym4<S>& g, V4<S> X, V4<S> Y, V4<S> Z, V4<S> u, V4<S> l) {
+    V4<S> ez = axpy(inner(g, Z, u), u, Z);
+    ez = divs(ez, sqrt_<false>(inner(g, ez, ez)));
+    V4<S> ex = axpy(-inner(g, X, ez), ez, axpy(inner(g, X, u), u, X));
+    ex = divs(ex, sqrt_<false>(inner(g, ex, ex)));
+    V4<S> ey = axpy(-inner(g, Y, ex), ex, axpy(-inner(g, Y, ez), ez, axpy(inner(g, Y, u), u, Y)));
+    ey = divs(ey, sqrt_<false>(inner(g, ey, ey)));
+    V4<S> r = scale(u, l.t);
+    r = axpy(l.x, ex, r);
+    r = axpy(l.y, ey, r);
+    r = axpy(l.z, ez, r);
+    return r;
+}
+
+// Keplerian emitter co-rotating with the equations' spin (Bardeen, Press &
+// Teukolsky 1972); see the oracle's kepler_velocity for the derivation.
+template <int METRIC, class S> __device__ __forceinline__ V4<S> kepler_velocity(S x, S z, S r, S a) {
+    if constexpr (METRIC == MINKOWSKI) {
+        return {S(0), S(0), S(0), S(1)};
+    } else {
+        S aa = METRIC == KERR ? (a < S(0) ? -a : a) : S(0);
+        S s = (METRIC == KERR && a < S(0)) ? S(-1) : S(1);
+        S sr = sqrt_<false>(r);
+        S r32 = r * sr;
+        S ut = (r32 + aa) / (sqrt_<false>(r32) * sqrt_<false>(r32 - S(3) * sr + S(2) * aa));
+        S w = ut * s / (r32 + aa);
+        return {z * w, S(0), -x * w, ut};
+    }
+}
+
+__device__ __forceinline__ float planck(float lambda_nm, float T) {
+    if (T <= 0.0f) return 0.0f;
+    float lm = lambda_nm * 1e-9f;
+    const float t0 = kPlanck * kC;
+    const float t1 = 2.0f * t0 * kC;
+    float t2 = lm * lm * lm * lm * lm;
+    float t3 = expf(t0 / (lm * kBoltzmann * T)) - 1.0f;
+    return (t1 / t2) * (1.0f / t3);
+}
+
+}  // namespace mgk

@@ -1,0 +1,96 @@
+// Launch-parameter structs shared by the kernels (mgk_kernels.hip) and the
+// C-ABI layer (mgk_api.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "magik_amd.h"
+
+namespace mgk {
+
+constexpr int kMaxBins = 64;
+
+struct RayGenParams {
+    int metric, precision;
+    uint32_t time_reversed;
+    float spin, mass_sign, length_scale;
+    float position[3];
+    float orientation[9];
+    float focal_mm, sensor_width_mm;
+    uint32_t W, H, camera_id;
+    float observer_velocity[3];
+    uint64_t seed;
+    uint32_t sample_begin, spp;
+    void* rays;
+    uint64_t stride;
+};
+
+struct TraceParams {
+    int metric, precision;
+    const void* rays;
+    uint64_t stride, n;
+    const uint32_t* queue;
+    mgk_hit_t* hits;
+    uint32_t* steps_out;
+    unsigned long long* head;    // work counter (zeroed by the launcher)
+    unsigned long long* stats;   // [0] steps [1] trials [2] fp64 trials
+    float spin, mass_sign, length_scale;
+    float h_max, abs_tol, rel_tol;
+    uint32_t n_max, step_limit;
+    float horizon_r_f;           // (1 + sqrt(1 - a^2)) * scalar, FP32 as the reference computes it
+    double horizon_r_d;
+    int spin_in_range;           // |a| <= 1 (metric_kerr.cuh:171)
+    float cam[3];
+    float cel_r2_f;              // celestial radius^2 (SI)
+    double cel_r2_d;
+    uint32_t flags;
+    float disk_r_in, disk_r_out;
+    float sphere_c[3], sphere_r;
+    double zone_rho2;            // mixed precision: FP64 trials while x^2+y^2+z^2 < this
+};
+
+struct ShadeParams {
+    uint32_t W, H, spp, sample_begin, camera_id;
+    uint64_t seed;
+    const mgk_hit_t* hits;
+    float* xyz;
+    uint32_t flags;
+    float disk_r_in, disk_T_in;
+    float sphere_T;
+    uint32_t sphere_checker;
+    float background_scale, background_T;
+    float star_T, cos_star_r;
+    const float4* stars;
+    const uint32_t* cell_start;
+    const uint32_t* cell_items;
+    uint32_t grid_nc, grid_np;
+    uint32_t n_hero;
+    float hero_min, hero_span;
+    uint32_t n_bin;
+    float bin_min, bin_width, sigma;
+    float irr_scale;             // radiance_to_irradiance factor per unit L * throughput
+    float weights[3 * kMaxBins];
+};
+
+struct GeodesicStepParams {
+    int metric;
+    uint32_t time_reversed, n_alloc;
+    float spin, mass_sign, length_scale, horizon_r, h_max, abs_tol, rel_tol;
+    uint32_t n_max;
+    int spin_in_range;
+    float position[3];
+    float rot[9];
+    float lpv[3];
+    int massive;
+    float speed;
+    magik_trace_geodesic_step_t* steps;
+    uint32_t* n_written;
+};
+
+hipError_t launch_generate_rays(const RayGenParams& p, hipStream_t s);
+hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s);
+hipError_t launch_shade(const ShadeParams& p, hipStream_t s);
+hipError_t launch_geodesic_steps(const GeodesicStepParams& p, hipStream_t s);
+
+}  // namespace mgk

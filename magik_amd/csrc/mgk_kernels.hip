@@ -1,0 +1,759 @@
+// HIP kernels of the wavefront spectral geodesic tracer for gfx950.
+//
+//   k_generate_rays  : pinhole ray + observer tetrad + null covariant momentum
+//                      per (pixel, sample), written SoA to HBM
+//                      (kernel_generate_thin_lens_ray.cu:51-77 fused with
+//                      kernel_generate_initial_conditions.cu:54-74).
+//   k_trace          : persistent RKF45 integrator.  Each wave64 pulls chunks
+//                      of ray indices from one device counter and refills idle
+//                      lanes by ballot/mbcnt, so lanes whose geodesics end early
+//                      take new rays instead of idling (replaces
+//                      kernel_advance_geodesic / kernel_intersect /
+//                      kernel_enforce_ray_limits and their host-driven queue
+//                      loop, wavefront.cpp:90-117).  One RKF45 trial per loop
+//                      iteration, so rejected trials in one lane do not stall
+//                      the others.
+//   k_shade          : per pixel, all samples: emission, g^5 boost, sensor
+//                      irradiance, Gaussian redistribution into spectral bins
+//                      (in LDS), fold to CIE XYZ (kernel_bxdf.cu emit path,
+//                      kernel_redistribute_radiance.cu:55-92,
+//                      kernel_splat_to_renderbuffer.cu:37-57).
+//   k_geodesic_steps : magik_trace_geodesic's per-step records for one ray.
+#include "mgk_device.hpp"
+#include "mgk_internal.hpp"
+
+namespace mgk {
+
+// ---------------------------------------------------------------------------
+// Small state helpers with an explicit, left-to-right evaluation order.
+// ---------------------------------------------------------------------------
+template <class S> __device__ __forceinline__ St<S> add(const St<S>& a, const St<S>& b) {
+    return {a.qx + b.qx, a.qy + b.qy, a.qz + b.qz, a.qt + b.qt, a.px + b.px, a.py + b.py, a.pz + b.pz, a.pt + b.pt};
+}
+template <class S> __device__ __forceinline__ St<S> sub(const St<S>& a, const St<S>& b) {
+    return {a.qx - b.qx, a.qy - b.qy, a.qz - b.qz, a.qt - b.qt, a.px - b.px, a.py - b.py, a.pz - b.pz, a.pt - b.pt};
+}
+template <class S> __device__ __forceinline__ St<S> mul(S s, const St<S>& a) {
+    return {s * a.qx, s * a.qy, s * a.qz, s * a.qt, s * a.px, s * a.py, s * a.pz, s * a.pt};
+}
+
+// RMS norm of relativity_utilities.cuh:15-29.
+template <class S>
+__device__ __forceinline__ S rms_norm(const St<S>& e, const St<S>& y0, const St<S>& y1, S at, S rt) {
+    auto c = [&](S ev, S a, S b) { return ev / (at + rt * fmax(fabs(a), fabs(b))); };
+    S s0 = c(e.qt, y0.qt, y1.qt), s1 = c(e.qx, y0.qx, y1.qx), s2 = c(e.qy, y0.qy, y1.qy), s3 = c(e.qz, y0.qz, y1.qz);
+    S s4 = c(e.pt, y0.pt, y1.pt), s5 = c(e.px, y0.px, y1.px), s6 = c(e.py, y0.py, y1.py), s7 = c(e.pz, y0.pz, y1.pz);
+    S total = (s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3 + s4 * s4 + s5 * s5 + s6 * s6 + s7 * s7) / S(8);
+    return sqrt_<false>(total);
+}
+
+__device__ __forceinline__ float powr(float a, float b) { return powf(a, b); }
+__device__ __forceinline__ double powr(double a, double b) { return pow(a, b); }
+
+// One RKF45 trial in the reference's arithmetic (internal_math.h:2167-2199),
+// contraction off so the FP32 instance rounds as the reference does.
+template <int METRIC, class S>
+__device__ __forceinline__ St<S> trial_reference(const St<S>& y0, S th, S a, S ms, S at, S rt, S h_max, S& h_new, bool& acc) {
+#pragma clang fp contract(off)
+    St<S> k1 = rhs_ref<METRIC, S>(y0, a, ms);
+    St<S> k2 = rhs_ref<METRIC, S>(add(y0, mul(S(1) / S(4) * th, k1)), a, ms);
+    St<S> k3 = rhs_ref<METRIC, S>(add(add(y0, mul(S(3) / S(32) * th, k1)), mul(S(9) / S(32) * th, k2)), a, ms);
+    St<S> k4 = rhs_ref<METRIC, S>(
+        add(sub(add(y0, mul(S(1932) / S(2197) * th, k1)), mul(S(7200) / S(2197) * th, k2)), mul(S(7296) / S(2197) * th, k3)), a, ms);
+    St<S> k5 = rhs_ref<METRIC, S>(
+        sub(add(sub(add(y0, mul(S(439) / S(216) * th, k1)), mul(S(8) * th, k2)), mul(S(3680) / S(513) * th, k3)),
+            mul(S(845) / S(4104) * th, k4)),
+        a, ms);
+    St<S> k6 = rhs_ref<METRIC, S>(
+        sub(add(sub(add(sub(y0, mul(S(8) / S(27) * th, k1)), mul(S(2) * th, k2)), mul(S(3544) / S(2565) * th, k3)),
+                mul(S(1859) / S(4104) * th, k4)),
+            mul(S(11) / S(40) * th, k5)),
+        a, ms);
+    St<S> y1 = add(sub(add(add(add(y0, mul(S(16) / S(135) * th, k1)), mul(S(6656) / S(12825) * th, k3)),
+                           mul(S(28561) / S(56430) * th, k4)),
+                       mul(S(9) / S(50) * th, k5)),
+                   mul(S(2) / S(55) * th, k6));
+    St<S> err = sub(sub(add(add(mul(S(-1) / S(360) * th, k1), mul(S(128) / S(4275) * th, k3)), mul(S(2197) / S(75240) * th, k4)),
+                        mul(S(1) / S(50) * th, k5)),
+                    mul(S(2) / S(55) * th, k6));
+    S rms = rms_norm(err, y0, y1, at, rt);
+    S rms_safe = rms > S(1e-30f) ? rms : S(1e-30f);
+    h_new = S(0.9f) * th * powr(S(1) / rms_safe, S(0.2f));
+    if (h_new > h_max) h_new = h_max;
+    acc = rms_safe < S(1);
+    return y1;
+}
+
+// Fehlberg tableau, and its 5th-order (b) and error (e = 4th - 5th) weights.
+constexpr double A21 = 1.0 / 4, A31 = 3.0 / 32, A32 = 9.0 / 32;
+constexpr double A41 = 1932.0 / 2197, A42 = -7200.0 / 2197, A43 = 7296.0 / 2197;
+constexpr double A51 = 439.0 / 216, A52 = -8.0, A53 = 3680.0 / 513, A54 = -845.0 / 4104;
+constexpr double A61 = -8.0 / 27, A62 = 2.0, A63 = -3544.0 / 2565, A64 = 1859.0 / 4104, A65 = -11.0 / 40;
+constexpr double B1 = 16.0 / 135, B3 = 6656.0 / 12825, B4 = 28561.0 / 56430, B5 = -9.0 / 50, B6 = 2.0 / 55;
+constexpr double E1 = -1.0 / 360, E3 = 128.0 / 4275, E4 = 2197.0 / 75240, E5 = -1.0 / 50, E6 = -2.0 / 55;
+// beta_j = sum_i b_i a_ij and eps_j = sum_i e_i a_ij: the weights of the stage
+// momentum derivatives inside the position update once the stage momenta
+// p_i = p0 + h sum_j a_ij dp_j are substituted exactly.
+constexpr double BT1 = B3 * A31 + B4 * A41 + B5 * A51 + B6 * A61;
+constexpr double BT2 = B3 * A32 + B4 * A42 + B5 * A52 + B6 * A62;
+constexpr double BT3 = B4 * A43 + B5 * A53 + B6 * A63;
+constexpr double BT4 = B5 * A54 + B6 * A64;
+constexpr double BT5 = B6 * A65;
+constexpr double ET1 = E3 * A31 + E4 * A41 + E5 * A51 + E6 * A61;
+constexpr double ET2 = E3 * A32 + E4 * A42 + E5 * A52 + E6 * A62;
+constexpr double ET3 = E4 * A43 + E5 * A53 + E6 * A63;
+constexpr double ET4 = E5 * A54 + E6 * A64;
+constexpr double ET5 = E6 * A65;
+
+// One RKF45 trial of the mixed-precision path.  The six right-hand sides and
+// the stage states run in SS (float in the bulk, double in the near zone).
+// The update and the error estimate are assembled in FP64 from the split
+// right-hand side dq = P(p) - c: with the stage momenta substituted exactly,
+//   y1_q  = y0_q + h P(p0) + h^2 sum beta_j P(dp_j) - h sum b_i c_i
+//   err_q =                 h^2 sum eps_j  P(dp_j) - h sum e_i c_i
+// so the large, cancelling P(p) terms never pass through FP32 and the error
+// estimate carries no FP32 rounding noise of |p|.
+template <int METRIC, class SS, bool FAST>
+__device__ __forceinline__ void trial_split(const St<double>& y0, double h, float a, float ms, const TraceParams& P, St<double>& y1,
+                                            double& rms_out) {
+    const SS qx = SS(y0.qx), qy = SS(y0.qy), qz = SS(y0.qz);
+    const SS px = SS(y0.px), py = SS(y0.py), pz = SS(y0.pz), pt = SS(y0.pt);
+    const SS hs = SS(h), as = SS(a), mss = SS(ms);
+    // stage 1
+    Rhs<SS> r1 = rhs<METRIC, SS, FAST>(qx, qy, qz, px, py, pz, pt, as, mss);
+    SS q1x = px - r1.cx, q1y = py - r1.cy, q1z = pz - r1.cz;
+    // stage 2
+    SS c21 = SS(A21) * hs;
+    SS p2x = px + c21 * r1.dpx, p2y = py + c21 * r1.dpy, p2z = pz + c21 * r1.dpz;
+    Rhs<SS> r2 = rhs<METRIC, SS, FAST>(qx + c21 * q1x, qy + c21 * q1y, qz + c21 * q1z, p2x, p2y, p2z, pt, as, mss);
+    SS q2x = p2x - r2.cx, q2y = p2y - r2.cy, q2z = p2z - r2.cz;
+    // stage 3
+    SS c31 = SS(A31) * hs, c32 = SS(A32) * hs;
+    SS p3x = px + c31 * r1.dpx + c32 * r2.dpx, p3y = py + c31 * r1.dpy + c32 * r2.dpy, p3z = pz + c31 * r1.dpz + c32 * r2.dpz;
+    Rhs<SS> r3 = rhs<METRIC, SS, FAST>(qx + c31 * q1x + c32 * q2x, qy + c31 * q1y + c32 * q2y, qz + c31 * q1z + c32 * q2z, p3x, p3y,
+                                       p3z, pt, as, mss);
+    SS q3x = p3x - r3.cx, q3y = p3y - r3.cy, q3z = p3z - r3.cz;
+    // stage 4
+    SS c41 = SS(A41) * hs, c42 = SS(A42) * hs, c43 = SS(A43) * hs;
+    SS p4x = px + c41 * r1.dpx + c42 * r2.dpx + c43 * r3.dpx, p4y = py + c41 * r1.dpy + c42 * r2.dpy + c43 * r3.dpy,
+       p4z = pz + c41 * r1.dpz + c42 * r2.dpz + c43 * r3.dpz;
+    Rhs<SS> r4 = rhs<METRIC, SS, FAST>(qx + c41 * q1x + c42 * q2x + c43 * q3x, qy + c41 * q1y + c42 * q2y + c43 * q3y,
+                                       qz + c41 * q1z + c42 * q2z + c43 * q3z, p4x, p4y, p4z, pt, as, mss);
+    SS q4x = p4x - r4.cx, q4y = p4y - r4.cy, q4z = p4z - r4.cz;
+    // stage 5
+    SS c51 = SS(A51) * hs, c52 = SS(A52) * hs, c53 = SS(A53) * hs, c54 = SS(A54) * hs;
+    SS p5x = px + c51 * r1.dpx + c52 * r2.dpx + c53 * r3.dpx + c54 * r4.dpx,
+       p5y = py + c51 * r1.dpy + c52 * r2.dpy + c53 * r3.dpy + c54 * r4.dpy,
+       p5z = pz + c51 * r1.dpz + c52 * r2.dpz + c53 * r3.dpz + c54 * r4.dpz;
+    Rhs<SS> r5 = rhs<METRIC, SS, FAST>(qx + c51 * q1x + c52 * q2x + c53 * q3x + c54 * q4x,
+                                       qy + c51 * q1y + c52 * q2y + c53 * q3y + c54 * q4y,
+                                       qz + c51 * q1z + c52 * q2z + c53 * q3z + c54 * q4z, p5x, p5y, p5z, pt, as, mss);
+    SS q5x = p5x - r5.cx, q5y = p5y - r5.cy, q5z = p5z - r5.cz;
+    // stage 6
+    SS c61 = SS(A61) * hs, c62 = SS(A62) * hs, c63 = SS(A63) * hs, c64 = SS(A64) * hs, c65 = SS(A65) * hs;
+    SS p6x = px + c61 * r1.dpx + c62 * r2.dpx + c63 * r3.dpx + c64 * r4.dpx + c65 * r5.dpx,
+       p6y = py + c61 * r1.dpy + c62 * r2.dpy + c63 * r3.dpy + c64 * r4.dpy + c65 * r5.dpy,
+       p6z = pz + c61 * r1.dpz + c62 * r2.dpz + c63 * r3.dpz + c64 * r4.dpz + c65 * r5.dpz;
+    Rhs<SS> r6 = rhs<METRIC, SS, FAST>(qx + c61 * q1x + c62 * q2x + c63 * q3x + c64 * q4x + c65 * q5x,
+                                       qy + c61 * q1y + c62 * q2y + c63 * q3y + c64 * q4y + c65 * q5y,
+                                       qz + c61 * q1z + c62 * q2z + c63 * q3z + c64 * q4z + c65 * q5z, p6x, p6y, p6z, pt, as, mss);
+
+    // FP64 assembly.
+    const double hh = h * h;
+    auto dp5 = [&](double w1, double w2, double w3, double w4, double w5, SS d1, SS d2, SS d3, SS d4, SS d5) {
+        return w1 * double(d1) + w2 * double(d2) + w3 * double(d3) + w4 * double(d4) + w5 * double(d5);
+    };
+    auto cb = [&](SS c1, SS c3, SS c4, SS c5, SS c6) {
+        return B1 * double(c1) + B3 * double(c3) + B4 * double(c4) + B5 * double(c5) + B6 * double(c6);
+    };
+    auto ce = [&](SS c1, SS c3, SS c4, SS c5, SS c6) {
+        return E1 * double(c1) + E3 * double(c3) + E4 * double(c4) + E5 * double(c5) + E6 * double(c6);
+    };
+    double bpx = B1 * double(r1.dpx) + B3 * double(r3.dpx) + B4 * double(r4.dpx) + B5 * double(r5.dpx) + B6 * double(r6.dpx);
+    double bpy = B1 * double(r1.dpy) + B3 * double(r3.dpy) + B4 * double(r4.dpy) + B5 * double(r5.dpy) + B6 * double(r6.dpy);
+    double bpz = B1 * double(r1.dpz) + B3 * double(r3.dpz) + B4 * double(r4.dpz) + B5 * double(r5.dpz) + B6 * double(r6.dpz);
+    double epx = E1 * double(r1.dpx) + E3 * double(r3.dpx) + E4 * double(r4.dpx) + E5 * double(r5.dpx) + E6 * double(r6.dpx);
+    double epy = E1 * double(r1.dpy) + E3 * double(r3.dpy) + E4 * double(r4.dpy) + E5 * double(r5.dpy) + E6 * double(r6.dpy);
+    double epz = E1 * double(r1.dpz) + E3 * double(r3.dpz) + E4 * double(r4.dpz) + E5 * double(r5.dpz) + E6 * double(r6.dpz);
+    double btx = dp5(BT1, BT2, BT3, BT4, BT5, r1.dpx, r2.dpx, r3.dpx, r4.dpx, r5.dpx);
+    double bty = dp5(BT1, BT2, BT3, BT4, BT5, r1.dpy, r2.dpy, r3.dpy, r4.dpy, r5.dpy);
+    double btz = dp5(BT1, BT2, BT3, BT4, BT5, r1.dpz, r2.dpz, r3.dpz, r4.dpz, r5.dpz);
+    double etx = dp5(ET1, ET2, ET3, ET4, ET5, r1.dpx, r2.dpx, r3.dpx, r4.dpx, r5.dpx);
+    double ety = dp5(ET1, ET2, ET3, ET4, ET5, r1.dpy, r2.dpy, r3.dpy, r4.dpy, r5.dpy);
+    double etz = dp5(ET1, ET2, ET3, ET4, ET5, r1.dpz, r2.dpz, r3.dpz, r4.dpz, r5.dpz);
+    double cbx = cb(r1.cx, r3.cx, r4.cx, r5.cx, r6.cx), cby = cb(r1.cy, r3.cy, r4.cy, r5.cy, r6.cy);
+    double cbz = cb(r1.cz, r3.cz, r4.cz, r5.cz, r6.cz), cbt = cb(r1.ct, r3.ct, r4.ct, r5.ct, r6.ct);
+    double cex = ce(r1.cx, r3.cx, r4.cx, r5.cx, r6.cx), cey = ce(r1.cy, r3.cy, r4.cy, r5.cy, r6.cy);
+    double cez = ce(r1.cz, r3.cz, r4.cz, r5.cz, r6.cz), cet = ce(r1.ct, r3.ct, r4.ct, r5.ct, r6.ct);
+
+    y1.px = y0.px + h * bpx;
+    y1.py = y0.py + h * bpy;
+    y1.pz = y0.pz + h * bpz;
+    y1.pt = y0.pt;
+    y1.qx = y0.qx + h * y0.px + hh * btx - h * cbx;
+    y1.qy = y0.qy + h * y0.py + hh * bty - h * cby;
+    y1.qz = y0.qz + h * y0.pz + hh * btz - h * cbz;
+    y1.qt = y0.qt - h * y0.pt - h * cbt;
+    St<double> e;
+    e.px = h * epx; e.py = h * epy; e.pz = h * epz; e.pt = 0.0;
+    e.qx = hh * etx - h * cex; e.qy = hh * ety - h * cey; e.qz = hh * etz - h * cez; e.qt = -h * cet;
+    rms_out = rms_norm(e, y0, y1, double(P.abs_tol), double(P.rel_tol));
+}
+
+// Null-constraint renormalisation (FP64): rescale the spatial momentum by the
+// root s ~ 1 of H(s) = 1/2 g^{mu nu} p_mu p_nu = 0 with p_t (the conserved
+// energy) held fixed.  With g^{mu nu} = eta - f L L, L = (k, -1):
+//   (A - f B^2) s^2 + 2 f B p_t s - (1 + f) p_t^2 = 0,  A = |p|^2, B = k.p.
+template <int METRIC>
+__device__ __forceinline__ void renormalise_null(St<double>& y, double a, double ms) {
+    if constexpr (METRIC == MINKOWSKI) {
+        double A = y.px * y.px + y.py * y.py + y.pz * y.pz;
+        double s = fabs(y.pt) / sqrt(A);
+        if (fabs(s - 1.0) < 1e-3) { y.px *= s; y.py *= s; y.pz *= s; }
+    } else {
+        double f, kx, ky, kz;
+        if constexpr (METRIC == KERR) {
+            double r = kerr_schild_r(y.qx, y.qy, y.qz, a);
+            double r2 = r * r;
+            f = 2.0 * r2 * r / (r2 * r2 + a * a * y.qy * y.qy);
+            double id = 1.0 / (r2 + a * a);
+            kx = (r * y.qx - a * y.qz) * id;
+            ky = y.qy / r;
+            kz = (r * y.qz + a * y.qx) * id;
+        } else {
+            double r = sqrt(y.qx * y.qx + y.qy * y.qy + y.qz * y.qz);
+            double ir = 1.0 / r;
+            f = 2.0 * ms * ir;
+            kx = y.qx * ir; ky = y.qy * ir; kz = y.qz * ir;
+        }
+        double A = y.px * y.px + y.py * y.py + y.pz * y.pz;
+        double B = kx * y.px + ky * y.py + kz * y.pz;
+        double al = A - f * B * B, be = 2.0 * f * B * y.pt, ga = -(1.0 + f) * y.pt * y.pt;
+        double disc = be * be - 4.0 * al * ga;
+        if (!(disc >= 0.0) || al == 0.0) return;
+        double qq = -0.5 * (be + copysign(sqrt(disc), be));
+        double s1 = qq / al, s2 = qq != 0.0 ? ga / qq : s1;
+        double s = fabs(s1 - 1.0) < fabs(s2 - 1.0) ? s1 : s2;
+        if (fabs(s - 1.0) < 1e-3) { y.px *= s; y.py *= s; y.pz *= s; }
+    }
+}
+
+// Chord tests of one completed step (kernel_intersect.cu:55-138 extended with
+// the disk and the emissive sphere), then the horizon (in_domain) and the
+// step limit (kernel_enforce_ray_limits.cu:48, n_step > limit).  Returns the
+// class and fills the hit record.
+template <int METRIC, class S>
+__device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, uint32_t n_step, S contraction, const TraceParams& P,
+                                             mgk_hit_t& rec) {
+    constexpr bool F32 = sizeof(S) == 4;
+    const S a = METRIC == KERR ? S(P.spin) : S(0);
+    S best = S(1e30f);
+    uint32_t cls = MGK_HIT_NONE;
+    S r_hit = S(0);
+    if (P.flags & MGK_SCENE_DISK) {
+        S ya = y0.qy, yb = y1.qy;
+        if ((ya > S(0) && yb <= S(0)) || (ya < S(0) && yb >= S(0))) {
+            S t = ya / (ya - yb);
+            S x = y0.qx + t * (y1.qx - y0.qx), z = y0.qz + t * (y1.qz - y0.qz);
+            S rho2 = x * x + z * z - a * a;
+            if (rho2 > S(0)) {
+                S r = sqrt_<false>(rho2);
+                if (!(r < S(P.disk_r_in) || r > S(P.disk_r_out)) && t < best) { best = t; cls = MGK_HIT_DISK; r_hit = r; }
+            }
+        }
+    }
+    if (P.flags & MGK_SCENE_SPHERE) {
+        S ox = y0.qx - S(P.sphere_c[0]), oy = y0.qy - S(P.sphere_c[1]), oz = y0.qz - S(P.sphere_c[2]);
+        S dx = y1.qx - y0.qx, dy = y1.qy - y0.qy, dz = y1.qz - y0.qz;
+        S A = dx * dx + dy * dy + dz * dz, B = ox * dx + oy * dy + oz * dz;
+        S Cc = ox * ox + oy * oy + oz * oz - S(P.sphere_r) * S(P.sphere_r);
+        S disc = B * B - A * Cc;
+        if (A > S(0) && disc >= S(0)) {
+            S sd = sqrt_<false>(disc);
+            S t = (-B - sd) / A;
+            if (t < S(0)) t = (-B + sd) / A;
+            if (t >= S(0) && t <= S(1) && t < best) { best = t; cls = MGK_HIT_SPHERE; }
+        }
+    }
+    {
+        const S L = S(P.length_scale);
+        S ex = y1.qx * L - S(P.cam[0]), ey = y1.qy * L - S(P.cam[1]), ez = y1.qz * L - S(P.cam[2]);
+        S d2 = ex * ex + ey * ey + ez * ez;
+        S r2 = F32 ? S(P.cel_r2_f) : S(P.cel_r2_d);
+        if (!(d2 < r2) && S(1) < best) { best = S(1); cls = MGK_HIT_ESCAPE; }
+    }
+    if (cls == MGK_HIT_NONE) {
+        if constexpr (METRIC == KERR) {
+            if (P.spin_in_range) {
+                S r = kerr_schild_r(y1.qx, y1.qy, y1.qz, a);
+                S hr = F32 ? S(P.horizon_r_f) : S(P.horizon_r_d);
+                if (!(hr < r)) cls = MGK_HIT_HORIZON;
+            }
+        } else if constexpr (METRIC == SCHWARZSCHILD) {
+            S r = sqrt_<false>(y1.qx * y1.qx + y1.qy * y1.qy + y1.qz * y1.qz);
+            if (P.mass_sign < 0.0f) { if (r < S(1e-3f)) cls = MGK_HIT_HORIZON; }
+            else if (r <= (F32 ? S(P.horizon_r_f) : S(P.horizon_r_d))) cls = MGK_HIT_HORIZON;
+        }
+    }
+    if (cls == MGK_HIT_NONE && n_step > P.step_limit) cls = MGK_HIT_STEP_LIMIT;
+    if (cls == MGK_HIT_NONE) return cls;
+    rec.cls = cls;
+    rec.g = 0.0f; rec.a0 = 0.0f; rec.a1 = 0.0f;
+    if (cls == MGK_HIT_DISK || cls == MGK_HIT_SPHERE) {
+        V4<S> ph{y0.px + best * (y1.px - y0.px), y0.py + best * (y1.py - y0.py), y0.pz + best * (y1.pz - y0.pz),
+                 y0.pt + best * (y1.pt - y0.pt)};
+        S hx = y0.qx + best * (y1.qx - y0.qx), hy = y0.qy + best * (y1.qy - y0.qy), hz = y0.qz + best * (y1.qz - y0.qz);
+        V4<S> u = cls == MGK_HIT_DISK ? kepler_velocity<METRIC, S>(hx, hz, r_hit, a)
+                                      : reference_observer<METRIC, S>(hx, hy, hz, a, S(P.mass_sign));
+        S g = contraction / dot4(u, ph);
+        if (isnan(g) || isinf(g)) g = S(1e-10f);
+        rec.g = float(g);
+        if (cls == MGK_HIT_DISK) {
+            rec.a0 = float(r_hit);
+            rec.a1 = float(atan2(hz, hx));
+        } else {
+            S nx = hx - S(P.sphere_c[0]), ny = hy - S(P.sphere_c[1]), nz = hz - S(P.sphere_c[2]);
+            S l = sqrt_<false>(nx * nx + ny * ny + nz * nz);
+            rec.a0 = float(ny / l);
+            rec.a1 = float(atan2(nz / l, nx / l));
+        }
+    } else if (cls == MGK_HIT_ESCAPE) {
+        S dx = y1.qx - y0.qx, dy = y1.qy - y0.qy, dz = y1.qz - y0.qz;
+        S l = sqrt_<false>(dx * dx + dy * dy + dz * dz);
+        rec.a0 = float(dy / l);
+        rec.a1 = float(atan2(dz / l, dx / l));
+        S g = contraction / y1.pt;
+        if (isnan(g) || isinf(g)) g = S(1e-10f);
+        rec.g = float(g);
+    }
+    return cls;
+}
+
+template <class S> __device__ __forceinline__ S load_c(const void* rays, uint64_t stride, int c, uint64_t i) {
+    return reinterpret_cast<const S*>(rays)[uint64_t(c) * stride + i];
+}
+
+constexpr uint32_t kChunk = 64;   // ray indices taken from the device counter per wave refill
+
+// Persistent integrator.  T is the storage / state type: float for the FP32
+// reference path, double for the mixed and FP64 paths.
+template <int METRIC, int PREC>
+__global__ void __launch_bounds__(256) k_trace(TraceParams P) {
+    using S = typename std::conditional<PREC == MGK_PRECISION_FP32_REFERENCE, float, double>::type;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    St<S> y{};
+    S h = S(0), contraction = S(0);
+    bool active = false;
+    uint64_t ray = 0;
+    uint32_t n_step = 0, n_sub = 0;
+    uint64_t steps_acc = 0, trials_acc = 0, fp64_acc = 0;
+
+    uint64_t chunk_next = 0, chunk_end = 0;   // wave-uniform
+    bool exhausted = false;                   // wave-uniform
+
+    for (;;) {
+        // ---- refill idle lanes (wave-uniform control flow) ----
+        uint64_t idle = __ballot(!active);
+        while (idle != 0 && !exhausted) {
+            if (chunk_next >= chunk_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(P.head, (unsigned long long)kChunk);
+                base = __shfl(base, 0);
+                if (base >= P.n) { exhausted = true; break; }
+                chunk_next = base;
+                chunk_end = base + kChunk < P.n ? base + kChunk : P.n;
+            }
+            uint32_t avail = uint32_t(chunk_end - chunk_next);
+            uint32_t cnt = uint32_t(__popcll(idle));
+            uint32_t take = cnt < avail ? cnt : avail;
+            if (!active) {
+                uint32_t rank = uint32_t(__popcll(idle & lt_mask));
+                if (rank < take) {
+                    uint64_t slot = chunk_next + rank;
+                    ray = P.queue ? uint64_t(P.queue[slot]) : slot;
+                    y.qx = load_c<S>(P.rays, P.stride, 0, ray);
+                    y.qy = load_c<S>(P.rays, P.stride, 1, ray);
+                    y.qz = load_c<S>(P.rays, P.stride, 2, ray);
+                    y.qt = load_c<S>(P.rays, P.stride, 3, ray);
+                    y.px = load_c<S>(P.rays, P.stride, 4, ray);
+                    y.py = load_c<S>(P.rays, P.stride, 5, ray);
+                    y.pz = load_c<S>(P.rays, P.stride, 6, ray);
+                    y.pt = load_c<S>(P.rays, P.stride, 7, ray);
+                    h = load_c<S>(P.rays, P.stride, 8, ray);
+                    contraction = load_c<S>(P.rays, P.stride, 9, ray);
+                    active = true;
+                    n_step = 0;
+                    n_sub = 0;
+                }
+            }
+            chunk_next += take;
+            idle = __ballot(!active);
+        }
+        if (__ballot(active) == 0) break;
+        if (!active) continue;
+
+        // ---- one trial ----
+        St<S> y1;
+        bool step_done;
+        if constexpr (METRIC == MINKOWSKI) {
+            // explicit Euler with the stored h (metric_minkowski.cuh:49-53)
+            y1 = y;
+            y1.qx = y.qx + h * y.px; y1.qy = y.qy + h * y.py; y1.qz = y.qz + h * y.pz; y1.qt = y.qt + h * -y.pt;
+            step_done = true;
+            ++trials_acc;
+        } else if constexpr (PREC == MGK_PRECISION_MIXED) {
+            double rms;
+            double r2 = y.qx * y.qx + y.qy * y.qy + y.qz * y.qz;
+            bool near = r2 < P.zone_rho2;   // rho^2 = r^2 + a^2 sin^2(theta): zone_rho2 = zone_r^2 + a^2 covers r < zone_r
+            if (near) {
+                trial_split<METRIC, double, false>(y, h, P.spin, P.mass_sign, P, y1, rms);
+                ++fp64_acc;
+            } else {
+                trial_split<METRIC, float, true>(y, h, P.spin, P.mass_sign, P, y1, rms);
+            }
+            ++trials_acc;
+            double rs = rms > 1e-30 ? rms : 1e-30;
+            double fac = double(__expf(-0.2f * __logf(float(rs))));
+            double hn = 0.9 * h * fac;
+            if (hn > double(P.h_max)) hn = double(P.h_max);
+            h = hn;
+            ++n_sub;
+            step_done = (rs < 1.0) || (n_sub >= P.n_max);
+        } else {
+            S hn;
+            bool acc;
+            y1 = trial_reference<METRIC, S>(y, h, S(P.spin), S(P.mass_sign), S(P.abs_tol), S(P.rel_tol), S(P.h_max), hn, acc);
+            ++trials_acc;
+            if (PREC == MGK_PRECISION_FP64) ++fp64_acc;
+            h = hn;
+            ++n_sub;
+            step_done = acc || (n_sub >= P.n_max);
+        }
+        if (!step_done) continue;
+
+        // ---- step complete: renormalise, classify ----
+        ++n_step;
+        n_sub = 0;
+        ++steps_acc;
+        if constexpr (PREC == MGK_PRECISION_MIXED) renormalise_null<METRIC>(y1, double(P.spin), double(P.mass_sign));
+        mgk_hit_t rec;
+        uint32_t cls = classify<METRIC, S>(y, y1, n_step, contraction, P, rec);
+        if (cls != MGK_HIT_NONE) {
+            P.hits[ray] = rec;
+            if (P.steps_out) P.steps_out[ray] = n_step;
+            active = false;
+        } else {
+            y = y1;
+        }
+    }
+
+    // ---- statistics: wave reduction, one atomic per counter per wave ----
+    for (int off = 32; off > 0; off >>= 1) {
+        steps_acc += __shfl_xor(steps_acc, off);
+        trials_acc += __shfl_xor(trials_acc, off);
+        fp64_acc += __shfl_xor(fp64_acc, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&P.stats[0], (unsigned long long)steps_acc);
+        atomicAdd(&P.stats[1], (unsigned long long)trials_acc);
+        atomicAdd(&P.stats[2], (unsigned long long)fp64_acc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ray generation + initial conditions.
+// ---------------------------------------------------------------------------
+template <int METRIC, class S>
+__device__ __forceinline__ void gen_one(const RayGenParams& P, uint64_t i) {
+#pragma clang fp contract(off)
+    const uint32_t npix = P.W * P.H;
+    const uint32_t pixel = uint32_t(i % npix);
+    const uint32_t sample = P.sample_begin + uint32_t(i / npix);
+    const uint32_t px = pixel % P.W, py = pixel / P.W;
+    float u[4];
+    path_uniforms(P.seed, P.camera_id, pixel, sample, 0, u);
+    // sensor point (device_sensor_utilities.cuh:35-80)
+    const S sw = S(P.sensor_width_mm);
+    S inv_s = sw / sqrt_<false>(S(float(P.W * P.W + P.H * P.H)));
+    S width_mm = S(float(P.W)) * inv_s, height_mm = S(float(P.H)) * inv_s;
+    S area = (width_mm * height_mm) / S(1E6f);
+    S side = sqrt_<false>(area / S(float(P.W * P.H)));
+    S ju = (S(u[0]) - S(0.5)) * side, jv = (S(u[1]) - S(0.5)) * side;
+    S uu = -((S(float(px)) / S(float(P.W))) - S(0.5));
+    S vv = -(S(float(py)) / S(float(P.H)) - S(0.5));
+    S ratio = S(float(P.H)) / S(float(P.W));
+    if (ratio < S(1)) vv = vv * ratio; else uu = uu / ratio;
+    S sx = ((-uu * sw) / S(1000)) + ju, sy = ((-vv * sw) / S(1000)) + jv;
+    // pinhole (thin_lens.cuh:60-76): origin at the lens point, direction lens - sensor
+    S f_m = S(P.focal_mm) / S(1000);
+    S dx = S(0) - sx, dy = S(0) - sy, dz = -f_m - S(0);
+    S dl = sqrt_<false>(dx * dx + dy * dy + dz * dz);
+    dx = dx / dl; dy = dy / dl; dz = dz / dl;
+    const float* m = P.orientation;
+    S ox = (S(m[0]) * S(0) + S(m[1]) * S(0) + S(m[2]) * -f_m) + S(P.position[0]);
+    S oy = (S(m[3]) * S(0) + S(m[4]) * S(0) + S(m[5]) * -f_m) + S(P.position[1]);
+    S oz = (S(m[6]) * S(0) + S(m[7]) * S(0) + S(m[8]) * -f_m) + S(P.position[2]);
+    // initial conditions (kernel_generate_initial_conditions.cu:54-74)
+    const S L = S(P.length_scale), a = S(P.spin), ms = S(P.mass_sign);
+    S qx = ox / L, qy = oy / L, qz = oz / L, qt = S(1) / (L / S(kC));
+    Sym4<S> g = metric_g<METRIC, S>(qx, qy, qz, a, ms, false);
+    V4<S> uref = reference_observer<METRIC, S>(qx, qy, qz, a, ms);
+    S wx = S(P.observer_velocity[0]) / S(kC), wy = S(P.observer_velocity[1]) / S(kC), wz = S(P.observer_velocity[2]) / S(kC);
+    if (P.time_reversed) { wx = -wx; wy = -wy; wz = -wz; }
+    V4<S> X{S(1), S(0), S(0), S(0)}, Y{S(0), S(1), S(0), S(0)}, Z{S(0), S(0), S(1), S(0)};
+    V4<S> uobs = tetrad_apply(g, X, Y, Z, uref, V4<S>{wx, wy, wz, sqrt_<false>((wx * wx + wy * wy + wz * wz) - S(-1))});
+    V4<S> CX{S(m[0]), S(m[3]), S(m[6]), S(0)}, CY{S(m[1]), S(m[4]), S(m[7]), S(0)}, CZ{S(m[2]), S(m[5]), S(m[8]), S(0)};
+    S lx = dx / L, ly = dy / L, lz = dz / L;
+    V4<S> pup = tetrad_apply(g, CX, CY, CZ, uobs, V4<S>{lx, ly, lz, sqrt_<false>((lx * lx + ly * ly + lz * lz) - S(0))});
+    V4<S> pm = lower(g, pup);
+    S* out = reinterpret_cast<S*>(P.rays);
+    const uint64_t st = P.stride;
+    out[0 * st + i] = qx; out[1 * st + i] = qy; out[2 * st + i] = qz; out[3 * st + i] = qt;
+    out[4 * st + i] = pm.x; out[5 * st + i] = pm.y; out[6 * st + i] = pm.z; out[7 * st + i] = pm.t;
+    out[8 * st + i] = METRIC == MINKOWSKI ? S(1000) : S(1);
+    out[9 * st + i] = dot4(uobs, pm);
+}
+
+template <int METRIC, class S> __global__ void __launch_bounds__(256) k_generate_rays(RayGenParams P) {
+    const uint64_t n = uint64_t(P.W) * P.H * P.spp;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        gen_one<METRIC, S>(P, i);
+}
+
+// ---------------------------------------------------------------------------
+// Shading: one thread per pixel, bins in LDS (bin-major, thread-minor).
+// ---------------------------------------------------------------------------
+constexpr int kShadeBlock = 128;
+
+__device__ __forceinline__ float star_lookup(const ShadeParams& P, float c, float ph) {
+    float sn = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+    float dx = sn * cosf(ph), dz = sn * sinf(ph);
+    uint32_t ic = min(P.grid_nc - 1, uint32_t(fmaxf(0.0f, (c + 1.0f) * 0.5f * float(P.grid_nc))));
+    uint32_t ip = min(P.grid_np - 1, uint32_t(fmaxf(0.0f, (ph + kPi) * (0.5f / kPi) * float(P.grid_np))));
+    uint32_t cell = ic * P.grid_np + ip;
+    float acc = 0.0f;
+    for (uint32_t k = P.cell_start[cell]; k < P.cell_start[cell + 1]; ++k) {
+        float4 s = P.stars[P.cell_items[k]];
+        if (s.x * dx + s.y * c + s.z * dz >= P.cos_star_r) acc += s.w;
+    }
+    return acc;
+}
+
+__global__ void __launch_bounds__(kShadeBlock) k_shade(ShadeParams P) {
+    __shared__ float bins[kMaxBins * kShadeBlock];
+    const uint32_t npix = P.W * P.H;
+    const uint32_t pixel = blockIdx.x * kShadeBlock + threadIdx.x;
+    for (uint32_t i = 0; i < P.n_bin; ++i) bins[i * kShadeBlock + threadIdx.x] = 0.0f;
+    if (pixel < npix) {
+        const float inv_sqrt2_sigma = 1.0f / (P.sigma * 1.41421356237f);
+        for (uint32_t s = 0; s < P.spp; ++s) {
+            mgk_hit_t h = P.hits[uint64_t(s) * npix + pixel];
+            if (h.cls != MGK_HIT_DISK && h.cls != MGK_HIT_SPHERE && h.cls != MGK_HIT_ESCAPE) continue;
+            float u[4];
+            path_uniforms(P.seed, P.camera_id, pixel, P.sample_begin + s, 0, u);
+            const float g = h.g, g5 = g * g * g * g * g;
+            float emit_scale = 0.0f, T = 0.0f, star = 0.0f;
+            if (h.cls == MGK_HIT_DISK) {
+                T = P.disk_T_in * powf(h.a0 / P.disk_r_in, -0.75f);
+                emit_scale = 1.0f;
+            } else if (h.cls == MGK_HIT_SPHERE) {
+                float th = acosf(fmaxf(-1.0f, fminf(1.0f, h.a0)));
+                float ph = h.a1 + kPi;
+                uint32_t n = P.sphere_checker;
+                uint32_t ci = min(n - 1, uint32_t(th / kPi * float(n)));
+                uint32_t cj = min(n - 1, uint32_t(ph / (2.0f * kPi) * float(n)));
+                T = P.sphere_T;
+                emit_scale = ((ci + cj) & 1u) ? 0.0f : 1.0f;
+            } else {
+                if (P.flags & MGK_SCENE_STARS) star = star_lookup(P, h.a0, h.a1);
+            }
+            for (uint32_t k = 0; k < P.n_hero; ++k) {
+                float f = u[2] + float(k) / float(P.n_hero);
+                f -= floorf(f);
+                const float lam = P.hero_min + P.hero_span * f;
+                const float le = lam * g;
+                float L;
+                if (h.cls == MGK_HIT_ESCAPE) {
+                    L = 0.0f;
+                    if (P.flags & MGK_SCENE_BACKGROUND) L += P.background_scale * planck(le, P.background_T);
+                    if (star > 0.0f) L += star * planck(le, P.star_T);
+                } else {
+                    L = emit_scale > 0.0f ? planck(le, T) : 0.0f;
+                }
+                L *= g5;
+                if (isnan(L) || isinf(L) || L < 0.0f) L = 0.0f;
+                if (!(L > 0.0f)) continue;
+                const float E = L * P.irr_scale;
+                float x1 = P.bin_min;
+                float prev = 0.5f * (1.0f + erff((x1 - lam) * inv_sqrt2_sigma));
+                for (uint32_t i = 0; i < P.n_bin; ++i) {
+                    x1 += P.bin_width;
+                    float cur = 0.5f * (1.0f + erff((x1 - lam) * inv_sqrt2_sigma));
+                    bins[i * kShadeBlock + threadIdx.x] += E * (cur - prev);
+                    prev = cur;
+                }
+            }
+        }
+        float X = 0.0f, Yv = 0.0f, Zv = 0.0f;
+        for (uint32_t i = 0; i < P.n_bin; ++i) {
+            float b = bins[i * kShadeBlock + threadIdx.x];
+            X += b * P.weights[3 * i];
+            Yv += b * P.weights[3 * i + 1];
+            Zv += b * P.weights[3 * i + 2];
+        }
+        const float inv = 1.0f / float(P.spp);
+        P.xyz[3 * uint64_t(pixel) + 0] = X * inv;
+        P.xyz[3 * uint64_t(pixel) + 1] = Yv * inv;
+        P.xyz[3 * uint64_t(pixel) + 2] = Zv * inv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// magik_trace_geodesic: one ray, FP32 reference arithmetic, every step
+// recorded (relativity/host_trace_geodesic.cu:13-130).
+// ---------------------------------------------------------------------------
+template <int METRIC> __global__ void k_geodesic_steps(GeodesicStepParams P) {
+#pragma clang fp contract(off)
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    using S = float;
+    const S L = P.length_scale, a = P.spin, ms = P.mass_sign;
+    const float* m = P.rot;
+    S dx = m[0] * S(0) + m[1] * S(0) + m[2] * S(-1);
+    S dy = m[3] * S(0) + m[4] * S(0) + m[5] * S(-1);
+    S dz = m[6] * S(0) + m[7] * S(0) + m[8] * S(-1);
+    S sq = S(0);
+    if (P.massive) {
+        S f = P.speed / kC;
+        dx = dx * f; dy = dy * f; dz = dz * f;
+        sq = S(-1);
+    }
+    S qx = P.position[0] / L, qy = P.position[1] / L, qz = P.position[2] / L, qt = S(0) / (L / kC);
+    Sym4<S> g = metric_g<METRIC, S>(qx, qy, qz, a, ms, false);
+    V4<S> uref = reference_observer<METRIC, S>(qx, qy, qz, a, ms);
+    S wx = P.lpv[0] / kC, wy = P.lpv[1] / kC, wz = P.lpv[2] / kC;
+    if (P.time_reversed) { wx = -wx; wy = -wy; wz = -wz; }
+    V4<S> RX{m[0], m[3], m[6], S(0)}, RY{m[1], m[4], m[7], S(0)}, RZ{m[2], m[5], m[8], S(0)};
+    V4<S> u = tetrad_apply(g, RX, RY, RZ, uref, V4<S>{wx, wy, wz, sqrt_<false>((wx * wx + wy * wy + wz * wz) - S(-1))});
+    V4<S> X{S(1), S(0), S(0), S(0)}, Y{S(0), S(1), S(0), S(0)}, Z{S(0), S(0), S(1), S(0)};
+    S lx = dx / L, ly = dy / L, lz = dz / L;
+    V4<S> pup = tetrad_apply(g, X, Y, Z, u, V4<S>{lx, ly, lz, sqrt_<false>((lx * lx + ly * ly + lz * lz) - sq)});
+    V4<S> pm = lower(g, pup);
+    St<S> y{qx, qy, qz, qt, pm.x, pm.y, pm.z, pm.t};
+    St<S> d = rhs_ref<METRIC, S>(y, a, ms);
+    auto put = [&](uint32_t k, const St<S>& s, const St<S>& dd, S con, S aff, S dh) {
+        magik_trace_geodesic_step_t& o = P.steps[k];
+        o.four_position[0] = s.qx; o.four_position[1] = s.qy; o.four_position[2] = s.qz; o.four_position[3] = s.qt;
+        o.four_momentum[0] = s.px; o.four_momentum[1] = s.py; o.four_momentum[2] = s.pz; o.four_momentum[3] = s.pt;
+        o.moment_derivative[0] = dd.px; o.moment_derivative[1] = dd.py; o.moment_derivative[2] = dd.pz; o.moment_derivative[3] = dd.pt;
+        o.four_vector_contraction = con;
+        o.affine_parameter = aff;
+        o.delta_affine_parameter = dh;
+    };
+    const S contraction = dot4(u, pm);
+    S h = METRIC == MINKOWSKI ? S(1000) : S(1);
+    uint32_t n = 0;
+    if (P.n_alloc > 0) {
+        put(0, y, d, contraction, S(0), h);
+        n = 1;
+    }
+    S affine = S(0);
+    while (n < P.n_alloc) {
+        St<S> y1;
+        S ht;
+        if constexpr (METRIC == MINKOWSKI) {
+            ht = h;
+            y1 = add(y, mul(h, rhs_ref<METRIC, S>(y, a, ms)));
+        } else {
+            for (uint32_t i = 0; i < P.n_max; ++i) {
+                S hn;
+                bool acc;
+                y1 = trial_reference<METRIC, S>(y, h, a, ms, S(P.abs_tol), S(P.rel_tol), S(P.h_max), hn, acc);
+                ht = h;
+                h = hn;
+                if (acc) break;
+            }
+        }
+        St<S> d1 = rhs_ref<METRIC, S>(y1, a, ms);
+        affine = affine + ht;
+        put(n, y1, d1, contraction, affine, h);
+        ++n;
+        y = y1;
+        bool inside = false;
+        if constexpr (METRIC == KERR) {
+            if (P.spin_in_range) inside = !(P.horizon_r < kerr_schild_r(y1.qx, y1.qy, y1.qz, a));
+        } else if constexpr (METRIC == SCHWARZSCHILD) {
+            S r = sqrt_<false>(y1.qx * y1.qx + y1.qy * y1.qy + y1.qz * y1.qz);
+            inside = ms < S(0) ? (r < S(1e-3f)) : (r <= P.horizon_r);
+        }
+        if (inside) break;
+    }
+    for (uint32_t k = 0; k < n; ++k)
+        for (int c = 0; c < 4; ++c) P.steps[k].four_position[c] *= L;
+    *P.n_written = n;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers (compile-time dispatch over the metric plugin, as the reference's
+// dispatch_list over supported_metrics, relativity_utilities.cuh:197-202).
+// ---------------------------------------------------------------------------
+template <template <int> class K, class Fn> static hipError_t dispatch_metric(int metric, Fn&& fn) {
+    switch (metric) {
+        case MINKOWSKI: return fn(std::integral_constant<int, MINKOWSKI>{});
+        case SCHWARZSCHILD: return fn(std::integral_constant<int, SCHWARZSCHILD>{});
+        case KERR: return fn(std::integral_constant<int, KERR>{});
+        default: return hipErrorInvalidValue;
+    }
+}
+template <int> struct tag_t {};
+
+hipError_t launch_generate_rays(const RayGenParams& p, hipStream_t s) {
+    const uint64_t n = uint64_t(p.W) * p.H * p.spp;
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536ull * 16) blocks = 65536ull * 16;
+    return dispatch_metric<tag_t>(p.metric, [&](auto m) -> hipError_t {
+        constexpr int M = decltype(m)::value;
+        if (p.precision == MGK_PRECISION_FP32_REFERENCE)
+            hipLaunchKernelGGL((k_generate_rays<M, float>), dim3(unsigned(blocks)), dim3(256), 0, s, p);
+        else
+            hipLaunchKernelGGL((k_generate_rays<M, double>), dim3(unsigned(blocks)), dim3(256), 0, s, p);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s) {
+    return dispatch_metric<tag_t>(p.metric, [&](auto m) -> hipError_t {
+        constexpr int M = decltype(m)::value;
+        switch (p.precision) {
+            case MGK_PRECISION_FP32_REFERENCE:
+                hipLaunchKernelGGL((k_trace<M, MGK_PRECISION_FP32_REFERENCE>), dim3(grid), dim3(256), 0, s, p);
+                break;
+            case MGK_PRECISION_MIXED:
+                hipLaunchKernelGGL((k_trace<M, MGK_PRECISION_MIXED>), dim3(grid), dim3(256), 0, s, p);
+                break;
+            default:
+                hipLaunchKernelGGL((k_trace<M, MGK_PRECISION_FP64>), dim3(grid), dim3(256), 0, s, p);
+                break;
+        }
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_shade(const ShadeParams& p, hipStream_t s) {
+    const uint32_t npix = p.W * p.H;
+    if (npix == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shade, dim3((npix + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_geodesic_steps(const GeodesicStepParams& p, hipStream_t s) {
+    return dispatch_metric<tag_t>(p.metric, [&](auto m) -> hipError_t {
+        constexpr int M = decltype(m)::value;
+        hipLaunchKernelGGL((k_geodesic_steps<M>), dim3(1), dim3(64), 0, s, p);
+        return hipGetLastError();
+    });
+}
+
+}  // namespace mgk

@@ -1,0 +1,211 @@
+"""ctypes front of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Loads ``oracle/liboracle.so`` (the C++ restatement of the reference's per-ray
+arithmetic) and, where it was built, ``oracle/_ref/libmagik_ref.so`` (the
+reference's own host-compilable headers).  Imported only by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from magik_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libmagik_ref.so")
+
+F32, F64 = 0, 1
+
+_D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_U = C.POINTER(C.c_uint32)
+_SIG = {
+    "orc_philox4x32_10": (None, [_U, _U, _U]),
+    "orc_path_uniforms": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _F]),
+    "orc_rhs": (None, [C.c_int, C.c_int, C.c_double, _D, _D]),
+    "orc_metric": (None, [C.c_int, C.POINTER(abi.Integrator), _D, C.c_int, _D]),
+    "orc_kerr_schild_r": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "orc_solve_step": (None, [C.c_int, C.POINTER(abi.Integrator), _D, _D, _D, _D, _U]),
+    "orc_rkf45_oscillator": (None, [C.c_int, _D, _D, _D, C.c_double, C.c_uint32, C.c_double, C.c_double, _D, _U]),
+    "orc_generate_rays": (C.c_int, [C.c_int, C.POINTER(abi.Integrator), C.POINTER(abi.Camera), C.c_uint64, _U, _U,
+                                    C.c_uint64, _D]),
+    "orc_trace_rays": (C.c_int, [C.c_int, C.POINTER(abi.Integrator), C.POINTER(abi.Scene), _F, _D, C.c_uint64,
+                                 C.c_void_p, _U, _U, C.c_int]),
+    "orc_bin_weights": (None, [C.c_uint32, C.c_float, C.c_float, _F, C.c_uint32, C.c_float, C.c_float, _F]),
+    "orc_planck": (C.c_double, [C.c_int, C.c_double, C.c_double]),
+    "orc_gaussian_cdf": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double]),
+    "orc_shade": (C.c_int, [C.POINTER(abi.Scene), C.POINTER(abi.Spectral), C.POINTER(abi.Camera), C.c_uint64,
+                            C.c_uint32, C.c_uint32, C.c_uint32, _U, C.c_void_p, _F, _D]),
+    "orc_host_trace_geodesic": (C.c_int, [C.POINTER(abi.Container), abi.InitialConditions, abi.MetricParameters,
+                                          abi.IntegratorParameters]),
+    "orc_star_catalogue": (None, [C.c_uint64, C.c_uint32, C.c_float, _F]),
+    "orc_rotation": (None, [C.c_float, C.c_float, C.c_float, _F]),
+    "orc_sample_cmf": (None, [_F, C.c_uint32, C.c_float, _F]),
+    "orc_layout": (None, [C.POINTER(C.c_size_t)]),
+}
+
+_REF_SIG = {
+    "ref_rkf45_oscillator": (None, [_D, _F, _F, C.c_float, C.c_uint, C.c_float, C.c_float, _D]),
+    "ref_rkf45_kerr": (None, [_D, C.c_float, _F, _F, C.c_float, C.c_uint, C.c_float, C.c_float, _D]),
+    "ref_planck_f32": (C.c_float, [C.c_float, C.c_float]),
+    "ref_planck_f64": (C.c_double, [C.c_double, C.c_double]),
+    "ref_gaussian_cdf_f32": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+    "ref_white_led": (C.c_float, [C.c_float]),
+    "ref_rotate": (None, [C.c_float, C.c_float, C.c_float, _F]),
+    "ref_sample_cmf": (None, [C.c_float, _F]),
+    "ref_cmf_table": (C.c_uint, [_F]),
+    "ref_layout": (None, [C.POINTER(C.c_size_t)]),
+}
+
+_lib = None
+_ref = None
+
+
+def build() -> None:
+    """Compile liboracle.so (and _ref when the reference tree is present)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    if os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        for n, (r, a) in _SIG.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+def ref():
+    """The reference-header harness, or None where it could not be built."""
+    global _ref
+    if _ref is None and os.path.exists(REF_LIB):
+        L = C.CDLL(REF_LIB)
+        for n, (r, a) in _REF_SIG.items():
+            f = getattr(L, n)
+            f.restype = r
+            f.argtypes = a
+        _ref = L
+    return _ref
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_F)
+
+
+def uptr(a: np.ndarray):
+    assert a.dtype == np.uint32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_U)
+
+
+def rhs(prec: int, metric: int, arg: float, y) -> np.ndarray:
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(8)
+    lib().orc_rhs(prec, metric, arg, dptr(y), dptr(out))
+    return out
+
+
+def metric_tensor(prec: int, integ, q, inverse: bool) -> np.ndarray:
+    """Full 4x4 matrix in index order (x, y, z, t)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    o = np.zeros(10)
+    lib().orc_metric(prec, C.byref(integ), dptr(q), int(inverse), dptr(o))
+    tt, tx, ty, tz, xx, xy, xz, yy, yz, zz = o
+    return np.array([[xx, xy, xz, tx], [xy, yy, yz, ty], [xz, yz, zz, tz], [tx, ty, tz, tt]])
+
+
+def solve_step(prec: int, integ, y, h: float):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    y1 = np.zeros(8)
+    hh, ht = C.c_double(h), C.c_double(0)
+    tr = C.c_uint32(0)
+    lib().orc_solve_step(prec, C.byref(integ), dptr(y), C.byref(hh), C.byref(ht), dptr(y1), C.byref(tr))
+    return y1, hh.value, ht.value, tr.value
+
+
+def generate_rays(prec: int, integ, cam, seed: int, pixels, samples) -> np.ndarray:
+    pixels = np.ascontiguousarray(pixels, dtype=np.uint32)
+    samples = np.ascontiguousarray(samples, dtype=np.uint32)
+    out = np.zeros((len(pixels), 10))
+    abi.check(lib().orc_generate_rays(prec, C.byref(integ), C.byref(cam), seed, uptr(pixels), uptr(samples),
+                                      len(pixels), dptr(out)), "orc_generate_rays")
+    return out
+
+
+HIT_DTYPE = np.dtype([("cls", np.uint32), ("g", np.float32), ("a0", np.float32), ("a1", np.float32)])
+
+
+def trace_rays(prec: int, integ, scene, cam_pos, rays: np.ndarray, threads: int = 0):
+    rays = np.ascontiguousarray(rays, dtype=np.float64)
+    n = rays.shape[0]
+    hits = np.zeros(n, dtype=HIT_DTYPE)
+    steps = np.zeros(n, dtype=np.uint32)
+    trials = np.zeros(n, dtype=np.uint32)
+    cp = np.ascontiguousarray(cam_pos, dtype=np.float32)
+    abi.check(lib().orc_trace_rays(prec, C.byref(integ), C.byref(scene), fptr(cp), dptr(rays), n,
+                                   hits.ctypes.data_as(C.c_void_p), uptr(steps), uptr(trials), threads),
+              "orc_trace_rays")
+    return hits, steps, trials
+
+
+def shade(scene, spec, cam, seed: int, sample_begin: int, spp: int, pixel_ids, hits: np.ndarray,
+          weights: np.ndarray) -> np.ndarray:
+    pixel_ids = np.ascontiguousarray(pixel_ids, dtype=np.uint32)
+    hits = np.ascontiguousarray(hits)
+    assert hits.dtype == HIT_DTYPE and hits.size == spp * len(pixel_ids)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    out = np.zeros((len(pixel_ids), 3))
+    abi.check(lib().orc_shade(C.byref(scene), C.byref(spec), C.byref(cam), seed, sample_begin, spp, len(pixel_ids),
+                              uptr(pixel_ids), hits.ctypes.data_as(C.c_void_p), fptr(w), dptr(out)), "orc_shade")
+    return out
+
+
+def bin_weights(n_bin: int, bmin: float, bmax: float, table: np.ndarray) -> np.ndarray:
+    tab = np.ascontiguousarray(table, dtype=np.float32)
+    W = np.zeros((n_bin, 3), dtype=np.float32)
+    lib().orc_bin_weights(n_bin, bmin, bmax, fptr(tab), tab.shape[0], 360.0, 830.0, fptr(W))
+    return W
+
+
+def star_catalogue(seed: int, n: int, scale: float) -> np.ndarray:
+    out = np.zeros((n, 4), dtype=np.float32)
+    lib().orc_star_catalogue(seed, n, scale, fptr(out))
+    return out
+
+
+def cie_table() -> np.ndarray:
+    """The CIE 1931 2-degree table the product ships (magik_amd/data)."""
+    path = os.path.join(os.path.dirname(HERE), "magik_amd", "data", "cie1931_2deg_1nm.csv")
+    rows = [l.strip().split(",") for l in open(path) if l.strip() and not l.startswith("#")]
+    return np.array([[float(x) for x in r[1:]] for r in rows], dtype=np.float32)
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(uptr(c), uptr(k), uptr(o))
+    return o
+
+
+def path_uniforms(seed: int, cam: int, pixel: int, sample: int, block: int = 0) -> np.ndarray:
+    u = np.zeros(4, dtype=np.float32)
+    lib().orc_path_uniforms(seed, cam, pixel, sample, block, fptr(u))
+    return u

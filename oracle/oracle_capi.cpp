@@ -1,0 +1,405 @@
+// C entry points of the CPU oracle (test infrastructure only: loaded by tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the
+// product).  prec = 0 runs the FP32 reference restatement, prec = 1 the
+// all-FP64 oracle.  Structs are the ones of include/magik_amd.h so a test can
+// hand the same settings to the oracle and to the HIP library.
+#include "oracle.hpp"
+#include "magik_amd.h"
+
+#include <omp.h>
+#include <cstddef>
+#include <cstdio>
+
+using namespace oracle;
+
+namespace {
+
+template <class R> MetricParams<R> params_from(const mgk_integrator_t* in) {
+    MetricParams<R> p = make_params<R>(in->metric_type, in->time_reversed != 0, in->mass_kg, in->spin,
+                                       in->event_horizon_scalar, in->abs_tol, in->rel_tol, in->n_max_substep);
+    p.h_max = R(in->h_max);
+    return p;
+}
+
+template <class R> Scene<R> scene_from(const mgk_scene_t* s, uint32_t step_limit) {
+    Scene<R> sc;
+    sc.celestial_radius_m = R(s->celestial_radius_m);
+    sc.disk = (s->flags & MGK_SCENE_DISK) != 0;
+    sc.disk_r_in = R(s->disk_r_in);
+    sc.disk_r_out = R(s->disk_r_out);
+    sc.disk_T_in = R(s->disk_T_in);
+    sc.sphere = (s->flags & MGK_SCENE_SPHERE) != 0;
+    sc.sphere_center = {R(s->sphere_center[0]), R(s->sphere_center[1]), R(s->sphere_center[2])};
+    sc.sphere_radius = R(s->sphere_radius);
+    sc.sphere_T = R(s->sphere_T);
+    sc.sphere_checker = s->sphere_checker;
+    sc.background_scale = R(s->background_scale);
+    sc.background_T = R(s->background_T);
+    sc.step_limit = step_limit;
+    return sc;
+}
+
+template <class R> Camera<R> camera_from(const mgk_camera_t* c) {
+    Camera<R> cam;
+    cam.position = {R(c->position[0]), R(c->position[1]), R(c->position[2])};
+    for (int i = 0; i < 9; ++i) cam.orientation.m[i] = R(c->orientation[i]);
+    cam.focal_mm = R(c->focal_length_mm);
+    cam.sensor_width_mm = R(c->sensor_width_mm);
+    cam.W = c->x_resolution;
+    cam.H = c->y_resolution;
+    cam.observer_velocity = {R(c->observer_velocity[0]), R(c->observer_velocity[1]), R(c->observer_velocity[2])};
+    return cam;
+}
+
+template <class R> State<R> load_state(const double* y) {
+    return {{R(y[0]), R(y[1]), R(y[2]), R(y[3])}, {R(y[4]), R(y[5]), R(y[6]), R(y[7])}};
+}
+template <class R> void store_state(const State<R>& s, double* y) {
+    y[0] = double(s.q.x); y[1] = double(s.q.y); y[2] = double(s.q.z); y[3] = double(s.q.t);
+    y[4] = double(s.p.x); y[5] = double(s.p.y); y[6] = double(s.p.z); y[7] = double(s.p.t);
+}
+
+// Ray generation + IC for one (pixel, sample): kernel_generate_thin_lens_ray.cu:51-77
+// followed by kernel_generate_initial_conditions.cu:54-74.
+template <class R>
+void gen_ray(const MetricParams<R>& p, const Camera<R>& cam, uint64_t seed, uint32_t cam_id, uint32_t pixel, uint32_t sample,
+             double* out10) {
+    float u[4];
+    path_uniforms(seed, cam_id, pixel, sample, 0, u);
+    uint32_t px = pixel % cam.W, py = pixel / cam.W;
+    V3<R> sensor = sensor_position<R>(cam.sensor_width_mm, cam.W, cam.H, px, py, R(u[0]), R(u[1]));
+    V3<R> dir, orig;
+    pinhole_ray(sensor, cam.focal_mm, &dir, &orig);
+    orig = mul(cam.orientation, orig) + cam.position;
+    RayIC<R> ic = initial_conditions(cam, p, dir, orig);
+    store_state(ic.y, out10);
+    out10[8] = double(ic.h);
+    out10[9] = double(ic.contraction);
+}
+
+template <class R> void trace_one(const MetricParams<R>& p, const Scene<R>& sc, V3<R> origin, const double* ray, mgk_hit_t* hit,
+                                  uint32_t* steps, uint32_t* trials) {
+    RayIC<R> ic;
+    ic.y = load_state<R>(ray);
+    ic.h = R(ray[8]);
+    ic.contraction = R(ray[9]);
+    Hit<R> h = trace_ray(ic, p, sc, origin);
+    hit->cls = h.cls;
+    hit->g = float(h.g);
+    hit->a0 = float(h.a0);
+    hit->a1 = float(h.a1);
+    if (steps) *steps = h.steps;
+    if (trials) *trials = h.trials;
+}
+
+// Emission of a terminated path at observed wavelength lam, before the g^5
+// boost (matter_examples.cuh:60-95 for the emitter contract; the disk,
+// sphere, background and stars are this build's scene additions).
+double emission(const mgk_scene_t* s, const StarIndex* stars, const mgk_hit_t& h, double lam_emit) {
+    switch (h.cls) {
+        case MGK_HIT_DISK: {
+            double T = double(s->disk_T_in) * std::pow(double(h.a0) / double(s->disk_r_in), -0.75);
+            return planck<double>(lam_emit, T);
+        }
+        case MGK_HIT_SPHERE: {
+            double th = std::acos(std::fmax(-1.0, std::fmin(1.0, double(h.a0))));
+            double ph = double(h.a1) + double(kPi);
+            uint32_t n = s->sphere_checker;
+            uint32_t ci = std::min(n - 1, uint32_t(th / double(kPi) * n));
+            uint32_t cj = std::min(n - 1, uint32_t(ph / (2.0 * double(kPi)) * n));
+            return ((ci + cj) & 1u) ? 0.0 : planck<double>(lam_emit, double(s->sphere_T));
+        }
+        case MGK_HIT_ESCAPE: {
+            double v = 0;
+            if (s->flags & MGK_SCENE_BACKGROUND) v += double(s->background_scale) * planck<double>(lam_emit, double(s->background_T));
+            if ((s->flags & MGK_SCENE_STARS) && stars) {
+                double c = double(h.a0), sn = std::sqrt(std::fmax(0.0, 1.0 - c * c));
+                double b = stars->lookup(sn * std::cos(double(h.a1)), c, sn * std::sin(double(h.a1)));
+                if (b > 0) v += b * planck<double>(lam_emit, double(s->star_T));
+            }
+            return v;
+        }
+        default:
+            return 0.0;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { philox4x32_10(ctr, key, out); }
+
+void orc_path_uniforms(uint64_t seed, uint32_t cam, uint32_t pixel, uint32_t sample, uint32_t block, float u[4]) {
+    path_uniforms(seed, cam, pixel, sample, block, u);
+}
+
+// Right-hand side of the geodesic equation; arg = spin for Kerr (as used by the
+// equations), mass sign for Schwarzschild.
+void orc_rhs(int prec, int metric, double arg, const double y[8], double out[8]) {
+    if (prec == 0) {
+        State<float> s = load_state<float>(y), r;
+        r = metric == kKerr ? kerr_rhs(s, float(arg)) : metric == kSchwarzschild ? schwarzschild_rhs(s, float(arg)) : minkowski_rhs(s);
+        store_state(r, out);
+    } else {
+        State<double> s = load_state<double>(y), r;
+        r = metric == kKerr ? kerr_rhs(s, arg) : metric == kSchwarzschild ? schwarzschild_rhs(s, arg) : minkowski_rhs(s);
+        store_state(r, out);
+    }
+}
+
+// g_{mu nu} (inverse = 0) or g^{mu nu} (inverse = 1) as {tt,tx,ty,tz,xx,xy,xz,yy,yz,zz}.
+void orc_metric(int prec, const mgk_integrator_t* integ, const double q[4], int inverse, double out[10]) {
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        MetricParams<R> p = params_from<R>(integ);
+        Sym4<R> g = compute_g(V4<R>{R(q[0]), R(q[1]), R(q[2]), R(q[3])}, p, inverse != 0);
+        R v[10] = {g.tt, g.tx, g.ty, g.tz, g.xx, g.xy, g.xz, g.yy, g.yz, g.zz};
+        for (int i = 0; i < 10; ++i) out[i] = double(v[i]);
+    };
+    if (prec == 0) run(0.0f); else run(0.0);
+}
+
+double orc_kerr_schild_r(int prec, double x, double y, double z, double a) {
+    if (prec == 0) return double(kerr_schild_r<float>(float(x), float(y), float(z), float(a)));
+    return kerr_schild_r<double>(x, y, z, a);
+}
+
+// One adaptive step (metric<T>::solve_step): returns y1, updates h, h_taken.
+void orc_solve_step(int prec, const mgk_integrator_t* integ, const double y[8], double* h, double* h_taken, double y1[8],
+                    uint32_t* trials) {
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        MetricParams<R> p = params_from<R>(integ);
+        R hh = R(*h), ht = 0;
+        State<R> r = solve_step(load_state<R>(y), p, &hh, &ht, trials);
+        store_state(r, y1);
+        *h = double(hh);
+        *h_taken = double(ht);
+    };
+    if (prec == 0) run(0.0f); else run(0.0);
+}
+
+// RKF45 on the harmonic oscillator y'' = -y embedded in the 8-wide state
+// (q.x = y, p.x = y'), for controller checks against the reference driver.
+void orc_rkf45_oscillator(int prec, const double y[8], double* h, double* h_taken, double h_max, uint32_t n_max,
+                          double abs_tol, double rel_tol, double y1[8], uint32_t* trials) {
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        auto f = [](const State<R>& s) {
+            State<R> d{};
+            d.q.x = s.p.x;
+            d.p.x = -s.q.x;
+            d.q.t = R(1);
+            return d;
+        };
+        R hh = R(*h), ht = 0;
+        State<R> r = rkf45_step<R>(f, load_state<R>(y), &hh, &ht, R(h_max), n_max, R(abs_tol), R(rel_tol), trials);
+        store_state(r, y1);
+        *h = double(hh);
+        *h_taken = double(ht);
+    };
+    if (prec == 0) run(0.0f); else run(0.0);
+}
+
+// Rays for (pixel[i], sample[i]); out is n x 10 (x y z t px py pz pt h u.p).
+int orc_generate_rays(int prec, const mgk_integrator_t* integ, const mgk_camera_t* cam, uint64_t seed, const uint32_t* pixel,
+                      const uint32_t* sample, uint64_t n, double* out) {
+    if (!integ || !cam || !pixel || !sample || !out) return MAGIK_ERROR_INVALID_POINTER;
+    if (prec == 0) {
+        MetricParams<float> p = params_from<float>(integ);
+        Camera<float> c = camera_from<float>(cam);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < int64_t(n); ++i) gen_ray(p, c, seed, cam->camera_id, pixel[i], sample[i], out + 10 * i);
+    } else {
+        MetricParams<double> p = params_from<double>(integ);
+        Camera<double> c = camera_from<double>(cam);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < int64_t(n); ++i) gen_ray(p, c, seed, cam->camera_id, pixel[i], sample[i], out + 10 * i);
+    }
+    return MAGIK_SUCCESS;
+}
+
+// Trace n rays (n x 10 doubles, as produced above) to termination with
+// `threads` OpenMP threads (0 = all).
+int orc_trace_rays(int prec, const mgk_integrator_t* integ, const mgk_scene_t* scene, const float cam_pos[3], const double* rays,
+                   uint64_t n, mgk_hit_t* hits, uint32_t* steps, uint32_t* trials, int threads) {
+    if (!integ || !scene || !cam_pos || !rays || !hits) return MAGIK_ERROR_INVALID_POINTER;
+    if (threads > 0) omp_set_num_threads(threads);
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        MetricParams<R> p = params_from<R>(integ);
+        Scene<R> sc = scene_from<R>(scene, integ->step_limit);
+        V3<R> o{R(cam_pos[0]), R(cam_pos[1]), R(cam_pos[2])};
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t i = 0; i < int64_t(n); ++i)
+            trace_one(p, sc, o, rays + 10 * i, hits + i, steps ? steps + i : nullptr, trials ? trials + i : nullptr);
+    };
+    if (prec == 0) run(0.0f); else run(0.0);
+    return MAGIK_SUCCESS;
+}
+
+// Bin CMF weights (host_color_utilities.cuh:145-178) from a CIE table (n x 3).
+void orc_bin_weights(uint32_t n_bin, float bmin, float bmax, const float* tab, uint32_t n_tab, float tmin, float tmax, float* W) {
+    bin_weights(W, n_bin, bmin, bmax, tab, n_tab, tmin, tmax);
+}
+
+double orc_planck(int prec, double lam_nm, double T) {
+    if (prec == 0) return double(planck<float>(float(lam_nm), float(T)));
+    return planck<double>(lam_nm, T);
+}
+
+double orc_gaussian_cdf(int prec, double mean, double x, double sigma) {
+    if (prec == 0) return double(norm_gaussian_cdf<float>(float(mean), float(x), float(sigma)));
+    return norm_gaussian_cdf<double>(mean, x, sigma);
+}
+
+// Shade pixels: hits[s * n_pix + j] is sample s of pixel pixel_ids[j]; weights
+// are n_bin x 3 XYZ bin weights.  Per sample the hero wavelengths are
+// stratified from the block-0 uniform u[2]; each is redshifted (lambda g), its
+// emission boosted by g^5 (bxdf_utilities.cuh:156-163), converted to sensor
+// irradiance (device_color_utilities.cuh:22-29) and spread over the bins by
+// Gaussian-CDF differences (kernel_redistribute_radiance.cu:64-88); the bins
+// are accumulated over samples and folded with the weights / spp
+// (device_color_utilities.cuh:40-54).  Computed in double.
+int orc_shade(const mgk_scene_t* scene, const mgk_spectral_t* spec, const mgk_camera_t* cam, uint64_t seed, uint32_t sample_begin,
+              uint32_t spp, uint32_t n_pix, const uint32_t* pixel_ids, const mgk_hit_t* hits, const float* weights, double* xyz) {
+    if (!scene || !spec || !cam || !pixel_ids || !hits || !weights || !xyz) return MAGIK_ERROR_INVALID_POINTER;
+    StarIndex idx;
+    bool have_stars = (scene->flags & MGK_SCENE_STARS) && scene->n_star > 0;
+    if (have_stars) idx.build(star_catalogue(scene->star_seed, scene->n_star, scene->star_scale), scene->star_radius_rad);
+    double inv_b = double(cam->sensor_width_mm) / std::sqrt(double(float(cam->x_resolution * cam->x_resolution + cam->y_resolution * cam->y_resolution)));
+    double sw = double(cam->x_resolution) * inv_b, sh = double(cam->y_resolution) * inv_b;
+    double span = double(spec->hero_max_nm) - double(spec->hero_min_nm);
+    double bw = (double(spec->bin_max_nm) - double(spec->bin_min_nm)) / double(spec->n_bin);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t j = 0; j < int64_t(n_pix); ++j) {
+        double bins[64] = {0};
+        for (uint32_t s = 0; s < spp; ++s) {
+            const mgk_hit_t& h = hits[uint64_t(s) * n_pix + j];
+            float u[4];
+            path_uniforms(seed, cam->camera_id, pixel_ids[j], sample_begin + s, 0, u);
+            double g = double(h.g);
+            for (uint32_t k = 0; k < spec->n_hero; ++k) {
+                double f = double(u[2]) + double(k) / double(spec->n_hero);
+                f -= std::floor(f);
+                double lam = double(spec->hero_min_nm) + span * f;
+                double L = emission(scene, have_stars ? &idx : nullptr, h, lam * g) * (g * g * g * g * g);
+                if (std::isnan(L) || std::isinf(L) || L < 0) L = 0;
+                double thr = span / double(spec->n_hero);
+                if (!(L > 0)) continue;
+                double E = radiance_to_irradiance<double>(L, thr, sw, sh, spec->iso, spec->fps, spec->shutter_angle,
+                                                          cam->x_resolution, cam->y_resolution);
+                for (uint32_t i = 0; i < spec->n_bin; ++i) {
+                    double x1 = double(spec->bin_min_nm) + double(i) * bw, x2 = x1 + bw;
+                    bins[i] += E * (norm_gaussian_cdf<double>(lam, x2, spec->sigma_nm) - norm_gaussian_cdf<double>(lam, x1, spec->sigma_nm));
+                }
+            }
+        }
+        double c[3] = {0, 0, 0};
+        for (uint32_t i = 0; i < spec->n_bin; ++i)
+            for (int k = 0; k < 3; ++k) c[k] += bins[i] * double(weights[3 * i + k]);
+        for (int k = 0; k < 3; ++k) xyz[3 * j + k] = c[k] / double(spp);
+    }
+    return MAGIK_SUCCESS;
+}
+
+// Restatement of magik_trace_geodesic (src/magik.cpp:196-207 with
+// relativity/host_trace_geodesic.cu:13-144) in FP32: the host-API initial
+// conditions (rotate(x, y, z) applied as rotation(alpha = x_rot, beta = y_rot,
+// gamma = z_rot); rotated seeds for the observer, identity seeds for the
+// momentum; t0 = 0), then solve_step / rhs per step until the horizon or the
+// container is full, then positions scaled to SI.
+int orc_host_trace_geodesic(magik_trace_geodesic_container_t* c, magik_trace_geodesic_inital_conditions_t ic,
+                            magik_trace_geodesic_metric_parameters_t mp, magik_trace_geodesic_integrator_parameters_t ip) {
+    if (!c || !c->steps) return MAGIK_ERROR_INVALID_POINTER;
+    if (c->capacity != sizeof(magik_trace_geodesic_step_t) * c->n_allocated_step)
+        return MAGIK_ERROR_HOST_TRACE_ALLOCATED_STEPS_AND_CAPACITY_MISMATCH;
+    if (int(mp.metric_type) != kMinkowski && int(mp.metric_type) != kSchwarzschild && int(mp.metric_type) != kKerr) return MAGIK_ERROR_UNKNOWN;
+    using R = float;
+    float mass = int(mp.metric_type) == kKerr ? mp.kerr_mass : mp.schwarzschild_mass;
+    MetricParams<R> p = make_params<R>(mp.metric_type, mp.time_reversed, mass, mp.kerr_spin, mp.event_horizon_scalar,
+                                       ip.integrator_absolute_error_tolerance, ip.integrator_relative_error_tolerance,
+                                       ip.n_integrator_max_substep);
+    M3<R> m = rotation<R>(ic.x_rotation, ic.y_rotation, ic.z_rotation);
+    V3<R> dir = mul(m, V3<R>{0, 0, R(-1)});
+    V3<R> bx = mul(m, V3<R>{R(1), 0, 0}), by = mul(m, V3<R>{0, R(1), 0}), bz = mul(m, V3<R>{0, 0, R(1)});
+    Seeds<R> rs;
+    rs.X = {bx.x, bx.y, bx.z, 0};
+    rs.Y = {by.x, by.y, by.z, 0};
+    rs.Z = {bz.x, bz.y, bz.z, 0};
+    R sq = 0;
+    V3<R> lpv{ic.local_proper_velocity_x, ic.local_proper_velocity_y, ic.local_proper_velocity_z};
+    if (ic.test_particle_type == MASSIVE) {
+        sq = R(-1);
+        dir = dir * (ic.massive_particle_speed / kSpeedOfLight);
+    }
+    R L = length_scale(p);
+    V4<R> q{ic.spatial_position_x / L, ic.spatial_position_y / L, ic.spatial_position_z / L, R(0) / (L / kSpeedOfLight)};
+    V4<R> u = rest_frame_four_velocity(q, rs, lpv / kSpeedOfLight, p);
+    V4<R> pm = direction_to_momentum(q, Seeds<R>{}, dir / L, u, sq, p);
+    State<R> y0{q, pm};
+    State<R> ys = rhs(y0, p);
+    c->n_written = 0;
+    auto put = [&](magik_trace_geodesic_step_t& st, const State<R>& y, const State<R>& d, R contraction, R affine, R dh) {
+        st.four_position[0] = y.q.x; st.four_position[1] = y.q.y; st.four_position[2] = y.q.z; st.four_position[3] = y.q.t;
+        st.four_momentum[0] = y.p.x; st.four_momentum[1] = y.p.y; st.four_momentum[2] = y.p.z; st.four_momentum[3] = y.p.t;
+        st.moment_derivative[0] = d.p.x; st.moment_derivative[1] = d.p.y; st.moment_derivative[2] = d.p.z; st.moment_derivative[3] = d.p.t;
+        st.four_vector_contraction = contraction;
+        st.affine_parameter = affine;
+        st.delta_affine_parameter = dh;
+    };
+    if (c->n_allocated_step == 0) return MAGIK_SUCCESS;
+    put(c->steps[0], y0, ys, dot4(u, pm), R(0), initial_h(p));
+    c->n_written = 1;
+    while (c->n_written < c->n_allocated_step) {
+        auto& prev = c->steps[c->n_written - 1];
+        State<R> a{{prev.four_position[0], prev.four_position[1], prev.four_position[2], prev.four_position[3]},
+                   {prev.four_momentum[0], prev.four_momentum[1], prev.four_momentum[2], prev.four_momentum[3]}};
+        R h = prev.delta_affine_parameter, ht = 0;
+        State<R> b = solve_step(a, p, &h, &ht);
+        State<R> d = rhs(b, p);
+        put(c->steps[c->n_written], b, d, prev.four_vector_contraction, prev.affine_parameter + ht, h);
+        c->n_written++;
+        if (in_domain(b.q, p)) break;
+    }
+    for (uint32_t i = 0; i < c->n_written; i++)
+        for (int k = 0; k < 4; ++k) c->steps[i].four_position[k] *= L;
+    return MAGIK_SUCCESS;
+}
+
+void orc_rotation(float a, float b, float g, float out[9]) {
+    M3<float> m = rotation<float>(a, b, g);
+    for (int i = 0; i < 9; ++i) out[i] = m.m[i];
+}
+
+void orc_sample_cmf(const float* tab, uint32_t n, float lam, float out[3]) { sample_cmf(tab, n, 360.0f, 830.0f, lam, out); }
+
+// sizeof / offsetof of the magik.h structs as include/magik_amd.h declares them.
+void orc_layout(size_t* out) {
+    size_t i = 0;
+    out[i++] = sizeof(magik_trace_geodesic_metric_parameters_t);
+    out[i++] = offsetof(magik_trace_geodesic_metric_parameters_t, time_reversed);
+    out[i++] = offsetof(magik_trace_geodesic_metric_parameters_t, event_horizon_scalar);
+    out[i++] = offsetof(magik_trace_geodesic_metric_parameters_t, kerr_spin);
+    out[i++] = sizeof(magik_trace_geodesic_integrator_parameters_t);
+    out[i++] = sizeof(magik_trace_geodesic_inital_conditions_t);
+    out[i++] = offsetof(magik_trace_geodesic_inital_conditions_t, massive_particle_speed);
+    out[i++] = sizeof(magik_trace_geodesic_step_t);
+    out[i++] = offsetof(magik_trace_geodesic_step_t, four_vector_contraction);
+    out[i++] = sizeof(magik_trace_geodesic_container_t);
+    out[i++] = offsetof(magik_trace_geodesic_container_t, n_written);
+    out[i++] = offsetof(magik_trace_geodesic_container_t, steps);
+    out[i++] = (size_t)MAGIK_ERROR_HOST_TRACE_ALLOCATED_STEPS_AND_CAPACITY_MISMATCH;
+    out[i++] = (size_t)MAGIK_ERROR_UNKNOWN_METRIC;
+}
+
+// Star catalogue exactly as the oracle generates it (n x 4: dx dy dz brightness).
+void orc_star_catalogue(uint64_t seed, uint32_t n, float scale, float* out) {
+    std::vector<Star> s = star_catalogue(seed, n, scale);
+    for (uint32_t i = 0; i < n; ++i) {
+        out[4 * i] = s[i].dx; out[4 * i + 1] = s[i].dy; out[4 * i + 2] = s[i].dz; out[4 * i + 3] = s[i].brightness;
+    }
+}
+
+}  // extern "C"
