@@ -36,6 +36,8 @@ constexpr float kGravitational = 6.67430E-11f;
 
 template <class R> inline R sqrt_(R v) { return std::sqrt(v); }
 template <class R> inline R pow_(R a, R b) { return std::pow(a, b); }
+template <class R> inline R abs_(R v) { return std::fabs(v); }
+template <class R> inline R max_(R a, R b) { return std::fmax(a, b); }
 
 // ---------------------------------------------------------------------------
 // Vectors.  4-vectors keep t LAST ({x,y,z,t}, internal_math.h:909-915); the
@@ -104,7 +106,7 @@ template <class R> inline State<R> operator*(R s, State<R> a) { return {s * a.q,
 // order t,x,y,z of position then momentum (relativity_utilities.cuh:15-29).
 template <class R>
 inline R rms_error(const State<R>& e, const State<R>& y0, const State<R>& y1, R abs_tol, R rel_tol) {
-    auto c = [&](R ev, R a, R b) { return ev / (abs_tol + rel_tol * std::fmax(std::fabs(a), std::fabs(b))); };
+    auto c = [&](R ev, R a, R b) { return ev / (abs_tol + rel_tol * max_(abs_(a), abs_(b))); };
     R s0 = c(e.q.t, y0.q.t, y1.q.t), s1 = c(e.q.x, y0.q.x, y1.q.x), s2 = c(e.q.y, y0.q.y, y1.q.y), s3 = c(e.q.z, y0.q.z, y1.q.z);
     R s4 = c(e.p.t, y0.p.t, y1.p.t), s5 = c(e.p.x, y0.p.x, y1.p.x), s6 = c(e.p.y, y0.p.y, y1.p.y), s7 = c(e.p.z, y0.p.z, y1.p.z);
     R total = (s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3 + s4 * s4 + s5 * s5 + s6 * s6 + s7 * s7) / R(8);
@@ -622,7 +624,7 @@ template <class R> inline V4<R> kepler_velocity(V4<R> q, R r, const MetricParams
 // Planck's law in W sr^-1 m^-3 with lambda in nm (internal_math.h:2511-2526).
 template <class R> inline R planck(R lambda_nm, R T) {
     if (T <= R(0)) return R(0);
-    R lm = lambda_nm * R(1e-9f);
+    R lm = lambda_nm * R(1e-9);
     const R t0 = R(kPlanck) * R(kSpeedOfLight);
     const R t1 = R(2) * t0 * R(kSpeedOfLight);
     R t2 = lm * lm * lm * lm * lm;

@@ -1,0 +1,183 @@
+"""HIP path vs the CPU oracle on identical seeded rays (through the C-ABI).
+
+Bars (floating point, so by tolerance):
+  * ray generation: FP32 reference path within 4 ulp of the FP32 oracle,
+    FP64 path within 1e-12 relative of the FP64 oracle;
+  * tracing: termination class identical on >= 99.9 % of rays; for rays of the
+    same class, redshift g and hit coordinate within 1e-3 relative (FP32
+    reference vs FP32 oracle, mixed vs FP64 oracle) or 1e-9 (FP64 vs FP64);
+  * shading: per-pixel XYZ within 1e-4 relative of the FP64 oracle shade of
+    the same hits;
+  * mixed-precision frame vs FP64 oracle frame: mean relative per-pixel Y
+    error <= 1e-3.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import magik_amd as M
+from magik_amd import abi
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SEED = 1234
+SIZES = {"minkowski_sphere": (64, 64, 2), "schwarzschild_disk": (64, 64, 2), "kerr_disk_1080p": (96, 54, 2)}
+
+
+def setup(name, prec):
+    W, H, spp = SIZES[name]
+    integ, sc, spec, cam, _ = M.workload(name, W=W, H=H, spp=spp, precision=prec)
+    return integ, sc, spec, cam, spp
+
+
+def pix_samples(cam, spp):
+    n = cam.x_resolution * cam.y_resolution
+    return np.tile(np.arange(n, dtype=np.uint32), spp), np.repeat(np.arange(spp, dtype=np.uint32), n)
+
+
+@pytest.mark.parametrize("name", list(SIZES))
+@pytest.mark.parametrize("prec", [M.PREC_FP32_REFERENCE, M.PREC_FP64])
+def test_generate_rays_matches_oracle(gpu_ctx, name, prec):
+    integ, sc, spec, cam, spp = setup(name, prec)
+    rays = gpu_ctx.generate_rays(integ, cam, SEED, 0, spp)
+    torch.cuda.synchronize()
+    g = rays.cpu().numpy().T.astype(np.float64)
+    p, s = pix_samples(cam, spp)
+    ref = O.generate_rays(O.F32 if prec == M.PREC_FP32_REFERENCE else O.F64, integ, cam, SEED, p, s)
+    if prec == M.PREC_FP32_REFERENCE:
+        ulp = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(g - ref) <= 4 * ulp + 1e-30), np.max(np.abs(g - ref) / (ulp + 1e-30))
+    else:
+        assert np.allclose(g, ref, rtol=1e-12, atol=1e-300)
+
+
+def trace_both(ctx, name, prec, oracle_prec):
+    integ, sc, spec, cam, spp = setup(name, prec)
+    ctx.set_scene(sc)
+    rays = ctx.generate_rays(integ, cam, SEED, 0, spp)
+    hits, steps = ctx.trace(integ, sc, list(cam.position), rays)
+    torch.cuda.synchronize()
+    gh = M.hits_to_numpy(hits)
+    rh, rsteps, _ = O.trace_rays(oracle_prec, integ, sc, list(cam.position), rays.cpu().numpy().T.astype(np.float64))
+    return gh, steps.cpu().numpy(), rh, rsteps, (integ, sc, spec, cam, spp, hits)
+
+
+def compare_hits(gh, rh, rtol):
+    assert np.all(gh["cls"] != M.HIT_NONE)
+    same = gh["cls"] == rh["cls"]
+    assert same.mean() >= 0.999, f"class agreement {same.mean():.5f}"
+    m = same & (gh["cls"] != M.HIT_HORIZON) & (gh["cls"] != M.HIT_STEP_LIMIT)
+    for f in ("g", "a0"):
+        d = np.abs(gh[f][m].astype(np.float64) - rh[f][m]) / np.maximum(np.abs(rh[f][m]), 1e-6)
+        assert np.quantile(d, 0.999) <= rtol, (f, np.quantile(d, 0.999), d.max())
+
+
+@pytest.mark.parametrize("name", list(SIZES))
+def test_trace_fp32_reference_matches_fp32_oracle(gpu_ctx, name):
+    gh, gs, rh, rs, _ = trace_both(gpu_ctx, name, M.PREC_FP32_REFERENCE, O.F32)
+    compare_hits(gh, rh, 1e-3)
+    assert np.mean(gs == rs) >= 0.99
+
+
+@pytest.mark.parametrize("name", list(SIZES))
+def test_trace_fp64_matches_fp64_oracle(gpu_ctx, name):
+    gh, gs, rh, rs, _ = trace_both(gpu_ctx, name, M.PREC_FP64, O.F64)
+    compare_hits(gh, rh, 1e-9)
+    assert np.mean(gs == rs) >= 0.999
+
+
+@pytest.mark.parametrize("name", ["schwarzschild_disk", "kerr_disk_1080p"])
+def test_mixed_matches_fp64_oracle(gpu_ctx, name):
+    gh, gs, rh, rs, (integ, sc, spec, cam, spp, hits) = trace_both(gpu_ctx, name, M.PREC_MIXED, O.F64)
+    compare_hits(gh, rh, 1e-3)
+    # radiance: shade both hit sets with the same (GPU) shader, compare per-pixel Y
+    ref_hits = torch.from_numpy(rh.view(np.int32).reshape(-1, 4).copy()).to(hits.device)
+    y_gpu = gpu_ctx.shade(sc, spec, cam, SEED, 0, spp, hits)[..., 1].cpu().numpy().ravel().astype(np.float64)
+    y_ref = gpu_ctx.shade(sc, spec, cam, SEED, 0, spp, ref_hits)[..., 1].cpu().numpy().ravel().astype(np.float64)
+    eps = 1e-3 * y_ref.mean()
+    err = np.abs(y_gpu - y_ref) / np.maximum(y_ref, eps)
+    assert err.mean() <= 1e-3, err.mean()
+
+
+@pytest.mark.parametrize("name", list(SIZES))
+def test_shade_matches_oracle(gpu_ctx, name):
+    integ, sc, spec, cam, spp = setup(name, M.PREC_FP64)
+    gpu_ctx.set_scene(sc)
+    rays = gpu_ctx.generate_rays(integ, cam, SEED, 0, spp)
+    hits, _ = gpu_ctx.trace(integ, sc, list(cam.position), rays)
+    xyz = gpu_ctx.shade(sc, spec, cam, SEED, 0, spp, hits).cpu().numpy().reshape(-1, 3).astype(np.float64)
+    W, _ = M.color_weights(spec)
+    npx = cam.x_resolution * cam.y_resolution
+    ref = O.shade(sc, spec, cam, SEED, 0, spp, np.arange(npx), M.hits_to_numpy(hits), W)
+    scale = np.maximum(np.abs(ref), 1e-6 * np.abs(ref).max())
+    rel = np.abs(xyz - ref) / scale
+    assert np.quantile(rel, 0.999) <= 1e-4, np.quantile(rel, 0.999)
+
+
+def test_render_frame_matches_stages_and_is_deterministic(gpu_ctx):
+    integ, sc, spec, cam, spp = setup("kerr_disk_1080p", M.PREC_MIXED)
+    gpu_ctx.set_scene(sc)
+    a = gpu_ctx.render_frame(integ, sc, spec, cam, SEED, spp).clone()
+    b = gpu_ctx.render_frame(integ, sc, spec, cam, SEED, spp).clone()
+    rays = gpu_ctx.generate_rays(integ, cam, SEED, 0, spp)
+    hits, _ = gpu_ctx.trace(integ, sc, list(cam.position), rays)
+    c = gpu_ctx.shade(sc, spec, cam, SEED, 0, spp, hits)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b) and torch.equal(a, c)
+    st = gpu_ctx.stats()
+    assert st.n_rays == cam.x_resolution * cam.y_resolution * spp and st.n_trials >= st.n_steps > 0
+
+
+def test_trace_queue_subset(gpu_ctx):
+    integ, sc, spec, cam, spp = setup("kerr_disk_1080p", M.PREC_MIXED)
+    rays = gpu_ctx.generate_rays(integ, cam, SEED, 0, spp)
+    full, _ = gpu_ctx.trace(integ, sc, list(cam.position), rays)
+    n = rays.shape[1]
+    q = torch.arange(3, n, 7, dtype=torch.int32, device=rays.device)
+    part, _ = gpu_ctx.trace(integ, sc, list(cam.position), rays, n=q.numel(), queue=q)
+    torch.cuda.synchronize()
+    idx = q.long()
+    assert torch.equal(part[idx], full[idx])
+    mask = torch.ones(n, dtype=torch.bool, device=rays.device)
+    mask[idx] = False
+    assert torch.all(part[mask] == 0)
+
+
+def test_trace_empty_and_step_limit(gpu_ctx):
+    integ, sc, spec, cam, spp = setup("kerr_disk_1080p", M.PREC_MIXED)
+    rays = gpu_ctx.generate_rays(integ, cam, SEED, 0, spp)
+    h0, _ = gpu_ctx.trace(integ, sc, list(cam.position), rays, n=0)
+    torch.cuda.synchronize()
+    assert torch.all(h0 == 0)
+    integ.step_limit = 3
+    h, s = gpu_ctx.trace(integ, sc, list(cam.position), rays)
+    gh = M.hits_to_numpy(h)
+    st = s.cpu().numpy()
+    assert np.all(st <= 4)
+    assert np.all((gh["cls"] == M.HIT_STEP_LIMIT) == (st == 4))
+    assert np.any(gh["cls"] == M.HIT_STEP_LIMIT)
+
+
+@pytest.mark.parametrize("metric", [M.KERR, M.SCHWARZSCHILD, M.MINKOWSKI])
+def test_magik_trace_geodesic_matches_host_restatement(metric):
+    lib = abi.load()
+    n = 400
+    mp = abi.MetricParameters(metric_type=metric, time_reversed=False, event_horizon_scalar=1.01,
+                              schwarzschild_mass=6.7329546075e26, kerr_mass=6.7329546075e26, kerr_spin=0.7)
+    ip = abi.IntegratorParameters(n_integrator_max_substep=56, integrator_absolute_error_tolerance=1e-5,
+                                  integrator_relative_error_tolerance=1e-5)
+    ic = abi.InitialConditions(test_particle_type=abi.MASSLESS, spatial_position_x=2.0, spatial_position_y=0.3,
+                               spatial_position_z=9.0, x_rotation=0.1, y_rotation=0.05, z_rotation=-0.2)
+    out = []
+    for fn in (lib.magik_trace_geodesic, O.lib().orc_host_trace_geodesic):
+        steps = (abi.Step * n)()
+        cont = abi.Container(capacity=C.sizeof(abi.Step) * n, n_allocated_step=n, n_written=0, steps=steps)
+        assert fn(C.byref(cont), ic, mp, ip) == abi.MAGIK_SUCCESS
+        arr = np.frombuffer(steps, dtype=np.float32).reshape(n, 15)[: cont.n_written].copy()
+        out.append(arr)
+    g, r = out
+    assert g.shape == r.shape, (g.shape, r.shape)
+    assert np.allclose(g, r, rtol=1e-4, atol=1e-6 * np.abs(r).max())

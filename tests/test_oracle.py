@@ -1,0 +1,330 @@
+"""The CPU oracle, pinned.
+
+1. Against the reference's own arithmetic: golden vectors written by
+   tests/golden/make_golden.py from the reference headers (internal_math.h,
+   magik.h, color_matching_functions.cuh) -- bit-exact in FP32.  Where the
+   reference tree is present the harness is also called directly.
+2. Against published known answers (Random123 Philox4x32-10 vectors).
+3. Against mathematics the reference does not contain: Hamilton's equations
+   from finite differences of H = 1/2 g^{mu nu} p p, g g^-1 = 1, a = 0 Kerr ==
+   Schwarzschild, the horizon radius, null and timelike normalisation of the
+   initial conditions, conserved E and L_y along a geodesic, the static
+   redshift sqrt(g_tt(e)/g_tt(o)), the RKF45 order on a harmonic oscillator,
+   Wien's law, the CIE weight normalisation and the analytic flat-space scene.
+"""
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import magik_amd as M
+from magik_amd import abi
+from oracle import oracle as O
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.json")))
+
+
+def bits(x):
+    return int(np.float32(x).view(np.uint32))
+
+
+def kerr_integ(a=0.9, prec=M.PREC_FP64, tr=False):
+    return M.integrator(metric=M.KERR, precision=prec, spin=a, time_reversed=tr)
+
+
+# ---------------------------------------------------------------------------
+# 1. golden vectors of the reference arithmetic
+# ---------------------------------------------------------------------------
+def test_rkf45_oscillator_bitexact_with_reference_driver():
+    for v in GOLD["rkf45_oscillator"]:
+        y = np.array(v["y0"])
+        h, ht = C.c_double(v["h0"]), C.c_double(0)
+        y1 = np.zeros(8)
+        tr = C.c_uint32()
+        O.lib().orc_rkf45_oscillator(O.F32, O.dptr(y), C.byref(h), C.byref(ht), 1000.0, 56, v["tol"], v["tol"],
+                                     O.dptr(y1), C.byref(tr))
+        assert [bits(x) for x in y1] == v["y1_bits"]
+        assert bits(h.value) == v["h_bits"] and bits(ht.value) == v["h_taken_bits"]
+
+
+def test_rkf45_kerr_bitexact_with_reference_driver():
+    for v in GOLD["rkf45_kerr"]:
+        integ = M.integrator(metric=M.KERR, precision=M.PREC_FP32_REFERENCE, spin=v["a"], time_reversed=False,
+                             abs_tol=v["abs_tol"], rel_tol=v["rel_tol"])
+        y1, h, ht, _ = O.solve_step(O.F32, integ, v["y0"], v["h0"])
+        assert [bits(x) for x in y1] == v["y1_bits"]
+        assert bits(h) == v["h_bits"] and bits(ht) == v["h_taken_bits"]
+
+
+def test_planck_and_gaussian_cdf_bitexact():
+    for v in GOLD["planck"]:
+        assert bits(O.lib().orc_planck(O.F32, v["lam"], v["T"])) == v["f32_bits"]
+        assert O.lib().orc_planck(O.F64, v["lam"], v["T"]) == v["f64"]
+    for v in GOLD["gaussian_cdf"]:
+        assert bits(O.lib().orc_gaussian_cdf(O.F32, v["mean"], v["x"], v["sigma"])) == v["f32_bits"]
+
+
+def test_rotation_bitexact():
+    for v in GOLD["rotate"]:
+        m = (C.c_float * 9)()
+        O.lib().orc_rotation(*v["angles"], m)
+        assert [bits(x) for x in m] == v["m_bits"]
+
+
+def test_cie_table_and_interpolation_bitexact():
+    tab = O.cie_table()
+    assert tab.shape == (471, 3)
+    assert [bits(x) for x in tab.sum(axis=0, dtype=np.float64).astype(np.float32)] == GOLD["cmf_table_sum_bits"]
+    for v in GOLD["sample_cmf"]:
+        out = (C.c_float * 3)()
+        O.lib().orc_sample_cmf(O.fptr(tab), 471, v["lam"], out)
+        assert [bits(x) for x in out] == v["xyz_bits"]
+    assert np.allclose(tab[195], [0.5121, 1.0, 0.00575], atol=1e-4)   # 555 nm, CIE 1931
+
+
+def test_struct_layouts_match_magik_h():
+    lay = (C.c_size_t * 14)()
+    O.lib().orc_layout(lay)
+    assert list(lay) == GOLD["layout"]
+    mine = [C.sizeof(abi.MetricParameters), abi.MetricParameters.time_reversed.offset,
+            abi.MetricParameters.event_horizon_scalar.offset, abi.MetricParameters.kerr_spin.offset,
+            C.sizeof(abi.IntegratorParameters), C.sizeof(abi.InitialConditions),
+            abi.InitialConditions.massive_particle_speed.offset, C.sizeof(abi.Step),
+            abi.Step.four_vector_contraction.offset, C.sizeof(abi.Container), abi.Container.n_written.offset,
+            abi.Container.steps.offset, abi.MAGIK_ERROR_HOST_TRACE_ALLOCATED_STEPS_AND_CAPACITY_MISMATCH,
+            abi.MAGIK_ERROR_UNKNOWN_METRIC]
+    assert mine == GOLD["layout"]
+
+
+@pytest.mark.skipif(O.ref() is None, reason="reference tree not present (golden vectors cover this)")
+def test_oracle_against_live_reference_harness():
+    R = O.ref()
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        lam, T = float(rng.uniform(300, 900)), float(rng.uniform(500, 4e4))
+        assert bits(R.ref_planck_f32(lam, T)) == bits(O.lib().orc_planck(O.F32, lam, T))
+    for _ in range(50):
+        y = np.concatenate([rng.uniform(-40, 40, 3), [6e8], rng.normal(size=3), [-2.0]]).astype(np.float32).astype(float)
+        a = float(rng.uniform(-0.99, 0.99))
+        h0 = float(rng.choice([0.3, 3.0, 30.0]))
+        hh, ht = C.c_float(h0), C.c_float(0)
+        y1 = np.zeros(8)
+        R.ref_rkf45_kerr(O.dptr(y), a, C.byref(hh), C.byref(ht), 1000.0, 56, 1e-6, 1e-6, O.dptr(y1))
+        integ = M.integrator(metric=M.KERR, precision=M.PREC_FP32_REFERENCE, spin=a, time_reversed=False,
+                             abs_tol=1e-6, rel_tol=1e-6)
+        o1, oh, oht, _ = O.solve_step(O.F32, integ, y, h0)
+        assert np.array_equal(o1, y1) and oh == hh.value and oht == ht.value
+
+
+# ---------------------------------------------------------------------------
+# 2. published known answers
+# ---------------------------------------------------------------------------
+def test_philox_known_answers():
+    # Random123 (Salmon et al. SC'11) kat_vectors for philox4x32 10 rounds
+    assert list(O.philox([0, 0, 0, 0], [0, 0])) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert list(O.philox([0xffffffff] * 4, [0xffffffff] * 2)) == [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]
+    assert list(O.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0])) == \
+        [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]
+
+
+# ---------------------------------------------------------------------------
+# 3. mathematics
+# ---------------------------------------------------------------------------
+def hamiltonian(integ, y):
+    gi = O.metric_tensor(O.F64, integ, y[:4], inverse=True)
+    p = np.asarray(y[4:8])
+    return 0.5 * p @ gi @ p
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, -0.9, 0.99])
+def test_kerr_rhs_is_hamiltons_equations(a):
+    integ = kerr_integ(a)
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        q = np.concatenate([rng.uniform(-15, 15, 3), [3.0]])
+        if O.lib().orc_kerr_schild_r(O.F64, q[0], q[1], q[2], a) < 2.5:
+            continue
+        y = np.concatenate([q, rng.normal(size=4)])
+        d = O.rhs(O.F64, M.KERR, a, y)
+        eps = 1e-6
+        for i in range(3):   # dx^i/dl = dH/dp_i, dp_i/dl = -dH/dx^i
+            e = np.zeros(8); e[4 + i] = eps
+            dHdp = (hamiltonian(integ, y + e) - hamiltonian(integ, y - e)) / (2 * eps)
+            e = np.zeros(8); e[i] = eps
+            dHdq = (hamiltonian(integ, y + e) - hamiltonian(integ, y - e)) / (2 * eps)
+            assert d[i] == pytest.approx(dHdp, rel=1e-6, abs=1e-9)
+            assert d[4 + i] == pytest.approx(-dHdq, rel=1e-5, abs=1e-9)
+        e = np.zeros(8); e[7] = eps
+        assert d[3] == pytest.approx((hamiltonian(integ, y + e) - hamiltonian(integ, y - e)) / (2 * eps), rel=1e-6)
+        assert d[7] == 0.0
+
+
+def test_metric_inverse_and_horizon():
+    for a in (0.0, 0.3, -0.9, 0.999):
+        integ = kerr_integ(a)
+        for q in ([3.0, 1.0, -2.0, 0.0], [10.0, -5.0, 0.5, 0.0], [0.5, 2.5, 0.2, 0.0]):
+            g = O.metric_tensor(O.F64, integ, q, False)
+            gi = O.metric_tensor(O.F64, integ, q, True)
+            assert np.allclose(g @ gi, np.eye(4), atol=1e-12)
+        # g_tt = 0 on the ergosurface, Delta = 0 at r_+: f = 1 + a^2 y^2 / r^4 ... check r_+ solves r^2 - 2r + a^2 = 0
+        rp = 1 + math.sqrt(1 - a * a)
+        assert rp * rp - 2 * rp + a * a == pytest.approx(0, abs=1e-12)
+
+
+def test_kerr_a0_equals_schwarzschild():
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        y = np.concatenate([rng.uniform(-20, 20, 3), [1.0], rng.normal(size=4)])
+        assert np.allclose(O.rhs(O.F64, M.KERR, 0.0, y), O.rhs(O.F64, M.SCHWARZSCHILD, 1.0, y), rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("a", [0.0, 0.9, -0.99])
+def test_boyer_lindquist_camera_placement(a):
+    cam = M.camera_bl(50.0, 80.0, a, 25.0, 64, 36)
+    L = M.length_scale()
+    q = np.array(cam.position, dtype=np.float64) / L
+    assert O.lib().orc_kerr_schild_r(O.F64, q[0], q[1], q[2], a) == pytest.approx(50.0, rel=1e-6)
+    m = np.array(cam.orientation, dtype=np.float64).reshape(3, 3)
+    assert np.allclose(m @ m.T, np.eye(3), atol=1e-6) and np.linalg.det(m) == pytest.approx(1, abs=1e-6)
+    assert np.allclose(m[:, 2], q[:3] / np.linalg.norm(q[:3]), atol=1e-6)   # -z_cam looks at the origin
+    assert cam.focal_length_mm == pytest.approx(12 / math.tan(math.radians(12.5)), rel=1e-6)
+
+
+@pytest.mark.parametrize("name", ["schwarzschild_disk", "kerr_disk_1080p", "minkowski_sphere"])
+def test_initial_conditions_are_null(name):
+    integ, sc, spec, cam, spp = M.workload(name, W=16, H=9, spp=2, precision=M.PREC_FP64)
+    n = 16 * 9
+    rays = O.generate_rays(O.F64, integ, cam, 99, np.arange(n), np.zeros(n))
+    for r in rays:
+        H = hamiltonian(integ, r[:8])   # the metric is built with the same (time-reversed) spin
+        scale = 0.5 * (r[4:8] @ r[4:8])
+        assert abs(H) <= 1e-10 * scale
+        assert r[9] < 0            # u.p of a future-directed photon
+
+
+def test_schwarzschild_static_observer_is_unit_timelike():
+    integ = M.integrator(metric=M.SCHWARZSCHILD, precision=M.PREC_FP64)
+    for r in (3.0, 10.0, 100.0):
+        q = [0.0, 0.0, r, 0.0]
+        g = O.metric_tensor(O.F64, integ, q, False)
+        u = np.array([0, 0, 0, 1 / math.sqrt(1 - 2 / r)])
+        assert u @ g @ u == pytest.approx(-1, abs=1e-12)
+
+
+def test_conserved_energy_and_axial_momentum_along_geodesic():
+    a = 0.9
+    integ = M.integrator(metric=M.KERR, precision=M.PREC_FP64, spin=a, time_reversed=False, abs_tol=1e-10,
+                         rel_tol=1e-10)
+    y = np.array([0.0, 2.0, 30.0, 0.0, 0.05, -0.02, -1.0, 0.0])
+    gi = O.metric_tensor(O.F64, integ, y[:4], True)
+    # choose p_t so that H = 0
+    A, B, Cc = gi[3, 3], 2 * (gi[3, :3] @ y[4:7]), y[4:7] @ gi[:3, :3] @ y[4:7]
+    y[7] = (-B - math.sqrt(B * B - 4 * A * Cc)) / (2 * A)
+    E0, L0 = -y[7], y[6] * y[0] - y[4] * y[2]   # L_y = z p_x - x p_z up to sign
+    h = 1.0
+    for _ in range(300):
+        y, h, _, _ = O.solve_step(O.F64, integ, y, h)
+        if O.lib().orc_kerr_schild_r(O.F64, y[0], y[1], y[2], a) < 2.5:
+            break
+    assert -y[7] == E0                                           # dp_t = 0 exactly
+    assert y[6] * y[0] - y[4] * y[2] == pytest.approx(L0, abs=1e-7)
+    assert abs(hamiltonian(integ, y)) < 1e-7
+
+
+def test_rkf45_is_fifth_order_on_oscillator():
+    errs = []
+    for h in (0.2, 0.1):
+        y = np.array([1.0, 0, 0, 0, 0, 0, 0, 0])
+        hh, ht = C.c_double(h), C.c_double(0)
+        y1 = np.zeros(8)
+        tr = C.c_uint32()
+        O.lib().orc_rkf45_oscillator(O.F64, O.dptr(y), C.byref(hh), C.byref(ht), 1000.0, 1, 1.0, 0.0, O.dptr(y1),
+                                     C.byref(tr))
+        errs.append(abs(y1[0] - math.cos(h)))
+    # local error of a 5th-order step scales as h^6
+    assert errs[0] / errs[1] == pytest.approx(64, rel=0.15)
+
+
+def test_static_redshift_in_schwarzschild():
+    # camera static at r_o on +z looking at the hole; emissive sphere r_e centred on it
+    r_o, r_e = 40.0, 6.0
+    integ = M.integrator(metric=M.SCHWARZSCHILD, precision=M.PREC_FP64, abs_tol=1e-10, rel_tol=1e-10,
+                         time_reversed=True)
+    L = M.length_scale()
+    cam = M.camera_pinhole([0.0, 0.0, 0.0], [1, 0, 0, 0, 1, 0, 0, 0, 1], 8.0, 4, 4)
+    cam.position[2] = r_o * L + cam.focal_length_mm / 1000.0   # the ray origin is the lens point, f in front
+    sc = M.scene(celestial_radius_m=1000 * L, flags=M.SCENE_SPHERE, sphere_center=(0.0, 0.0, 0.0), sphere_radius=r_e)
+    rays = O.generate_rays(O.F64, integ, cam, 3, np.arange(16), np.zeros(16))
+    assert np.allclose(rays[:, 2], r_o, rtol=1e-6)
+    hits, _, _ = O.trace_rays(O.F64, integ, sc, list(cam.position), rays)
+    # static emitter, static observer: g = sqrt(g_tt(r_e) / g_tt(r_o)) on every path between them
+    assert np.all(hits["cls"] == M.HIT_SPHERE)
+    assert np.allclose(hits["g"], math.sqrt((1 - 2 / r_e) / (1 - 2 / r_o)), rtol=2e-7)
+
+
+def test_keplerian_emitter_is_unit_timelike_and_circular():
+    for a in (0.0, 0.9, -0.9):
+        a = float(np.float32(a))   # the library takes the spin in float
+        integ = M.integrator(metric=M.KERR, precision=M.PREC_FP64, spin=a, time_reversed=False)
+        for r in (M.r_isco(abs(a)) + 0.01, 8.0, 25.0):
+            # disk hit records carry r; rebuild u as the oracle does and check g(u,u) = -1
+            x, z = 0.0, math.sqrt(r * r + a * a)
+            s = -1.0 if a < 0 else 1.0
+            r32 = r ** 1.5
+            ut = (r32 + abs(a)) / (math.sqrt(r32) * math.sqrt(r32 - 3 * math.sqrt(r) + 2 * abs(a)))
+            w = ut * s / (r32 + abs(a))
+            u = np.array([z * w, 0.0, -x * w, ut])
+            g = O.metric_tensor(O.F64, integ, [x, 0.0, z, 0.0], False)
+            assert u @ g @ u == pytest.approx(-1, abs=1e-9)
+
+
+def test_wien_law_and_planck_scale():
+    lam = np.linspace(200, 2000, 20001)
+    for T in (4000.0, 6500.0, 1e4):
+        b = np.array([O.lib().orc_planck(O.F64, l, T) for l in lam])
+        assert lam[np.argmax(b)] * 1e-9 * T == pytest.approx(2.897771955e-3, rel=2e-4)
+
+
+def test_bin_weights_normalisation_and_cdf_partition():
+    W = O.bin_weights(24, 360.0, 830.0, O.cie_table())
+    bw = (830.0 - 360.0) / 24
+    assert float(W[:, 1].sum()) * bw == pytest.approx(1.0, rel=1e-5)
+    lam, sig = 550.0, 10.0
+    edges = 360.0 + bw * np.arange(25)
+    cdf = [O.lib().orc_gaussian_cdf(O.F64, lam, e, sig) for e in edges]
+    assert sum(np.diff(cdf)) == pytest.approx(cdf[-1] - cdf[0], abs=1e-15)
+
+
+def test_flat_space_scene_is_analytic():
+    integ, sc, spec, cam, spp = M.workload("minkowski_sphere", W=48, H=48, spp=1)
+    n = 48 * 48
+    rays = O.generate_rays(O.F64, integ, cam, 11, np.arange(n), np.zeros(n))
+    hits, steps, _ = O.trace_rays(O.F64, integ, sc, list(cam.position), rays)
+    for r, h in zip(rays, hits):
+        o = r[:3]
+        d = r[4:7] / np.linalg.norm(r[4:7])         # flat space: dx/dl = p
+        c = np.array([0.0, 0.0, 5.0])
+        b = d @ (o - c)
+        disc = b * b - ((o - c) @ (o - c) - 1.0)
+        expect = M.HIT_SPHERE if disc > 1e-9 and -b - math.sqrt(max(disc, 0)) > 0 else M.HIT_ESCAPE
+        if abs(disc) > 1e-6:
+            assert h["cls"] == expect
+        if h["cls"] == M.HIT_SPHERE:
+            t = -b - math.sqrt(disc)
+            nrm = (o + t * d - c)
+            assert h["a0"] == pytest.approx(nrm[1], abs=1e-5)
+            assert h["g"] == pytest.approx(1.0, abs=1e-12)
+
+
+def test_oracle_trace_fp32_and_fp64_agree_on_classes():
+    integ, sc, spec, cam, spp = M.workload("kerr_disk_1080p", W=48, H=27, spp=1)
+    n = 48 * 27
+    r32 = O.generate_rays(O.F32, integ, cam, 5, np.arange(n), np.zeros(n))
+    r64 = O.generate_rays(O.F64, integ, cam, 5, np.arange(n), np.zeros(n))
+    assert np.allclose(r32, r64, rtol=2e-6, atol=1e-6 * np.abs(r64).max(axis=0))
+    h32, _, _ = O.trace_rays(O.F32, integ, sc, list(cam.position), r32)
+    h64, _, _ = O.trace_rays(O.F64, integ, sc, list(cam.position), r64)
+    assert np.mean(h32["cls"] == h64["cls"]) >= 0.995
