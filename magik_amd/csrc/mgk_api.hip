@@ -131,6 +131,7 @@ void fill_trace_params(const mgk_integrator_t* in, const mgk_scene_t* sc, const 
     P.rel_tol = in->rel_tol;
     P.n_max = in->n_max_substep ? in->n_max_substep : 1;
     P.step_limit = in->step_limit;
+    P.horizon_scalar = in->event_horizon_scalar;
     if (in->metric_type == KERR) {
         float a = P.spin;
         P.spin_in_range = std::fabs(a) > 1.0f ? 0 : 1;
@@ -397,7 +398,10 @@ e_magik_result_types fill_shade_params(mgk_context* c, const mgk_scene_t* sc, co
     P.background_scale = sc->background_scale;
     P.background_T = sc->background_T;
     P.star_T = sc->star_T;
-    P.cos_star_r = std::cos(sc->star_radius_rad);
+    {
+        float ch = 2.0f * std::sin(0.5f * sc->star_radius_rad);
+        P.chord2_star_r = ch * ch;
+    }
     if ((sc->flags & MGK_SCENE_STARS) && sc->n_star > 0) {
         P.stars = reinterpret_cast<const float4*>(c->stars.ptr);
         P.cell_start = reinterpret_cast<const uint32_t*>(c->cell_start.ptr);

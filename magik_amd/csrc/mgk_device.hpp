@@ -245,12 +245,14 @@ __device__ __forceinline__ St<S> rhs_ref(const St<S>& s, S a, S mass_sign) {
 template <class S> struct Sym4 { S tt, tx, ty, tz, xx, xy, xz, yy, yz, zz; };
 
 template <class S> __device__ __forceinline__ S kerr_schild_r(S x, S y, S z, S a) {
+#pragma clang fp contract(off)
     S R = (x * x + y * y + z * z - a * a);
     S r2 = S(0.5) * R + S(0.5) * sqrt_<false>(R * R + S(4) * a * a * y * y);
     return sqrt_<false>(r2);
 }
 
 template <int METRIC, class S> __device__ __forceinline__ Sym4<S> metric_g(S x, S y, S z, S a, S mass_sign, bool inverse) {
+#pragma clang fp contract(off)
     if constexpr (METRIC == MINKOWSKI) {
         return {S(-1), S(0), S(0), S(0), S(1), S(0), S(0), S(1), S(0), S(1)};
     } else {
@@ -277,23 +279,31 @@ template <int METRIC, class S> __device__ __forceinline__ Sym4<S> metric_g(S x, 
 template <class S> struct V4 { S x, y, z, t; };
 
 template <class S> __device__ __forceinline__ S inner(const Sym4<S>& g, V4<S> u, V4<S> v) {
+#pragma clang fp contract(off)
     return g.tt * u.t * v.t + g.tx * (u.t * v.x + u.x * v.t) + g.ty * (u.t * v.y + u.y * v.t) + g.tz * (u.t * v.z + u.z * v.t) +
            g.xx * u.x * v.x + g.xy * (u.x * v.y + u.y * v.x) + g.xz * (u.x * v.z + u.z * v.x) + g.yy * u.y * v.y +
            g.yz * (u.y * v.z + u.z * v.y) + g.zz * u.z * v.z;
 }
 template <class S> __device__ __forceinline__ V4<S> lower(const Sym4<S>& g, V4<S> u) {
+#pragma clang fp contract(off)
     return {g.tx * u.t + g.xx * u.x + g.xy * u.y + g.xz * u.z, g.ty * u.t + g.xy * u.x + g.yy * u.y + g.yz * u.z,
             g.tz * u.t + g.xz * u.x + g.yz * u.y + g.zz * u.z, g.tt * u.t + g.tx * u.x + g.ty * u.y + g.tz * u.z};
 }
 template <class S> __device__ __forceinline__ V4<S> axpy(S s, V4<S> a, V4<S> b) {   // b + s a
+#pragma clang fp contract(off)
     return {b.x + s * a.x, b.y + s * a.y, b.z + s * a.z, b.t + s * a.t};
 }
+
 template <class S> __device__ __forceinline__ V4<S> scale(V4<S> a, S s) { return {a.x * s, a.y * s, a.z * s, a.t * s}; }
 template <class S> __device__ __forceinline__ V4<S> divs(V4<S> a, S s) { return {a.x / s, a.y / s, a.z / s, a.t / s}; }
-template <class S> __device__ __forceinline__ S dot4(V4<S> a, V4<S> b) { return ((a.x * b.x) + (a.y * b.y) + (a.z * b.z)) + (a.t * b.t); }
+template <class S> __device__ __forceinline__ S dot4(V4<S> a, V4<S> b) {
+#pragma clang fp contract(off)
+    return ((a.x * b.x) + (a.y * b.y) + (a.z * b.z)) + (a.t * b.t);
+}
 
 // Reference observer (metric_kerr.cuh:207-226; metric_schwarzschild.cuh:167-171).
 template <int METRIC, class S> __device__ __forceinline__ V4<S> reference_observer(S x, S y, S z, S a, S mass_sign) {
+#pragma clang fp contract(off)
     if constexpr (METRIC == KERR) {
         S r = kerr_schild_r(x, y, z, a);
         S r_sq = r * r, a_sq = a * a;
@@ -317,6 +327,7 @@ template <int METRIC, class S> __device__ __forceinline__ V4<S> reference_observ
 // returns the global vector of local components (l.x, l.y, l.z, l.t) in that frame.
 template <class S>
 __device__ __forceinline__ V4<S> tetrad_apply(const Sym4<S>& g, V4<S> X, V4<S> Y, V4<S> Z, V4<S> u, V4<S> l) {
+#pragma clang fp contract(off)
     V4<S> ez = axpy(inner(g, Z, u), u, Z);
     ez = divs(ez, sqrt_<false>(inner(g, ez, ez)));
     V4<S> ex = axpy(-inner(g, X, ez), ez, axpy(inner(g, X, u), u, X));
@@ -332,6 +343,21 @@ __device__ __forceinline__ V4<S> tetrad_apply(const Sym4<S>& g, V4<S> X, V4<S> Y
 
 // Keplerian emitter co-rotating with the equations' spin (Bardeen, Press &
 // Teukolsky 1972); see the oracle's kepler_velocity for the derivation.
+template <int METRIC, class S> __device__ __forceinline__ void kepler_ut_w(S r, S a, S& ut, S& w) {
+#pragma clang fp contract(off)
+    if constexpr (METRIC == MINKOWSKI) {
+        ut = S(1);
+        w = S(0);
+    } else {
+        S aa = METRIC == KERR ? (a < S(0) ? -a : a) : S(0);
+        S s = (METRIC == KERR && a < S(0)) ? S(-1) : S(1);
+        S sr = sqrt_<false>(r);
+        S r32 = r * sr;
+        ut = (r32 + aa) / (sqrt_<false>(r32) * sqrt_<false>(r32 - S(3) * sr + S(2) * aa));
+        w = ut * s / (r32 + aa);
+    }
+}
+
 template <int METRIC, class S> __device__ __forceinline__ V4<S> kepler_velocity(S x, S z, S r, S a) {
     if constexpr (METRIC == MINKOWSKI) {
         return {S(0), S(0), S(0), S(1)};
@@ -354,6 +380,57 @@ __device__ __forceinline__ float planck(float lambda_nm, float T) {
     float t2 = lm * lm * lm * lm * lm;
     float t3 = expf(t0 / (lm * kBoltzmann * T)) - 1.0f;
     return (t1 / t2) * (1.0f / t3);
+}
+
+}  // namespace mgk
+
+namespace mgk {
+
+// ---------------------------------------------------------------------------
+// Surrogate feature space.  A camera at Boyer-Lindquist (r, theta) of a Kerr
+// hole with spin a sits at the Kerr-Schild point (0, r cos theta,
+// sqrt(r^2 + a^2) sin theta), looks at the origin with +y up, and a ray's
+// screen coordinates are alpha = r d_x / (-d_z), beta = r d_y / (-d_z) for its
+// camera-local unit direction d.  ic_from_features builds the ray's initial
+// conditions exactly as the render path does (reference observer, camera
+// tetrad, t0 = c/L, |p| ~ 1/L).
+// ---------------------------------------------------------------------------
+struct Camframe {
+    double P[3];           // dimensionless position
+    double X[3], Y[3], Z[3];   // camera axes (columns of the orientation)
+};
+
+__host__ __device__ inline void bl_camera_frame(double a, double r, double theta, Camframe& f) {
+    f.P[0] = 0.0;
+    f.P[1] = r * cos(theta);
+    f.P[2] = sqrt(r * r + a * a) * sin(theta);
+    double n = sqrt(f.P[1] * f.P[1] + f.P[2] * f.P[2]);
+    f.Z[0] = 0.0; f.Z[1] = f.P[1] / n; f.Z[2] = f.P[2] / n;
+    double d = f.Z[1];
+    double y0 = -d * f.Z[0], y1 = 1.0 - d * f.Z[1], y2 = -d * f.Z[2];
+    double yl = sqrt(y0 * y0 + y1 * y1 + y2 * y2);
+    f.Y[0] = y0 / yl; f.Y[1] = y1 / yl; f.Y[2] = y2 / yl;
+    f.X[0] = f.Y[1] * f.Z[2] - f.Y[2] * f.Z[1];
+    f.X[1] = f.Y[2] * f.Z[0] - f.Y[0] * f.Z[2];
+    f.X[2] = f.Y[0] * f.Z[1] - f.Y[1] * f.Z[0];
+}
+
+// Initial conditions of a Kerr ray from a camera frame and a camera-local unit
+// direction d (the metric's spin `a_eq` is the time-reversed one).
+template <class S>
+__device__ __forceinline__ void ic_from_frame(const Camframe& f, S a_eq, S L, const S d[3], St<S>& y, S& contraction, V4<S>& u_out) {
+    S qx = S(f.P[0]), qy = S(f.P[1]), qz = S(f.P[2]), qt = S(1) / (L / S(kC));
+    Sym4<S> g = metric_g<KERR, S>(qx, qy, qz, a_eq, S(1), false);
+    V4<S> uref = reference_observer<KERR, S>(qx, qy, qz, a_eq, S(1));
+    V4<S> X{S(1), S(0), S(0), S(0)}, Y{S(0), S(1), S(0), S(0)}, Z{S(0), S(0), S(1), S(0)};
+    V4<S> u = tetrad_apply(g, X, Y, Z, uref, V4<S>{S(0), S(0), S(0), sqrt_<false>(S(0) - S(-1))});
+    V4<S> CX{S(f.X[0]), S(f.X[1]), S(f.X[2]), S(0)}, CY{S(f.Y[0]), S(f.Y[1]), S(f.Y[2]), S(0)}, CZ{S(f.Z[0]), S(f.Z[1]), S(f.Z[2]), S(0)};
+    S lx = d[0] / L, ly = d[1] / L, lz = d[2] / L;
+    V4<S> pup = tetrad_apply(g, CX, CY, CZ, u, V4<S>{lx, ly, lz, sqrt_<false>(lx * lx + ly * ly + lz * lz)});
+    V4<S> pm = lower(g, pup);
+    y = St<S>{qx, qy, qz, qt, pm.x, pm.y, pm.z, pm.t};
+    contraction = dot4(u, pm);
+    u_out = u;
 }
 
 }  // namespace mgk

@@ -48,6 +48,8 @@ struct TraceParams {
     float disk_r_in, disk_r_out;
     float sphere_c[3], sphere_r;
     double zone_rho2;            // mixed precision: FP64 trials while x^2+y^2+z^2 < this
+    int per_ray;                 // spin in ray component 10; escape sphere about each ray's origin
+    float horizon_scalar;
 };
 
 struct ShadeParams {
@@ -60,7 +62,7 @@ struct ShadeParams {
     float sphere_T;
     uint32_t sphere_checker;
     float background_scale, background_T;
-    float star_T, cos_star_r;
+    float star_T, chord2_star_r;
     const float4* stars;
     const uint32_t* cell_start;
     const uint32_t* cell_items;

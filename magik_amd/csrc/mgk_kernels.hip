@@ -28,18 +28,22 @@ namespace mgk {
 // Small state helpers with an explicit, left-to-right evaluation order.
 // ---------------------------------------------------------------------------
 template <class S> __device__ __forceinline__ St<S> add(const St<S>& a, const St<S>& b) {
+#pragma clang fp contract(off)
     return {a.qx + b.qx, a.qy + b.qy, a.qz + b.qz, a.qt + b.qt, a.px + b.px, a.py + b.py, a.pz + b.pz, a.pt + b.pt};
 }
 template <class S> __device__ __forceinline__ St<S> sub(const St<S>& a, const St<S>& b) {
+#pragma clang fp contract(off)
     return {a.qx - b.qx, a.qy - b.qy, a.qz - b.qz, a.qt - b.qt, a.px - b.px, a.py - b.py, a.pz - b.pz, a.pt - b.pt};
 }
 template <class S> __device__ __forceinline__ St<S> mul(S s, const St<S>& a) {
+#pragma clang fp contract(off)
     return {s * a.qx, s * a.qy, s * a.qz, s * a.qt, s * a.px, s * a.py, s * a.pz, s * a.pt};
 }
 
 // RMS norm of relativity_utilities.cuh:15-29.
 template <class S>
 __device__ __forceinline__ S rms_norm(const St<S>& e, const St<S>& y0, const St<S>& y1, S at, S rt) {
+#pragma clang fp contract(off)
     auto c = [&](S ev, S a, S b) { return ev / (at + rt * fmax(fabs(a), fabs(b))); };
     S s0 = c(e.qt, y0.qt, y1.qt), s1 = c(e.qx, y0.qx, y1.qx), s2 = c(e.qy, y0.qy, y1.qy), s3 = c(e.qz, y0.qz, y1.qz);
     S s4 = c(e.pt, y0.pt, y1.pt), s5 = c(e.px, y0.px, y1.px), s6 = c(e.py, y0.py, y1.py), s7 = c(e.pz, y0.pz, y1.pz);
@@ -238,15 +242,55 @@ __device__ __forceinline__ void renormalise_null(St<double>& y, double a, double
     }
 }
 
+// Per-ray spacetime constants: the spin of the equations, the scaled horizon
+// radius, the FP64-zone bound and the centre of the escape sphere.  Uniform
+// over a launch for a single camera; per ray for batched cameras and for the
+// surrogate's training rays (spin stored as ray component 10).
+template <class S> struct Lane {
+    S a, hr;
+    int in_range;
+    double zone_rho2;
+    S cx, cy, cz;
+};
+
+// Outer (retrograde) photon-orbit radius 2 (1 + cos(2/3 acos|a|)) (Bardeen,
+// Press & Teukolsky 1972).
+__device__ __forceinline__ double photon_orbit_outer(double a) {
+    return 2.0 * (1.0 + cos((2.0 / 3.0) * acos(fmin(1.0, fabs(a)))));
+}
+
+template <int METRIC, class S> __device__ __forceinline__ void lane_constants(const TraceParams& P, S a, Lane<S>& ln) {
+    constexpr bool F32 = sizeof(S) == 4;
+    ln.a = a;
+    if (!P.per_ray) {
+        ln.hr = F32 ? S(P.horizon_r_f) : S(P.horizon_r_d);
+        ln.in_range = P.spin_in_range;
+        ln.zone_rho2 = P.zone_rho2;
+        return;
+    }
+    if constexpr (METRIC == KERR) {
+        ln.in_range = fabs(a) > S(1) ? 0 : 1;
+        ln.hr = (S(1) + sqrt_<false>(S(1) - a * a)) * S(P.horizon_scalar);
+        double ad = double(a);
+        double rp = 1.0 + sqrt(fmax(0.0, 1.0 - ad * ad));
+        double zr = fmax(1.5 * photon_orbit_outer(ad), rp + 0.1);
+        ln.zone_rho2 = zr * zr + ad * ad;
+    } else {
+        ln.in_range = 1;
+        ln.hr = F32 ? S(P.horizon_r_f) : S(P.horizon_r_d);
+        ln.zone_rho2 = P.zone_rho2;
+    }
+}
+
 // Chord tests of one completed step (kernel_intersect.cu:55-138 extended with
 // the disk and the emissive sphere), then the horizon (in_domain) and the
 // step limit (kernel_enforce_ray_limits.cu:48, n_step > limit).  Returns the
 // class and fills the hit record.
 template <int METRIC, class S>
 __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, uint32_t n_step, S contraction, const TraceParams& P,
-                                             mgk_hit_t& rec) {
-    constexpr bool F32 = sizeof(S) == 4;
-    const S a = METRIC == KERR ? S(P.spin) : S(0);
+                                             const Lane<S>& ln, mgk_hit_t& rec) {
+#pragma clang fp contract(off)
+    const S a = METRIC == KERR ? ln.a : S(0);
     S best = S(1e30f);
     uint32_t cls = MGK_HIT_NONE;
     S r_hit = S(0);
@@ -277,22 +321,21 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
     }
     {
         const S L = S(P.length_scale);
-        S ex = y1.qx * L - S(P.cam[0]), ey = y1.qy * L - S(P.cam[1]), ez = y1.qz * L - S(P.cam[2]);
+        S ex = y1.qx * L - ln.cx, ey = y1.qy * L - ln.cy, ez = y1.qz * L - ln.cz;
         S d2 = ex * ex + ey * ey + ez * ez;
-        S r2 = F32 ? S(P.cel_r2_f) : S(P.cel_r2_d);
+        S r2 = sizeof(S) == 4 ? S(P.cel_r2_f) : S(P.cel_r2_d);
         if (!(d2 < r2) && S(1) < best) { best = S(1); cls = MGK_HIT_ESCAPE; }
     }
     if (cls == MGK_HIT_NONE) {
         if constexpr (METRIC == KERR) {
-            if (P.spin_in_range) {
+            if (ln.in_range) {
                 S r = kerr_schild_r(y1.qx, y1.qy, y1.qz, a);
-                S hr = F32 ? S(P.horizon_r_f) : S(P.horizon_r_d);
-                if (!(hr < r)) cls = MGK_HIT_HORIZON;
+                if (!(ln.hr < r)) cls = MGK_HIT_HORIZON;
             }
         } else if constexpr (METRIC == SCHWARZSCHILD) {
             S r = sqrt_<false>(y1.qx * y1.qx + y1.qy * y1.qy + y1.qz * y1.qz);
             if (P.mass_sign < 0.0f) { if (r < S(1e-3f)) cls = MGK_HIT_HORIZON; }
-            else if (r <= (F32 ? S(P.horizon_r_f) : S(P.horizon_r_d))) cls = MGK_HIT_HORIZON;
+            else if (r <= ln.hr) cls = MGK_HIT_HORIZON;
         }
     }
     if (cls == MGK_HIT_NONE && n_step > P.step_limit) cls = MGK_HIT_STEP_LIMIT;
@@ -303,9 +346,18 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
         V4<S> ph{y0.px + best * (y1.px - y0.px), y0.py + best * (y1.py - y0.py), y0.pz + best * (y1.pz - y0.pz),
                  y0.pt + best * (y1.pt - y0.pt)};
         S hx = y0.qx + best * (y1.qx - y0.qx), hy = y0.qy + best * (y1.qy - y0.qy), hz = y0.qz + best * (y1.qz - y0.qz);
-        V4<S> u = cls == MGK_HIT_DISK ? kepler_velocity<METRIC, S>(hx, hz, r_hit, a)
-                                      : reference_observer<METRIC, S>(hx, hy, hz, a, S(P.mass_sign));
-        S g = contraction / dot4(u, ph);
+        S g;
+        if (cls == MGK_HIT_DISK) {
+            // u.p of the circular emitter from the conserved p_t and p_phi = z p_x - x p_z of the
+            // step's end point: exact along the geodesic, free of chord interpolation error
+            S ut, w;
+            kepler_ut_w<METRIC, S>(r_hit, a, ut, w);
+            S lphi = y1.qz * y1.px - y1.qx * y1.pz;
+            g = contraction / (ut * y1.pt + w * lphi);
+        } else {
+            V4<S> u = reference_observer<METRIC, S>(hx, hy, hz, a, S(P.mass_sign));
+            g = contraction / dot4(u, ph);
+        }
         if (isnan(g) || isinf(g)) g = S(1e-10f);
         rec.g = float(g);
         if (cls == MGK_HIT_DISK) {
@@ -345,6 +397,9 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
 
     St<S> y{};
     S h = S(0), contraction = S(0);
+    Lane<S> ln;
+    lane_constants<METRIC, S>(P, S(P.spin), ln);
+    ln.cx = S(P.cam[0]); ln.cy = S(P.cam[1]); ln.cz = S(P.cam[2]);
     bool active = false;
     uint64_t ray = 0;
     uint32_t n_step = 0, n_sub = 0;
@@ -383,6 +438,11 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
                     y.pt = load_c<S>(P.rays, P.stride, 7, ray);
                     h = load_c<S>(P.rays, P.stride, 8, ray);
                     contraction = load_c<S>(P.rays, P.stride, 9, ray);
+                    if (P.per_ray) {
+                        lane_constants<METRIC, S>(P, METRIC == KERR ? load_c<S>(P.rays, P.stride, 10, ray) : S(0), ln);
+                        const S L = S(P.length_scale);
+                        ln.cx = y.qx * L; ln.cy = y.qy * L; ln.cz = y.qz * L;   // escape sphere about the ray's origin
+                    }
                     active = true;
                     n_step = 0;
                     n_sub = 0;
@@ -406,12 +466,12 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
         } else if constexpr (PREC == MGK_PRECISION_MIXED) {
             double rms;
             double r2 = y.qx * y.qx + y.qy * y.qy + y.qz * y.qz;
-            bool near = r2 < P.zone_rho2;   // rho^2 = r^2 + a^2 sin^2(theta): zone_rho2 = zone_r^2 + a^2 covers r < zone_r
+            bool near = r2 < ln.zone_rho2;   // rho^2 = r^2 + a^2 sin^2(theta): zone_rho2 = zone_r^2 + a^2 covers r < zone_r
             if (near) {
-                trial_split<METRIC, double, false>(y, h, P.spin, P.mass_sign, P, y1, rms);
+                trial_split<METRIC, double, false>(y, h, float(ln.a), P.mass_sign, P, y1, rms);
                 ++fp64_acc;
             } else {
-                trial_split<METRIC, float, true>(y, h, P.spin, P.mass_sign, P, y1, rms);
+                trial_split<METRIC, float, true>(y, h, float(ln.a), P.mass_sign, P, y1, rms);
             }
             ++trials_acc;
             double rs = rms > 1e-30 ? rms : 1e-30;
@@ -424,7 +484,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
         } else {
             S hn;
             bool acc;
-            y1 = trial_reference<METRIC, S>(y, h, S(P.spin), S(P.mass_sign), S(P.abs_tol), S(P.rel_tol), S(P.h_max), hn, acc);
+            y1 = trial_reference<METRIC, S>(y, h, ln.a, S(P.mass_sign), S(P.abs_tol), S(P.rel_tol), S(P.h_max), hn, acc);
             ++trials_acc;
             if (PREC == MGK_PRECISION_FP64) ++fp64_acc;
             h = hn;
@@ -437,9 +497,9 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
         ++n_step;
         n_sub = 0;
         ++steps_acc;
-        if constexpr (PREC == MGK_PRECISION_MIXED) renormalise_null<METRIC>(y1, double(P.spin), double(P.mass_sign));
+        if constexpr (PREC == MGK_PRECISION_MIXED) renormalise_null<METRIC>(y1, double(ln.a), double(P.mass_sign));
         mgk_hit_t rec;
-        uint32_t cls = classify<METRIC, S>(y, y1, n_step, contraction, P, rec);
+        uint32_t cls = classify<METRIC, S>(y, y1, n_step, contraction, P, ln, rec);
         if (cls != MGK_HIT_NONE) {
             P.hits[ray] = rec;
             if (P.steps_out) P.steps_out[ray] = n_step;
@@ -536,7 +596,8 @@ __device__ __forceinline__ float star_lookup(const ShadeParams& P, float c, floa
     float acc = 0.0f;
     for (uint32_t k = P.cell_start[cell]; k < P.cell_start[cell + 1]; ++k) {
         float4 s = P.stars[P.cell_items[k]];
-        if (s.x * dx + s.y * c + s.z * dz >= P.cos_star_r) acc += s.w;
+        float ex = s.x - dx, ey = s.y - c, ez = s.z - dz;   // chord: |s - d|^2 <= (2 sin(rho/2))^2
+        if (ex * ex + ey * ey + ez * ez <= P.chord2_star_r) acc += s.w;
     }
     return acc;
 }

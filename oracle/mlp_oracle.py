@@ -1,0 +1,97 @@
+"""NumPy oracle of the surrogate MLP (TEST INFRASTRUCTURE ONLY).
+
+Mirrors the quantisation points of the HIP kernels in
+magik_amd/csrc/mgk_mlp.hip so that forward outputs and gradients can be
+compared at tight tolerance:
+
+    h0 = bf16(x)                                (features padded to 16)
+    z_l = h_{l-1} @ bf16(W_l) + b_l              (FP32 accumulation), l = 1..4
+    h_l = bf16(silu(z_l))
+    y   = h_4 @ bf16(W_5) + b_5                  (FP32)
+
+Backward (the kernels' order):
+    g_5 = bf16(dy);  dW_5 = h_4^T g_5;  db_5 = sum dy
+    dz_l = (g_{l+1} @ bf16(W_{l+1})^T) * silu'(z_l);  g_l = bf16(dz_l)
+    dW_l = h_{l-1}^T g_l;  db_l = sum dz_l
+
+Accumulation here is float64; the kernels accumulate in FP32, so agreement is
+to ~1e-5 relative, far inside the bf16 quantisation the two share.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+IN_PAD = 16
+HIDDEN = 256
+N_HIDDEN = 4
+OUT = 8
+OUT_PAD = 32
+
+
+def bf16(x: np.ndarray) -> np.ndarray:
+    """Round float32 to bfloat16 (nearest-even), returned as float32."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    rnd = ((u >> 16) & 1) + 0x7FFF
+    u = ((u + rnd) >> 16) << 16
+    out = u.astype(np.uint32).view(np.float32)
+    nan = np.isnan(a)
+    out = np.where(nan, a, out)
+    return out.reshape(np.shape(x))
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def layer_shapes():
+    dims = [IN_PAD] + [HIDDEN] * N_HIDDEN + [OUT_PAD]
+    return [(dims[i], dims[i + 1]) for i in range(len(dims) - 1)]
+
+
+def forward(W: list, b: list, x: np.ndarray, keep: bool = False):
+    """W[l]: (in, out) float32 master weights (padded shapes); x: (n, 16)."""
+    h = bf16(x.astype(np.float32))
+    hs, zs = [h], []
+    for l in range(len(W)):
+        z = h.astype(np.float64) @ bf16(W[l]).astype(np.float64) + b[l].astype(np.float64)
+        z = z.astype(np.float32)
+        if l < len(W) - 1:
+            zs.append(z)
+            h = bf16(silu(z.astype(np.float64)).astype(np.float32))
+            hs.append(h)
+        else:
+            y = z
+    if keep:
+        return y, hs, zs
+    return y
+
+
+def backward(W: list, hs: list, zs: list, dy: np.ndarray):
+    """Gradients (dW, db) for dL/dy = dy (n, 32)."""
+    n_l = len(W)
+    dW, db = [None] * n_l, [None] * n_l
+    g = bf16(dy.astype(np.float32))
+    dz = dy.astype(np.float64)
+    for l in range(n_l - 1, -1, -1):
+        dW[l] = (hs[l].astype(np.float64).T @ g.astype(np.float64)).astype(np.float32)
+        db[l] = dz.sum(axis=0).astype(np.float32)
+        if l > 0:
+            dh = g.astype(np.float64) @ bf16(W[l]).astype(np.float64).T
+            dz = (dh * dsilu(zs[l - 1].astype(np.float64))).astype(np.float32).astype(np.float64)
+            g = bf16(dz.astype(np.float32))
+    return dW, db
+
+
+def exact_forward(W, b, x):
+    """Float64 forward without any bf16 rounding (for finite-difference checks)."""
+    h = x.astype(np.float64)
+    for l in range(len(W)):
+        z = h @ W[l].astype(np.float64) + b[l].astype(np.float64)
+        h = silu(z) if l < len(W) - 1 else z
+    return h

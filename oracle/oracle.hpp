@@ -672,8 +672,16 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
             if (cls == HIT_DISK || cls == HIT_SPHERE) {
                 V4<R> qh = y0.q + best * (y1.q - y0.q);
                 V4<R> ph = y0.p + best * (y1.p - y0.p);
-                V4<R> u = cls == HIT_DISK ? kepler_velocity(qh, r_hit, p) : reference_observer(qh, p);
-                hit.g = redshift_factor(ic.contraction, u, ph);
+                if (cls == HIT_DISK) {
+                    // circular emitter: u.p = u^t p_t + u^phi p_phi with the conserved p_t and
+                    // p_phi = z p_x - x p_z of the step's end point
+                    V4<R> u = kepler_velocity(V4<R>{R(0), R(0), R(1), R(0)}, r_hit, p);   // (w, 0, 0, u^t)
+                    R lphi = y1.q.z * y1.p.x - y1.q.x * y1.p.z;
+                    R z = ic.contraction / (u.t * y1.p.t + u.x * lphi);
+                    hit.g = (std::isnan(z) || std::isinf(z)) ? R(1e-10f) : z;
+                } else {
+                    hit.g = redshift_factor(ic.contraction, reference_observer(qh, p), ph);
+                }
                 if (cls == HIT_DISK) {
                     hit.a0 = r_hit;
                     hit.a1 = std::atan2(qh.z, qh.x);
@@ -817,8 +825,11 @@ struct StarIndex {
         double acc = 0;
         auto lo = std::lower_bound(sorted.begin(), sorted.end(), float(y - rho - 1e-6),
                                    [](const Star& a, float v) { return a.dy < v; });
-        for (auto it = lo; it != sorted.end() && it->dy <= y + rho + 1e-6; ++it)
-            if (double(it->dx) * x + double(it->dy) * y + double(it->dz) * z >= double(cos_rho)) acc += it->brightness;
+        const double ch = 2.0 * std::sin(0.5 * double(rho)), ch2 = ch * ch;
+        for (auto it = lo; it != sorted.end() && it->dy <= y + rho + 1e-6; ++it) {
+            double ex = double(it->dx) - x, ey = double(it->dy) - y, ez = double(it->dz) - z;
+            if (ex * ex + ey * ey + ez * ez <= ch2) acc += it->brightness;
+        }
         return acc;
     }
 };
