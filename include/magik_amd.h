@@ -249,8 +249,8 @@ MGK_API e_magik_result_types mgk_set_scene(mgk_context_handle ctx, const mgk_sce
  * kernel_generate_initial_conditions.cu:54-74) for every pixel and the samples
  * [sample_begin, sample_begin + spp).  Ray i = s * W * H + py * W + px is
  * written as SoA: d_rays[c * ray_stride + i] for c = 0..9 =
- * x, y, z, t, p_x, p_y, p_z, p_t, h0, u.p.  Element type is double for
- * MGK_PRECISION_FP64 and float otherwise. */
+ * x, y, z, t, p_x, p_y, p_z, p_t, h0, u.p.  Element type is float for
+ * MGK_PRECISION_FP32_REFERENCE and double for MIXED and FP64. */
 MGK_API e_magik_result_types mgk_generate_rays(mgk_context_handle ctx, const mgk_integrator_t* integ, const mgk_camera_t* cam,
                                                uint64_t seed, uint32_t sample_begin, uint32_t spp, void* d_rays,
                                                uint64_t ray_stride, void* stream);
@@ -294,6 +294,94 @@ MGK_API e_magik_result_types mgk_color_weights(const mgk_spectral_t* spec, float
  * with the given sensor width.  mass_kg sets the length scale. */
 MGK_API e_magik_result_types mgk_camera_boyer_lindquist(float r_obs, float theta_deg, float spin, float mass_kg,
                                                         float fov_deg, uint32_t x_res, uint32_t y_res, mgk_camera_t* out);
+
+
+/* ---------------------------------------------------------------------------
+ * 3. Learned surrogate (no counterpart in the reference: the MLP of the
+ *    tracer's Kerr path).  5 features (spin a, observer r and theta in
+ *    Boyer-Lindquist, screen alpha, beta) -> 4 x 256 SiLU -> outputs
+ *    (horizon / disk logits, normalised log r of the disk hit, escape
+ *    direction cos(theta), sin(phi), cos(phi)); bf16 weights and activations,
+ *    FP32 accumulation on the matrix cores.
+ * ------------------------------------------------------------------------- */
+typedef struct mgk_surrogate* mgk_surrogate_handle;
+
+typedef struct mgk_surrogate_ranges_t {
+    float spin_min, spin_max;             /* physical spin a (before time reversal) */
+    float r_min, r_max;                   /* observer BL radius, units of M */
+    float theta_min_deg, theta_max_deg;   /* observer inclination */
+    float screen_max;                     /* |alpha|, |beta| <= screen_max (units of M) */
+    float disk_r_max;                     /* largest disk radius the r output must span */
+} mgk_surrogate_ranges_t;
+
+typedef struct mgk_train_config_t {
+    uint32_t batch;
+    uint32_t epochs;
+    float lr, lr_final;                   /* Adam, cosine decay */
+    float w_r, w_d;                       /* loss weights: disk radius, escape direction */
+} mgk_train_config_t;
+
+/* Fallback rules of the surrogate render path: a path is traced by RKF45
+ * instead of being answered by the network when any rule selects it. */
+typedef struct mgk_fallback_t {
+    float min_confidence;                 /* max class probability below this (0.9) */
+    float critical_margin;                /* within this screen distance (M) of the critical curve (0.05) */
+    float direction_margin;               /* escape: predicted direction within this angle (rad) of a star's edge */
+    float disk_edge_margin;               /* disk: predicted r within this fraction of r_in or r_out */
+    uint32_t judge_stride;                /* every judge_stride-th path is also traced by RKF45 (0 = none) */
+    uint32_t trace_disk;                  /* 1: every disk-class path is traced */
+} mgk_fallback_t;
+
+typedef struct mgk_surrogate_stats_t {
+    uint64_t n_paths;
+    uint64_t n_fallback;                  /* paths traced by RKF45 (judge paths included) */
+    uint64_t n_fallback_confidence;       /* ... selected by the first rule that fired */
+    uint64_t n_fallback_critical;
+    uint64_t n_fallback_disk;
+    uint64_t n_fallback_star;
+    uint64_t n_fallback_domain;           /* screen point outside the trained range */
+    uint64_t n_judge;                     /* judge paths the network answered */
+    uint64_t judge_class_agree;           /* ... whose class matched RKF45 */
+    uint64_t judge_n_disk, judge_n_escape;
+    double judge_mean_rel_r;              /* mean |r_s - r| / r over judge paths that are disk in both */
+    double judge_mean_rel_g;              /* mean |g_s - g| / g over the same */
+    double judge_mean_dir, judge_max_dir; /* escape direction error (rad) over judge paths that escape in both */
+    uint64_t n_trials;                    /* RKF45 trials of the fallback trace */
+    double ms_raygen, ms_mlp, ms_trace, ms_shade, ms_total;   /* device time of the stages */
+} mgk_surrogate_stats_t;
+
+MGK_API e_magik_result_types mgk_surrogate_create(int device, uint64_t seed, const mgk_surrogate_ranges_t* ranges,
+                                                  mgk_surrogate_handle* out);
+MGK_API e_magik_result_types mgk_surrogate_destroy(mgk_surrogate_handle s);
+/* Parameter counts of the padded layout (weights 16x256, 3 x 256x256, 256x32). */
+MGK_API e_magik_result_types mgk_surrogate_n_params(uint64_t* n_weights, uint64_t* n_biases);
+MGK_API e_magik_result_types mgk_surrogate_get_weights(mgk_surrogate_handle s, float* weights, float* biases);
+MGK_API e_magik_result_types mgk_surrogate_set_weights(mgk_surrogate_handle s, const float* weights, const float* biases);
+/* Inference of n feature rows (n x 16 normalised features) -> n x 8 outputs. */
+MGK_API e_magik_result_types mgk_surrogate_forward(mgk_surrogate_handle s, const float* d_x, uint64_t n, float* d_y, void* stream);
+/* Forward (training kernels) + backward from a given dL/dy (n x 32): the
+ * gradients land in host arrays (weights, biases layout); d_y (optional, n x 32)
+ * receives the forward outputs.  For gradient checks. */
+MGK_API e_magik_result_types mgk_surrogate_gradients(mgk_surrogate_handle s, const float* d_x, const float* d_dy, uint32_t n, float* d_y,
+                                                     float* weight_grad, float* bias_grad);
+/* One Adam step on a batch (targets n x 8); losses[4] = mean CE, disk, dir, accuracy. */
+MGK_API e_magik_result_types mgk_surrogate_train_step(mgk_surrogate_handle s, const float* d_x, const float* d_t, uint32_t n, float lr,
+                                                      float w_r, float w_d, float* losses);
+MGK_API e_magik_result_types mgk_surrogate_train(mgk_surrogate_handle s, const float* d_x, const float* d_t, uint64_t n,
+                                                 const mgk_train_config_t* cfg, float* epoch_losses);
+/* Training data: n seeded rays uniform over the ranges, traced by the FP64
+ * RKF45 path (integ must be Kerr / FP64); d_rays n x 11 doubles (SoA),
+ * features n x 16, targets n x 8, hits n. */
+MGK_API e_magik_result_types mgk_surrogate_generate_data(mgk_context_handle ctx, mgk_surrogate_handle s, const mgk_integrator_t* integ,
+                                                         const mgk_scene_t* scene, uint64_t seed, uint64_t n, double* d_rays,
+                                                         float* d_x, float* d_t, mgk_hit_t* d_hits, void* stream);
+/* Whole frame with the surrogate: MLP inference for every path, RKF45 (the
+ * integ settings, mixed precision) for the paths the fallback rules select
+ * and for the judge sample, then the same spectral shading. */
+MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, mgk_surrogate_handle s, const mgk_integrator_t* integ,
+                                                        const mgk_scene_t* scene, const mgk_spectral_t* spec, const mgk_camera_t* cam,
+                                                        uint64_t seed, uint32_t spp, const mgk_fallback_t* fb, float* d_xyz,
+                                                        mgk_surrogate_stats_t* stats, void* stream);
 
 #ifdef __cplusplus
 }

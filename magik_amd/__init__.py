@@ -228,3 +228,140 @@ def color_weights(spec):
     abi.check(abi.load().mgk_color_weights(C.byref(spec), W.ctypes.data_as(C.POINTER(C.c_float)),
                                            M.ctypes.data_as(C.POINTER(C.c_float))), "mgk_color_weights")
     return W, M.reshape(3, 3)
+
+
+# ---------------------------------------------------------------------------
+# Surrogate (learned path)
+# ---------------------------------------------------------------------------
+LAYER_SHAPES = [(16, 256), (256, 256), (256, 256), (256, 256), (256, 32)]
+
+
+def surrogate_ranges(spin=(0.0, 0.99), r=(20.0, 100.0), theta=(10.0, 89.0), screen=15.0, disk_r_max=30.0):
+    return abi.SurrogateRanges(spin_min=spin[0], spin_max=spin[1], r_min=r[0], r_max=r[1], theta_min_deg=theta[0],
+                               theta_max_deg=theta[1], screen_max=screen, disk_r_max=disk_r_max)
+
+
+class Surrogate:
+    """The bf16 MLP surrogate on one device."""
+
+    def __init__(self, device: int = 0, seed: int = 1, ranges=None):
+        self.lib = abi.load()
+        self.ranges = ranges or surrogate_ranges()
+        h = C.c_void_p()
+        abi.check(self.lib.mgk_surrogate_create(device, seed, C.byref(self.ranges), C.byref(h)), "mgk_surrogate_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            abi.check(self.lib.mgk_surrogate_destroy(self.h), "mgk_surrogate_destroy")
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_weights(self):
+        import numpy as np
+        nw, nb = C.c_uint64(), C.c_uint64()
+        self.lib.mgk_surrogate_n_params(C.byref(nw), C.byref(nb))
+        w = np.zeros(nw.value, dtype=np.float32)
+        b = np.zeros(nb.value, dtype=np.float32)
+        abi.check(self.lib.mgk_surrogate_get_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float)),
+                                                     b.ctypes.data_as(C.POINTER(C.c_float))), "get_weights")
+        return split_params(w, b)
+
+    def set_weights(self, W, b):
+        import numpy as np
+        w = np.concatenate([np.ascontiguousarray(x, dtype=np.float32).ravel() for x in W])
+        bb = np.concatenate([np.ascontiguousarray(x, dtype=np.float32).ravel() for x in b])
+        abi.check(self.lib.mgk_surrogate_set_weights(self.h, w.ctypes.data_as(C.POINTER(C.c_float)),
+                                                     bb.ctypes.data_as(C.POINTER(C.c_float))), "set_weights")
+
+    def forward(self, x):
+        import torch
+        x = x.contiguous()
+        y = torch.zeros((x.shape[0], 8), dtype=torch.float32, device=x.device)
+        abi.check(self.lib.mgk_surrogate_forward(self.h, C.c_void_p(x.data_ptr()), x.shape[0], C.c_void_p(y.data_ptr()),
+                                                 Context._stream()), "mgk_surrogate_forward")
+        return y
+
+    def gradients(self, x, dy):
+        import numpy as np
+        import torch
+        n = x.shape[0]
+        y = torch.zeros((n, 32), dtype=torch.float32, device=x.device)
+        nw, nb = C.c_uint64(), C.c_uint64()
+        self.lib.mgk_surrogate_n_params(C.byref(nw), C.byref(nb))
+        gw = np.zeros(nw.value, dtype=np.float32)
+        gb = np.zeros(nb.value, dtype=np.float32)
+        torch.cuda.synchronize()
+        abi.check(self.lib.mgk_surrogate_gradients(self.h, C.c_void_p(x.contiguous().data_ptr()),
+                                                   C.c_void_p(dy.contiguous().data_ptr()), n, C.c_void_p(y.data_ptr()),
+                                                   gw.ctypes.data_as(C.POINTER(C.c_float)),
+                                                   gb.ctypes.data_as(C.POINTER(C.c_float))), "mgk_surrogate_gradients")
+        return y, split_params(gw, gb)
+
+    def train_step(self, x, t, lr, w_r=10.0, w_d=1.0):
+        import numpy as np
+        import torch
+        torch.cuda.synchronize()
+        losses = np.zeros(4, dtype=np.float32)
+        abi.check(self.lib.mgk_surrogate_train_step(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()), x.shape[0],
+                                                    lr, w_r, w_d, losses.ctypes.data_as(C.POINTER(C.c_float))),
+                  "mgk_surrogate_train_step")
+        return losses
+
+    def train(self, x, t, batch=8192, epochs=4, lr=2e-3, lr_final=2e-5, w_r=10.0, w_d=1.0):
+        import numpy as np
+        import torch
+        torch.cuda.synchronize()
+        cfg = abi.TrainConfig(batch=batch, epochs=epochs, lr=lr, lr_final=lr_final, w_r=w_r, w_d=w_d)
+        out = np.zeros((epochs, 4), dtype=np.float32)
+        abi.check(self.lib.mgk_surrogate_train(self.h, C.c_void_p(x.data_ptr()), C.c_void_p(t.data_ptr()), x.shape[0],
+                                               C.byref(cfg), out.ctypes.data_as(C.POINTER(C.c_float))),
+                  "mgk_surrogate_train")
+        return out
+
+    def generate_data(self, ctx, integ, scene, seed: int, n: int):
+        import torch
+        rays = torch.zeros((11, n), dtype=torch.float64, device=f"cuda:{ctx.device}")
+        x = torch.zeros((n, 16), dtype=torch.float32, device=rays.device)
+        t = torch.zeros((n, 8), dtype=torch.float32, device=rays.device)
+        hits = torch.zeros((n, 4), dtype=torch.int32, device=rays.device)
+        abi.check(self.lib.mgk_surrogate_generate_data(ctx.h, self.h, C.byref(integ), C.byref(scene), seed, n,
+                                                       C.c_void_p(rays.data_ptr()), C.c_void_p(x.data_ptr()),
+                                                       C.c_void_p(t.data_ptr()), C.c_void_p(hits.data_ptr()),
+                                                       Context._stream()), "mgk_surrogate_generate_data")
+        return rays, x, t, hits
+
+
+def split_params(w, b):
+    W, B, ow, ob = [], [], 0, 0
+    for k, n in LAYER_SHAPES:
+        W.append(w[ow:ow + k * n].reshape(k, n))
+        B.append(b[ob:ob + n])
+        ow += k * n
+        ob += n
+    return W, B
+
+
+def fallback(min_confidence=0.9, critical_margin=0.05, direction_margin=2e-3, disk_edge_margin=0.02, judge_stride=1024,
+             trace_disk=False) -> abi.Fallback:
+    return abi.Fallback(min_confidence=min_confidence, critical_margin=critical_margin,
+                        direction_margin=direction_margin, disk_edge_margin=disk_edge_margin,
+                        judge_stride=judge_stride, trace_disk=int(trace_disk))
+
+
+def render_frame_surrogate(ctx: Context, sur: Surrogate, integ, sc, spec, cam, seed: int, spp: int, fb=None, out=None):
+    """Whole frame through the surrogate with RKF45 fallback; returns (xyz, stats dict)."""
+    import torch
+    if out is None:
+        out = torch.zeros((cam.y_resolution, cam.x_resolution, 3), dtype=torch.float32, device=f"cuda:{ctx.device}")
+    st = abi.SurrogateStats()
+    fb = fb or fallback()
+    abi.check(ctx.lib.mgk_render_frame_surrogate(ctx.h, sur.h, C.byref(integ), C.byref(sc), C.byref(spec), C.byref(cam),
+                                                 seed, spp, C.byref(fb), C.c_void_p(out.data_ptr()), C.byref(st),
+                                                 Context._stream()), "mgk_render_frame_surrogate")
+    return out, st.as_dict()

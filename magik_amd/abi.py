@@ -100,6 +100,33 @@ class TraceStats(C.Structure):  # mgk_trace_stats_t
                 ("n_trials_fp64", C.c_uint64), ("ms", C.c_double)]
 
 
+class SurrogateRanges(C.Structure):  # mgk_surrogate_ranges_t
+    _fields_ = [(n, C.c_float) for n in ("spin_min", "spin_max", "r_min", "r_max", "theta_min_deg", "theta_max_deg",
+                                         "screen_max", "disk_r_max")]
+
+
+class TrainConfig(C.Structure):  # mgk_train_config_t
+    _fields_ = [("batch", C.c_uint32), ("epochs", C.c_uint32), ("lr", C.c_float), ("lr_final", C.c_float),
+                ("w_r", C.c_float), ("w_d", C.c_float)]
+
+
+class Fallback(C.Structure):  # mgk_fallback_t
+    _fields_ = [("min_confidence", C.c_float), ("critical_margin", C.c_float), ("direction_margin", C.c_float),
+                ("disk_edge_margin", C.c_float), ("judge_stride", C.c_uint32), ("trace_disk", C.c_uint32)]
+
+
+class SurrogateStats(C.Structure):  # mgk_surrogate_stats_t
+    _fields_ = [(n, C.c_uint64) for n in ("n_paths", "n_fallback", "n_fallback_confidence", "n_fallback_critical",
+                                          "n_fallback_disk", "n_fallback_star", "n_fallback_domain", "n_judge",
+                                          "judge_class_agree", "judge_n_disk", "judge_n_escape")] + \
+               [(n, C.c_double) for n in ("judge_mean_rel_r", "judge_mean_rel_g", "judge_mean_dir", "judge_max_dir")] + \
+               [("n_trials", C.c_uint64)] + \
+               [(n, C.c_double) for n in ("ms_raygen", "ms_mlp", "ms_trace", "ms_shade", "ms_total")]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # Every symbol include/magik_amd.h declares, with its ctypes signature.
 _P = C.c_void_p
 _R = C.c_int32
@@ -122,6 +149,21 @@ SIGNATURES = {
     "mgk_color_weights": (_R, [C.POINTER(Spectral), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "mgk_camera_boyer_lindquist": (_R, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32,
                                         C.c_uint32, C.POINTER(Camera)]),
+    "mgk_surrogate_create": (_R, [C.c_int, C.c_uint64, C.POINTER(SurrogateRanges), C.POINTER(_P)]),
+    "mgk_surrogate_destroy": (_R, [_P]),
+    "mgk_surrogate_n_params": (_R, [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "mgk_surrogate_get_weights": (_R, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mgk_surrogate_set_weights": (_R, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mgk_surrogate_forward": (_R, [_P, _P, C.c_uint64, _P, _P]),
+    "mgk_surrogate_gradients": (_R, [_P, _P, _P, C.c_uint32, _P, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mgk_surrogate_train_step": (_R, [_P, _P, _P, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                      C.POINTER(C.c_float)]),
+    "mgk_surrogate_train": (_R, [_P, _P, _P, C.c_uint64, C.POINTER(TrainConfig), C.POINTER(C.c_float)]),
+    "mgk_surrogate_generate_data": (_R, [_P, _P, C.POINTER(Integrator), C.POINTER(Scene), C.c_uint64, C.c_uint64,
+                                         _P, _P, _P, _P, _P]),
+    "mgk_render_frame_surrogate": (_R, [_P, _P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(Spectral),
+                                        C.POINTER(Camera), C.c_uint64, C.c_uint32, C.POINTER(Fallback), _P,
+                                        C.POINTER(SurrogateStats), _P]),
 }
 
 _lib = None
@@ -135,6 +177,8 @@ def load():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
         lib = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
+            if not hasattr(lib, name):
+                raise RuntimeError(f"{LIB_PATH} does not export {name}: rebuild it")
             fn = getattr(lib, name)
             fn.restype = res
             fn.argtypes = args

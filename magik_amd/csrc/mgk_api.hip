@@ -18,6 +18,7 @@
 #include "magik_amd.h"
 #include "mgk_device.hpp"
 #include "mgk_internal.hpp"
+#include "mgk_surrogate.hpp"
 
 namespace {
 
@@ -73,8 +74,16 @@ struct mgk_context {
     uint64_t star_key = ~0ull;
     uint32_t star_n = 0;
     float star_rho = -1.0f, star_scale = 0.0f;
+    // star grid widened by a margin (surrogate star-edge test)
+    DeviceBuffer band_start, band_items;
+    uint64_t band_key = ~0ull;
+    uint32_t band_n = 0, band_nc = 0, band_np = 0;
+    float band_rho = -1.0f;
     // frame buffers
     DeviceBuffer rays, hits;
+    // zone-split integration: parked states and the two hand-off queues
+    DeviceBuffer zone_state, zone_steps, zone_q0, zone_q1;
+    int last_passes = 0;
 };
 
 namespace {
@@ -170,13 +179,64 @@ int trace_grid(mgk_context* c, const TraceParams& P) {
     return int(want);
 }
 
+// Mixed precision: alternate the far-zone (FP32) and near-zone (FP64) passes
+// until no ray is handed over.  Reads one queue count per pass (a 8-byte copy).
+e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t s) {
+    const uint64_t stride = P.stride;
+    HIP_TRY(c->zone_state.reserve(stride * 9 * sizeof(double)));
+    HIP_TRY(c->zone_steps.reserve(stride * sizeof(uint32_t)));
+    HIP_TRY(c->zone_q0.reserve(std::max<uint64_t>(P.n, 1) * sizeof(uint32_t)));
+    HIP_TRY(c->zone_q1.reserve(std::max<uint64_t>(P.n, 1) * sizeof(uint32_t)));
+    uint32_t* q[2] = {reinterpret_cast<uint32_t*>(c->zone_q0.ptr), reinterpret_cast<uint32_t*>(c->zone_q1.ptr)};
+    unsigned long long* cnt[2] = {c->d_counters + 4, c->d_counters + 5};
+    const uint32_t* caller_queue = P.queue;
+    uint64_t n_in = P.n;
+    bool near = false;
+    int pass = 0;
+    const int max_passes = 64;
+    for (;;) {
+        ZoneParams Z;
+        std::memset(&Z, 0, sizeof(Z));
+        Z.from_ic = pass == 0;
+        Z.no_handoff = pass + 1 >= max_passes;
+        Z.in_queue = pass == 0 ? caller_queue : q[(pass + 1) & 1];
+        Z.n_in = n_in;
+        Z.out_queue = q[pass & 1];
+        Z.out_count = cnt[pass & 1];
+        Z.state = reinterpret_cast<double*>(c->zone_state.ptr);
+        Z.state_steps = reinterpret_cast<uint32_t*>(c->zone_steps.ptr);
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c->d_counters, 1);   // head
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, cnt[pass & 1], 1);
+        HIP_TRY(hipGetLastError());
+        TraceParams Q = P;
+        Q.n = n_in;
+        HIP_TRY(launch_trace_zone(Q, Z, near || Z.no_handoff, trace_grid(c, Q), s));
+        ++pass;
+        unsigned long long next = 0;
+        HIP_TRY(hipMemcpyAsync(&next, cnt[(pass - 1) & 1], sizeof(next), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (next == 0 || Z.no_handoff) break;
+        n_in = next;
+        near = !near;
+    }
+    c->last_passes = pass;
+    return MAGIK_SUCCESS;
+}
+
 e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s) {
     P.head = c->d_counters;
     P.stats = c->d_counters + 1;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c->d_counters, 4);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev0, s));
-    if (P.n > 0) HIP_TRY(launch_trace(P, trace_grid(c, P), s));
+    if (P.n > 0) {
+        if (P.precision == MGK_PRECISION_MIXED && P.metric != MINKOWSKI) {
+            e_magik_result_types r = do_trace_zones(c, P, s);
+            if (r != MAGIK_SUCCESS) return r;
+        } else {
+            HIP_TRY(launch_trace(P, trace_grid(c, P), s));
+        }
+    }
     HIP_TRY(hipEventRecord(c->ev1, s));
     c->timed = true;
     c->last_n = P.n;
@@ -374,6 +434,27 @@ e_magik_result_types ensure_stars(mgk_context* c, const mgk_scene_t* sc) {
     return MAGIK_SUCCESS;
 }
 
+e_magik_result_types ensure_star_band(mgk_context* c, const mgk_scene_t* sc, float rho) {
+    e_magik_result_types r = ensure_stars(c, sc);
+    if (r != MAGIK_SUCCESS) return r;
+    if (c->band_key == sc->star_seed && c->band_n == sc->n_star && c->band_rho == rho) return MAGIK_SUCCESS;
+    std::vector<float4> st;
+    star_catalogue(sc->star_seed, sc->n_star, sc->star_scale, st);
+    std::vector<uint32_t> start, items;
+    const uint32_t nc = 256, np = 512;
+    star_grid(st, rho, nc, np, start, items);
+    HIP_TRY(c->band_start.reserve(start.size() * sizeof(uint32_t)));
+    HIP_TRY(c->band_items.reserve(std::max<size_t>(1, items.size()) * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(c->band_start.ptr, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (!items.empty()) HIP_TRY(hipMemcpy(c->band_items.ptr, items.data(), items.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    c->band_nc = nc;
+    c->band_np = np;
+    c->band_key = sc->star_seed;
+    c->band_n = sc->n_star;
+    c->band_rho = rho;
+    return MAGIK_SUCCESS;
+}
+
 e_magik_result_types fill_shade_params(mgk_context* c, const mgk_scene_t* sc, const mgk_spectral_t* sp, const mgk_camera_t* cam,
                                        uint64_t seed, uint32_t sample_begin, uint32_t spp, const mgk_hit_t* hits, float* xyz,
                                        ShadeParams& P) {
@@ -486,7 +567,7 @@ MGK_API e_magik_result_types mgk_context_create(int device, mgk_context_handle* 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MGK_ERROR_HIP);
     c->n_cu = prop.multiProcessorCount;
-    if (hipMalloc(&c->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess) return fail(MAGIK_ERROR_DEVICE_ALLOCATION_FAILED);
+    if (hipMalloc(&c->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess) return fail(MAGIK_ERROR_DEVICE_ALLOCATION_FAILED);
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(MGK_ERROR_HIP);
     *out = c;
     return MAGIK_SUCCESS;
@@ -504,6 +585,10 @@ MGK_API e_magik_result_types mgk_context_destroy(mgk_context_handle c) {
     c->cell_items.release();
     c->rays.release();
     c->hits.release();
+    c->zone_state.release();
+    c->zone_steps.release();
+    c->zone_q0.release();
+    c->zone_q1.release();
     delete c;
     return MAGIK_SUCCESS;
 }
@@ -545,6 +630,28 @@ MGK_API e_magik_result_types mgk_trace_rays(mgk_context_handle c, const mgk_inte
     P.steps_out = d_steps;
     return do_trace(c, P, hipStream_t(stream));
 }
+
+}  // extern "C"
+
+e_magik_result_types mgk_trace_rays_per_ray(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc, const float origin[3],
+                                            const double* d_rays, uint64_t stride, uint64_t n, mgk_hit_t* d_hits, hipStream_t stream) {
+    if (!c || !sc || !d_hits || (n > 0 && !d_rays)) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = check_integrator(in);
+    if (r != MAGIK_SUCCESS) return r;
+    if (in->precision == MGK_PRECISION_FP32_REFERENCE || n > stride) return MGK_ERROR_INVALID_ARGUMENT;
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    TraceParams P;
+    fill_trace_params(in, sc, origin, P);
+    P.rays = d_rays;
+    P.stride = stride;
+    P.n = n;
+    P.hits = d_hits;
+    P.per_ray = 1;
+    P.disk_isco = 1;
+    return do_trace(c, P, stream);
+}
+
+extern "C" {
 
 MGK_API e_magik_result_types mgk_last_trace_stats(mgk_context_handle c, mgk_trace_stats_t* out) {
     if (!c || !out) return MAGIK_ERROR_INVALID_POINTER;
@@ -606,6 +713,28 @@ MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle c, const mgk_in
     HIP_TRY(launch_shade(S, s));
     return MAGIK_SUCCESS;
 }
+
+}  // extern "C"
+
+e_magik_result_types mgk_star_band_view(mgk_context_handle c, const mgk_scene_t* sc, float margin, StarView* out) {
+    if (!c || !sc || !out) return MAGIK_ERROR_INVALID_POINTER;
+    std::memset(out, 0, sizeof(*out));
+    if (!(sc->flags & MGK_SCENE_STARS) || sc->n_star == 0) return MAGIK_SUCCESS;
+    if (!(margin >= 0.0f) || !(sc->star_radius_rad > 0.0f)) return MGK_ERROR_INVALID_ARGUMENT;
+    e_magik_result_types r = set_device(c);
+    if (r != MAGIK_SUCCESS) return r;
+    const float rho = sc->star_radius_rad + margin;
+    if ((r = ensure_star_band(c, sc, rho)) != MAGIK_SUCCESS) return r;
+    out->stars = reinterpret_cast<const float4*>(c->stars.ptr);
+    out->cell_start = reinterpret_cast<const uint32_t*>(c->band_start.ptr);
+    out->cell_items = reinterpret_cast<const uint32_t*>(c->band_items.ptr);
+    out->nc = c->band_nc;
+    out->np = c->band_np;
+    out->rho = rho;
+    return MAGIK_SUCCESS;
+}
+
+extern "C" {
 
 MGK_API e_magik_result_types mgk_color_weights(const mgk_spectral_t* sp, float* weights_xyz, float* xyz_to_scene_out) {
     if (!sp) return MAGIK_ERROR_INVALID_POINTER;

@@ -28,12 +28,38 @@ template <bool FAST> __device__ __forceinline__ float rcp(float x) {
     if constexpr (FAST) return __builtin_amdgcn_rcpf(x);
     else return 1.0f / x;
 }
-template <bool FAST> __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+// FP64 FAST: v_rcp_f64 / v_rsq_f64 seeds refined by Newton steps to full
+// double precision (a few ulp, not IEEE-rounded), far cheaper than the IEEE
+// division / square-root sequences.
+template <bool FAST> __device__ __forceinline__ double rcp(double x) {
+    if constexpr (FAST) {
+        double y = __builtin_amdgcn_rcp(x);
+        double e = fma(-x, y, 1.0);
+        y = fma(y, e, y);
+        e = fma(-x, y, 1.0);
+        return fma(y, e, y);
+    } else {
+        return 1.0 / x;
+    }
+}
 template <bool FAST> __device__ __forceinline__ float sqrt_(float x) {
     if constexpr (FAST) return __builtin_amdgcn_sqrtf(x);
     else return __fsqrt_rn(x);
 }
-template <bool FAST> __device__ __forceinline__ double sqrt_(double x) { return __dsqrt_rn(x); }
+template <bool FAST> __device__ __forceinline__ double sqrt_(double x) {
+    if constexpr (FAST) {
+        // y ~ 1/sqrt(x): two Newton steps, then s = x y with one correction
+        double y = __builtin_amdgcn_rsq(x);
+        double h = 0.5 * x;
+        y = y * fma(-h * y, y, 1.5);
+        y = y * fma(-h * y, y, 1.5);
+        double s = x * y;
+        double r = fma(-s, s, x);
+        return x > 0.0 ? fma(r, 0.5 * y, s) : (x == 0.0 ? 0.0 : __dsqrt_rn(x));
+    } else {
+        return __dsqrt_rn(x);
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al., SC'11).  Counter (pixel, sample, block, 0),
@@ -385,6 +411,29 @@ __device__ __forceinline__ float planck(float lambda_nm, float T) {
 }  // namespace mgk
 
 namespace mgk {
+
+// Jittered sensor point of pixel (px, py) in metres on the z = 0 sensor plane
+// (device_sensor_utilities.cuh:35-80: u = -(x/W - 0.5), v = -(y/H - 0.5)
+// scaled by the aspect ratio, the sensor width as the full width, jitter of
+// one pixel side from the diagonal geometry).  Shared by the ray generator
+// and the surrogate's screen coordinates.
+template <class S>
+__device__ __forceinline__ void sensor_point(uint32_t W, uint32_t H, float sensor_width_mm, uint32_t px, uint32_t py, float u0,
+                                             float u1, S& sx, S& sy) {
+#pragma clang fp contract(off)
+    const S sw = S(sensor_width_mm);
+    S inv_s = sw / sqrt_<false>(S(float(W * W + H * H)));
+    S width_mm = S(float(W)) * inv_s, height_mm = S(float(H)) * inv_s;
+    S area = (width_mm * height_mm) / S(1E6f);
+    S side = sqrt_<false>(area / S(float(W * H)));
+    S ju = (S(u0) - S(0.5)) * side, jv = (S(u1) - S(0.5)) * side;
+    S uu = -((S(float(px)) / S(float(W))) - S(0.5));
+    S vv = -(S(float(py)) / S(float(H)) - S(0.5));
+    S ratio = S(float(H)) / S(float(W));
+    if (ratio < S(1)) vv = vv * ratio; else uu = uu / ratio;
+    sx = ((-uu * sw) / S(1000)) + ju;
+    sy = ((-vv * sw) / S(1000)) + jv;
+}
 
 // ---------------------------------------------------------------------------
 // Surrogate feature space.  A camera at Boyer-Lindquist (r, theta) of a Kerr

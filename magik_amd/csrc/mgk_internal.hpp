@@ -50,6 +50,19 @@ struct TraceParams {
     double zone_rho2;            // mixed precision: FP64 trials while x^2+y^2+z^2 < this
     int per_ray;                 // spin in ray component 10; escape sphere about each ray's origin
     float horizon_scalar;
+    int disk_isco;               // per-ray rays: the disk starts at the ISCO of the ray's spin
+};
+
+// One pass of the zone-split mixed-precision integrator.
+struct ZoneParams {
+    int from_ic;                     // load rays from the IC buffer (first pass) instead of the parked state
+    int no_handoff;                  // integrate to termination regardless of the zone
+    const uint32_t* in_queue;        // ray ids of this pass (NULL: 0 .. n_in-1)
+    uint64_t n_in;
+    uint32_t* out_queue;             // rays handed to the other zone
+    unsigned long long* out_count;
+    double* state;                   // parked states, 9 x stride doubles (q, p, h)
+    uint32_t* state_steps;           // parked step counts
 };
 
 struct ShadeParams {
@@ -90,9 +103,25 @@ struct GeodesicStepParams {
     uint32_t* n_written;
 };
 
+// Device view of a star catalogue's sky grid (256 bins in cos(theta) x 512 in
+// phi, CSR): every star is listed in each cell its disc of radius `rho` meets.
+struct StarView {
+    const float4* stars;          // unit direction (x, y = cos(theta), z) and brightness
+    const uint32_t* cell_start;
+    const uint32_t* cell_items;
+    uint32_t nc, np;
+    float rho;
+};
+
 hipError_t launch_generate_rays(const RayGenParams& p, hipStream_t s);
 hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s);
+hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s);
 hipError_t launch_geodesic_steps(const GeodesicStepParams& p, hipStream_t s);
 
 }  // namespace mgk
+
+// Internal (not exported): a sky grid of the scene's stars whose discs are
+// widened to star_radius_rad + margin (the surrogate's star-edge test); the
+// grid stays valid until the next call on the context.
+e_magik_result_types mgk_star_band_view(mgk_context_handle ctx, const mgk_scene_t* scene, float margin, mgk::StarView* out);

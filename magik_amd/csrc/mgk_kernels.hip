@@ -117,90 +117,69 @@ constexpr double ET5 = E6 * A65;
 //   err_q =                 h^2 sum eps_j  P(dp_j) - h sum e_i c_i
 // so the large, cancelling P(p) terms never pass through FP32 and the error
 // estimate carries no FP32 rounding noise of |p|.
+constexpr double kA[6][5] = {{0, 0, 0, 0, 0},
+                              {A21, 0, 0, 0, 0},
+                              {A31, A32, 0, 0, 0},
+                              {A41, A42, A43, 0, 0},
+                              {A51, A52, A53, A54, 0},
+                              {A61, A62, A63, A64, A65}};
+constexpr double kB[6] = {B1, 0, B3, B4, B5, B6};
+constexpr double kE[6] = {E1, 0, E3, E4, E5, E6};
+constexpr double kBT[6] = {BT1, BT2, BT3, BT4, BT5, 0};
+constexpr double kET[6] = {ET1, ET2, ET3, ET4, ET5, 0};
+
 template <int METRIC, class SS, bool FAST>
 __device__ __forceinline__ void trial_split(const St<double>& y0, double h, float a, float ms, const TraceParams& P, St<double>& y1,
                                             double& rms_out) {
     const SS qx = SS(y0.qx), qy = SS(y0.qy), qz = SS(y0.qz);
     const SS px = SS(y0.px), py = SS(y0.py), pz = SS(y0.pz), pt = SS(y0.pt);
     const SS hs = SS(h), as = SS(a), mss = SS(ms);
-    // stage 1
-    Rhs<SS> r1 = rhs<METRIC, SS, FAST>(qx, qy, qz, px, py, pz, pt, as, mss);
-    SS q1x = px - r1.cx, q1y = py - r1.cy, q1z = pz - r1.cz;
-    // stage 2
-    SS c21 = SS(A21) * hs;
-    SS p2x = px + c21 * r1.dpx, p2y = py + c21 * r1.dpy, p2z = pz + c21 * r1.dpz;
-    Rhs<SS> r2 = rhs<METRIC, SS, FAST>(qx + c21 * q1x, qy + c21 * q1y, qz + c21 * q1z, p2x, p2y, p2z, pt, as, mss);
-    SS q2x = p2x - r2.cx, q2y = p2y - r2.cy, q2z = p2z - r2.cz;
-    // stage 3
-    SS c31 = SS(A31) * hs, c32 = SS(A32) * hs;
-    SS p3x = px + c31 * r1.dpx + c32 * r2.dpx, p3y = py + c31 * r1.dpy + c32 * r2.dpy, p3z = pz + c31 * r1.dpz + c32 * r2.dpz;
-    Rhs<SS> r3 = rhs<METRIC, SS, FAST>(qx + c31 * q1x + c32 * q2x, qy + c31 * q1y + c32 * q2y, qz + c31 * q1z + c32 * q2z, p3x, p3y,
-                                       p3z, pt, as, mss);
-    SS q3x = p3x - r3.cx, q3y = p3y - r3.cy, q3z = p3z - r3.cz;
-    // stage 4
-    SS c41 = SS(A41) * hs, c42 = SS(A42) * hs, c43 = SS(A43) * hs;
-    SS p4x = px + c41 * r1.dpx + c42 * r2.dpx + c43 * r3.dpx, p4y = py + c41 * r1.dpy + c42 * r2.dpy + c43 * r3.dpy,
-       p4z = pz + c41 * r1.dpz + c42 * r2.dpz + c43 * r3.dpz;
-    Rhs<SS> r4 = rhs<METRIC, SS, FAST>(qx + c41 * q1x + c42 * q2x + c43 * q3x, qy + c41 * q1y + c42 * q2y + c43 * q3y,
-                                       qz + c41 * q1z + c42 * q2z + c43 * q3z, p4x, p4y, p4z, pt, as, mss);
-    SS q4x = p4x - r4.cx, q4y = p4y - r4.cy, q4z = p4z - r4.cz;
-    // stage 5
-    SS c51 = SS(A51) * hs, c52 = SS(A52) * hs, c53 = SS(A53) * hs, c54 = SS(A54) * hs;
-    SS p5x = px + c51 * r1.dpx + c52 * r2.dpx + c53 * r3.dpx + c54 * r4.dpx,
-       p5y = py + c51 * r1.dpy + c52 * r2.dpy + c53 * r3.dpy + c54 * r4.dpy,
-       p5z = pz + c51 * r1.dpz + c52 * r2.dpz + c53 * r3.dpz + c54 * r4.dpz;
-    Rhs<SS> r5 = rhs<METRIC, SS, FAST>(qx + c51 * q1x + c52 * q2x + c53 * q3x + c54 * q4x,
-                                       qy + c51 * q1y + c52 * q2y + c53 * q3y + c54 * q4y,
-                                       qz + c51 * q1z + c52 * q2z + c53 * q3z + c54 * q4z, p5x, p5y, p5z, pt, as, mss);
-    SS q5x = p5x - r5.cx, q5y = p5y - r5.cy, q5z = p5z - r5.cz;
-    // stage 6
-    SS c61 = SS(A61) * hs, c62 = SS(A62) * hs, c63 = SS(A63) * hs, c64 = SS(A64) * hs, c65 = SS(A65) * hs;
-    SS p6x = px + c61 * r1.dpx + c62 * r2.dpx + c63 * r3.dpx + c64 * r4.dpx + c65 * r5.dpx,
-       p6y = py + c61 * r1.dpy + c62 * r2.dpy + c63 * r3.dpy + c64 * r4.dpy + c65 * r5.dpy,
-       p6z = pz + c61 * r1.dpz + c62 * r2.dpz + c63 * r3.dpz + c64 * r4.dpz + c65 * r5.dpz;
-    Rhs<SS> r6 = rhs<METRIC, SS, FAST>(qx + c61 * q1x + c62 * q2x + c63 * q3x + c64 * q4x + c65 * q5x,
-                                       qy + c61 * q1y + c62 * q2y + c63 * q3y + c64 * q4y + c65 * q5y,
-                                       qz + c61 * q1z + c62 * q2z + c63 * q3z + c64 * q4z + c65 * q5z, p6x, p6y, p6z, pt, as, mss);
-
-    // FP64 assembly.
+    // Stage sums are folded forward as soon as a stage is evaluated, so only the
+    // pending partial states of the later stages stay live (not all six k's).
+    SS Sq[6][3], Sp[6][3];
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Sq[j][c] = Sp[j][c] = SS(0);
+    double bp[3] = {0, 0, 0}, ep[3] = {0, 0, 0}, bt[3] = {0, 0, 0}, et[3] = {0, 0, 0};
+    double cb[4] = {0, 0, 0, 0}, ce[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const SS sqx = qx + Sq[i][0], sqy = qy + Sq[i][1], sqz = qz + Sq[i][2];
+        const SS spx = px + Sp[i][0], spy = py + Sp[i][1], spz = pz + Sp[i][2];
+        const Rhs<SS> r = rhs<METRIC, SS, FAST>(sqx, sqy, sqz, spx, spy, spz, pt, as, mss);
+        const SS kx = spx - r.cx, ky = spy - r.cy, kz = spz - r.cz;
+#pragma unroll
+        for (int j = i + 1; j < 6; ++j) {
+            const SS c = SS(kA[j][i]) * hs;
+            Sq[j][0] += c * kx; Sq[j][1] += c * ky; Sq[j][2] += c * kz;
+            Sp[j][0] += c * r.dpx; Sp[j][1] += c * r.dpy; Sp[j][2] += c * r.dpz;
+        }
+        const double dpx = double(r.dpx), dpy = double(r.dpy), dpz = double(r.dpz);
+        if (kB[i] != 0.0) {
+            bp[0] += kB[i] * dpx; bp[1] += kB[i] * dpy; bp[2] += kB[i] * dpz;
+            ep[0] += kE[i] * dpx; ep[1] += kE[i] * dpy; ep[2] += kE[i] * dpz;
+            cb[0] += kB[i] * double(r.cx); cb[1] += kB[i] * double(r.cy); cb[2] += kB[i] * double(r.cz); cb[3] += kB[i] * double(r.ct);
+            ce[0] += kE[i] * double(r.cx); ce[1] += kE[i] * double(r.cy); ce[2] += kE[i] * double(r.cz); ce[3] += kE[i] * double(r.ct);
+        }
+        if (kBT[i] != 0.0) {
+            bt[0] += kBT[i] * dpx; bt[1] += kBT[i] * dpy; bt[2] += kBT[i] * dpz;
+            et[0] += kET[i] * dpx; et[1] += kET[i] * dpy; et[2] += kET[i] * dpz;
+        }
+    }
+    // FP64 assembly (see above).
     const double hh = h * h;
-    auto dp5 = [&](double w1, double w2, double w3, double w4, double w5, SS d1, SS d2, SS d3, SS d4, SS d5) {
-        return w1 * double(d1) + w2 * double(d2) + w3 * double(d3) + w4 * double(d4) + w5 * double(d5);
-    };
-    auto cb = [&](SS c1, SS c3, SS c4, SS c5, SS c6) {
-        return B1 * double(c1) + B3 * double(c3) + B4 * double(c4) + B5 * double(c5) + B6 * double(c6);
-    };
-    auto ce = [&](SS c1, SS c3, SS c4, SS c5, SS c6) {
-        return E1 * double(c1) + E3 * double(c3) + E4 * double(c4) + E5 * double(c5) + E6 * double(c6);
-    };
-    double bpx = B1 * double(r1.dpx) + B3 * double(r3.dpx) + B4 * double(r4.dpx) + B5 * double(r5.dpx) + B6 * double(r6.dpx);
-    double bpy = B1 * double(r1.dpy) + B3 * double(r3.dpy) + B4 * double(r4.dpy) + B5 * double(r5.dpy) + B6 * double(r6.dpy);
-    double bpz = B1 * double(r1.dpz) + B3 * double(r3.dpz) + B4 * double(r4.dpz) + B5 * double(r5.dpz) + B6 * double(r6.dpz);
-    double epx = E1 * double(r1.dpx) + E3 * double(r3.dpx) + E4 * double(r4.dpx) + E5 * double(r5.dpx) + E6 * double(r6.dpx);
-    double epy = E1 * double(r1.dpy) + E3 * double(r3.dpy) + E4 * double(r4.dpy) + E5 * double(r5.dpy) + E6 * double(r6.dpy);
-    double epz = E1 * double(r1.dpz) + E3 * double(r3.dpz) + E4 * double(r4.dpz) + E5 * double(r5.dpz) + E6 * double(r6.dpz);
-    double btx = dp5(BT1, BT2, BT3, BT4, BT5, r1.dpx, r2.dpx, r3.dpx, r4.dpx, r5.dpx);
-    double bty = dp5(BT1, BT2, BT3, BT4, BT5, r1.dpy, r2.dpy, r3.dpy, r4.dpy, r5.dpy);
-    double btz = dp5(BT1, BT2, BT3, BT4, BT5, r1.dpz, r2.dpz, r3.dpz, r4.dpz, r5.dpz);
-    double etx = dp5(ET1, ET2, ET3, ET4, ET5, r1.dpx, r2.dpx, r3.dpx, r4.dpx, r5.dpx);
-    double ety = dp5(ET1, ET2, ET3, ET4, ET5, r1.dpy, r2.dpy, r3.dpy, r4.dpy, r5.dpy);
-    double etz = dp5(ET1, ET2, ET3, ET4, ET5, r1.dpz, r2.dpz, r3.dpz, r4.dpz, r5.dpz);
-    double cbx = cb(r1.cx, r3.cx, r4.cx, r5.cx, r6.cx), cby = cb(r1.cy, r3.cy, r4.cy, r5.cy, r6.cy);
-    double cbz = cb(r1.cz, r3.cz, r4.cz, r5.cz, r6.cz), cbt = cb(r1.ct, r3.ct, r4.ct, r5.ct, r6.ct);
-    double cex = ce(r1.cx, r3.cx, r4.cx, r5.cx, r6.cx), cey = ce(r1.cy, r3.cy, r4.cy, r5.cy, r6.cy);
-    double cez = ce(r1.cz, r3.cz, r4.cz, r5.cz, r6.cz), cet = ce(r1.ct, r3.ct, r4.ct, r5.ct, r6.ct);
-
-    y1.px = y0.px + h * bpx;
-    y1.py = y0.py + h * bpy;
-    y1.pz = y0.pz + h * bpz;
+    y1.px = y0.px + h * bp[0];
+    y1.py = y0.py + h * bp[1];
+    y1.pz = y0.pz + h * bp[2];
     y1.pt = y0.pt;
-    y1.qx = y0.qx + h * y0.px + hh * btx - h * cbx;
-    y1.qy = y0.qy + h * y0.py + hh * bty - h * cby;
-    y1.qz = y0.qz + h * y0.pz + hh * btz - h * cbz;
-    y1.qt = y0.qt - h * y0.pt - h * cbt;
+    y1.qx = y0.qx + h * y0.px + hh * bt[0] - h * cb[0];
+    y1.qy = y0.qy + h * y0.py + hh * bt[1] - h * cb[1];
+    y1.qz = y0.qz + h * y0.pz + hh * bt[2] - h * cb[2];
+    y1.qt = y0.qt - h * y0.pt - h * cb[3];
     St<double> e;
-    e.px = h * epx; e.py = h * epy; e.pz = h * epz; e.pt = 0.0;
-    e.qx = hh * etx - h * cex; e.qy = hh * ety - h * cey; e.qz = hh * etz - h * cez; e.qt = -h * cet;
+    e.px = h * ep[0]; e.py = h * ep[1]; e.pz = h * ep[2]; e.pt = 0.0;
+    e.qx = hh * et[0] - h * ce[0]; e.qy = hh * et[1] - h * ce[1]; e.qz = hh * et[2] - h * ce[2]; e.qt = -h * ce[3];
     rms_out = rms_norm(e, y0, y1, double(P.abs_tol), double(P.rel_tol));
 }
 
@@ -251,7 +230,22 @@ template <class S> struct Lane {
     int in_range;
     double zone_rho2;
     S cx, cy, cz;
+    S disk_r_in;
 };
+
+// Prograde ISCO radius (Bardeen, Press & Teukolsky 1972), units of M.
+__device__ inline float isco_radius_f(float a) {
+    a = fabsf(a);
+    float z1 = 1.0f + cbrtf(1.0f - a * a) * (cbrtf(1.0f + a) + cbrtf(1.0f - a));
+    float z2 = sqrtf(3.0f * a * a + z1 * z1);
+    return 3.0f + z2 - sqrtf((3.0f - z1) * (3.0f + z1 + 2.0f * z2));
+}
+__host__ __device__ inline double isco_radius(double a) {
+    a = fabs(a);
+    double z1 = 1.0 + cbrt(1.0 - a * a) * (cbrt(1.0 + a) + cbrt(1.0 - a));
+    double z2 = sqrt(3.0 * a * a + z1 * z1);
+    return 3.0 + z2 - sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
+}
 
 // Outer (retrograde) photon-orbit radius 2 (1 + cos(2/3 acos|a|)) (Bardeen,
 // Press & Teukolsky 1972).
@@ -262,6 +256,7 @@ __device__ __forceinline__ double photon_orbit_outer(double a) {
 template <int METRIC, class S> __device__ __forceinline__ void lane_constants(const TraceParams& P, S a, Lane<S>& ln) {
     constexpr bool F32 = sizeof(S) == 4;
     ln.a = a;
+    ln.disk_r_in = (P.per_ray && P.disk_isco) ? S(isco_radius_f(float(a))) : S(P.disk_r_in);
     if (!P.per_ray) {
         ln.hr = F32 ? S(P.horizon_r_f) : S(P.horizon_r_d);
         ln.in_range = P.spin_in_range;
@@ -271,14 +266,62 @@ template <int METRIC, class S> __device__ __forceinline__ void lane_constants(co
     if constexpr (METRIC == KERR) {
         ln.in_range = fabs(a) > S(1) ? 0 : 1;
         ln.hr = (S(1) + sqrt_<false>(S(1) - a * a)) * S(P.horizon_scalar);
-        double ad = double(a);
-        double rp = 1.0 + sqrt(fmax(0.0, 1.0 - ad * ad));
-        double zr = fmax(1.5 * photon_orbit_outer(ad), rp + 0.1);
-        ln.zone_rho2 = zr * zr + ad * ad;
+        const float af = float(a);
+        const float rp = 1.0f + sqrtf(fmaxf(0.0f, 1.0f - af * af));
+        const float rph = 2.0f * (1.0f + cosf((2.0f / 3.0f) * acosf(fminf(1.0f, fabsf(af)))));
+        const double zr = double(fmaxf(1.5f * rph, rp + 0.1f)) * 1.0001;   // a hair wider than the host's FP64 bound
+        ln.zone_rho2 = zr * zr + double(af) * double(af);
     } else {
         ln.in_range = 1;
         ln.hr = F32 ? S(P.horizon_r_f) : S(P.horizon_r_d);
         ln.zone_rho2 = P.zone_rho2;
+    }
+}
+
+// Hit record of a terminated path (g, a0, a1).  The azimuths are taken with
+// FP32 atan2 (they are stored in FP32): an FP64 atan2 here alone would cost the
+// integration loop ~45 VGPRs.
+template <int METRIC, class S>
+__device__ __forceinline__ void hit_record(const St<S>& y0, const St<S>& y1, uint32_t cls, S best, S r_hit, S contraction,
+                                        const TraceParams& P, S a, mgk_hit_t& rec) {
+#pragma clang fp contract(off)
+    rec.cls = cls;
+    rec.g = 0.0f; rec.a0 = 0.0f; rec.a1 = 0.0f;
+    if (cls == MGK_HIT_DISK || cls == MGK_HIT_SPHERE) {
+        V4<S> ph{y0.px + best * (y1.px - y0.px), y0.py + best * (y1.py - y0.py), y0.pz + best * (y1.pz - y0.pz),
+                 y0.pt + best * (y1.pt - y0.pt)};
+        S hx = y0.qx + best * (y1.qx - y0.qx), hy = y0.qy + best * (y1.qy - y0.qy), hz = y0.qz + best * (y1.qz - y0.qz);
+        S g;
+        if (cls == MGK_HIT_DISK) {
+            // u.p of the circular emitter from the conserved p_t and p_phi = z p_x - x p_z of the
+            // step's end point: exact along the geodesic, free of chord interpolation error
+            S ut, w;
+            kepler_ut_w<METRIC, S>(r_hit, a, ut, w);
+            S lphi = y1.qz * y1.px - y1.qx * y1.pz;
+            g = contraction / (ut * y1.pt + w * lphi);
+        } else {
+            V4<S> u = reference_observer<METRIC, S>(hx, hy, hz, a, S(P.mass_sign));
+            g = contraction / dot4(u, ph);
+        }
+        if (isnan(g) || isinf(g)) g = S(1e-10f);
+        rec.g = float(g);
+        if (cls == MGK_HIT_DISK) {
+            rec.a0 = float(r_hit);
+            rec.a1 = atan2f(float(hz), float(hx));
+        } else {
+            S nx = hx - S(P.sphere_c[0]), ny = hy - S(P.sphere_c[1]), nz = hz - S(P.sphere_c[2]);
+            S l = sqrt_<false>(nx * nx + ny * ny + nz * nz);
+            rec.a0 = float(ny / l);
+            rec.a1 = atan2f(float(nz / l), float(nx / l));
+        }
+    } else if (cls == MGK_HIT_ESCAPE) {
+        S dx = y1.qx - y0.qx, dy = y1.qy - y0.qy, dz = y1.qz - y0.qz;
+        S l = sqrt_<false>(dx * dx + dy * dy + dz * dz);
+        rec.a0 = float(dy / l);
+        rec.a1 = atan2f(float(dz / l), float(dx / l));
+        S g = contraction / y1.pt;
+        if (isnan(g) || isinf(g)) g = S(1e-10f);
+        rec.g = float(g);
     }
 }
 
@@ -302,7 +345,7 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
             S rho2 = x * x + z * z - a * a;
             if (rho2 > S(0)) {
                 S r = sqrt_<false>(rho2);
-                if (!(r < S(P.disk_r_in) || r > S(P.disk_r_out)) && t < best) { best = t; cls = MGK_HIT_DISK; r_hit = r; }
+                if (!(r < ln.disk_r_in || r > S(P.disk_r_out)) && t < best) { best = t; cls = MGK_HIT_DISK; r_hit = r; }
             }
         }
     }
@@ -340,44 +383,7 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
     }
     if (cls == MGK_HIT_NONE && n_step > P.step_limit) cls = MGK_HIT_STEP_LIMIT;
     if (cls == MGK_HIT_NONE) return cls;
-    rec.cls = cls;
-    rec.g = 0.0f; rec.a0 = 0.0f; rec.a1 = 0.0f;
-    if (cls == MGK_HIT_DISK || cls == MGK_HIT_SPHERE) {
-        V4<S> ph{y0.px + best * (y1.px - y0.px), y0.py + best * (y1.py - y0.py), y0.pz + best * (y1.pz - y0.pz),
-                 y0.pt + best * (y1.pt - y0.pt)};
-        S hx = y0.qx + best * (y1.qx - y0.qx), hy = y0.qy + best * (y1.qy - y0.qy), hz = y0.qz + best * (y1.qz - y0.qz);
-        S g;
-        if (cls == MGK_HIT_DISK) {
-            // u.p of the circular emitter from the conserved p_t and p_phi = z p_x - x p_z of the
-            // step's end point: exact along the geodesic, free of chord interpolation error
-            S ut, w;
-            kepler_ut_w<METRIC, S>(r_hit, a, ut, w);
-            S lphi = y1.qz * y1.px - y1.qx * y1.pz;
-            g = contraction / (ut * y1.pt + w * lphi);
-        } else {
-            V4<S> u = reference_observer<METRIC, S>(hx, hy, hz, a, S(P.mass_sign));
-            g = contraction / dot4(u, ph);
-        }
-        if (isnan(g) || isinf(g)) g = S(1e-10f);
-        rec.g = float(g);
-        if (cls == MGK_HIT_DISK) {
-            rec.a0 = float(r_hit);
-            rec.a1 = float(atan2(hz, hx));
-        } else {
-            S nx = hx - S(P.sphere_c[0]), ny = hy - S(P.sphere_c[1]), nz = hz - S(P.sphere_c[2]);
-            S l = sqrt_<false>(nx * nx + ny * ny + nz * nz);
-            rec.a0 = float(ny / l);
-            rec.a1 = float(atan2(nz / l, nx / l));
-        }
-    } else if (cls == MGK_HIT_ESCAPE) {
-        S dx = y1.qx - y0.qx, dy = y1.qy - y0.qy, dz = y1.qz - y0.qz;
-        S l = sqrt_<false>(dx * dx + dy * dy + dz * dz);
-        rec.a0 = float(dy / l);
-        rec.a1 = float(atan2(dz / l, dx / l));
-        S g = contraction / y1.pt;
-        if (isnan(g) || isinf(g)) g = S(1e-10f);
-        rec.g = float(g);
-    }
+    hit_record<METRIC, S>(y0, y1, cls, best, r_hit, contraction, P, a, rec);
     return cls;
 }
 
@@ -466,9 +472,12 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
         } else if constexpr (PREC == MGK_PRECISION_MIXED) {
             double rms;
             double r2 = y.qx * y.qx + y.qy * y.qy + y.qz * y.qz;
-            bool near = r2 < ln.zone_rho2;   // rho^2 = r^2 + a^2 sin^2(theta): zone_rho2 = zone_r^2 + a^2 covers r < zone_r
-            if (near) {
-                trial_split<METRIC, double, false>(y, h, float(ln.a), P.mass_sign, P, y1, rms);
+            // rho^2 = r^2 + a^2 sin^2(theta): zone_rho2 = zone_r^2 + a^2 covers r < zone_r.  The choice is
+            // made per wave: if any active lane is in the zone, the whole wave runs the FP64 trial, so a
+            // wave never executes both trial bodies.
+            bool near = r2 < ln.zone_rho2;
+            if (__ballot(near) != 0) {
+                trial_split<METRIC, double, true>(y, h, float(ln.a), P.mass_sign, P, y1, rms);
                 ++fp64_acc;
             } else {
                 trial_split<METRIC, float, true>(y, h, float(ln.a), P.mass_sign, P, y1, rms);
@@ -523,6 +532,146 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
 }
 
 // ---------------------------------------------------------------------------
+// Mixed precision, split by zone.  The far-zone kernel runs FP32 trials, the
+// near-zone kernel FP64 trials; a ray whose completed step crosses the zone
+// boundary is parked (state to HBM) and appended to the other kernel's queue,
+// compacted per wave with ballot + mbcnt and one atomic per wave.  The host
+// alternates the two until both queues are empty.  A ray's arithmetic thus
+// depends on its own position only, never on the rays sharing its wave.
+// ---------------------------------------------------------------------------
+template <int METRIC, bool NEAR>
+__global__ void __launch_bounds__(256, 2) k_trace_zone(TraceParams P, ZoneParams Z) {
+    using S = double;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    St<S> y{};
+    S h = S(0), contraction = S(0);
+    Lane<S> ln;
+    lane_constants<METRIC, S>(P, S(P.spin), ln);
+    ln.cx = S(P.cam[0]); ln.cy = S(P.cam[1]); ln.cz = S(P.cam[2]);
+    bool active = false, handoff = false;
+    uint64_t ray = 0;
+    uint32_t n_step = 0, n_sub = 0;
+    uint64_t steps_acc = 0, trials_acc = 0, fp64_acc = 0;
+    uint64_t chunk_next = 0, chunk_end = 0;
+    bool exhausted = false;
+    // hysteresis: enter the near zone at rho^2 < zone, leave it at rho^2 > 1.21 zone
+    auto in_zone = [&](const St<S>& q) {
+        const S r2 = q.qx * q.qx + q.qy * q.qy + q.qz * q.qz;
+        return NEAR ? !(r2 > 1.21 * ln.zone_rho2) : (r2 < ln.zone_rho2);
+    };
+
+    for (;;) {
+        // ---- park the rays that crossed the zone boundary (wave-uniform) ----
+        const uint64_t ho = __ballot(handoff);
+        if (ho != 0) {
+            unsigned long long base = 0;
+            const int leader = __ffsll((unsigned long long)ho) - 1;
+            if (int(lane) == leader) base = atomicAdd(Z.out_count, (unsigned long long)__popcll(ho));
+            base = __shfl(base, leader);
+            if (handoff) {
+                const uint64_t pos = base + uint64_t(__popcll(ho & lt_mask));
+                Z.out_queue[pos] = uint32_t(ray);
+                double* st = Z.state;
+                const uint64_t sd = P.stride;
+                st[0 * sd + ray] = y.qx; st[1 * sd + ray] = y.qy; st[2 * sd + ray] = y.qz; st[3 * sd + ray] = y.qt;
+                st[4 * sd + ray] = y.px; st[5 * sd + ray] = y.py; st[6 * sd + ray] = y.pz; st[7 * sd + ray] = y.pt;
+                st[8 * sd + ray] = h;
+                Z.state_steps[ray] = n_step;
+                handoff = false;
+                active = false;
+            }
+        }
+        // ---- refill idle lanes ----
+        uint64_t idle = __ballot(!active);
+        while (idle != 0 && !exhausted) {
+            if (chunk_next >= chunk_end) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(P.head, (unsigned long long)kChunk);
+                base = __shfl(base, 0);
+                if (base >= Z.n_in) { exhausted = true; break; }
+                chunk_next = base;
+                chunk_end = base + kChunk < Z.n_in ? base + kChunk : Z.n_in;
+            }
+            const uint32_t avail = uint32_t(chunk_end - chunk_next);
+            const uint32_t cnt = uint32_t(__popcll(idle));
+            const uint32_t take = cnt < avail ? cnt : avail;
+            if (!active) {
+                const uint32_t rank = uint32_t(__popcll(idle & lt_mask));
+                if (rank < take) {
+                    const uint64_t slot = chunk_next + rank;
+                    ray = Z.in_queue ? uint64_t(Z.in_queue[slot]) : slot;
+                    const double* src = Z.from_ic ? reinterpret_cast<const double*>(P.rays) : Z.state;
+                    y.qx = src[0 * P.stride + ray]; y.qy = src[1 * P.stride + ray]; y.qz = src[2 * P.stride + ray];
+                    y.qt = src[3 * P.stride + ray]; y.px = src[4 * P.stride + ray]; y.py = src[5 * P.stride + ray];
+                    y.pz = src[6 * P.stride + ray]; y.pt = src[7 * P.stride + ray]; h = src[8 * P.stride + ray];
+                    n_step = Z.from_ic ? 0u : Z.state_steps[ray];
+                    contraction = load_c<S>(P.rays, P.stride, 9, ray);
+                    if (P.per_ray) {
+                        lane_constants<METRIC, S>(P, METRIC == KERR ? load_c<S>(P.rays, P.stride, 10, ray) : S(0), ln);
+                        const S L = S(P.length_scale);
+                        const double* ic = reinterpret_cast<const double*>(P.rays);
+                        ln.cx = ic[0 * P.stride + ray] * L; ln.cy = ic[1 * P.stride + ray] * L; ln.cz = ic[2 * P.stride + ray] * L;
+                    }
+                    n_sub = 0;
+                    active = true;
+                    handoff = !Z.no_handoff && (in_zone(y) != NEAR);
+                }
+            }
+            chunk_next += take;
+            idle = __ballot(!active);
+        }
+        if (__ballot(active) == 0) break;
+        if (!active || handoff) continue;
+
+        // ---- one trial ----
+        St<S> y1;
+        double rms;
+        if constexpr (NEAR) {
+            trial_split<METRIC, double, true>(y, h, float(ln.a), P.mass_sign, P, y1, rms);
+            ++fp64_acc;
+        } else {
+            trial_split<METRIC, float, true>(y, h, float(ln.a), P.mass_sign, P, y1, rms);
+        }
+        ++trials_acc;
+        const double rs = rms > 1e-30 ? rms : 1e-30;
+        double hn = 0.9 * h * double(__expf(-0.2f * __logf(float(rs))));
+        if (hn > double(P.h_max)) hn = double(P.h_max);
+        h = hn;
+        ++n_sub;
+        if (!((rs < 1.0) || (n_sub >= P.n_max))) continue;
+
+        // ---- step complete ----
+        ++n_step;
+        n_sub = 0;
+        ++steps_acc;
+        renormalise_null<METRIC>(y1, double(ln.a), double(P.mass_sign));
+        mgk_hit_t rec;
+        const uint32_t cls = classify<METRIC, S>(y, y1, n_step, contraction, P, ln, rec);
+        if (cls != MGK_HIT_NONE) {
+            P.hits[ray] = rec;
+            if (P.steps_out) P.steps_out[ray] = n_step;
+            active = false;
+        } else {
+            y = y1;
+            handoff = !Z.no_handoff && (in_zone(y) != NEAR);
+        }
+    }
+
+    for (int off = 32; off > 0; off >>= 1) {
+        steps_acc += __shfl_xor(steps_acc, off);
+        trials_acc += __shfl_xor(trials_acc, off);
+        fp64_acc += __shfl_xor(fp64_acc, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&P.stats[0], (unsigned long long)steps_acc);
+        atomicAdd(&P.stats[1], (unsigned long long)trials_acc);
+        atomicAdd(&P.stats[2], (unsigned long long)fp64_acc);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Ray generation + initial conditions.
 // ---------------------------------------------------------------------------
 template <int METRIC, class S>
@@ -535,17 +684,8 @@ __device__ __forceinline__ void gen_one(const RayGenParams& P, uint64_t i) {
     float u[4];
     path_uniforms(P.seed, P.camera_id, pixel, sample, 0, u);
     // sensor point (device_sensor_utilities.cuh:35-80)
-    const S sw = S(P.sensor_width_mm);
-    S inv_s = sw / sqrt_<false>(S(float(P.W * P.W + P.H * P.H)));
-    S width_mm = S(float(P.W)) * inv_s, height_mm = S(float(P.H)) * inv_s;
-    S area = (width_mm * height_mm) / S(1E6f);
-    S side = sqrt_<false>(area / S(float(P.W * P.H)));
-    S ju = (S(u[0]) - S(0.5)) * side, jv = (S(u[1]) - S(0.5)) * side;
-    S uu = -((S(float(px)) / S(float(P.W))) - S(0.5));
-    S vv = -(S(float(py)) / S(float(P.H)) - S(0.5));
-    S ratio = S(float(P.H)) / S(float(P.W));
-    if (ratio < S(1)) vv = vv * ratio; else uu = uu / ratio;
-    S sx = ((-uu * sw) / S(1000)) + ju, sy = ((-vv * sw) / S(1000)) + jv;
+    S sx, sy;
+    sensor_point<S>(P.W, P.H, P.sensor_width_mm, px, py, u[0], u[1], sx, sy);
     // pinhole (thin_lens.cuh:60-76): origin at the lens point, direction lens - sensor
     S f_m = S(P.focal_mm) / S(1000);
     S dx = S(0) - sx, dy = S(0) - sy, dz = -f_m - S(0);
@@ -798,6 +938,15 @@ hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s) {
                 hipLaunchKernelGGL((k_trace<M, MGK_PRECISION_FP64>), dim3(grid), dim3(256), 0, s, p);
                 break;
         }
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s) {
+    return dispatch_metric<tag_t>(p.metric, [&](auto m) -> hipError_t {
+        constexpr int M = decltype(m)::value;
+        if (near) hipLaunchKernelGGL((k_trace_zone<M, true>), dim3(grid), dim3(256), 0, s, p, z);
+        else hipLaunchKernelGGL((k_trace_zone<M, false>), dim3(grid), dim3(256), 0, s, p, z);
         return hipGetLastError();
     });
 }

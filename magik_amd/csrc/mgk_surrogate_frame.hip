@@ -1,0 +1,514 @@
+// The surrogate render path (mgk_render_frame_surrogate): every path of the
+// frame is answered by the MLP unless a fallback rule sends it to the RKF45
+// integrator; a held-out judge sample is traced as well and compared.
+//
+//   k_sur_features : pixel + sample -> screen point (alpha, beta) of the lens
+//                    point's camera frame -> normalised features (n x 16)
+//   k_mlp_infer    : (mgk_mlp.hip) features -> outputs (n x 8)
+//   k_sur_decode   : outputs -> class / disk radius / escape direction, the
+//                    fallback rules, the hit record of answered paths, and
+//                    the compacted queue of paths to trace (one atomic per
+//                    wave, ballot ranks within it)
+//   mgk_trace_rays : the queue, written back into the same hit slots
+//   k_sur_judge    : surrogate vs RKF45 on the judge sample
+//   mgk_shade      : the unchanged spectral shading of every hit
+//
+// Redshift of an answered path comes from the ray's conserved quantities:
+// g = (u.p)_camera / p_t for an escaping ray and
+// g = (u.p)_camera / (u^t p_t + Omega u^t p_phi) for a circular Keplerian
+// emitter at the predicted disk radius, with p_t and p_phi = z p_x - x p_z
+// read from the initial conditions (both are conserved along a Kerr geodesic
+// in Kerr-Schild coordinates with the spin along +y).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "magik_amd.h"
+#include "mgk_device.hpp"
+#include "mgk_internal.hpp"
+#include "mgk_mlp.hpp"
+#include "mgk_surrogate.hpp"
+
+using namespace mgk;
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            std::fprintf(stderr, "magik_amd: %s failed: %s\n", #expr, hipGetErrorString(e_));   \
+            return MGK_ERROR_HIP;                                                                \
+        }                                                                                        \
+    } while (0)
+
+namespace {
+
+constexpr uint32_t kNotAnswered = 0xFFFFFFFFu;
+
+// device counters of one frame (u64 slots; the double sums live in the
+// same buffer, bit-cast)
+enum {
+    C_QUEUE = 0, C_CONF, C_CRIT, C_DISK, C_STAR, C_DOMAIN, C_JUDGE, C_AGREE, C_JDISK, C_JESC,
+    C_SUM_R, C_SUM_G, C_SUM_DIR, C_MAX_DIR, C_N
+};
+
+struct FeatureParams {
+    SurrogateRanges R;
+    uint32_t W, H, camera_id, spp;
+    uint64_t seed, n;
+    float sensor_width_mm, focal_mm;
+    double a, r_lens, th_lens_deg;
+    float* X;
+};
+
+__global__ void __launch_bounds__(256) k_sur_features(FeatureParams P) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    const uint32_t npix = P.W * P.H;
+    const uint32_t pixel = uint32_t(i % npix), sample = uint32_t(i / npix);
+    float u[4];
+    path_uniforms(P.seed, P.camera_id, pixel, sample, 0, u);
+    double sx, sy;
+    sensor_point<double>(P.W, P.H, P.sensor_width_mm, pixel % P.W, pixel / P.W, u[0], u[1], sx, sy);
+    // camera-local direction (-sx, -sy, -f): alpha = r d_x / (-d_z)
+    const double f_m = double(P.focal_mm) / 1000.0;
+    const double al = P.r_lens * (-sx) / f_m, be = P.r_lens * (-sy) / f_m;
+    float x[kMlpInPad];
+    normalise_features(P.R, P.a, P.r_lens, P.th_lens_deg, al, be, x);
+    float4* dst = reinterpret_cast<float4*>(P.X + i * kMlpInPad);
+    for (int k = 0; k < kMlpInPad / 4; ++k) dst[k] = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
+}
+
+struct DecodeParams {
+    SurrogateRanges R;
+    uint64_t n;
+    const float* X;
+    const float* Y;
+    const double* rays;          // 10 x n doubles (q, p, h0, u.p)
+    mgk_hit_t* hits;
+    uint32_t* queue;
+    mgk_hit_t* pred;             // judge predictions (one per judge slot)
+    unsigned long long* cnt;
+    const float* crit;           // kCritN screen radii of the critical curve (< 0: none)
+    StarView stars;
+    float star_chord2_in, star_chord2_out;
+    uint32_t flags;
+    double a_eq, disk_r_in, disk_r_out;
+    float min_conf, crit_margin, edge_margin;
+    int star_test;
+    uint32_t judge_stride, trace_disk;
+};
+
+// Any star whose edge lies within the band [chord_in, chord_out] of direction d.
+__device__ bool near_star_edge(const DecodeParams& P, float dx, float dy, float dz, float ph) {
+    const StarView& S = P.stars;
+    const uint32_t ic = min(S.nc - 1, uint32_t(fmaxf(0.0f, (dy + 1.0f) * 0.5f * float(S.nc))));
+    const uint32_t ip = min(S.np - 1, uint32_t(fmaxf(0.0f, (ph + kPi) * (0.5f / kPi) * float(S.np))));
+    const uint32_t cell = ic * S.np + ip;
+    for (uint32_t k = S.cell_start[cell]; k < S.cell_start[cell + 1]; ++k) {
+        const float4 s = S.stars[S.cell_items[k]];
+        const float ex = s.x - dx, ey = s.y - dy, ez = s.z - dz;
+        const float c2 = ex * ex + ey * ey + ez * ez;
+        if (c2 > P.star_chord2_in && c2 <= P.star_chord2_out) return true;
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(256) k_sur_decode(DecodeParams P) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const bool live = i < P.n;
+    bool fallback = false;
+    int reason = -1;
+    mgk_hit_t h{MGK_HIT_NONE, 0.0f, 0.0f, 0.0f};
+    if (live) {
+        const float4 y0 = reinterpret_cast<const float4*>(P.Y + i * kMlpOut)[0];
+        const float4 y1 = reinterpret_cast<const float4*>(P.Y + i * kMlpOut)[1];
+        // logits (captured, disk, escape = 0)
+        const float mx = fmaxf(fmaxf(y0.x, y0.y), 0.0f);
+        const float e0 = __expf(y0.x - mx), e1 = __expf(y0.y - mx), e2 = __expf(-mx);
+        const float inv = 1.0f / (e0 + e1 + e2);
+        const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
+        const int cls = (p0 >= p1 && p0 >= p2) ? 0 : (p1 >= p2 ? 1 : 2);
+        const float conf = fmaxf(p0, fmaxf(p1, p2));
+        const float al = P.X[i * kMlpInPad + 3] * float(P.R.s_max), be = P.X[i * kMlpInPad + 4] * float(P.R.s_max);
+        if (fabsf(al) > float(P.R.s_max) || fabsf(be) > float(P.R.s_max)) {
+            fallback = true; reason = C_DOMAIN;
+        } else if (conf < P.min_conf) {
+            fallback = true; reason = C_CONF;
+        } else {
+            const float rho = sqrtf(al * al + be * be), psi = atan2f(be, al);
+            float fk = (psi + kPi) * (float(kCritN) / (2.0f * kPi));
+            int k0 = int(floorf(fk));
+            const float t = fk - float(k0);
+            k0 = ((k0 % kCritN) + kCritN) % kCritN;
+            const int k1 = (k0 + 1) % kCritN;
+            const float c0 = P.crit[k0], c1 = P.crit[k1];
+            if (c0 >= 0.0f && c1 >= 0.0f && fabsf(rho - (c0 + t * (c1 - c0))) < P.crit_margin) {
+                fallback = true; reason = C_CRIT;
+            }
+        }
+        if (!fallback) {
+            const double con = P.rays[9 * P.n + i], pt = P.rays[7 * P.n + i];
+            if (cls == 0) {
+                h.cls = MGK_HIT_HORIZON;
+            } else if (cls == 1) {
+                const double r = exp(P.R.lnr_mid + double(y0.z) * P.R.lnr_half);
+                if (P.trace_disk || r < P.disk_r_in * (1.0 + P.edge_margin) || r > P.disk_r_out * (1.0 - P.edge_margin)) {
+                    fallback = true; reason = C_DISK;
+                } else {
+                    double ut, w;
+                    kepler_ut_w<KERR, double>(r, P.a_eq, ut, w);
+                    const double qx = P.rays[0 * P.n + i], qz = P.rays[2 * P.n + i];
+                    const double px = P.rays[4 * P.n + i], pz = P.rays[6 * P.n + i];
+                    double g = con / (ut * pt + w * (qz * px - qx * pz));
+                    if (isnan(g) || isinf(g)) g = 1e-10;
+                    h = mgk_hit_t{MGK_HIT_DISK, float(g), float(r), 0.0f};
+                }
+            } else {
+                const float c = fminf(1.0f, fmaxf(-1.0f, y0.w));
+                const float ph = atan2f(y1.x, y1.y);
+                const float sn = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+                if (P.star_test && near_star_edge(P, sn * cosf(ph), c, sn * sinf(ph), ph)) {
+                    fallback = true; reason = C_STAR;
+                } else {
+                    double g = con / pt;
+                    if (isnan(g) || isinf(g)) g = 1e-10;
+                    h = mgk_hit_t{MGK_HIT_ESCAPE, float(g), c, ph};
+                }
+            }
+        }
+        if (fallback) atomicAdd(&P.cnt[reason], 1ull);
+        if (P.judge_stride && (i % P.judge_stride) == 0) {
+            mgk_hit_t pr = h;
+            if (fallback) pr.cls = kNotAnswered;
+            P.pred[i / P.judge_stride] = pr;
+            fallback = true;
+        }
+        if (!fallback) P.hits[i] = h;
+    }
+    // compacted queue of the paths to trace
+    const uint64_t fb = __ballot(live && fallback);
+    if (fb != 0) {
+        unsigned long long base = 0;
+        const int leader = __ffsll((unsigned long long)fb) - 1;
+        if (int(lane) == leader) base = atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)__popcll(fb));
+        base = __shfl(base, leader);
+        if (live && fallback) P.queue[base + __popcll(fb & lt_mask)] = uint32_t(i);
+    }
+}
+
+__device__ inline void atomic_max_double(unsigned long long* p, double v) {
+    // non-negative doubles order like their bit patterns
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+__global__ void __launch_bounds__(256) k_sur_judge(const mgk_hit_t* pred, const mgk_hit_t* hits, uint64_t n_slots, uint32_t stride,
+                                                   unsigned long long* cnt) {
+    const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= n_slots) return;
+    const mgk_hit_t p = pred[j];
+    if (p.cls == kNotAnswered) return;
+    const mgk_hit_t h = hits[j * stride];
+    atomicAdd(&cnt[C_JUDGE], 1ull);
+    if (p.cls != h.cls) return;
+    atomicAdd(&cnt[C_AGREE], 1ull);
+    double* sums = reinterpret_cast<double*>(cnt);
+    if (h.cls == MGK_HIT_DISK) {
+        atomicAdd(&cnt[C_JDISK], 1ull);
+        atomicAdd(&sums[C_SUM_R], fabs(double(p.a0) - double(h.a0)) / fabs(double(h.a0)));
+        atomicAdd(&sums[C_SUM_G], fabs(double(p.g) - double(h.g)) / fabs(double(h.g)));
+    } else if (h.cls == MGK_HIT_ESCAPE) {
+        atomicAdd(&cnt[C_JESC], 1ull);
+        const double sp = sqrt(fmax(0.0, 1.0 - double(p.a0) * p.a0)), sh = sqrt(fmax(0.0, 1.0 - double(h.a0) * h.a0));
+        const double dx = sp * cos(double(p.a1)) - sh * cos(double(h.a1)), dy = double(p.a0) - double(h.a0);
+        const double dz = sp * sin(double(p.a1)) - sh * sin(double(h.a1));
+        const double chord = sqrt(dx * dx + dy * dy + dz * dz);
+        const double ang = 2.0 * asin(fmin(1.0, 0.5 * chord));
+        atomicAdd(&sums[C_SUM_DIR], ang);
+        atomic_max_double(&cnt[C_MAX_DIR], ang);
+    }
+}
+
+// Critical curve by bisection along each screen angle: captured at rho = 0,
+// not captured at rho = s_max, traced in FP64 without disk or stars.
+struct CritParams {
+    SurrogateRanges R;
+    double L, a, r, th_deg;
+    int time_reversed;
+    int mode;                    // 0: rho = 0, 1: rho = hi, 2: midpoint
+    double* lo;
+    double* hi;
+    int* ok;
+    double* rays;
+};
+
+__global__ void k_crit_rays(CritParams P) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kCritN) return;
+    const double psi = -3.14159265358979323846 + 2.0 * 3.14159265358979323846 * double(k) / double(kCritN);
+    const double rho = P.mode == 0 ? 0.0 : (P.mode == 1 ? P.hi[k] : 0.5 * (P.lo[k] + P.hi[k]));
+    write_feature_ray(P.R, P.L, P.time_reversed, P.a, P.r, P.th_deg, rho * cos(psi), rho * sin(psi), P.rays, kCritN, uint64_t(k),
+                      nullptr);
+}
+
+__global__ void k_crit_update(CritParams P, const mgk_hit_t* hits, float* table) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kCritN) return;
+    // a ray that orbits until the step limit counts as captured
+    const bool captured = hits[k].cls == MGK_HIT_HORIZON || hits[k].cls == MGK_HIT_STEP_LIMIT;
+    if (P.mode == 0) {
+        P.ok[k] = captured ? 1 : 0;
+    } else if (P.mode == 1) {
+        P.ok[k] = P.ok[k] && !captured;
+    } else if (P.mode == 2) {
+        const double mid = 0.5 * (P.lo[k] + P.hi[k]);
+        if (captured) P.lo[k] = mid; else P.hi[k] = mid;
+    }
+    if (table) table[k] = P.ok[k] ? float(0.5 * (P.lo[k] + P.hi[k])) : -1.0f;
+}
+
+__global__ void k_crit_init(double* lo, double* hi, double s_max) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= kCritN) return;
+    lo[k] = 0.0;
+    hi[k] = s_max;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0ull;
+}
+
+double kerr_r(double x, double y, double z, double a) {   // Kerr-Schild r with the spin along +y
+    const double rho2 = x * x + y * y + z * z - a * a;
+    return std::sqrt(0.5 * (rho2 + std::sqrt(rho2 * rho2 + 4.0 * a * a * y * y)));
+}
+
+double isco(double a) {   // prograde ISCO (Bardeen, Press & Teukolsky 1972)
+    a = std::fabs(a);
+    const double z1 = 1.0 + std::cbrt(1.0 - a * a) * (std::cbrt(1.0 + a) + std::cbrt(1.0 - a));
+    const double z2 = std::sqrt(3.0 * a * a + z1 * z1);
+    return 3.0 + z2 - std::sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
+}
+
+e_magik_result_types ensure_critical_curve(mgk_surrogate* s, mgk_context_handle ctx, const mgk_integrator_t* integ,
+                                           const mgk_scene_t* scene, double L, double r, double th_deg, hipStream_t st) {
+    const double key[6] = {double(integ->spin), r, th_deg, s->ranges.s_max, double(scene->celestial_radius_m),
+                           double(integ->time_reversed)};
+    if (std::memcmp(key, s->crit_key, sizeof(key)) == 0) return MAGIK_SUCCESS;
+    HIP_TRY(s->crit.reserve(kCritN * sizeof(float)));
+    HIP_TRY(s->crit_lo.reserve(kCritN * sizeof(double)));
+    HIP_TRY(s->crit_hi.reserve(kCritN * sizeof(double)));
+    HIP_TRY(s->crit_ok.reserve(kCritN * sizeof(int)));
+    HIP_TRY(s->crit_rays.reserve(size_t(kCritN) * 11 * sizeof(double)));
+    HIP_TRY(s->crit_hits.reserve(kCritN * sizeof(mgk_hit_t)));
+    mgk_integrator_t it = *integ;
+    it.precision = MGK_PRECISION_FP64;
+    mgk_scene_t sc{};
+    sc.celestial_radius_m = scene->celestial_radius_m;
+    sc.flags = 0;
+    CritParams P;
+    P.R = s->ranges;
+    P.L = L;
+    P.a = integ->spin;
+    P.r = r;
+    P.th_deg = th_deg;
+    P.time_reversed = integ->time_reversed ? 1 : 0;
+    P.lo = s->crit_lo.as<double>();
+    P.hi = s->crit_hi.as<double>();
+    P.ok = s->crit_ok.as<int>();
+    P.rays = s->crit_rays.as<double>();
+    const dim3 grid(kCritN / 256), block(256);
+    hipLaunchKernelGGL(k_crit_init, grid, block, 0, st, P.lo, P.hi, s->ranges.s_max);
+    HIP_TRY(hipGetLastError());
+    const float origin[3] = {0.0f, 0.0f, 0.0f};
+    const int iters = 22;   // s_max / 2^22 ~ 4e-6 M
+    for (int step = 0; step < iters + 2; ++step) {
+        P.mode = step < 2 ? step : 2;
+        hipLaunchKernelGGL(k_crit_rays, grid, block, 0, st, P);
+        HIP_TRY(hipGetLastError());
+        e_magik_result_types rr = mgk_trace_rays_per_ray(ctx, &it, &sc, origin, P.rays, kCritN, kCritN, s->crit_hits.as<mgk_hit_t>(), st);
+        if (rr != MAGIK_SUCCESS) return rr;
+        hipLaunchKernelGGL(k_crit_update, grid, block, 0, st, P, s->crit_hits.as<const mgk_hit_t>(),
+                           step == iters + 1 ? s->crit.as<float>() : nullptr);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    std::memcpy(s->crit_key, key, sizeof(key));
+    return MAGIK_SUCCESS;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms : 0.0f;
+}
+
+}  // namespace
+
+extern "C" {
+
+MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, mgk_surrogate_handle s, const mgk_integrator_t* integ,
+                                                        const mgk_scene_t* scene, const mgk_spectral_t* spec, const mgk_camera_t* cam,
+                                                        uint64_t seed, uint32_t spp, const mgk_fallback_t* fb, float* d_xyz,
+                                                        mgk_surrogate_stats_t* stats, void* stream) {
+    if (!ctx || !s || !integ || !scene || !spec || !cam || !fb || !d_xyz) return MAGIK_ERROR_INVALID_POINTER;
+    if (integ->metric_type != KERR || integ->precision == MGK_PRECISION_FP32_REFERENCE || spp == 0) return MGK_ERROR_INVALID_ARGUMENT;
+    if (cam->observer_velocity[0] != 0.0f || cam->observer_velocity[1] != 0.0f || cam->observer_velocity[2] != 0.0f)
+        return MGK_ERROR_INVALID_ARGUMENT;
+    if (!(fb->min_confidence >= 0.0f) || !(fb->critical_margin >= 0.0f) || !(fb->direction_margin >= 0.0f) ||
+        !(fb->disk_edge_margin >= 0.0f))
+        return MGK_ERROR_INVALID_ARGUMENT;
+    const SurrogateRanges& R = s->ranges;
+    const double a = integ->spin;
+    if (a < R.a_lo || a > R.a_hi) return MGK_ERROR_INVALID_ARGUMENT;
+    // the disk the network was trained on: r_ISCO(a) .. disk_r_max
+    if ((scene->flags & MGK_SCENE_DISK) &&
+        (std::fabs(double(scene->disk_r_in) - isco(a)) > 1e-4 * isco(a) || std::fabs(double(scene->disk_r_out) - R.disk_r_max) > 1e-4 * R.disk_r_max))
+        return MGK_ERROR_INVALID_ARGUMENT;
+    if (scene->flags & (MGK_SCENE_SPHERE | MGK_SCENE_BACKGROUND)) return MGK_ERROR_INVALID_ARGUMENT;
+    // lens point of the camera in units of M (the rays' origin) and its Boyer-Lindquist (r, theta)
+    const double L = double(kG * integ->mass_kg / (kC * kC));
+    const double f_m = double(cam->focal_length_mm) / 1000.0;
+    const float* m = cam->orientation;
+    const double o[3] = {(double(m[2]) * -f_m + double(cam->position[0])) / L, (double(m[5]) * -f_m + double(cam->position[1])) / L,
+                         (double(m[8]) * -f_m + double(cam->position[2])) / L};
+    const double r_lens = kerr_r(o[0], o[1], o[2], a);
+    const double th_lens = std::acos(std::max(-1.0, std::min(1.0, o[1] / r_lens)));
+    const double th_deg = th_lens * 180.0 / 3.14159265358979323846;
+    if (r_lens < R.r_lo || r_lens > R.r_hi || th_deg < R.th_lo || th_deg > R.th_hi) return MGK_ERROR_INVALID_ARGUMENT;
+    // the camera must be the Boyer-Lindquist camera of that point (looking at the hole, +y up)
+    Camframe f;
+    bl_camera_frame(a, r_lens, th_lens, f);
+    for (int row = 0; row < 3; ++row) {
+        if (std::fabs(m[3 * row + 0] - f.X[row]) > 1e-4 || std::fabs(m[3 * row + 1] - f.Y[row]) > 1e-4 ||
+            std::fabs(m[3 * row + 2] - f.Z[row]) > 1e-4)
+            return MGK_ERROR_INVALID_ARGUMENT;
+    }
+    if (std::fabs(o[0]) > 1e-5 * r_lens) return MGK_ERROR_INVALID_ARGUMENT;
+    HIP_TRY(hipSetDevice(s->device));
+    hipStream_t st = hipStream_t(stream);
+    for (hipEvent_t& e : s->ev)
+        if (!e) HIP_TRY(hipEventCreate(&e));
+
+    const uint64_t n = uint64_t(cam->x_resolution) * cam->y_resolution * spp;
+    const uint64_t n_slots = fb->judge_stride ? (n + fb->judge_stride - 1) / fb->judge_stride : 0;
+    HIP_TRY(s->fr_rays.reserve(n * 10 * sizeof(double)));
+    HIP_TRY(s->fr_x.reserve(n * kMlpInPad * sizeof(float)));
+    HIP_TRY(s->fr_y.reserve(n * kMlpOut * sizeof(float)));
+    HIP_TRY(s->fr_hits.reserve(n * sizeof(mgk_hit_t)));
+    HIP_TRY(s->fr_queue.reserve(n * sizeof(uint32_t)));
+    HIP_TRY(s->fr_pred.reserve(std::max<uint64_t>(1, n_slots) * sizeof(mgk_hit_t)));
+    HIP_TRY(s->fr_counters.reserve(C_N * sizeof(unsigned long long)));
+    if (n > 0xFFFFFFFFull) return MGK_ERROR_CAPACITY_EXCEEDED;
+
+    e_magik_result_types r = ensure_critical_curve(s, ctx, integ, scene, L, r_lens, th_deg, st);
+    if (r != MAGIK_SUCCESS) return r;
+    StarView sv;
+    const bool star_test = (scene->flags & MGK_SCENE_STARS) && scene->n_star > 0 && fb->direction_margin > 0.0f;
+    std::memset(&sv, 0, sizeof(sv));
+    if (star_test && (r = mgk_star_band_view(ctx, scene, fb->direction_margin, &sv)) != MAGIK_SUCCESS) return r;
+    unsigned long long* cnt = s->fr_counters.as<unsigned long long>();
+
+    HIP_TRY(hipEventRecord(s->ev[0], st));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, cnt, int(C_N));
+    HIP_TRY(hipGetLastError());
+    if ((r = mgk_generate_rays(ctx, integ, cam, seed, 0, spp, s->fr_rays.ptr, n, st)) != MAGIK_SUCCESS) return r;
+    HIP_TRY(hipEventRecord(s->ev[1], st));
+
+    FeatureParams F;
+    F.R = R;
+    F.W = cam->x_resolution; F.H = cam->y_resolution; F.camera_id = cam->camera_id; F.spp = spp;
+    F.seed = seed; F.n = n;
+    F.sensor_width_mm = cam->sensor_width_mm; F.focal_mm = cam->focal_length_mm;
+    F.a = a; F.r_lens = r_lens; F.th_lens_deg = th_deg;
+    F.X = s->fr_x.as<float>();
+    const unsigned blocks = unsigned((n + 255) / 256);
+    hipLaunchKernelGGL(k_sur_features, dim3(blocks), dim3(256), 0, st, F);
+    HIP_TRY(hipGetLastError());
+    if ((r = mgk_surrogate_forward(s, s->fr_x.as<float>(), n, s->fr_y.as<float>(), st)) != MAGIK_SUCCESS) return r;
+
+    DecodeParams D;
+    std::memset(&D, 0, sizeof(D));
+    D.R = R;
+    D.n = n;
+    D.X = s->fr_x.as<const float>();
+    D.Y = s->fr_y.as<const float>();
+    D.rays = s->fr_rays.as<const double>();
+    D.hits = s->fr_hits.as<mgk_hit_t>();
+    D.queue = s->fr_queue.as<uint32_t>();
+    D.pred = s->fr_pred.as<mgk_hit_t>();
+    D.cnt = cnt;
+    D.crit = s->crit.as<const float>();
+    D.stars = sv;
+    D.star_test = star_test ? 1 : 0;
+    if (star_test) {
+        const double rin = std::max(0.0, double(scene->star_radius_rad) - fb->direction_margin);
+        const double rout = double(scene->star_radius_rad) + fb->direction_margin;
+        D.star_chord2_in = float(4.0 * std::sin(0.5 * rin) * std::sin(0.5 * rin));
+        D.star_chord2_out = float(4.0 * std::sin(0.5 * rout) * std::sin(0.5 * rout));
+    }
+    D.flags = scene->flags;
+    D.a_eq = integ->time_reversed ? -a : a;
+    D.disk_r_in = scene->disk_r_in;
+    D.disk_r_out = scene->disk_r_out;
+    D.min_conf = fb->min_confidence;
+    D.crit_margin = fb->critical_margin;
+    D.edge_margin = fb->disk_edge_margin;
+    D.judge_stride = fb->judge_stride;
+    D.trace_disk = fb->trace_disk;
+    hipLaunchKernelGGL(k_sur_decode, dim3(blocks), dim3(256), 0, st, D);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(s->ev[2], st));
+
+    unsigned long long nq = 0;
+    HIP_TRY(hipMemcpyAsync(&nq, cnt + C_QUEUE, sizeof(nq), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (nq > 0 && (r = mgk_trace_rays(ctx, integ, scene, cam->position, s->fr_rays.ptr, n, nq, s->fr_queue.as<uint32_t>(),
+                                      s->fr_hits.as<mgk_hit_t>(), nullptr, st)) != MAGIK_SUCCESS)
+        return r;
+    if (n_slots) {
+        hipLaunchKernelGGL(k_sur_judge, dim3(unsigned((n_slots + 255) / 256)), dim3(256), 0, st, s->fr_pred.as<const mgk_hit_t>(),
+                           s->fr_hits.as<const mgk_hit_t>(), n_slots, fb->judge_stride, cnt);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(s->ev[3], st));
+    if ((r = mgk_shade(ctx, scene, spec, cam, seed, 0, spp, s->fr_hits.as<mgk_hit_t>(), d_xyz, st)) != MAGIK_SUCCESS) return r;
+    HIP_TRY(hipEventRecord(s->ev[4], st));
+
+    if (stats) {
+        HIP_TRY(hipStreamSynchronize(st));
+        unsigned long long h[C_N];
+        HIP_TRY(hipMemcpy(h, cnt, sizeof(h), hipMemcpyDeviceToHost));
+        double dsum[C_N];
+        std::memcpy(dsum, h, sizeof(h));
+        std::memset(stats, 0, sizeof(*stats));
+        stats->n_paths = n;
+        stats->n_fallback = h[C_QUEUE];
+        stats->n_fallback_confidence = h[C_CONF];
+        stats->n_fallback_critical = h[C_CRIT];
+        stats->n_fallback_disk = h[C_DISK];
+        stats->n_fallback_star = h[C_STAR];
+        stats->n_fallback_domain = h[C_DOMAIN];
+        stats->n_judge = h[C_JUDGE];
+        stats->judge_class_agree = h[C_AGREE];
+        stats->judge_n_disk = h[C_JDISK];
+        stats->judge_n_escape = h[C_JESC];
+        stats->judge_mean_rel_r = h[C_JDISK] ? dsum[C_SUM_R] / double(h[C_JDISK]) : 0.0;
+        stats->judge_mean_rel_g = h[C_JDISK] ? dsum[C_SUM_G] / double(h[C_JDISK]) : 0.0;
+        stats->judge_mean_dir = h[C_JESC] ? dsum[C_SUM_DIR] / double(h[C_JESC]) : 0.0;
+        stats->judge_max_dir = dsum[C_MAX_DIR];
+        mgk_trace_stats_t ts;
+        if (nq > 0 && mgk_last_trace_stats(ctx, &ts) == MAGIK_SUCCESS) stats->n_trials = ts.n_trials;
+        stats->ms_raygen = elapsed(s->ev[0], s->ev[1]);
+        stats->ms_mlp = elapsed(s->ev[1], s->ev[2]);
+        stats->ms_trace = elapsed(s->ev[2], s->ev[3]);
+        stats->ms_shade = elapsed(s->ev[3], s->ev[4]);
+        stats->ms_total = elapsed(s->ev[0], s->ev[4]);
+    }
+    return MAGIK_SUCCESS;
+}
+
+}  // extern "C"

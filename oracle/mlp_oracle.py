@@ -54,16 +54,22 @@ def layer_shapes():
     return [(dims[i], dims[i + 1]) for i in range(len(dims) - 1)]
 
 
-def forward(W: list, b: list, x: np.ndarray, keep: bool = False):
-    """W[l]: (in, out) float32 master weights (padded shapes); x: (n, 16)."""
-    h = bf16(x.astype(np.float32))
+def _q(x, quant):
+    return bf16(x) if quant else x
+
+
+def forward(W: list, b: list, x: np.ndarray, keep: bool = False, quant: bool = True):
+    """W[l]: (in, out) float32 master weights (padded shapes); x: (n, 16).
+    quant=False drops every bf16 rounding and float32 cast (float64 throughout)."""
+    dt = np.float32 if quant else np.float64
+    h = _q(x.astype(dt), quant)
     hs, zs = [h], []
     for l in range(len(W)):
-        z = h.astype(np.float64) @ bf16(W[l]).astype(np.float64) + b[l].astype(np.float64)
-        z = z.astype(np.float32)
+        z = h.astype(np.float64) @ _q(W[l].astype(dt), quant).astype(np.float64) + b[l].astype(np.float64)
+        z = z.astype(dt)
         if l < len(W) - 1:
             zs.append(z)
-            h = bf16(silu(z.astype(np.float64)).astype(np.float32))
+            h = _q(silu(z.astype(np.float64)).astype(dt), quant)
             hs.append(h)
         else:
             y = z
@@ -72,20 +78,54 @@ def forward(W: list, b: list, x: np.ndarray, keep: bool = False):
     return y
 
 
-def backward(W: list, hs: list, zs: list, dy: np.ndarray):
+def backward(W: list, hs: list, zs: list, dy: np.ndarray, quant: bool = True):
     """Gradients (dW, db) for dL/dy = dy (n, 32)."""
+    dt = np.float32 if quant else np.float64
     n_l = len(W)
     dW, db = [None] * n_l, [None] * n_l
-    g = bf16(dy.astype(np.float32))
+    g = _q(dy.astype(dt), quant)
     dz = dy.astype(np.float64)
     for l in range(n_l - 1, -1, -1):
-        dW[l] = (hs[l].astype(np.float64).T @ g.astype(np.float64)).astype(np.float32)
-        db[l] = dz.sum(axis=0).astype(np.float32)
+        dW[l] = (hs[l].astype(np.float64).T @ g.astype(np.float64)).astype(dt)
+        db[l] = dz.sum(axis=0).astype(dt)
         if l > 0:
-            dh = g.astype(np.float64) @ bf16(W[l]).astype(np.float64).T
-            dz = (dh * dsilu(zs[l - 1].astype(np.float64))).astype(np.float32).astype(np.float64)
-            g = bf16(dz.astype(np.float32))
+            dh = g.astype(np.float64) @ _q(W[l].astype(dt), quant).astype(np.float64).T
+            dz = (dh * dsilu(zs[l - 1].astype(np.float64))).astype(dt).astype(np.float64)
+            g = _q(dz.astype(dt), quant)
     return dW, db
+
+
+def loss(y: np.ndarray, t: np.ndarray, w_r: float, w_d: float):
+    """The training loss of the kernels (k_loss): logits (y0 captured, y1 disk,
+    0 escape) -> cross-entropy on the class t0; disk rows regress y2 -> t1;
+    escape rows regress y3..y5 -> t2..t4.  Returns (sums [ce, r, dir, correct],
+    dL/dy of the batch-mean loss, (n, 32))."""
+    y = y.astype(np.float64)
+    n = y.shape[0]
+    cls = t[:, 0].astype(int)
+    logits = np.stack([y[:, 0], y[:, 1], np.zeros(n)], axis=1)
+    mx = logits.max(axis=1, keepdims=True)
+    e = np.exp(logits - mx)
+    p = e / e.sum(axis=1, keepdims=True)
+    pc = p[np.arange(n), cls]
+    ce = -np.log(np.maximum(pc, 1e-30))
+    correct = (p.argmax(axis=1) == cls).astype(np.float64)
+    d = np.zeros((n, OUT_PAD))
+    onehot = np.eye(3)[cls]
+    d[:, 0] = (p[:, 0] - onehot[:, 0]) / n
+    d[:, 1] = (p[:, 1] - onehot[:, 1]) / n
+    disk = cls == 1
+    er = y[:, 2] - t[:, 1]
+    lr = np.where(disk, w_r * er * er, 0.0)
+    d[:, 2] = np.where(disk, 2 * w_r * er / n, 0.0)
+    esc = cls == 2
+    ldir = np.zeros(n)
+    for c in range(3):
+        ec = y[:, 3 + c] - t[:, 2 + c]
+        ldir += np.where(esc, w_d * ec * ec, 0.0)
+        d[:, 3 + c] = np.where(esc, 2 * w_d * ec / n, 0.0)
+    sums = np.array([ce.sum(), lr.sum(), ldir.sum(), correct.sum()])
+    return sums, d
 
 
 def exact_forward(W, b, x):

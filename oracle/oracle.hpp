@@ -270,7 +270,9 @@ inline MetricParams<R> make_params(int metric, bool time_reversed, float mass_kg
 // (metric_kerr.cuh:154-157); 1 for Minkowski (metric_minkowski.cuh:44-47).
 template <class R> inline R length_scale(const MetricParams<R>& p) {
     if (p.metric == kMinkowski) return R(1);
-    return R(kGravitational) * p.mass_kg / (R(kSpeedOfLight) * R(kSpeedOfLight));
+    // evaluated in float for every state type: the FP64 paths promote the float value
+    const float m = float(p.mass_kg);
+    return R(float(kGravitational) * m / (float(kSpeedOfLight) * float(kSpeedOfLight)));
 }
 template <class R> inline R initial_h(const MetricParams<R>& p) { return p.metric == kMinkowski ? R(1000) : R(1); }
 

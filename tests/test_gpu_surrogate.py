@@ -1,0 +1,234 @@
+"""GPU tests of the learned surrogate through the C-ABI: the bf16 MFMA
+kernels against the NumPy oracle, training, data generation, and the
+surrogate render path against the plain RKF45 render."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import magik_amd as M
+from oracle import mlp_oracle as MO
+
+pytestmark = pytest.mark.gpu
+
+A, R_OBS, TH_OBS, FOV = 0.9, 50.0, 80.0, 25.0
+
+
+def random_net(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    W, b = [], []
+    for l, (k, n) in enumerate(M.LAYER_SHAPES):
+        fan = 5 if l == 0 else k
+        w = rng.normal(0, scale * math.sqrt(2.0 / fan), size=(k, n)).astype(np.float32)
+        if l == 0:
+            w[5:, :] = 0
+        if l == len(M.LAYER_SHAPES) - 1:
+            w[:, 8:] = 0
+        W.append(w)
+        bb = rng.normal(0, 0.1, size=n).astype(np.float32)
+        if l == len(M.LAYER_SHAPES) - 1:
+            bb[8:] = 0
+        b.append(bb)
+    return W, b
+
+
+def features(rng, n):
+    x = np.zeros((n, 16), dtype=np.float32)
+    x[:, :5] = rng.uniform(-1, 1, size=(n, 5))
+    return x
+
+
+@pytest.fixture(scope="module")
+def sur():
+    s = M.Surrogate(0, seed=7)
+    yield s
+    s.close()
+
+
+def test_weights_roundtrip(sur):
+    W, b = random_net(1)
+    sur.set_weights(W, b)
+    W2, b2 = sur.get_weights()
+    for l in range(len(W)):
+        np.testing.assert_array_equal(W[l], W2[l])
+        np.testing.assert_array_equal(b[l], b2[l])
+
+
+@pytest.mark.parametrize("n", [1, 127, 1000, 4096])
+def test_fused_inference_matches_oracle(sur, n):
+    import torch
+    W, b = random_net(2)
+    sur.set_weights(W, b)
+    x = features(np.random.default_rng(n), n)
+    y = sur.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    ref = MO.forward(W, b, x)[:, :8]
+    # FP32 accumulation order differs from the oracle's float64: an activation
+    # can round to the neighbouring bf16 value, so compare at bf16 resolution
+    err = np.abs(y - ref)
+    assert err.max() <= 2e-2 * (1 + np.abs(ref).max()), err.max()
+    assert np.median(err) <= 1e-3 * (1 + np.abs(ref).max())
+
+
+def test_gradients_match_oracle(sur):
+    import torch
+    W, b = random_net(3)
+    sur.set_weights(W, b)
+    rng = np.random.default_rng(4)
+    n = 4096 + 96   # several k-split chunks and a ragged tail
+    x = features(rng, n)
+    dy = np.zeros((n, 32), dtype=np.float32)
+    dy[:, :8] = rng.normal(0, 1.0 / n, size=(n, 8))
+    y, (gW, gb) = sur.gradients(torch.from_numpy(x).cuda(), torch.from_numpy(dy).cuda())
+    yr, hs, zs = MO.forward(W, b, x, keep=True)
+    dWr, dbr = MO.backward(W, hs, zs, dy)
+    assert np.abs(y.cpu().numpy() - yr).max() <= 2e-2 * (1 + np.abs(yr).max())
+    for l in range(len(W)):
+        rel = np.linalg.norm(gW[l] - dWr[l]) / max(np.linalg.norm(dWr[l]), 1e-30)
+        assert rel <= 1e-2, (l, rel)
+        relb = np.linalg.norm(gb[l] - dbr[l]) / max(np.linalg.norm(dbr[l]), 1e-30)
+        assert relb <= 1e-2, (l, relb)
+
+
+def test_train_step_loss_and_adam(sur):
+    """One step: the reported losses equal the oracle's, and the weights move
+    by Adam's first step (lr * sign(grad) where the gradient is non-zero)."""
+    import torch
+    W, b = random_net(5)
+    sur.set_weights(W, b)
+    rng = np.random.default_rng(6)
+    n = 2048
+    x = features(rng, n)
+    t = np.zeros((n, 8), dtype=np.float32)
+    t[:, 0] = rng.integers(0, 3, size=n)
+    t[:, 1:5] = rng.uniform(-1, 1, size=(n, 4))
+    fresh = M.Surrogate(0, seed=9)
+    try:
+        fresh.set_weights(W, b)
+        lr = 1e-3
+        losses = fresh.train_step(torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda(), lr, 10.0, 1.0)
+        yr = MO.forward(W, b, x)
+        sums, _ = MO.loss(yr, t, 10.0, 1.0)
+        np.testing.assert_allclose(losses[:3], sums[:3] / n, rtol=2e-2, atol=1e-4)
+        W2, _ = fresh.get_weights()
+        step = W2[2] - W[2]
+        moved = np.abs(step) > 0
+        assert moved.mean() > 0.5
+        # Adam's first step is lr * g / (|g| + eps): at most lr, and ~lr where |g| >> eps
+        assert np.abs(step).max() <= 1.02 * lr
+        assert np.median(np.abs(step[moved])) >= 0.9 * lr
+    finally:
+        fresh.close()
+
+
+def kerr_camera_ranges(width=0.0):
+    return M.surrogate_ranges(spin=(A - width, A + width), r=(R_OBS - 0.5, R_OBS + 0.5), theta=(TH_OBS - 0.5, TH_OBS + 0.5),
+                              screen=12.0, disk_r_max=30.0)
+
+
+def training_scene():
+    return M.scene(celestial_radius_m=1000.0 * M.length_scale(), flags=M.SCENE_DISK, disk_r_in=M.r_isco(A),
+                   disk_r_out=30.0)
+
+
+def test_generate_data_targets_and_initial_conditions(gpu_ctx):
+    import torch
+    s = M.Surrogate(0, seed=11, ranges=kerr_camera_ranges())
+    try:
+        integ = M.integrator(metric=M.KERR, precision=M.PREC_FP64, spin=A)
+        n = 4096
+        rays, x, t, hits = s.generate_data(gpu_ctx, integ, training_scene(), seed=5, n=n)
+        torch.cuda.synchronize()
+        h = M.hits_to_numpy(hits)
+        t = t.cpu().numpy()
+        x = x.cpu().numpy()
+        rays = rays.cpu().numpy()
+        # classes present and mapped: horizon / none / step-limit -> 0, disk -> 1, escape -> 2
+        cls = np.where(h["cls"] == M.HIT_DISK, 1, np.where(h["cls"] == M.HIT_ESCAPE, 2, 0))
+        np.testing.assert_array_equal(t[:, 0], cls)
+        assert {0, 1, 2} <= set(cls.tolist())
+        d = cls == 1
+        lnr = np.log(h["a0"][d].astype(np.float64))
+        mid, half = 0.5 * math.log(30.0), 0.5 * math.log(30.0)
+        np.testing.assert_allclose(t[d, 1], (lnr - mid) / half, atol=1e-5)
+        e = cls == 2
+        np.testing.assert_allclose(t[e, 2], h["a0"][e], atol=1e-6)
+        np.testing.assert_allclose(t[e, 3] ** 2 + t[e, 4] ** 2, 1.0, atol=1e-5)
+        # features inside [-1, 1], padding zero; spin column = the time-reversed metric spin
+        assert np.abs(x[:, :5]).max() <= 1.0 + 1e-6 and np.all(x[:, 5:] == 0)
+        np.testing.assert_allclose(rays[10], -A, atol=1e-6)
+        # the camera sits at the BL point of its (r, theta) and every photon is null
+        from oracle import oracle as OR
+        for i in range(0, n, 512):
+            x_, y_, z_ = rays[0:3, i]
+            rho2 = x_ * x_ + y_ * y_ + z_ * z_ - A * A
+            r_bl = math.sqrt(0.5 * (rho2 + math.sqrt(rho2 * rho2 + 4 * A * A * y_ * y_)))
+            assert abs(r_bl - R_OBS) <= 0.5 + 1e-9
+            g_inv = OR.metric_tensor(1, integ, rays[0:4, i], inverse=True)
+            p = rays[4:8, i]
+            assert abs(p @ g_inv @ p) <= 1e-9 * (p @ p)
+    finally:
+        s.close()
+
+
+@pytest.fixture(scope="module")
+def trained(gpu_ctx):
+    """A surrogate specialised to one Kerr camera, trained briefly on the GPU."""
+    import torch
+    s = M.Surrogate(0, seed=13, ranges=kerr_camera_ranges())
+    integ = M.integrator(metric=M.KERR, precision=M.PREC_FP64, spin=A)
+    n = 1 << 18
+    _, x, t, _ = s.generate_data(gpu_ctx, integ, training_scene(), seed=21, n=n)
+    hist = s.train(x, t, batch=4096, epochs=12, lr=2e-3, lr_final=5e-5)
+    del x, t
+    torch.cuda.empty_cache()
+    yield s, hist
+    s.close()
+
+
+def test_training_reduces_loss(trained):
+    _, hist = trained
+    assert hist[-1, 0] < 0.5 * hist[0, 0], hist
+    assert hist[-1, 3] > 0.9, hist   # class accuracy of the last batch
+
+
+def frame_setup(W=192, H=108):
+    integ, sc, spec, cam, _ = M.workload("kerr_disk_1080p", W=W, H=H, spp=2)
+    return integ, sc, spec, cam
+
+
+def test_surrogate_all_fallback_equals_plain_render(gpu_ctx, trained):
+    """With every path sent to RKF45 the frame is the plain render, bit for bit."""
+    import torch
+    s, _ = trained
+    integ, sc, spec, cam = frame_setup()
+    gpu_ctx.set_scene(sc)
+    plain = gpu_ctx.render_frame(integ, sc, spec, cam, seed=3, spp=2)
+    fb = M.fallback(min_confidence=1.5, judge_stride=0)
+    xyz, st = M.render_frame_surrogate(gpu_ctx, s, integ, sc, spec, cam, seed=3, spp=2, fb=fb)
+    torch.cuda.synchronize()
+    assert st["n_fallback"] == st["n_paths"] == 192 * 108 * 2
+    assert torch.equal(xyz, plain)
+
+
+def test_surrogate_judge_and_gain(gpu_ctx, trained):
+    """What the trained network gains over the method without it: most paths
+    are answered by the network, so the frame needs far fewer RKF45 trials,
+    while the judge sample (traced by RKF45 as well) agrees on the class and
+    bounds the disk-radius and redshift error of the answered paths."""
+    import torch
+    s, _ = trained
+    integ, sc, spec, cam = frame_setup()
+    gpu_ctx.set_scene(sc)
+    gpu_ctx.render_frame(integ, sc, spec, cam, seed=4, spp=2)
+    plain_trials = gpu_ctx.stats().n_trials
+    fb = M.fallback(judge_stride=7)
+    xyz, st = M.render_frame_surrogate(gpu_ctx, s, integ, sc, spec, cam, seed=4, spp=2, fb=fb)
+    torch.cuda.synchronize()
+    n = st["n_paths"]
+    assert st["n_fallback"] < 0.4 * n, st
+    assert st["n_trials"] < 0.5 * plain_trials, (st["n_trials"], plain_trials)
+    assert st["n_judge"] > 0.5 * n / 7, st
+    assert st["judge_class_agree"] >= 0.99 * st["n_judge"], st
+    assert st["judge_n_disk"] > 0 and st["judge_mean_rel_r"] < 0.05 and st["judge_mean_rel_g"] < 0.05, st
+    assert np.isfinite(xyz.cpu().numpy()).all()
