@@ -2,14 +2,23 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kerr_disk_1080p]
 
-A step is one full frame through the C-ABI (mgk_render_frame: ray generation +
-persistent RKF45 integration + spectral shading) of synthetic seeded rays.
-Prints one JSON line: rays/s (completed paths per second of wall time over K
-frames), geodesic steps/s, RKF45 trials/s, the roofline of the integration
-kernel (algorithmic FP32 FLOPs per launch / its HIP-event time on the
-library's stream), the radiance error of the frame's rays against the FP64
-path on a seeded 2^16-ray subset, and the CPU baseline (the oracle's FP32
-restatement of the reference integrator on this host's cores).
+A step is one full frame through the C-ABI of synthetic seeded rays, timed on
+two paths:
+
+  plain      mgk_render_frame: ray generation + persistent RKF45 integration
+             (FP32 bulk, FP64 near zone and controller) + spectral shading
+  surrogate  mgk_render_frame_surrogate: ray generation + fused bf16 MLP
+             inference + decode with the fallback rules + RKF45 for the
+             fallback queue and the judge sample + the same shading; the
+             network is trained first on the GPU (untimed setup, reported)
+
+Each path's per-ray radiance error is measured against the FP64 integrator
+on a seeded 2^16-ray subset of the frame with identical per-ray samples.
+`value` is the rays/s of the fastest path whose error is within 1e-3; the
+JSON line carries both paths, the roofline of the integration kernel
+(algorithmic FP32 FLOPs per frame / its HIP-event time) and of the inference
+kernel (MLP FLOPs / its HIP-event time), and the CPU baseline (the oracle's
+FP32 restatement of the reference integrator on this host's cores).
 """
 from __future__ import annotations
 
@@ -26,34 +35,48 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Kerr rays/s (1080p, a=0.9): MLP surrogate vs plain RKF45 at ≤1e-3 radiance err"
-PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector/matrix peak (vendor figure)
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak (vendor figure, counts packed FP32)
+PEAK_FP32_UNPACKED_TFLOPS = 78.6  # the same without packed FP32 (this build compiles packed FP32 off)
+PEAK_BF16_TFLOPS = 2500.0         # MI355X dense bf16 MFMA peak
+ERROR_BAR = 1e-3
 # Algorithmic operations of one reference RKF45 trial (6 Kerr right-hand sides,
 # stage sums, error norm, controller), counted by oracle/opcount.cpp:
 # 453 add/sub + 921 mul + 40 div + 13 sqrt + 1 pow.
 FLOP_PER_TRIAL = {2: 1428, 1: 1272, 0: 0}
+# MLP forward, 5 -> 4 x 256 SiLU -> 6 outputs: 2 (5*256 + 3*256*256 + 256*6)
+FLOP_PER_INFERENCE = 398848
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def radiance_error(ctx, M, integ, sc, spec, cam, spp, seed, n_sub=1 << 16):
-    """Mean relative per-ray Y error of `integ` against the FP64 path on a
-    seeded subset of the frame's rays, with identical per-ray samples."""
+def subset(n, seed, n_sub=1 << 16):
+    rng = np.random.default_rng(seed ^ 0xE44)
+    return np.sort(rng.choice(n, size=min(n_sub, n), replace=False)).astype(np.int64)
+
+
+def radiance_error(ctx, M, integ, sc, spec, cam, spp, seed, test_hits=None, n_sub=1 << 16):
+    """Mean relative per-ray Y error against the FP64 path on a seeded subset of
+    the frame's rays, with identical per-ray samples.  test_hits: the frame's
+    hit records of the path under test ((n, 4) device tensor); when None the
+    subset is traced with `integ`."""
     import torch
     rays = ctx.generate_rays(integ, cam, seed, 0, spp)           # float64 storage (mixed / FP64)
     n = rays.shape[1]
-    rng = np.random.default_rng(seed ^ 0xE44)
-    q = np.sort(rng.choice(n, size=min(n_sub, n), replace=False)).astype(np.int32)
-    qt = torch.from_numpy(q).to(rays.device)
+    q = subset(n, seed, n_sub)
+    qt = torch.from_numpy(q.astype(np.int32)).to(rays.device)
     out = {}
-    for name, prec in (("test", integ.precision), ("fp64", M.PREC_FP64)):
+    runs = [("fp64", M.PREC_FP64)] + ([("test", integ.precision)] if test_hits is None else [])
+    for name, prec in runs:
         it = M.integrator()
         C.memmove(C.byref(it), C.byref(integ), C.sizeof(integ))
         it.precision = prec
         hits, _ = ctx.trace(it, sc, list(cam.position), rays, n=len(q), queue=qt, want_steps=False)
         out[name] = hits[qt.long()].contiguous()
         del hits
+    if test_hits is not None:
+        out["test"] = test_hits[torch.from_numpy(q).to(test_hits.device)].contiguous()
     side = int(np.sqrt(len(q)))
     assert side * side == len(q)
     vcam = M.abi.Camera()
@@ -67,7 +90,8 @@ def radiance_error(ctx, M, integ, sc, spec, cam, spp, seed, n_sub=1 << 16):
     torch.cuda.empty_cache()
     eps = 1e-3 * max(y["fp64"].mean(), 1e-30)
     err = np.abs(y["test"] - y["fp64"]) / np.maximum(y["fp64"], eps)
-    return float(err.mean()), float(np.mean(cls_t == cls_r)), len(q)
+    return {"mean_rel_Y": float(err.mean()), "p99_rel_Y": float(np.quantile(err, 0.99)),
+            "class_agreement": float(np.mean(cls_t == cls_r)), "rays": len(q)}
 
 
 def cpu_baseline(M, integ, sc, cam, spp, seed, budget_s=12.0):
@@ -88,12 +112,28 @@ def cpu_baseline(M, integ, sc, cam, spp, seed, budget_s=12.0):
         return time.perf_counter() - t, int(steps.sum()), int(trials.sum())
 
     dt, _, _ = run(1 << 14)
-    n = int(min(1 << 22, max(1 << 16, (1 << 14) * budget_s / max(dt, 1e-3))))
+    n = int(min(1 << 25, max(1 << 16, (1 << 14) * budget_s / max(dt, 1e-3))))
     dt, steps, trials = run(n)
     return {"value": n / dt, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"{n} seeded rays of the same 1920x1080x32 Kerr frame, FP32 reference integrator "
                       f"(oracle/oracle.hpp) with OpenMP over rays, {dt:.1f} s",
             "steps_per_s": steps / dt, "trials_per_s": trials / dt}
+
+
+def time_frames(render, steps, warmup, sync_stats=None):
+    import torch
+    for i in range(warmup):
+        render(i)
+        if sync_stats:
+            sync_stats()
+        log(f"  warmup {i} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per = []
+    for i in range(steps):
+        per.append(render(1000 + i))
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, per
 
 
 def main():
@@ -106,7 +146,22 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--train-rays", type=int, default=1 << 24)
+    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a copy kernel over BYTES three times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)   # an elementwise kernel: reads and writes BYTES each
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 3}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every GPU box here has one GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -123,61 +178,165 @@ def main():
     ctx.set_scene(sc)
     out = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
     log(f"bench: {args.workload} {W}x{H}x{spp} = {n_paths} paths, precision {args.precision}")
-    for i in range(args.warmup):
-        ctx.render_frame(integ, sc, spec, cam, args.seed, spp, out)
-        ctx.stats()
-        log(f"warmup {i} done")
-    torch.cuda.synchronize()
-    kern_ms, steps, trials, fp64 = 0.0, 0, 0, 0
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ctx.render_frame(integ, sc, spec, cam, args.seed + i, spp, out)
-        st = ctx.stats()                      # waits for this frame's integration kernel
-        kern_ms += st.ms
-        steps += st.n_steps
-        trials += st.n_trials
-        fp64 += st.n_trials_fp64
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    log(f"timed {args.steps} frames in {wall:.3f} s")
-    assert torch.isfinite(out).all()
 
-    flop = FLOP_PER_TRIAL[integ.metric_type] * trials / args.steps
-    achieved = flop / (kern_ms / args.steps * 1e-3) / 1e12
+    # ---- plain RKF45 path ----
+    acc = {"ms": 0.0, "steps": 0, "trials": 0, "fp64": 0}
+
+    def plain_frame(seed_off):
+        ctx.render_frame(integ, sc, spec, cam, args.seed + seed_off, spp, out)
+        st = ctx.stats()                      # waits for this frame's integration
+        return st
+
+    log("plain RKF45 frames")
+    wall, per = time_frames(plain_frame, args.steps, args.warmup)
+    for st in per:
+        acc["ms"] += st.ms
+        acc["steps"] += st.n_steps
+        acc["trials"] += st.n_trials
+        acc["fp64"] += st.n_trials_fp64
+    assert torch.isfinite(out).all()
+    plain_rate = n_paths * args.steps / wall
+    log(f"  {args.steps} frames in {wall:.3f} s: {plain_rate:.4e} rays/s")
+    flop = FLOP_PER_TRIAL[integ.metric_type] * acc["trials"] / args.steps
+    trace_ms = acc["ms"] / args.steps
+    achieved = flop / (trace_ms * 1e-3) / 1e12
+    plain = {"rays_per_s": plain_rate, "ms_per_frame": wall / args.steps * 1e3,
+             "geodesic_steps_per_s": acc["steps"] / wall, "rkf45_trials_per_s": acc["trials"] / wall,
+             "fp64_trial_fraction": acc["fp64"] / max(acc["trials"], 1), "trace_ms_per_frame": trace_ms}
+    if not args.no_error:
+        plain["radiance_err_vs_fp64"] = radiance_error(ctx, M, integ, sc, spec, cam, spp, args.seed)
+        log(f"  radiance error {plain['radiance_err_vs_fp64']}")
+
+    # ---- surrogate path ----
+    sur_res, mlp_roof = None, None
+    if not args.no_surrogate and args.workload == "kerr_disk_1080p" and prec != M.PREC_FP32_REFERENCE:
+        sur_res, mlp_roof = bench_surrogate(M, ctx, args, integ, sc, spec, cam, spp, n_paths, out)
+
+    if sur_res:
+        for v in sur_res["configs"].values():
+            v["speedup_vs_plain"] = v["rays_per_s"] / plain["rays_per_s"]
+
+    # ---- headline: the fastest path within the error bar ----
+    cands = [("plain_rkf45", plain["rays_per_s"], plain.get("radiance_err_vs_fp64", {}).get("mean_rel_Y"))]
+    if sur_res:
+        for k, v in sur_res["configs"].items():
+            cands.append((f"surrogate_{k}", v["rays_per_s"], v.get("radiance_err_vs_fp64", {}).get("mean_rel_Y")))
+    ok = [c for c in cands if c[2] is not None and c[2] <= ERROR_BAR] or cands[:1]
+    best = max(ok, key=lambda c: c[1])
+    value = best[1]
     res = {
         "metric": METRIC,
-        "value": n_paths * args.steps / wall,
+        "value": value,
         "unit": "rays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": n_paths / value * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if prec != M.PREC_FP64 else "f64",
-        "data": "synthetic (seeded Philox rays, Kerr a=0.9 disk + 1e5-star field scene)",
+        "data": "synthetic (seeded Philox rays, Kerr a=0.9 disk + 1e5-star field scene; surrogate trained on "
+                "seeded FP64-traced rays)",
         "config": {"workload": f"{args.workload}: Kerr a=0.9, r_obs=50M, theta=80 deg, FOV 25 deg, "
                                f"{W}x{H}x{spp} spp, 4 hero wavelengths, tol 1e-7, cap 1e4",
-                   "path": f"plain RKF45 ({args.precision})", "paths_per_frame": n_paths},
-        "geodesic_steps_per_s": steps / wall,
-        "rkf45_trials_per_s": trials / wall,
-        "fp64_trial_fraction": fp64 / max(trials, 1),
-        "trace_kernel_ms": kern_ms / args.steps,
+                   "paths_per_frame": n_paths, "value_path": best[0], "error_bar": ERROR_BAR},
+        "plain": plain,
+        "surrogate": sur_res,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "note": "integration kernel: FP32 vector ALU (no MFMA on this path); algorithmic FLOPs = "
-                             f"{FLOP_PER_TRIAL[integ.metric_type]} per RKF45 trial x trials per launch"},
+                     "frac": achieved / PEAK_FP32_TFLOPS, "frac_of_unpacked_peak": achieved / PEAK_FP32_UNPACKED_TFLOPS,
+                     "traffic": None, "kernel": "k_trace_zone (far + near passes of one frame)",
+                     "note": "integration kernel of the plain path: FP32 vector ALU (no MFMA on this path); "
+                             f"algorithmic FLOPs = {FLOP_PER_TRIAL[integ.metric_type]} per RKF45 trial x trials "
+                             "per frame / HIP-event time of the frame's integration on the library stream"},
+        "roofline_mlp": mlp_roof,
     }
-    if not args.no_error:
-        e, agree, n = radiance_error(ctx, M, integ, sc, spec, cam, spp, args.seed)
-        res["radiance_err_vs_fp64"] = {"mean_rel_Y": e, "class_agreement": agree, "rays": n}
-        log(f"radiance error {e:.3e}, class agreement {agree:.5f}")
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(M, integ, sc, cam, spp, args.seed)
         log(f"cpu baseline {res['cpu_baseline']['value']:.3e} rays/s")
     ctx.close()
     print(json.dumps(res), flush=True)
+
+
+def bench_surrogate(M, ctx, args, integ, sc, spec, cam, spp, n_paths, out):
+    import torch
+    log("surrogate: training data + training (untimed setup)")
+    ranges = M.surrogate_ranges(spin=(0.0, 0.99), r=(20.0, 100.0), theta=(10.0, 89.0), screen=12.0,
+                                disk_r_max=float(sc.disk_r_out))
+    sur = M.Surrogate(0, seed=args.seed, ranges=ranges)
+    fp64 = M.integrator(metric=M.KERR, precision=M.PREC_FP64, spin=integ.spin)
+    tsc = M.scene(celestial_radius_m=sc.celestial_radius_m, flags=M.SCENE_DISK, disk_r_in=sc.disk_r_in,
+                  disk_r_out=sc.disk_r_out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, x, t, _ = sur.generate_data(ctx, fp64, tsc, seed=args.seed + 17, n=args.train_rays)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    hist = sur.train(x, t, batch=8192, epochs=args.epochs, lr=2e-3, lr_final=2e-5)
+    t_train = time.perf_counter() - t0
+    del x, t
+    torch.cuda.empty_cache()
+    log(f"  data {t_data:.2f} s, training {t_train:.2f} s, last epoch {hist[-1].tolist()}")
+    res = {"training": {"rays": args.train_rays, "epochs": args.epochs, "data_s": t_data, "train_s": t_train,
+                        "samples_per_s": args.train_rays * args.epochs / t_train,
+                        "ranges": "a in [0, 0.99], r_obs in [20, 100] M, theta in [10, 89] deg, |alpha|,|beta| <= 12 M",
+                        "last_epoch": {"ce": float(hist[-1, 0]), "disk_r": float(hist[-1, 1]),
+                                       "dir": float(hist[-1, 2]), "accuracy": float(hist[-1, 3])}},
+           "configs": {}}
+    hits = torch.zeros((n_paths, 4), dtype=torch.int32, device="cuda:0")
+    rules = {"default": M.fallback(),
+             "trace_disk": M.fallback(trace_disk=True)}
+    for name, fb in rules.items():
+        log(f"surrogate frames ({name})")
+        stats = []
+
+        def frame(seed_off):
+            _, st = M.render_frame_surrogate(ctx, sur, integ, sc, spec, cam, args.seed + seed_off, spp, fb=fb,
+                                             out=out, stats=True)
+            stats.append(st)
+            return st
+
+        wall, per = time_frames(frame, args.steps, args.warmup)
+        rate = n_paths * args.steps / wall
+        st = per[-1]
+        cfg = {"rays_per_s": rate, "ms_per_frame": wall / args.steps * 1e3, "speedup_vs_plain": None,
+               "fallback_fraction": st["n_fallback"] / st["n_paths"],
+               "stage_ms": {k: st[k] for k in ("ms_raygen", "ms_mlp", "ms_trace", "ms_shade", "ms_total")},
+               "stats": st}
+        log(f"  {rate:.4e} rays/s, fallback {cfg['fallback_fraction']:.4f}, stages {cfg['stage_ms']}")
+        if not args.no_error:
+            M.render_frame_surrogate(ctx, sur, integ, sc, spec, cam, args.seed, spp, fb=fb, out=out, hits=hits,
+                                     stats=False)
+            cfg["radiance_err_vs_fp64"] = radiance_error(ctx, M, integ, sc, spec, cam, spp, args.seed, test_hits=hits)
+            log(f"  radiance error {cfg['radiance_err_vs_fp64']}")
+        res["configs"][name] = cfg
+    del hits
+    torch.cuda.empty_cache()
+    # inference kernel alone: the fused MLP over one frame's worth of feature rows
+    xf = torch.zeros((n_paths, 16), dtype=torch.float32, device="cuda:0")
+    xf[:, :5].uniform_(-1.0, 1.0)
+    yf = sur.forward(xf)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 3
+    e0.record()
+    for _ in range(reps):
+        sur.forward(xf)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    achieved = FLOP_PER_INFERENCE * n_paths / (ms * 1e-3) / 1e12
+    mlp_roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_mlp_infer",
+                "ms_per_frame": ms,
+                "note": f"{FLOP_PER_INFERENCE} FLOP per path (5 -> 4x256 SiLU -> 6) x {n_paths} paths / "
+                        "HIP-event time of the kernel on the launch stream"}
+    log(f"  k_mlp_infer {ms:.2f} ms per frame, {achieved:.1f} TFLOP/s")
+    del xf, yf
+    sur.close()
+    torch.cuda.empty_cache()
+    return res, mlp_roof
 
 
 if __name__ == "__main__":

@@ -376,12 +376,15 @@ MGK_API e_magik_result_types mgk_surrogate_generate_data(mgk_context_handle ctx,
                                                          const mgk_scene_t* scene, uint64_t seed, uint64_t n, double* d_rays,
                                                          float* d_x, float* d_t, mgk_hit_t* d_hits, void* stream);
 /* Whole frame with the surrogate: MLP inference for every path, RKF45 (the
- * integ settings, mixed precision) for the paths the fallback rules select
- * and for the judge sample, then the same spectral shading. */
+ * integ settings) for the paths the fallback rules select and for the judge
+ * sample, then the same spectral shading.  The camera must be a
+ * Boyer-Lindquist camera (mgk_camera_boyer_lindquist) inside the trained
+ * ranges, the scene's disk the trained one (r_ISCO .. disk_r_max).  d_hits
+ * (optional, W*H*spp records, sample-major) receives every path's hit. */
 MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, mgk_surrogate_handle s, const mgk_integrator_t* integ,
                                                         const mgk_scene_t* scene, const mgk_spectral_t* spec, const mgk_camera_t* cam,
                                                         uint64_t seed, uint32_t spp, const mgk_fallback_t* fb, float* d_xyz,
-                                                        mgk_surrogate_stats_t* stats, void* stream);
+                                                        mgk_hit_t* d_hits, mgk_surrogate_stats_t* stats, void* stream);
 
 #ifdef __cplusplus
 }

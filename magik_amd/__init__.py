@@ -354,14 +354,18 @@ def fallback(min_confidence=0.9, critical_margin=0.05, direction_margin=2e-3, di
                         judge_stride=judge_stride, trace_disk=int(trace_disk))
 
 
-def render_frame_surrogate(ctx: Context, sur: Surrogate, integ, sc, spec, cam, seed: int, spp: int, fb=None, out=None):
-    """Whole frame through the surrogate with RKF45 fallback; returns (xyz, stats dict)."""
+def render_frame_surrogate(ctx: Context, sur: Surrogate, integ, sc, spec, cam, seed: int, spp: int, fb=None, out=None,
+                           hits=None, stats: bool = True):
+    """Whole frame through the surrogate with RKF45 fallback; returns (xyz, stats dict or None).
+    `hits` (optional (W*H*spp, 4) int32 device tensor) receives every path's hit record."""
     import torch
     if out is None:
         out = torch.zeros((cam.y_resolution, cam.x_resolution, 3), dtype=torch.float32, device=f"cuda:{ctx.device}")
     st = abi.SurrogateStats()
     fb = fb or fallback()
     abi.check(ctx.lib.mgk_render_frame_surrogate(ctx.h, sur.h, C.byref(integ), C.byref(sc), C.byref(spec), C.byref(cam),
-                                                 seed, spp, C.byref(fb), C.c_void_p(out.data_ptr()), C.byref(st),
-                                                 Context._stream()), "mgk_render_frame_surrogate")
-    return out, st.as_dict()
+                                                 seed, spp, C.byref(fb), C.c_void_p(out.data_ptr()),
+                                                 C.c_void_p(hits.data_ptr() if hits is not None else None),
+                                                 C.byref(st) if stats else None, Context._stream()),
+              "mgk_render_frame_surrogate")
+    return out, (st.as_dict() if stats else None)

@@ -162,7 +162,7 @@ SIGNATURES = {
     "mgk_surrogate_generate_data": (_R, [_P, _P, C.POINTER(Integrator), C.POINTER(Scene), C.c_uint64, C.c_uint64,
                                          _P, _P, _P, _P, _P]),
     "mgk_render_frame_surrogate": (_R, [_P, _P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(Spectral),
-                                        C.POINTER(Camera), C.c_uint64, C.c_uint32, C.POINTER(Fallback), _P,
+                                        C.POINTER(Camera), C.c_uint64, C.c_uint32, C.POINTER(Fallback), _P, _P,
                                         C.POINTER(SurrogateStats), _P]),
 }
 
@@ -175,6 +175,13 @@ def load():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        # One HIP runtime per process: when torch (which carries its own HIP
+        # and HSA runtime) is present, load it first so that the library binds
+        # to that runtime instead of starting a second one beside it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             if not hasattr(lib, name):

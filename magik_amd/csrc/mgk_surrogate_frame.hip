@@ -180,7 +180,6 @@ __global__ void __launch_bounds__(256) k_sur_decode(DecodeParams P) {
                 }
             }
         }
-        if (fallback) atomicAdd(&P.cnt[reason], 1ull);
         if (P.judge_stride && (i % P.judge_stride) == 0) {
             mgk_hit_t pr = h;
             if (fallback) pr.cls = kNotAnswered;
@@ -188,6 +187,11 @@ __global__ void __launch_bounds__(256) k_sur_decode(DecodeParams P) {
             fallback = true;
         }
         if (!fallback) P.hits[i] = h;
+    }
+    // fallback reasons: one atomic per wave and reason
+    for (int c = C_CONF; c <= C_DOMAIN; ++c) {
+        const uint64_t m = __ballot(reason == c);
+        if (m != 0 && lane == uint32_t(__ffsll((unsigned long long)m) - 1)) atomicAdd(&P.cnt[c], (unsigned long long)__popcll(m));
     }
     // compacted queue of the paths to trace
     const uint64_t fb = __ballot(live && fallback);
@@ -353,7 +357,7 @@ extern "C" {
 MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, mgk_surrogate_handle s, const mgk_integrator_t* integ,
                                                         const mgk_scene_t* scene, const mgk_spectral_t* spec, const mgk_camera_t* cam,
                                                         uint64_t seed, uint32_t spp, const mgk_fallback_t* fb, float* d_xyz,
-                                                        mgk_surrogate_stats_t* stats, void* stream) {
+                                                        mgk_hit_t* d_hits, mgk_surrogate_stats_t* stats, void* stream) {
     if (!ctx || !s || !integ || !scene || !spec || !cam || !fb || !d_xyz) return MAGIK_ERROR_INVALID_POINTER;
     if (integ->metric_type != KERR || integ->precision == MGK_PRECISION_FP32_REFERENCE || spp == 0) return MGK_ERROR_INVALID_ARGUMENT;
     if (cam->observer_velocity[0] != 0.0f || cam->observer_velocity[1] != 0.0f || cam->observer_velocity[2] != 0.0f)
@@ -398,7 +402,8 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     HIP_TRY(s->fr_rays.reserve(n * 10 * sizeof(double)));
     HIP_TRY(s->fr_x.reserve(n * kMlpInPad * sizeof(float)));
     HIP_TRY(s->fr_y.reserve(n * kMlpOut * sizeof(float)));
-    HIP_TRY(s->fr_hits.reserve(n * sizeof(mgk_hit_t)));
+    if (!d_hits) HIP_TRY(s->fr_hits.reserve(n * sizeof(mgk_hit_t)));
+    mgk_hit_t* hits = d_hits ? d_hits : s->fr_hits.as<mgk_hit_t>();
     HIP_TRY(s->fr_queue.reserve(n * sizeof(uint32_t)));
     HIP_TRY(s->fr_pred.reserve(std::max<uint64_t>(1, n_slots) * sizeof(mgk_hit_t)));
     HIP_TRY(s->fr_counters.reserve(C_N * sizeof(unsigned long long)));
@@ -437,7 +442,7 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.X = s->fr_x.as<const float>();
     D.Y = s->fr_y.as<const float>();
     D.rays = s->fr_rays.as<const double>();
-    D.hits = s->fr_hits.as<mgk_hit_t>();
+    D.hits = hits;
     D.queue = s->fr_queue.as<uint32_t>();
     D.pred = s->fr_pred.as<mgk_hit_t>();
     D.cnt = cnt;
@@ -467,15 +472,15 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     HIP_TRY(hipMemcpyAsync(&nq, cnt + C_QUEUE, sizeof(nq), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (nq > 0 && (r = mgk_trace_rays(ctx, integ, scene, cam->position, s->fr_rays.ptr, n, nq, s->fr_queue.as<uint32_t>(),
-                                      s->fr_hits.as<mgk_hit_t>(), nullptr, st)) != MAGIK_SUCCESS)
+                                      hits, nullptr, st)) != MAGIK_SUCCESS)
         return r;
     if (n_slots) {
         hipLaunchKernelGGL(k_sur_judge, dim3(unsigned((n_slots + 255) / 256)), dim3(256), 0, st, s->fr_pred.as<const mgk_hit_t>(),
-                           s->fr_hits.as<const mgk_hit_t>(), n_slots, fb->judge_stride, cnt);
+                           hits, n_slots, fb->judge_stride, cnt);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(s->ev[3], st));
-    if ((r = mgk_shade(ctx, scene, spec, cam, seed, 0, spp, s->fr_hits.as<mgk_hit_t>(), d_xyz, st)) != MAGIK_SUCCESS) return r;
+    if ((r = mgk_shade(ctx, scene, spec, cam, seed, 0, spp, hits, d_xyz, st)) != MAGIK_SUCCESS) return r;
     HIP_TRY(hipEventRecord(s->ev[4], st));
 
     if (stats) {
