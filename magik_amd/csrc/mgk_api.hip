@@ -82,7 +82,7 @@ struct mgk_context {
     // frame buffers
     DeviceBuffer rays, hits;
     // zone-split integration: parked states and the two hand-off queues
-    DeviceBuffer zone_state, zone_steps, zone_q0, zone_q1;
+    DeviceBuffer zone_state[2], zone_steps[2], zone_q0, zone_q1;
     int last_passes = 0;
 };
 
@@ -182,11 +182,13 @@ int trace_grid(mgk_context* c, const TraceParams& P) {
 // Mixed precision: alternate the far-zone (FP32) and near-zone (FP64) passes
 // until no ray is handed over.  Reads one queue count per pass (a 8-byte copy).
 e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t s) {
-    const uint64_t stride = P.stride;
-    HIP_TRY(c->zone_state.reserve(stride * 9 * sizeof(double)));
-    HIP_TRY(c->zone_steps.reserve(stride * sizeof(uint32_t)));
-    HIP_TRY(c->zone_q0.reserve(std::max<uint64_t>(P.n, 1) * sizeof(uint32_t)));
-    HIP_TRY(c->zone_q1.reserve(std::max<uint64_t>(P.n, 1) * sizeof(uint32_t)));
+    const uint64_t cap = std::max<uint64_t>(P.n, 1);   // a pass hands off at most the rays it was given
+    for (int k = 0; k < 2; ++k) {
+        HIP_TRY(c->zone_state[k].reserve(cap * 9 * sizeof(double)));
+        HIP_TRY(c->zone_steps[k].reserve(cap * sizeof(uint32_t)));
+    }
+    HIP_TRY(c->zone_q0.reserve(cap * sizeof(uint32_t)));
+    HIP_TRY(c->zone_q1.reserve(cap * sizeof(uint32_t)));
     uint32_t* q[2] = {reinterpret_cast<uint32_t*>(c->zone_q0.ptr), reinterpret_cast<uint32_t*>(c->zone_q1.ptr)};
     unsigned long long* cnt[2] = {c->d_counters + 4, c->d_counters + 5};
     const uint32_t* caller_queue = P.queue;
@@ -203,8 +205,12 @@ e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t 
         Z.n_in = n_in;
         Z.out_queue = q[pass & 1];
         Z.out_count = cnt[pass & 1];
-        Z.state = reinterpret_cast<double*>(c->zone_state.ptr);
-        Z.state_steps = reinterpret_cast<uint32_t*>(c->zone_steps.ptr);
+        // pass p parks into buffer p % 2 and reads what pass p - 1 parked
+        Z.cap = cap;
+        Z.state_out = reinterpret_cast<double*>(c->zone_state[pass & 1].ptr);
+        Z.steps_out = reinterpret_cast<uint32_t*>(c->zone_steps[pass & 1].ptr);
+        Z.state_in = reinterpret_cast<const double*>(c->zone_state[(pass + 1) & 1].ptr);
+        Z.steps_in = reinterpret_cast<const uint32_t*>(c->zone_steps[(pass + 1) & 1].ptr);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c->d_counters, 1);   // head
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, cnt[pass & 1], 1);
         HIP_TRY(hipGetLastError());
@@ -585,8 +591,10 @@ MGK_API e_magik_result_types mgk_context_destroy(mgk_context_handle c) {
     c->cell_items.release();
     c->rays.release();
     c->hits.release();
-    c->zone_state.release();
-    c->zone_steps.release();
+    for (int k = 0; k < 2; ++k) {
+        c->zone_state[k].release();
+        c->zone_steps[k].release();
+    }
     c->zone_q0.release();
     c->zone_q1.release();
     delete c;

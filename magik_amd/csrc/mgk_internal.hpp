@@ -61,8 +61,11 @@ struct ZoneParams {
     uint64_t n_in;
     uint32_t* out_queue;             // rays handed to the other zone
     unsigned long long* out_count;
-    double* state;                   // parked states, 9 x stride doubles (q, p, h)
-    uint32_t* state_steps;           // parked step counts
+    uint64_t cap;                    // capacity of the queues and the parked-state arrays
+    const double* state_in;          // states parked by the previous pass, 9 x cap doubles (q, p, h), at the queue slot
+    const uint32_t* steps_in;        // their step counts
+    double* state_out;               // states parked by this pass, at their out-queue slot
+    uint32_t* steps_out;
 };
 
 struct ShadeParams {

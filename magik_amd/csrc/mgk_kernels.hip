@@ -571,14 +571,15 @@ __global__ void __launch_bounds__(256, 2) k_trace_zone(TraceParams P, ZoneParams
             if (int(lane) == leader) base = atomicAdd(Z.out_count, (unsigned long long)__popcll(ho));
             base = __shfl(base, leader);
             if (handoff) {
+                // parked at the out-queue slot: the next pass reads it back contiguously
                 const uint64_t pos = base + uint64_t(__popcll(ho & lt_mask));
                 Z.out_queue[pos] = uint32_t(ray);
-                double* st = Z.state;
-                const uint64_t sd = P.stride;
-                st[0 * sd + ray] = y.qx; st[1 * sd + ray] = y.qy; st[2 * sd + ray] = y.qz; st[3 * sd + ray] = y.qt;
-                st[4 * sd + ray] = y.px; st[5 * sd + ray] = y.py; st[6 * sd + ray] = y.pz; st[7 * sd + ray] = y.pt;
-                st[8 * sd + ray] = h;
-                Z.state_steps[ray] = n_step;
+                double* st = Z.state_out;
+                const uint64_t sd = Z.cap;
+                st[0 * sd + pos] = y.qx; st[1 * sd + pos] = y.qy; st[2 * sd + pos] = y.qz; st[3 * sd + pos] = y.qt;
+                st[4 * sd + pos] = y.px; st[5 * sd + pos] = y.py; st[6 * sd + pos] = y.pz; st[7 * sd + pos] = y.pt;
+                st[8 * sd + pos] = h;
+                Z.steps_out[pos] = n_step;
                 handoff = false;
                 active = false;
             }
@@ -602,11 +603,13 @@ __global__ void __launch_bounds__(256, 2) k_trace_zone(TraceParams P, ZoneParams
                 if (rank < take) {
                     const uint64_t slot = chunk_next + rank;
                     ray = Z.in_queue ? uint64_t(Z.in_queue[slot]) : slot;
-                    const double* src = Z.from_ic ? reinterpret_cast<const double*>(P.rays) : Z.state;
-                    y.qx = src[0 * P.stride + ray]; y.qy = src[1 * P.stride + ray]; y.qz = src[2 * P.stride + ray];
-                    y.qt = src[3 * P.stride + ray]; y.px = src[4 * P.stride + ray]; y.py = src[5 * P.stride + ray];
-                    y.pz = src[6 * P.stride + ray]; y.pt = src[7 * P.stride + ray]; h = src[8 * P.stride + ray];
-                    n_step = Z.from_ic ? 0u : Z.state_steps[ray];
+                    // initial conditions by ray id; parked states by queue slot
+                    const double* src = Z.from_ic ? reinterpret_cast<const double*>(P.rays) : Z.state_in;
+                    const uint64_t sd = Z.from_ic ? P.stride : Z.cap, at = Z.from_ic ? ray : slot;
+                    y.qx = src[0 * sd + at]; y.qy = src[1 * sd + at]; y.qz = src[2 * sd + at];
+                    y.qt = src[3 * sd + at]; y.px = src[4 * sd + at]; y.py = src[5 * sd + at];
+                    y.pz = src[6 * sd + at]; y.pt = src[7 * sd + at]; h = src[8 * sd + at];
+                    n_step = Z.from_ic ? 0u : Z.steps_in[slot];
                     contraction = load_c<S>(P.rays, P.stride, 9, ray);
                     if (P.per_ray) {
                         lane_constants<METRIC, S>(P, METRIC == KERR ? load_c<S>(P.rays, P.stride, 10, ray) : S(0), ln);

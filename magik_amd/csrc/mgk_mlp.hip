@@ -7,9 +7,13 @@
 //                    fused epilogues (bias + SiLU -> z, h; SiLU' -> dz, g).
 //                    Used for the training forward, backward-data and
 //                    weight-gradient products.
-//   k_mlp_infer    : the whole network for 128 rays per workgroup; the
-//                    activations never leave LDS, the weights stream from
-//                    L2 in MFMA fragment order (one 16-byte load per lane).
+//   k_mlp_infer    : the whole network for 256 rays per workgroup, computed
+//                    transposed (Z^T = W^T H^T: weights are the MFMA A
+//                    operand, rays the columns) so that a layer's FP32
+//                    accumulators, after bias + SiLU, are already the next
+//                    layer's bf16 B operand in registers; the hidden layers'
+//                    weights stream through a double-buffered LDS stage
+//                    shared by the workgroup's four waves.
 //   k_loss         : class cross-entropy + masked regressions -> dL/dy.
 //   k_adam         : Adam on the FP32 master weights, refreshes the bf16 copy.
 #include "mgk_internal.hpp"
@@ -115,15 +119,28 @@ __global__ void k_to_bf16(const float* x, __bf16* y, int64_t n) {
     for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) y[i] = __bf16(x[i]);
 }
 
-// Repack a row-major bf16 [K][N] weight into MFMA B-fragment order:
-// frag[((kstep * (N/32) + ntile) * 64 + lane) * 8 + j] = W[16 kstep + frag_k(lane, j)][32 ntile + lane % 32].
-__global__ void k_repack(const __bf16* W, int K, int N, __bf16* F) {
+// Input feature of k-slot (g, j) of k-step ks in the transposed inference:
+// the accumulator of output tile nt = ks / 2 holds, in lane group g, the
+// features 32 nt + 8 (i / 4) + 4 g + i % 4 (i = 0..15); k-step ks takes
+// i = 8 (ks % 2) + j of them as its k-slots 8 g + j.
+__device__ __forceinline__ int perm_k(int ks, int g, int j) {
+    return 32 * (ks >> 1) + 16 * (ks & 1) + 8 * (j >> 2) + 4 * g + (j & 3);
+}
+
+// Repack a row-major bf16 [K][N] weight into MFMA A-fragment order of the
+// transposed product (A = W^T): fragment (ks, nt) holds, for lane l and
+// j = 0..7, W[k][32 nt + l % 32] with k = 16 ks + 8 (l / 32) + j (natural,
+// the input layer) or k = perm_k(ks, l / 32, j) (layers fed by a previous
+// layer's accumulators).
+// F[((ks * (N / 32) + nt) * 64 + lane) * 8 + j].
+__global__ void k_repack(const __bf16* W, int K, int N, __bf16* F, int permuted) {
     const int64_t total = int64_t(K) * N;
     for (int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < total; o += int64_t(gridDim.x) * blockDim.x) {
         const int j = int(o & 7), lane = int((o >> 3) & 63);
         const int64_t t = o >> 9;
         const int nt = int(t % (N / 32)), ks = int(t / (N / 32));
-        F[o] = W[int64_t(16 * ks + frag_k(lane, j)) * N + 32 * nt + frag_rc(lane)];
+        const int k = permuted ? perm_k(ks, lane >> 5, j) : 16 * ks + 8 * (lane >> 5) + j;
+        F[o] = W[int64_t(k) * N + 32 * nt + (lane & 31)];
     }
 }
 
@@ -190,68 +207,132 @@ __global__ void k_adam(float* w, float* m, float* v, const float* g, int64_t n, 
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: 4 waves x 32 rays per workgroup.
+// Fused inference: 4 waves x 2 ray tiles x 32 rays per workgroup.
 // ---------------------------------------------------------------------------
-constexpr int kInferRays = 128;
+constexpr int kInferWaves = 4;
+constexpr int kInferTiles = 2;                            // 32-ray tiles per wave
+constexpr int kInferRays = kInferWaves * kInferTiles * 32;
+constexpr int kStepFrags = kMlpHidden / 32;               // A fragments per hidden k-step
 
-__global__ void __launch_bounds__(256) k_mlp_infer(InferArgs a) {
-    __shared__ __bf16 act[4][32][kMlpHidden + 8];   // +8: spread rows over LDS banks
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t r0 = int64_t(blockIdx.x) * kInferRays + w * 32;
-    __bf16(*A)[kMlpHidden + 8] = act[w];
-    // layer 1: K = 16 (one k-step), features straight from global
-    {
-        bf16x8 x;
-        const int64_t row = r0 + frag_rc(lane);
-        for (int j = 0; j < 8; ++j) {
-            const int k = frag_k(lane, j);
-            x[j] = __bf16(row < a.n ? a.X[row * kMlpInPad + k] : 0.0f);
-        }
-        const bf16x8* F = reinterpret_cast<const bf16x8*>(a.Wf[0]);
-        for (int nt = 0; nt < kMlpHidden / 32; ++nt) {
-            f32x16 c;
-            for (int i = 0; i < 16; ++i) c[i] = 0.0f;
-            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, F[nt * 64 + lane], c, 0, 0, 0);
-            const int col = nt * 32 + frag_rc(lane);
-            const float b = a.b[0][col];
-            for (int i = 0; i < 16; ++i) A[acc_row(lane, i)][col] = __bf16(silu(c[i] + b));
-        }
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+
+// bias + SiLU of one layer's accumulators into the next layer's B operand:
+// H[c][2 nt + h] holds elements i = 8 h .. 8 h + 7 of accumulator tile nt.
+__device__ __forceinline__ void infer_epilogue(const f32x16 (&acc)[kInferTiles][kStepFrags], const float* bias, int g,
+                                               bf16x8 (&H)[kInferTiles][2 * kStepFrags]) {
+#pragma unroll
+    for (int nt = 0; nt < kStepFrags; ++nt) {
+        float b[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) b[i] = bias[32 * nt + 8 * (i >> 2) + 4 * g + (i & 3)];
+#pragma unroll
+        for (int c = 0; c < kInferTiles; ++c)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) H[c][2 * nt + (i >> 3)][i & 7] = __bf16(silu_fast(acc[c][nt][i] + b[i]));
     }
-    // hidden layers 2..4: K = N = 256
-    for (int l = 1; l < kMlpLayers - 1; ++l) {
-        const bf16x8* F = reinterpret_cast<const bf16x8*>(a.Wf[l]);
-        f32x16 c[kMlpHidden / 32];
-        for (int nt = 0; nt < kMlpHidden / 32; ++nt)
-            for (int i = 0; i < 16; ++i) c[nt][i] = 0.0f;
-        for (int ks = 0; ks < kMlpHidden / 16; ++ks) {
-            bf16x8 x;
-            for (int j = 0; j < 8; ++j) x[j] = A[frag_rc(lane)][16 * ks + frag_k(lane, j)];
-            for (int nt = 0; nt < kMlpHidden / 32; ++nt)
-                c[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, F[(ks * (kMlpHidden / 32) + nt) * 64 + lane], c[nt], 0, 0, 0);
-        }
-        for (int nt = 0; nt < kMlpHidden / 32; ++nt) {
-            const int col = nt * 32 + frag_rc(lane);
-            const float b = a.b[l][col];
-            for (int i = 0; i < 16; ++i) A[acc_row(lane, i)][col] = __bf16(silu(c[nt][i] + b));
-        }
-    }
-    // output layer: N = 32 (8 used)
+}
+
+__global__ void __launch_bounds__(256, 1) k_mlp_infer(InferArgs a) {
+    __shared__ bf16x8 wbuf[2][kStepFrags * 64];          // one hidden k-step, double buffered (16 KB)
+    __shared__ float bias[4 * kMlpHidden + kMlpOutPad];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 5, r = lane & 31;
+    for (int k = tid; k < 4 * kMlpHidden; k += 256) bias[k] = a.b[k / kMlpHidden][k % kMlpHidden];
+    if (tid < kMlpOutPad) bias[4 * kMlpHidden + tid] = a.b[kMlpLayers - 1][tid];
     {
-        const bf16x8* F = reinterpret_cast<const bf16x8*>(a.Wf[kMlpLayers - 1]);
-        f32x16 c;
-        for (int i = 0; i < 16; ++i) c[i] = 0.0f;
-        for (int ks = 0; ks < kMlpHidden / 16; ++ks) {
-            bf16x8 x;
-            for (int j = 0; j < 8; ++j) x[j] = A[frag_rc(lane)][16 * ks + frag_k(lane, j)];
-            c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, F[ks * 64 + lane], c, 0, 0, 0);
-        }
-        const int col = frag_rc(lane);
-        if (col < kMlpOut) {
-            const float b = a.b[kMlpLayers - 1][col];
-            for (int i = 0; i < 16; ++i) {
-                const int64_t row = r0 + acc_row(lane, i);
-                if (row < a.n) a.Y[row * kMlpOut + col] = c[i] + b;
+        const bf16x8* F1 = reinterpret_cast<const bf16x8*>(a.Wf[1]);
+        wbuf[0][tid] = F1[tid];
+        wbuf[0][tid + 256] = F1[tid + 256];
+    }
+    const int64_t ray0 = int64_t(blockIdx.x) * kInferRays + w * (kInferTiles * 32);
+    bf16x8 H[kInferTiles][2 * kStepFrags];
+    f32x16 acc[kInferTiles][kStepFrags];
+    // layer 1 (K = 16): B = X^T, lane l holds ray l % 32, features 8 (l / 32) .. + 7
+    {
+        bf16x8 xb[kInferTiles];
+#pragma unroll
+        for (int c = 0; c < kInferTiles; ++c) {
+            const int64_t row = ray0 + 32 * c + r;
+            float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
+            if (row < a.n) {
+                const float4* src = reinterpret_cast<const float4*>(a.X + row * kMlpInPad + 8 * g);
+                lo = src[0];
+                hi = src[1];
             }
+            xb[c][0] = __bf16(lo.x); xb[c][1] = __bf16(lo.y); xb[c][2] = __bf16(lo.z); xb[c][3] = __bf16(lo.w);
+            xb[c][4] = __bf16(hi.x); xb[c][5] = __bf16(hi.y); xb[c][6] = __bf16(hi.z); xb[c][7] = __bf16(hi.w);
+        }
+        const bf16x8* F0 = reinterpret_cast<const bf16x8*>(a.Wf[0]);
+#pragma unroll
+        for (int nt = 0; nt < kStepFrags; ++nt) {
+            const bf16x8 A = F0[nt * 64 + lane];
+#pragma unroll
+            for (int c = 0; c < kInferTiles; ++c) {
+                f32x16 z;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) z[i] = 0.0f;
+                acc[c][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, xb[c], z, 0, 0, 0);
+            }
+        }
+    }
+    __syncthreads();                                      // bias and the first weight stage
+    infer_epilogue(acc, bias, g, H);
+    // hidden layers 2..4: 16 k-steps each; the stage of step s + 1 is loaded
+    // while step s runs, one barrier per step
+    for (int l = 1; l < kMlpLayers - 1; ++l) {
+#pragma unroll
+        for (int c = 0; c < kInferTiles; ++c)
+#pragma unroll
+            for (int nt = 0; nt < kStepFrags; ++nt)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) acc[c][nt][i] = 0.0f;
+#pragma unroll
+        for (int ks = 0; ks < 2 * kStepFrags; ++ks) {
+            const bool last = ks == 2 * kStepFrags - 1;
+            const bool has_next = !last || l + 1 < kMlpLayers - 1;
+            bf16x8 n0, n1;
+            if (has_next) {
+                const bf16x8* Fn = last ? reinterpret_cast<const bf16x8*>(a.Wf[l + 1])
+                                        : reinterpret_cast<const bf16x8*>(a.Wf[l]) + (ks + 1) * (kStepFrags * 64);
+                n0 = Fn[tid];
+                n1 = Fn[tid + 256];
+            }
+#pragma unroll
+            for (int nt = 0; nt < kStepFrags; ++nt) {
+                const bf16x8 A = wbuf[ks & 1][nt * 64 + lane];
+#pragma unroll
+                for (int c = 0; c < kInferTiles; ++c)
+                    acc[c][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, H[c][ks], acc[c][nt], 0, 0, 0);
+            }
+            if (has_next) {
+                wbuf[(ks + 1) & 1][tid] = n0;
+                wbuf[(ks + 1) & 1][tid + 256] = n1;
+            }
+            __syncthreads();
+        }
+        infer_epilogue(acc, bias + l * kMlpHidden, g, H);
+    }
+    // output layer: one 32-row tile (8 outputs used)
+    {
+        const bf16x8* F4 = reinterpret_cast<const bf16x8*>(a.Wf[kMlpLayers - 1]);
+        f32x16 o[kInferTiles];
+#pragma unroll
+        for (int c = 0; c < kInferTiles; ++c)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) o[c][i] = 0.0f;
+#pragma unroll
+        for (int ks = 0; ks < 2 * kStepFrags; ++ks) {
+            const bf16x8 A = F4[ks * 64 + lane];
+#pragma unroll
+            for (int c = 0; c < kInferTiles; ++c) o[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, H[c][ks], o[c], 0, 0, 0);
+        }
+        // lane group g holds outputs 4 g .. 4 g + 3 (accumulator rows i = 0..3) of ray l % 32
+        const float* bo = bias + 4 * kMlpHidden + 4 * g;
+#pragma unroll
+        for (int c = 0; c < kInferTiles; ++c) {
+            const int64_t row = ray0 + 32 * c + r;
+            if (row < a.n)
+                *reinterpret_cast<float4*>(a.Y + row * kMlpOut + 4 * g) =
+                    make_float4(o[c][0] + bo[0], o[c][1] + bo[1], o[c][2] + bo[2], o[c][3] + bo[3]);
         }
     }
 }
@@ -279,10 +360,10 @@ hipError_t launch_to_bf16(const float* x, __bf16* y, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_bf16, dim3(unsigned(blocks)), dim3(256), 0, s, x, y, n);
     return hipGetLastError();
 }
-hipError_t launch_repack(const __bf16* W, int K, int N, __bf16* F, hipStream_t s) {
+hipError_t launch_repack(const __bf16* W, int K, int N, __bf16* F, int permuted, hipStream_t s) {
     int64_t blocks = (int64_t(K) * N + 255) / 256;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_repack, dim3(unsigned(blocks)), dim3(256), 0, s, W, K, N, F);
+    hipLaunchKernelGGL(k_repack, dim3(unsigned(blocks)), dim3(256), 0, s, W, K, N, F, permuted);
     return hipGetLastError();
 }
 hipError_t launch_loss(const float* Y, const float* T, int n, float w_r, float w_d, float* dY, float* sums, hipStream_t s) {

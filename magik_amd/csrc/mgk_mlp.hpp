@@ -38,7 +38,7 @@ struct GemmArgs {
 struct InferArgs {
     const float* X;     // n x 16 normalised features
     int64_t n;
-    const __bf16* Wf[kMlpLayers];   // fragment-ordered bf16 weights
+    const __bf16* Wf[kMlpLayers];   // bf16 weights in the A-fragment order of k_repack
     const float* b[kMlpLayers];
     float* Y;           // n x 8 outputs
 };
@@ -47,7 +47,7 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
 hipError_t launch_colsum(const float* X, int M, int N, float* out, hipStream_t s);
 hipError_t launch_zero(float* p, int64_t n, hipStream_t s);
 hipError_t launch_to_bf16(const float* x, __bf16* y, int64_t n, hipStream_t s);
-hipError_t launch_repack(const __bf16* W, int K, int N, __bf16* F, hipStream_t s);
+hipError_t launch_repack(const __bf16* W, int K, int N, __bf16* F, int permuted, hipStream_t s);
 hipError_t launch_loss(const float* Y, const float* T, int n, float w_r, float w_d, float* dY, float* sums, hipStream_t s);
 hipError_t launch_adam(float* w, float* m, float* v, const float* g, int64_t n, float lr, int t, __bf16* wb, hipStream_t s);
 hipError_t launch_infer(const InferArgs& a, hipStream_t s);

@@ -170,7 +170,7 @@ e_magik_result_types train_backward(mgk_surrogate* s, int n, hipStream_t st) {
 }
 
 e_magik_result_types refresh_fragments(mgk_surrogate* s, hipStream_t st) {
-    for (int l = 0; l < kMlpLayers; ++l) HIP_TRY(launch_repack(s->Wb[l], kK[l], kN[l], s->Wf[l], st));
+    for (int l = 0; l < kMlpLayers; ++l) HIP_TRY(launch_repack(s->Wb[l], kK[l], kN[l], s->Wf[l], l > 0 ? 1 : 0, st));
     s->frag_dirty = false;
     return MAGIK_SUCCESS;
 }

@@ -115,93 +115,122 @@ __device__ bool near_star_edge(const DecodeParams& P, float dx, float dy, float 
     return false;
 }
 
-__global__ void __launch_bounds__(256) k_sur_decode(DecodeParams P) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// Paths decoded per workgroup; the workgroup compacts its fallback paths in
+// LDS and publishes them with one global atomic.
+constexpr int kDecodeBlock = 256;
+constexpr int kDecodeChunk = 16 * kDecodeBlock;
+
+__device__ __forceinline__ bool decode_one(const DecodeParams& P, uint64_t i, mgk_hit_t& h, int& reason) {
+    bool fallback = false;
+    reason = -1;
+    h = mgk_hit_t{MGK_HIT_NONE, 0.0f, 0.0f, 0.0f};
+    const float4 y0 = reinterpret_cast<const float4*>(P.Y + i * kMlpOut)[0];
+    const float4 y1 = reinterpret_cast<const float4*>(P.Y + i * kMlpOut)[1];
+    // logits (captured, disk, escape = 0)
+    const float mx = fmaxf(fmaxf(y0.x, y0.y), 0.0f);
+    const float e0 = __expf(y0.x - mx), e1 = __expf(y0.y - mx), e2 = __expf(-mx);
+    const float inv = 1.0f / (e0 + e1 + e2);
+    const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
+    const int cls = (p0 >= p1 && p0 >= p2) ? 0 : (p1 >= p2 ? 1 : 2);
+    const float conf = fmaxf(p0, fmaxf(p1, p2));
+    const float2 ab = *reinterpret_cast<const float2*>(P.X + i * kMlpInPad + 2);   // x[2], x[3]
+    const float be_n = P.X[i * kMlpInPad + 4];
+    const float al = ab.y * float(P.R.s_max), be = be_n * float(P.R.s_max);
+    if (fabsf(al) > float(P.R.s_max) || fabsf(be) > float(P.R.s_max)) {
+        fallback = true; reason = C_DOMAIN;
+    } else if (conf < P.min_conf) {
+        fallback = true; reason = C_CONF;
+    } else {
+        const float rho = sqrtf(al * al + be * be), psi = atan2f(be, al);
+        float fk = (psi + kPi) * (float(kCritN) / (2.0f * kPi));
+        int k0 = int(floorf(fk));
+        const float t = fk - float(k0);
+        k0 = ((k0 % kCritN) + kCritN) % kCritN;
+        const int k1 = (k0 + 1) % kCritN;
+        const float c0 = P.crit[k0], c1 = P.crit[k1];
+        if (c0 >= 0.0f && c1 >= 0.0f && fabsf(rho - (c0 + t * (c1 - c0))) < P.crit_margin) {
+            fallback = true; reason = C_CRIT;
+        }
+    }
+    if (fallback) return true;
+    const double con = P.rays[9 * P.n + i], pt = P.rays[7 * P.n + i];
+    if (cls == 0) {
+        h.cls = MGK_HIT_HORIZON;
+    } else if (cls == 1) {
+        const double r = exp(P.R.lnr_mid + double(y0.z) * P.R.lnr_half);
+        if (P.trace_disk || r < P.disk_r_in * (1.0 + P.edge_margin) || r > P.disk_r_out * (1.0 - P.edge_margin)) {
+            reason = C_DISK;
+            return true;
+        }
+        double ut, w;
+        kepler_ut_w<KERR, double>(r, P.a_eq, ut, w);
+        const double qx = P.rays[0 * P.n + i], qz = P.rays[2 * P.n + i];
+        const double px = P.rays[4 * P.n + i], pz = P.rays[6 * P.n + i];
+        double g = con / (ut * pt + w * (qz * px - qx * pz));
+        if (isnan(g) || isinf(g)) g = 1e-10;
+        h = mgk_hit_t{MGK_HIT_DISK, float(g), float(r), 0.0f};
+    } else {
+        const float c = fminf(1.0f, fmaxf(-1.0f, y0.w));
+        const float ph = atan2f(y1.x, y1.y);
+        const float sn = sqrtf(fmaxf(0.0f, 1.0f - c * c));
+        if (P.star_test && near_star_edge(P, sn * cosf(ph), c, sn * sinf(ph), ph)) {
+            reason = C_STAR;
+            return true;
+        }
+        double g = con / pt;
+        if (isnan(g) || isinf(g)) g = 1e-10;
+        h = mgk_hit_t{MGK_HIT_ESCAPE, float(g), c, ph};
+    }
+    return false;
+}
+
+__global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
+    __shared__ uint32_t qbuf[kDecodeChunk];
+    __shared__ uint32_t n_local;
+    __shared__ uint32_t reasons[C_DOMAIN + 1];
+    __shared__ unsigned long long base_s;
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const bool live = i < P.n;
-    bool fallback = false;
-    int reason = -1;
-    mgk_hit_t h{MGK_HIT_NONE, 0.0f, 0.0f, 0.0f};
-    if (live) {
-        const float4 y0 = reinterpret_cast<const float4*>(P.Y + i * kMlpOut)[0];
-        const float4 y1 = reinterpret_cast<const float4*>(P.Y + i * kMlpOut)[1];
-        // logits (captured, disk, escape = 0)
-        const float mx = fmaxf(fmaxf(y0.x, y0.y), 0.0f);
-        const float e0 = __expf(y0.x - mx), e1 = __expf(y0.y - mx), e2 = __expf(-mx);
-        const float inv = 1.0f / (e0 + e1 + e2);
-        const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
-        const int cls = (p0 >= p1 && p0 >= p2) ? 0 : (p1 >= p2 ? 1 : 2);
-        const float conf = fmaxf(p0, fmaxf(p1, p2));
-        const float al = P.X[i * kMlpInPad + 3] * float(P.R.s_max), be = P.X[i * kMlpInPad + 4] * float(P.R.s_max);
-        if (fabsf(al) > float(P.R.s_max) || fabsf(be) > float(P.R.s_max)) {
-            fallback = true; reason = C_DOMAIN;
-        } else if (conf < P.min_conf) {
-            fallback = true; reason = C_CONF;
-        } else {
-            const float rho = sqrtf(al * al + be * be), psi = atan2f(be, al);
-            float fk = (psi + kPi) * (float(kCritN) / (2.0f * kPi));
-            int k0 = int(floorf(fk));
-            const float t = fk - float(k0);
-            k0 = ((k0 % kCritN) + kCritN) % kCritN;
-            const int k1 = (k0 + 1) % kCritN;
-            const float c0 = P.crit[k0], c1 = P.crit[k1];
-            if (c0 >= 0.0f && c1 >= 0.0f && fabsf(rho - (c0 + t * (c1 - c0))) < P.crit_margin) {
-                fallback = true; reason = C_CRIT;
+    if (threadIdx.x == 0) n_local = 0;
+    if (threadIdx.x <= C_DOMAIN) reasons[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t begin = uint64_t(blockIdx.x) * kDecodeChunk;
+    for (int it = 0; it < kDecodeChunk / kDecodeBlock; ++it) {
+        const uint64_t i = begin + uint64_t(it) * kDecodeBlock + threadIdx.x;
+        const bool live = i < P.n;
+        bool fallback = false;
+        int reason = -1;
+        mgk_hit_t h;
+        if (live) {
+            fallback = decode_one(P, i, h, reason);
+            if (P.judge_stride && (i % P.judge_stride) == 0) {
+                mgk_hit_t pr = h;
+                if (fallback) pr.cls = kNotAnswered;
+                P.pred[i / P.judge_stride] = pr;
+                fallback = true;
             }
+            if (!fallback) P.hits[i] = h;
         }
-        if (!fallback) {
-            const double con = P.rays[9 * P.n + i], pt = P.rays[7 * P.n + i];
-            if (cls == 0) {
-                h.cls = MGK_HIT_HORIZON;
-            } else if (cls == 1) {
-                const double r = exp(P.R.lnr_mid + double(y0.z) * P.R.lnr_half);
-                if (P.trace_disk || r < P.disk_r_in * (1.0 + P.edge_margin) || r > P.disk_r_out * (1.0 - P.edge_margin)) {
-                    fallback = true; reason = C_DISK;
-                } else {
-                    double ut, w;
-                    kepler_ut_w<KERR, double>(r, P.a_eq, ut, w);
-                    const double qx = P.rays[0 * P.n + i], qz = P.rays[2 * P.n + i];
-                    const double px = P.rays[4 * P.n + i], pz = P.rays[6 * P.n + i];
-                    double g = con / (ut * pt + w * (qz * px - qx * pz));
-                    if (isnan(g) || isinf(g)) g = 1e-10;
-                    h = mgk_hit_t{MGK_HIT_DISK, float(g), float(r), 0.0f};
-                }
-            } else {
-                const float c = fminf(1.0f, fmaxf(-1.0f, y0.w));
-                const float ph = atan2f(y1.x, y1.y);
-                const float sn = sqrtf(fmaxf(0.0f, 1.0f - c * c));
-                if (P.star_test && near_star_edge(P, sn * cosf(ph), c, sn * sinf(ph), ph)) {
-                    fallback = true; reason = C_STAR;
-                } else {
-                    double g = con / pt;
-                    if (isnan(g) || isinf(g)) g = 1e-10;
-                    h = mgk_hit_t{MGK_HIT_ESCAPE, float(g), c, ph};
-                }
-            }
+        for (int c = C_CONF; c <= C_DOMAIN; ++c) {
+            const uint64_t m = __ballot(reason == c);
+            if (m != 0 && lane == 0) atomicAdd(&reasons[c], uint32_t(__popcll(m)));
         }
-        if (P.judge_stride && (i % P.judge_stride) == 0) {
-            mgk_hit_t pr = h;
-            if (fallback) pr.cls = kNotAnswered;
-            P.pred[i / P.judge_stride] = pr;
-            fallback = true;
+        const uint64_t fb = __ballot(live && fallback);
+        if (fb != 0) {
+            uint32_t off = 0;
+            if (lane == 0) off = atomicAdd(&n_local, uint32_t(__popcll(fb)));
+            off = __shfl(off, 0);
+            if (live && fallback) qbuf[off + __popcll(fb & lt_mask)] = uint32_t(i);
         }
-        if (!fallback) P.hits[i] = h;
     }
-    // fallback reasons: one atomic per wave and reason
-    for (int c = C_CONF; c <= C_DOMAIN; ++c) {
-        const uint64_t m = __ballot(reason == c);
-        if (m != 0 && lane == uint32_t(__ffsll((unsigned long long)m) - 1)) atomicAdd(&P.cnt[c], (unsigned long long)__popcll(m));
-    }
-    // compacted queue of the paths to trace
-    const uint64_t fb = __ballot(live && fallback);
-    if (fb != 0) {
-        unsigned long long base = 0;
-        const int leader = __ffsll((unsigned long long)fb) - 1;
-        if (int(lane) == leader) base = atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)__popcll(fb));
-        base = __shfl(base, leader);
-        if (live && fallback) P.queue[base + __popcll(fb & lt_mask)] = uint32_t(i);
-    }
+    __syncthreads();
+    const uint32_t n_q = n_local;
+    if (threadIdx.x == 0) base_s = n_q ? atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)n_q) : 0ull;
+    if (threadIdx.x >= C_CONF && threadIdx.x <= C_DOMAIN && reasons[threadIdx.x])
+        atomicAdd(&P.cnt[threadIdx.x], (unsigned long long)reasons[threadIdx.x]);
+    __syncthreads();
+    const unsigned long long base = base_s;
+    for (uint32_t k = threadIdx.x; k < n_q; k += kDecodeBlock) P.queue[base + k] = qbuf[k];
 }
 
 __device__ inline void atomic_max_double(unsigned long long* p, double v) {
@@ -464,7 +493,7 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.edge_margin = fb->disk_edge_margin;
     D.judge_stride = fb->judge_stride;
     D.trace_disk = fb->trace_disk;
-    hipLaunchKernelGGL(k_sur_decode, dim3(blocks), dim3(256), 0, st, D);
+    hipLaunchKernelGGL(k_sur_decode, dim3(unsigned((n + kDecodeChunk - 1) / kDecodeChunk)), dim3(kDecodeBlock), 0, st, D);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(s->ev[2], st));
 
