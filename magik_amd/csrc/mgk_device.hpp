@@ -28,16 +28,15 @@ template <bool FAST> __device__ __forceinline__ float rcp(float x) {
     if constexpr (FAST) return __builtin_amdgcn_rcpf(x);
     else return 1.0f / x;
 }
-// FP64 FAST: v_rcp_f64 / v_rsq_f64 seeds refined by Newton steps to full
-// double precision (a few ulp, not IEEE-rounded), far cheaper than the IEEE
-// division / square-root sequences.
+// FP64 FAST: v_rcp_f64 / v_rsq_f64 seeds refined by one Newton step (the
+// seed's relative error e becomes e^2, below 1e-13 for any seed good to
+// 2^-22), far cheaper than the IEEE division / square-root sequences; used
+// where the result only has to be far more accurate than the integration
+// tolerance (near-zone right-hand side, null renormalisation).
 template <bool FAST> __device__ __forceinline__ double rcp(double x) {
     if constexpr (FAST) {
-        double y = __builtin_amdgcn_rcp(x);
-        double e = fma(-x, y, 1.0);
-        y = fma(y, e, y);
-        e = fma(-x, y, 1.0);
-        return fma(y, e, y);
+        const double y = __builtin_amdgcn_rcp(x);
+        return fma(y, fma(-x, y, 1.0), y);
     } else {
         return 1.0 / x;
     }
@@ -48,10 +47,9 @@ template <bool FAST> __device__ __forceinline__ float sqrt_(float x) {
 }
 template <bool FAST> __device__ __forceinline__ double sqrt_(double x) {
     if constexpr (FAST) {
-        // y ~ 1/sqrt(x): two Newton steps, then s = x y with one correction
+        // y ~ 1/sqrt(x): one Newton step, then s = x y with one correction
         double y = __builtin_amdgcn_rsq(x);
         double h = 0.5 * x;
-        y = y * fma(-h * y, y, 1.5);
         y = y * fma(-h * y, y, 1.5);
         double s = x * y;
         double r = fma(-s, s, x);

@@ -51,6 +51,21 @@ __device__ __forceinline__ S rms_norm(const St<S>& e, const St<S>& y0, const St<
     return sqrt_<false>(total);
 }
 
+// The same norm for the split (mixed / near-zone) trials: the FP64 error
+// components and scales are formed first, their ratios only feed the
+// accept/reject test and the step-size factor, so FP32 ratios with a hardware
+// reciprocal are ample.
+__device__ __forceinline__ double rms_norm_split(const St<double>& e, const St<double>& y0, const St<double>& y1, double at, double rt) {
+    auto c = [&](double ev, double a, double b) {
+        const float sc = float(at + rt * fmax(fabs(a), fabs(b)));
+        return float(ev) * __builtin_amdgcn_rcpf(sc);
+    };
+    const float s0 = c(e.qt, y0.qt, y1.qt), s1 = c(e.qx, y0.qx, y1.qx), s2 = c(e.qy, y0.qy, y1.qy), s3 = c(e.qz, y0.qz, y1.qz);
+    const float s4 = c(e.pt, y0.pt, y1.pt), s5 = c(e.px, y0.px, y1.px), s6 = c(e.py, y0.py, y1.py), s7 = c(e.pz, y0.pz, y1.pz);
+    const float total = (s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3 + s4 * s4 + s5 * s5 + s6 * s6 + s7 * s7) * 0.125f;
+    return double(__builtin_amdgcn_sqrtf(total));
+}
+
 __device__ __forceinline__ float powr(float a, float b) { return powf(a, b); }
 __device__ __forceinline__ double powr(double a, double b) { return pow(a, b); }
 
@@ -141,8 +156,11 @@ __device__ __forceinline__ void trial_split(const St<double>& y0, double h, floa
     for (int j = 0; j < 6; ++j)
 #pragma unroll
         for (int c = 0; c < 3; ++c) Sq[j][c] = Sp[j][c] = SS(0);
-    double bp[3] = {0, 0, 0}, ep[3] = {0, 0, 0}, bt[3] = {0, 0, 0}, et[3] = {0, 0, 0};
-    double cb[4] = {0, 0, 0, 0}, ce[4] = {0, 0, 0, 0};
+    // Weighted sums of the stage values in the stage precision: none of them
+    // carries the large, cancelling P(p) term (that enters the FP64 assembly
+    // as h P(p0) only), so their rounding stays far below the tolerance.
+    SS bp[3] = {0, 0, 0}, ep[3] = {0, 0, 0}, bt[3] = {0, 0, 0}, et[3] = {0, 0, 0};
+    SS cb[4] = {0, 0, 0, 0}, ce[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
         const SS sqx = qx + Sq[i][0], sqy = qy + Sq[i][1], sqz = qz + Sq[i][2];
@@ -155,32 +173,34 @@ __device__ __forceinline__ void trial_split(const St<double>& y0, double h, floa
             Sq[j][0] += c * kx; Sq[j][1] += c * ky; Sq[j][2] += c * kz;
             Sp[j][0] += c * r.dpx; Sp[j][1] += c * r.dpy; Sp[j][2] += c * r.dpz;
         }
-        const double dpx = double(r.dpx), dpy = double(r.dpy), dpz = double(r.dpz);
         if (kB[i] != 0.0) {
-            bp[0] += kB[i] * dpx; bp[1] += kB[i] * dpy; bp[2] += kB[i] * dpz;
-            ep[0] += kE[i] * dpx; ep[1] += kE[i] * dpy; ep[2] += kE[i] * dpz;
-            cb[0] += kB[i] * double(r.cx); cb[1] += kB[i] * double(r.cy); cb[2] += kB[i] * double(r.cz); cb[3] += kB[i] * double(r.ct);
-            ce[0] += kE[i] * double(r.cx); ce[1] += kE[i] * double(r.cy); ce[2] += kE[i] * double(r.cz); ce[3] += kE[i] * double(r.ct);
+            const SS b = SS(kB[i]), ee = SS(kE[i]);
+            bp[0] += b * r.dpx; bp[1] += b * r.dpy; bp[2] += b * r.dpz;
+            ep[0] += ee * r.dpx; ep[1] += ee * r.dpy; ep[2] += ee * r.dpz;
+            cb[0] += b * r.cx; cb[1] += b * r.cy; cb[2] += b * r.cz; cb[3] += b * r.ct;
+            ce[0] += ee * r.cx; ce[1] += ee * r.cy; ce[2] += ee * r.cz; ce[3] += ee * r.ct;
         }
         if (kBT[i] != 0.0) {
-            bt[0] += kBT[i] * dpx; bt[1] += kBT[i] * dpy; bt[2] += kBT[i] * dpz;
-            et[0] += kET[i] * dpx; et[1] += kET[i] * dpy; et[2] += kET[i] * dpz;
+            const SS bt_i = SS(kBT[i]), et_i = SS(kET[i]);
+            bt[0] += bt_i * r.dpx; bt[1] += bt_i * r.dpy; bt[2] += bt_i * r.dpz;
+            et[0] += et_i * r.dpx; et[1] += et_i * r.dpy; et[2] += et_i * r.dpz;
         }
     }
     // FP64 assembly (see above).
     const double hh = h * h;
-    y1.px = y0.px + h * bp[0];
-    y1.py = y0.py + h * bp[1];
-    y1.pz = y0.pz + h * bp[2];
+    y1.px = y0.px + h * double(bp[0]);
+    y1.py = y0.py + h * double(bp[1]);
+    y1.pz = y0.pz + h * double(bp[2]);
     y1.pt = y0.pt;
-    y1.qx = y0.qx + h * y0.px + hh * bt[0] - h * cb[0];
-    y1.qy = y0.qy + h * y0.py + hh * bt[1] - h * cb[1];
-    y1.qz = y0.qz + h * y0.pz + hh * bt[2] - h * cb[2];
-    y1.qt = y0.qt - h * y0.pt - h * cb[3];
+    y1.qx = y0.qx + h * y0.px + (hh * double(bt[0]) - h * double(cb[0]));
+    y1.qy = y0.qy + h * y0.py + (hh * double(bt[1]) - h * double(cb[1]));
+    y1.qz = y0.qz + h * y0.pz + (hh * double(bt[2]) - h * double(cb[2]));
+    y1.qt = y0.qt - h * y0.pt - h * double(cb[3]);
     St<double> e;
-    e.px = h * ep[0]; e.py = h * ep[1]; e.pz = h * ep[2]; e.pt = 0.0;
-    e.qx = hh * et[0] - h * ce[0]; e.qy = hh * et[1] - h * ce[1]; e.qz = hh * et[2] - h * ce[2]; e.qt = -h * ce[3];
-    rms_out = rms_norm(e, y0, y1, double(P.abs_tol), double(P.rel_tol));
+    e.px = h * double(ep[0]); e.py = h * double(ep[1]); e.pz = h * double(ep[2]); e.pt = 0.0;
+    e.qx = hh * double(et[0]) - h * double(ce[0]); e.qy = hh * double(et[1]) - h * double(ce[1]);
+    e.qz = hh * double(et[2]) - h * double(ce[2]); e.qt = -h * double(ce[3]);
+    rms_out = rms_norm_split(e, y0, y1, double(P.abs_tol), double(P.rel_tol));
 }
 
 // Null-constraint renormalisation (FP64): rescale the spatial momentum by the
@@ -194,18 +214,21 @@ __device__ __forceinline__ void renormalise_null(St<double>& y, double a, double
         double s = fabs(y.pt) / sqrt(A);
         if (fabs(s - 1.0) < 1e-3) { y.px *= s; y.py *= s; y.pz *= s; }
     } else {
+        // divisions and square roots through the Newton-refined hardware
+        // seeds (a few ulp): the scale s only has to restore H = 0 far below
+        // the integration tolerance
         double f, kx, ky, kz;
         if constexpr (METRIC == KERR) {
-            double r = kerr_schild_r(y.qx, y.qy, y.qz, a);
-            double r2 = r * r;
-            f = 2.0 * r2 * r / (r2 * r2 + a * a * y.qy * y.qy);
-            double id = 1.0 / (r2 + a * a);
+            const double R = y.qx * y.qx + y.qy * y.qy + y.qz * y.qz - a * a;
+            const double r = sqrt_<true>(0.5 * R + 0.5 * sqrt_<true>(R * R + 4.0 * a * a * y.qy * y.qy));
+            const double r2 = r * r;
+            f = 2.0 * r2 * r * rcp<true>(r2 * r2 + a * a * y.qy * y.qy);
+            const double id = rcp<true>(r2 + a * a);
             kx = (r * y.qx - a * y.qz) * id;
-            ky = y.qy / r;
+            ky = y.qy * rcp<true>(r);
             kz = (r * y.qz + a * y.qx) * id;
         } else {
-            double r = sqrt(y.qx * y.qx + y.qy * y.qy + y.qz * y.qz);
-            double ir = 1.0 / r;
+            const double ir = rcp<true>(sqrt_<true>(y.qx * y.qx + y.qy * y.qy + y.qz * y.qz));
             f = 2.0 * ms * ir;
             kx = y.qx * ir; ky = y.qy * ir; kz = y.qz * ir;
         }
@@ -214,8 +237,8 @@ __device__ __forceinline__ void renormalise_null(St<double>& y, double a, double
         double al = A - f * B * B, be = 2.0 * f * B * y.pt, ga = -(1.0 + f) * y.pt * y.pt;
         double disc = be * be - 4.0 * al * ga;
         if (!(disc >= 0.0) || al == 0.0) return;
-        double qq = -0.5 * (be + copysign(sqrt(disc), be));
-        double s1 = qq / al, s2 = qq != 0.0 ? ga / qq : s1;
+        double qq = -0.5 * (be + copysign(sqrt_<true>(disc), be));
+        double s1 = qq * rcp<true>(al), s2 = qq != 0.0 ? ga * rcp<true>(qq) : s1;
         double s = fabs(s1 - 1.0) < fabs(s2 - 1.0) ? s1 : s2;
         if (fabs(s - 1.0) < 1e-3) { y.px *= s; y.py *= s; y.pz *= s; }
     }
@@ -329,7 +352,11 @@ __device__ __forceinline__ void hit_record(const St<S>& y0, const St<S>& y1, uin
 // the disk and the emissive sphere), then the horizon (in_domain) and the
 // step limit (kernel_enforce_ray_limits.cu:48, n_step > limit).  Returns the
 // class and fills the hit record.
-template <int METRIC, class S>
+// FAST_TESTS (the mixed-precision zone kernels): the per-step escape-sphere
+// and horizon threshold tests run in FP32; only rays within ~1e-7 of a
+// threshold can see a different step, and the hit itself is still recorded
+// from the FP64 state.
+template <int METRIC, class S, bool FAST_TESTS = false>
 __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, uint32_t n_step, S contraction, const TraceParams& P,
                                              const Lane<S>& ln, mgk_hit_t& rec) {
 #pragma clang fp contract(off)
@@ -362,7 +389,12 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
             if (t >= S(0) && t <= S(1) && t < best) { best = t; cls = MGK_HIT_SPHERE; }
         }
     }
-    {
+    if constexpr (FAST_TESTS) {
+        const float L = P.length_scale;
+        const float ex = float(y1.qx) * L - float(ln.cx), ey = float(y1.qy) * L - float(ln.cy), ez = float(y1.qz) * L - float(ln.cz);
+        const float d2 = ex * ex + ey * ey + ez * ez;
+        if (!(d2 < P.cel_r2_f) && S(1) < best) { best = S(1); cls = MGK_HIT_ESCAPE; }
+    } else {
         const S L = S(P.length_scale);
         S ex = y1.qx * L - ln.cx, ey = y1.qy * L - ln.cy, ez = y1.qz * L - ln.cz;
         S d2 = ex * ex + ey * ey + ez * ez;
@@ -372,8 +404,15 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
     if (cls == MGK_HIT_NONE) {
         if constexpr (METRIC == KERR) {
             if (ln.in_range) {
-                S r = kerr_schild_r(y1.qx, y1.qy, y1.qz, a);
-                if (!(ln.hr < r)) cls = MGK_HIT_HORIZON;
+                if constexpr (FAST_TESTS) {
+                    const float af = float(a), yf = float(y1.qy);
+                    const float R = float(y1.qx) * float(y1.qx) + yf * yf + float(y1.qz) * float(y1.qz) - af * af;
+                    const float r2 = 0.5f * R + 0.5f * __builtin_amdgcn_sqrtf(R * R + 4.0f * af * af * yf * yf);
+                    if (!(float(ln.hr) * float(ln.hr) < r2)) cls = MGK_HIT_HORIZON;
+                } else {
+                    S r = kerr_schild_r(y1.qx, y1.qy, y1.qz, a);
+                    if (!(ln.hr < r)) cls = MGK_HIT_HORIZON;
+                }
             }
         } else if constexpr (METRIC == SCHWARZSCHILD) {
             S r = sqrt_<false>(y1.qx * y1.qx + y1.qy * y1.qy + y1.qz * y1.qz);
@@ -540,7 +579,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
 // depends on its own position only, never on the rays sharing its wave.
 // ---------------------------------------------------------------------------
 template <int METRIC, bool NEAR>
-__global__ void __launch_bounds__(256, 2) k_trace_zone(TraceParams P, ZoneParams Z) {
+__global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P, ZoneParams Z) {
     using S = double;
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -651,7 +690,7 @@ __global__ void __launch_bounds__(256, 2) k_trace_zone(TraceParams P, ZoneParams
         ++steps_acc;
         renormalise_null<METRIC>(y1, double(ln.a), double(P.mass_sign));
         mgk_hit_t rec;
-        const uint32_t cls = classify<METRIC, S>(y, y1, n_step, contraction, P, ln, rec);
+        const uint32_t cls = classify<METRIC, S, true>(y, y1, n_step, contraction, P, ln, rec);
         if (cls != MGK_HIT_NONE) {
             P.hits[ray] = rec;
             if (P.steps_out) P.steps_out[ray] = n_step;
@@ -790,10 +829,16 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(ShadeParams P) {
                 if (isnan(L) || isinf(L) || L < 0.0f) L = 0.0f;
                 if (!(L > 0.0f)) continue;
                 const float E = L * P.irr_scale;
-                float x1 = P.bin_min;
+                // Only bins with an edge inside lambda +- 8 sigma: beyond that the
+                // FP32 normal CDF is exactly 0 or 1 (erf saturates at 3.92), so the
+                // bins outside receive exactly zero and are skipped.
+                const float w8 = 8.0f * P.sigma;
+                const int i0 = max(0, int(floorf((lam - w8 - P.bin_min) / P.bin_width)));
+                const int i1 = min(int(P.n_bin) - 1, int(floorf((lam + w8 - P.bin_min) / P.bin_width)));
+                float x1 = P.bin_min + float(i0) * P.bin_width;
                 float prev = 0.5f * (1.0f + erff((x1 - lam) * inv_sqrt2_sigma));
-                for (uint32_t i = 0; i < P.n_bin; ++i) {
-                    x1 += P.bin_width;
+                for (int i = i0; i <= i1; ++i) {
+                    x1 = P.bin_min + float(i + 1) * P.bin_width;
                     float cur = 0.5f * (1.0f + erff((x1 - lam) * inv_sqrt2_sigma));
                     bins[i * kShadeBlock + threadIdx.x] += E * (cur - prev);
                     prev = cur;
