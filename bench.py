@@ -60,23 +60,30 @@ def radiance_error(ctx, M, integ, sc, spec, cam, spp, seed, test_hits=None, n_su
     """Mean relative per-ray Y error against the FP64 path on a seeded subset of
     the frame's rays, with identical per-ray samples.  test_hits: the frame's
     hit records of the path under test ((n, 4) device tensor); when None the
-    subset is traced with `integ`."""
+    subset is traced with `integ`, and the endpoint-direction error and the
+    drift of the constants of motion (H, L, Carter's Q) are measured too."""
     import torch
-    rays = ctx.generate_rays(integ, cam, seed, 0, spp)           # float64 storage (mixed / FP64)
+    from oracle import invariants as INV
+    it64 = M.integrator()
+    C.memmove(C.byref(it64), C.byref(integ), C.sizeof(integ))
+    it64.precision = M.PREC_FP64
+    rays = ctx.generate_rays(it64, cam, seed, 0, spp)            # float64 initial conditions
+    # the FP32 reference path starts from its own float initial conditions
+    rays_test = rays if integ.precision != M.PREC_FP32_REFERENCE else ctx.generate_rays(integ, cam, seed, 0, spp)
     n = rays.shape[1]
     q = subset(n, seed, n_sub)
     qt = torch.from_numpy(q.astype(np.int32)).to(rays.device)
-    out = {}
-    runs = [("fp64", M.PREC_FP64)] + ([("test", integ.precision)] if test_hits is None else [])
-    for name, prec in runs:
-        it = M.integrator()
-        C.memmove(C.byref(it), C.byref(integ), C.sizeof(integ))
-        it.precision = prec
-        hits, _ = ctx.trace(it, sc, list(cam.position), rays, n=len(q), queue=qt, want_steps=False)
+    out, fin = {}, {}
+    runs = [("fp64", it64, rays)] + ([("test", integ, rays_test)] if test_hits is None else [])
+    for name, it, rr in runs:
+        hits, final = ctx.trace_final(it, sc, list(cam.position), rr, n=len(q), queue=qt)
         out[name] = hits[qt.long()].contiguous()
-        del hits
+        fin[name] = final[:, qt.long()].cpu().numpy()
+        del hits, final
+    del rays_test
     if test_hits is not None:
         out["test"] = test_hits[torch.from_numpy(q).to(test_hits.device)].contiguous()
+    ic = rays[:, qt.long()].cpu().numpy()
     side = int(np.sqrt(len(q)))
     assert side * side == len(q)
     vcam = M.abi.Camera()
@@ -84,14 +91,37 @@ def radiance_error(ctx, M, integ, sc, spec, cam, spp, seed, test_hits=None, n_su
     vcam.x_resolution = vcam.y_resolution = side
     y = {k: ctx.shade(sc, spec, vcam, seed, 0, 1, v)[..., 1].cpu().numpy().ravel().astype(np.float64)
          for k, v in out.items()}
-    cls_t = out["test"][:, 0].cpu().numpy()
-    cls_r = out["fp64"][:, 0].cpu().numpy()
+    ht = M.hits_to_numpy(out["test"])
+    hr = M.hits_to_numpy(out["fp64"])
     del rays
     torch.cuda.empty_cache()
     eps = 1e-3 * max(y["fp64"].mean(), 1e-30)
     err = np.abs(y["test"] - y["fp64"]) / np.maximum(y["fp64"], eps)
-    return {"mean_rel_Y": float(err.mean()), "p99_rel_Y": float(np.quantile(err, 0.99)),
-            "class_agreement": float(np.mean(cls_t == cls_r)), "rays": len(q)}
+    res = {"mean_rel_Y": float(err.mean()), "p99_rel_Y": float(np.quantile(err, 0.99)),
+           "class_agreement": float(np.mean(ht["cls"] == hr["cls"])), "rays": len(q)}
+    # endpoint direction of the rays that escape on both paths
+    esc = (ht["cls"] == M.HIT_ESCAPE) & (hr["cls"] == M.HIT_ESCAPE)
+    if esc.any():
+        def unit(h):
+            c = h["a0"].astype(np.float64)
+            s_ = np.sqrt(np.maximum(0.0, 1.0 - c * c))
+            return np.stack([s_ * np.cos(h["a1"]), c, s_ * np.sin(h["a1"])])
+        d = np.linalg.norm(unit(ht[esc]) - unit(hr[esc]), axis=0)
+        ang = 2.0 * np.arcsin(np.minimum(1.0, 0.5 * d))
+        res["endpoint_direction_err_rad"] = {"mean": float(ang.mean()), "p99": float(np.quantile(ang, 0.99)),
+                                             "max": float(ang.max()), "rays": int(esc.sum())}
+    # drift of the constants of motion between the initial conditions and termination
+    if integ.metric_type == M.KERR:
+        a_eq = float(np.float32(-integ.spin if integ.time_reversed else integ.spin))
+        drift = {}
+        for k, f in fin.items():
+            h, dl, dq = INV.drift(ic[0:3], ic[4:8], f[0:3], f[4:8], a_eq)
+            ok = np.isfinite(h) & np.isfinite(dq)
+            drift[k] = {"H_over_p2": {"mean_abs": float(np.abs(h[ok]).mean()), "max_abs": float(np.abs(h[ok]).max())},
+                        "L": {"mean_abs": float(np.abs(dl[ok]).mean()), "max_abs": float(np.abs(dl[ok]).max())},
+                        "Q": {"mean_abs": float(np.abs(dq[ok]).mean()), "max_abs": float(np.abs(dq[ok]).max())}}
+        res["constraint_drift"] = {"test" if k == "test" else "fp64": v for k, v in drift.items()}
+    return res
 
 
 def cpu_baseline(M, integ, sc, cam, spp, seed, budget_s=12.0):
@@ -206,6 +236,24 @@ def main():
     if not args.no_error:
         plain["radiance_err_vs_fp64"] = radiance_error(ctx, M, integ, sc, spec, cam, spp, args.seed)
         log(f"  radiance error {plain['radiance_err_vs_fp64']}")
+    if prec == M.PREC_MIXED:
+        # the plain path in the reference's FP32 arithmetic (the integrator the
+        # reference ships), for the speed ratios
+        it32 = M.integrator()
+        C.memmove(C.byref(it32), C.byref(integ), C.sizeof(integ))
+        it32.precision = M.PREC_FP32_REFERENCE
+        log("plain FP32 reference-arithmetic frames")
+
+        def ref_frame(seed_off):
+            ctx.render_frame(it32, sc, spec, cam, args.seed + seed_off, spp, out)
+            return ctx.stats()
+
+        wall32, _ = time_frames(ref_frame, max(1, min(args.steps, 2)), 1)
+        plain["fp32_reference_rays_per_s"] = n_paths * max(1, min(args.steps, 2)) / wall32
+        log(f"  {plain['fp32_reference_rays_per_s']:.4e} rays/s")
+        if not args.no_error:
+            plain["fp32_reference_radiance_err_vs_fp64"] = radiance_error(ctx, M, it32, sc, spec, cam, spp, args.seed)
+            log(f"  radiance error {plain['fp32_reference_radiance_err_vs_fp64']}")
 
     # ---- surrogate path ----
     sur_res, mlp_roof = None, None
@@ -215,6 +263,8 @@ def main():
     if sur_res:
         for v in sur_res["configs"].values():
             v["speedup_vs_plain"] = v["rays_per_s"] / plain["rays_per_s"]
+            if "fp32_reference_rays_per_s" in plain:
+                v["speedup_vs_plain_fp32_reference"] = v["rays_per_s"] / plain["fp32_reference_rays_per_s"]
 
     # ---- headline: the fastest path within the error bar ----
     cands = [("plain_rkf45", plain["rays_per_s"], plain.get("radiance_err_vs_fp64", {}).get("mean_rel_Y"))]

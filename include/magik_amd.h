@@ -264,6 +264,15 @@ MGK_API e_magik_result_types mgk_trace_rays(mgk_context_handle ctx, const mgk_in
                                             uint64_t n_rays, const uint32_t* d_queue, mgk_hit_t* d_hits,
                                             uint32_t* d_steps, void* stream);
 
+/* mgk_trace_rays that also writes each ray's state at termination (the end
+ * of its last step): d_final[c * ray_stride + i] for c = 0..7 = x, y, z, t,
+ * p_x, p_y, p_z, p_t (doubles, dimensionless).  For endpoint-direction and
+ * invariant-drift measurements against the FP64 path. */
+MGK_API e_magik_result_types mgk_trace_rays_final(mgk_context_handle ctx, const mgk_integrator_t* integ, const mgk_scene_t* scene,
+                                                  const float camera_position[3], const void* d_rays, uint64_t ray_stride,
+                                                  uint64_t n_rays, const uint32_t* d_queue, mgk_hit_t* d_hits, double* d_final,
+                                                  void* stream);
+
 /* Statistics of the last mgk_trace_rays / mgk_render_frame (synchronises). */
 MGK_API e_magik_result_types mgk_last_trace_stats(mgk_context_handle ctx, mgk_trace_stats_t* out);
 

@@ -186,6 +186,20 @@ class Context:
                                           self._stream()), "mgk_trace_rays")
         return hits, steps
 
+    def trace_final(self, integ, sc, cam_pos, rays, n: int | None = None, queue=None):
+        """Like trace(), returning (hits, final states (8, stride) float64)."""
+        import torch
+        stride = rays.shape[1]
+        n = stride if n is None else n
+        hits = torch.zeros((stride, 4), dtype=torch.int32, device=rays.device)
+        final = torch.zeros((8, stride), dtype=torch.float64, device=rays.device)
+        cp = (C.c_float * 3)(*cam_pos)
+        abi.check(self.lib.mgk_trace_rays_final(self.h, C.byref(integ), C.byref(sc), cp, C.c_void_p(rays.data_ptr()),
+                                                stride, n, C.c_void_p(queue.data_ptr() if queue is not None else None),
+                                                C.c_void_p(hits.data_ptr()), C.c_void_p(final.data_ptr()),
+                                                self._stream()), "mgk_trace_rays_final")
+        return hits, final
+
     def shade(self, sc, spec, cam, seed: int, sample_begin: int, spp: int, hits):
         import torch
         xyz = torch.zeros((cam.y_resolution, cam.x_resolution, 3), dtype=torch.float32, device=hits.device)

@@ -141,6 +141,8 @@ SIGNATURES = {
                                C.c_uint64, _P]),
     "mgk_trace_rays": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(C.c_float), _P, C.c_uint64,
                             C.c_uint64, _P, _P, _P, _P]),
+    "mgk_trace_rays_final": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(C.c_float), _P, C.c_uint64,
+                                  C.c_uint64, _P, _P, _P, _P]),
     "mgk_last_trace_stats": (_R, [_P, C.POINTER(TraceStats)]),
     "mgk_shade": (_R, [_P, C.POINTER(Scene), C.POINTER(Spectral), C.POINTER(Camera), C.c_uint64, C.c_uint32,
                        C.c_uint32, _P, _P, _P]),

@@ -184,7 +184,7 @@ int trace_grid(mgk_context* c, const TraceParams& P) {
 e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t s) {
     const uint64_t cap = std::max<uint64_t>(P.n, 1);   // a pass hands off at most the rays it was given
     for (int k = 0; k < 2; ++k) {
-        HIP_TRY(c->zone_state[k].reserve(cap * 9 * sizeof(double)));
+        HIP_TRY(c->zone_state[k].reserve(cap * kParked * sizeof(double)));
         HIP_TRY(c->zone_steps[k].reserve(cap * sizeof(uint32_t)));
     }
     HIP_TRY(c->zone_q0.reserve(cap * sizeof(uint32_t)));
@@ -636,6 +636,25 @@ MGK_API e_magik_result_types mgk_trace_rays(mgk_context_handle c, const mgk_inte
     P.queue = d_queue;
     P.hits = d_hits;
     P.steps_out = d_steps;
+    return do_trace(c, P, hipStream_t(stream));
+}
+
+MGK_API e_magik_result_types mgk_trace_rays_final(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc,
+                                                  const float cam_pos[3], const void* d_rays, uint64_t stride, uint64_t n,
+                                                  const uint32_t* d_queue, mgk_hit_t* d_hits, double* d_final, void* stream) {
+    if (!c || !sc || !cam_pos || !d_hits || !d_final || (n > 0 && !d_rays)) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = check_integrator(in);
+    if (r != MAGIK_SUCCESS) return r;
+    if (!d_queue && n > stride) return MGK_ERROR_CAPACITY_EXCEEDED;
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    TraceParams P;
+    fill_trace_params(in, sc, cam_pos, P);
+    P.rays = d_rays;
+    P.stride = stride;
+    P.n = n;
+    P.queue = d_queue;
+    P.hits = d_hits;
+    P.final_state = d_final;
     return do_trace(c, P, hipStream_t(stream));
 }
 

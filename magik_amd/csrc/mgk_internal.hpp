@@ -33,6 +33,7 @@ struct TraceParams {
     const uint32_t* queue;
     mgk_hit_t* hits;
     uint32_t* steps_out;
+    double* final_state;         // optional: the state (q, p) at termination, 8 x stride doubles
     unsigned long long* head;    // work counter (zeroed by the launcher)
     unsigned long long* stats;   // [0] steps [1] trials [2] fp64 trials
     float spin, mass_sign, length_scale;
@@ -53,6 +54,9 @@ struct TraceParams {
     int disk_isco;               // per-ray rays: the disk starts at the ISCO of the ray's spin
 };
 
+// Doubles parked per handed-off ray: q (4), p (4), h, u.p, L0, Q0.
+constexpr int kParked = 12;
+
 // One pass of the zone-split mixed-precision integrator.
 struct ZoneParams {
     int from_ic;                     // load rays from the IC buffer (first pass) instead of the parked state
@@ -62,7 +66,7 @@ struct ZoneParams {
     uint32_t* out_queue;             // rays handed to the other zone
     unsigned long long* out_count;
     uint64_t cap;                    // capacity of the queues and the parked-state arrays
-    const double* state_in;          // states parked by the previous pass, 9 x cap doubles (q, p, h), at the queue slot
+    const double* state_in;          // states parked by the previous pass, kParked x cap doubles, at the queue slot
     const uint32_t* steps_in;        // their step counts
     double* state_out;               // states parked by this pass, at their out-queue slot
     uint32_t* steps_out;

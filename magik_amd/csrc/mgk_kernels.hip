@@ -244,6 +244,86 @@ __device__ __forceinline__ void renormalise_null(St<double>& y, double a, double
     }
 }
 
+// Constants of motion of a Kerr (or positive-mass Schwarzschild) null ray in
+// Kerr-Schild coordinates with the spin along +y: the axial angular momentum
+// L = z p_x - x p_z and Carter's constant
+//   Q = p_theta^2 + cos^2(theta) (L^2 / sin^2(theta) - a^2 E^2),
+// cos(theta) = y / r, p_theta = cot(theta) (x p_x + z p_z) - r sin(theta) p_y
+// (x and z scale with sin(theta) at fixed r and azimuth, y = r cos(theta)),
+// E = -p_t.  Carter (1968).
+struct Invariants {
+    double r, f, kx, ky, kz, c, s2;
+    double L, Q, pth;
+};
+
+template <int METRIC>
+__device__ __forceinline__ bool invariants(const St<double>& y, double a, Invariants& v) {
+    const double a2 = METRIC == KERR ? a * a : 0.0;
+    const double R = y.qx * y.qx + y.qy * y.qy + y.qz * y.qz - a2;
+    v.r = sqrt_<true>(0.5 * R + 0.5 * sqrt_<true>(R * R + 4.0 * a2 * y.qy * y.qy));
+    const double r2 = v.r * v.r, ir = rcp<true>(v.r);
+    v.f = 2.0 * r2 * v.r * rcp<true>(r2 * r2 + a2 * y.qy * y.qy);
+    const double id = rcp<true>(r2 + a2);
+    v.kx = (v.r * y.qx - a * y.qz) * id;
+    v.ky = y.qy * ir;
+    v.kz = (v.r * y.qz + a * y.qx) * id;
+    v.c = y.qy * ir;
+    v.s2 = 1.0 - v.c * v.c;
+    if (!(v.s2 > 1e-6)) return false;    // near the spin axis theta is singular
+    const double s = sqrt_<true>(v.s2), cot = v.c * rcp<true>(s);
+    v.L = y.qz * y.px - y.qx * y.pz;
+    v.pth = cot * (y.qx * y.px + y.qz * y.pz) - v.r * s * y.py;
+    v.Q = v.pth * v.pth + v.c * v.c * (v.L * v.L * rcp<true>(v.s2) - a2 * y.pt * y.pt);
+    return true;
+}
+
+// Renormalisation of the constants of motion after a completed step (FP64):
+// one minimum-norm Newton step on the spatial momentum (position and
+// E = -p_t held) back onto H = 0, L = L0, Q = Q0, i.e. p -= J^T (J J^T)^-1 c
+// with c = (H, L - L0, Q - Q0) and J their gradients in p.  Where the three
+// gradients are nearly dependent (turning points, the spin axis) only the
+// null constraint is restored.
+template <int METRIC>
+__device__ __forceinline__ void renormalise_constants(St<double>& y, double a, double ms, double L0, double Q0) {
+    if constexpr (METRIC == MINKOWSKI) {
+        renormalise_null<METRIC>(y, a, ms);
+    } else {
+        if (METRIC == SCHWARZSCHILD && ms < 0.0) { renormalise_null<METRIC>(y, a, ms); return; }
+        Invariants v;
+        if (!invariants<METRIC>(y, a, v)) { renormalise_null<METRIC>(y, a, ms); return; }
+        const double f = METRIC == KERR ? v.f : 2.0 * ms * rcp<true>(v.r);
+        const double kp = v.kx * y.px + v.ky * y.py + v.kz * y.pz - y.pt;
+        const double H = 0.5 * (y.px * y.px + y.py * y.py + y.pz * y.pz - y.pt * y.pt - f * kp * kp);
+        const double gH[3] = {y.px - f * kp * v.kx, y.py - f * kp * v.ky, y.pz - f * kp * v.kz};
+        const double gL[3] = {y.qz, 0.0, -y.qx};
+        const double s = sqrt_<true>(v.s2), cot = v.c * rcp<true>(s);
+        const double wl = 2.0 * v.c * v.c * v.L * rcp<true>(v.s2);
+        const double gQ[3] = {2.0 * v.pth * cot * y.qx + wl * gL[0], -2.0 * v.pth * v.r * s,
+                              2.0 * v.pth * cot * y.qz + wl * gL[2]};
+        auto dot = [](const double* u, const double* w) { return u[0] * w[0] + u[1] * w[1] + u[2] * w[2]; };
+        const double g00 = dot(gH, gH), g01 = dot(gH, gL), g02 = dot(gH, gQ);
+        const double g11 = dot(gL, gL), g12 = dot(gL, gQ), g22 = dot(gQ, gQ);
+        const double c00 = g11 * g22 - g12 * g12, c01 = g02 * g12 - g01 * g22, c02 = g01 * g12 - g02 * g11;
+        const double det = g00 * c00 + g01 * c01 + g02 * c02;
+        // rows within ~0.01 rad of dependence amplify the residuals: null step only
+        if (!(det > 1e-4 * g00 * g11 * g22)) { renormalise_null<METRIC>(y, a, ms); return; }
+        const double c11 = g00 * g22 - g02 * g02, c12 = g01 * g02 - g00 * g12, c22 = g00 * g11 - g01 * g01;
+        const double e0 = H, e1 = v.L - L0, e2 = v.Q - Q0;
+        const double id = rcp<true>(det);
+        const double l0 = (c00 * e0 + c01 * e1 + c02 * e2) * id;
+        const double l1 = (c01 * e0 + c11 * e1 + c12 * e2) * id;
+        const double l2 = (c02 * e0 + c12 * e1 + c22 * e2) * id;
+        const double dx = l0 * gH[0] + l1 * gL[0] + l2 * gQ[0];
+        const double dy = l0 * gH[1] + l1 * gL[1] + l2 * gQ[1];
+        const double dz = l0 * gH[2] + l1 * gL[2] + l2 * gQ[2];
+        // a correction beyond ~1e-5 |p| is no integration error (those are ~tol = 1e-7):
+        // a degenerate solve, keep the null step only
+        const double p2 = y.px * y.px + y.py * y.py + y.pz * y.pz;
+        if (!(dx * dx + dy * dy + dz * dz < 1e-10 * p2)) { renormalise_null<METRIC>(y, a, ms); return; }
+        y.px -= dx; y.py -= dy; y.pz -= dz;
+    }
+}
+
 // Per-ray spacetime constants: the spin of the equations, the scaled horizon
 // radius, the FP64-zone bound and the centre of the escape sphere.  Uniform
 // over a launch for a single camera; per ray for batched cameras and for the
@@ -426,6 +506,13 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
     return cls;
 }
 
+template <class S> __device__ __forceinline__ void write_final(const TraceParams& P, uint64_t ray, const St<S>& y) {
+    double* f = P.final_state;
+    const uint64_t sd = P.stride;
+    f[0 * sd + ray] = double(y.qx); f[1 * sd + ray] = double(y.qy); f[2 * sd + ray] = double(y.qz); f[3 * sd + ray] = double(y.qt);
+    f[4 * sd + ray] = double(y.px); f[5 * sd + ray] = double(y.py); f[6 * sd + ray] = double(y.pz); f[7 * sd + ray] = double(y.pt);
+}
+
 template <class S> __device__ __forceinline__ S load_c(const void* rays, uint64_t stride, int c, uint64_t i) {
     return reinterpret_cast<const S*>(rays)[uint64_t(c) * stride + i];
 }
@@ -551,6 +638,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
         if (cls != MGK_HIT_NONE) {
             P.hits[ray] = rec;
             if (P.steps_out) P.steps_out[ray] = n_step;
+            if (P.final_state) write_final(P, ray, y1);
             active = false;
         } else {
             y = y1;
@@ -586,6 +674,7 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
 
     St<S> y{};
     S h = S(0), contraction = S(0);
+    S L0 = S(0), Q0 = S(0);   // the ray's axial angular momentum and Carter constant
     Lane<S> ln;
     lane_constants<METRIC, S>(P, S(P.spin), ln);
     ln.cx = S(P.cam[0]); ln.cy = S(P.cam[1]); ln.cz = S(P.cam[2]);
@@ -618,6 +707,7 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
                 st[0 * sd + pos] = y.qx; st[1 * sd + pos] = y.qy; st[2 * sd + pos] = y.qz; st[3 * sd + pos] = y.qt;
                 st[4 * sd + pos] = y.px; st[5 * sd + pos] = y.py; st[6 * sd + pos] = y.pz; st[7 * sd + pos] = y.pt;
                 st[8 * sd + pos] = h;
+                st[9 * sd + pos] = contraction; st[10 * sd + pos] = L0; st[11 * sd + pos] = Q0;
                 Z.steps_out[pos] = n_step;
                 handoff = false;
                 active = false;
@@ -649,12 +739,22 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
                     y.qt = src[3 * sd + at]; y.px = src[4 * sd + at]; y.py = src[5 * sd + at];
                     y.pz = src[6 * sd + at]; y.pt = src[7 * sd + at]; h = src[8 * sd + at];
                     n_step = Z.from_ic ? 0u : Z.steps_in[slot];
-                    contraction = load_c<S>(P.rays, P.stride, 9, ray);
+                    contraction = src[9 * sd + at];
                     if (P.per_ray) {
                         lane_constants<METRIC, S>(P, METRIC == KERR ? load_c<S>(P.rays, P.stride, 10, ray) : S(0), ln);
                         const S L = S(P.length_scale);
                         const double* ic = reinterpret_cast<const double*>(P.rays);
                         ln.cx = ic[0 * P.stride + ray] * L; ln.cy = ic[1 * P.stride + ray] * L; ln.cz = ic[2 * P.stride + ray] * L;
+                    }
+                    if (Z.from_ic) {
+                        // constants of motion of the initial conditions (NaN: not defined on the spin axis)
+                        Invariants v;
+                        const bool ok = METRIC != MINKOWSKI && invariants<METRIC>(y, double(ln.a), v);
+                        L0 = ok ? v.L : S(__builtin_nan(""));
+                        Q0 = ok ? v.Q : S(__builtin_nan(""));
+                    } else {
+                        L0 = src[10 * sd + at];
+                        Q0 = src[11 * sd + at];
                     }
                     n_sub = 0;
                     active = true;
@@ -688,12 +788,14 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
         ++n_step;
         n_sub = 0;
         ++steps_acc;
-        renormalise_null<METRIC>(y1, double(ln.a), double(P.mass_sign));
+        if (isnan(Q0)) renormalise_null<METRIC>(y1, double(ln.a), double(P.mass_sign));
+        else renormalise_constants<METRIC>(y1, double(ln.a), double(P.mass_sign), L0, Q0);
         mgk_hit_t rec;
         const uint32_t cls = classify<METRIC, S, true>(y, y1, n_step, contraction, P, ln, rec);
         if (cls != MGK_HIT_NONE) {
             P.hits[ray] = rec;
             if (P.steps_out) P.steps_out[ray] = n_step;
+            if (P.final_state) write_final(P, ray, y1);
             active = false;
         } else {
             y = y1;

@@ -189,3 +189,31 @@ def test_magik_trace_geodesic_matches_host_restatement(metric):
     k = min(10, len(g), len(r))
     assert np.allclose(g[:k, :4], r[:k, :4], rtol=1e-3, atol=1e-3 * np.abs(r[:k, :4]).max())
     assert abs(len(g) - len(r)) <= max(3, 0.1 * len(r))
+
+
+def test_trace_final_states(gpu_ctx):
+    """mgk_trace_rays_final: same hits as mgk_trace_rays, null final states,
+    escaping rays end outside the celestial sphere, horizon rays inside 1.01 r+."""
+    import torch
+    from oracle import invariants as INV
+    integ, sc, spec, cam, _ = M.workload("kerr_disk_1080p", W=64, H=36, spp=2)
+    gpu_ctx.set_scene(sc)
+    rays = gpu_ctx.generate_rays(integ, cam, 5, 0, 2)
+    h0, _ = gpu_ctx.trace(integ, sc, list(cam.position), rays, want_steps=False)
+    h1, fin = gpu_ctx.trace_final(integ, sc, list(cam.position), rays)
+    torch.cuda.synchronize()
+    assert torch.equal(h0, h1)
+    f = fin.cpu().numpy()
+    hits = M.hits_to_numpy(h1)
+    a_eq = float(np.float32(-integ.spin))
+    h, _, _ = INV.drift(f[0:3], f[4:8], f[0:3], f[4:8], a_eq)
+    assert np.abs(h).max() < 1e-6
+    L = M.length_scale()
+    cam_m = np.array(list(cam.position))[:, None]
+    d = np.linalg.norm(f[0:3] * L - cam_m, axis=0)
+    esc = hits["cls"] == M.HIT_ESCAPE
+    assert esc.any() and np.all(d[esc] >= sc.celestial_radius_m * (1 - 1e-6))
+    hor = hits["cls"] == M.HIT_HORIZON
+    r = INV.kerr_r(f[0], f[1], f[2], a_eq)
+    rp = 1 + np.sqrt(1 - a_eq * a_eq)
+    assert hor.any() and np.all(r[hor] <= 1.01 * rp * (1 + 1e-6))
