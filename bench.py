@@ -201,6 +201,9 @@ def main():
     import magik_amd as M
 
     prec = {"mixed": M.PREC_MIXED, "fp32_reference": M.PREC_FP32_REFERENCE, "fp64": M.PREC_FP64}[args.precision]
+    if args.workload == "batch64":
+        bench_batch(M, args, prec)
+        return
     integ, sc, spec, cam, spp = M.workload(args.workload, precision=prec)
     W, H = cam.x_resolution, cam.y_resolution
     n_paths = W * H * spp
@@ -304,6 +307,40 @@ def main():
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(M, integ, sc, cam, spp, args.seed)
         log(f"cpu baseline {res['cpu_baseline']['value']:.3e} rays/s")
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
+def bench_batch(M, args, prec):
+    """The batched uneven workload: 64 seeded cameras (spin, pose, resolution
+    per camera) integrated in one persistent launch per frame."""
+    import torch
+    integ, sc, spec, cams, spp = M.batch_workload(seed=args.seed, precision=prec)
+    ctx = M.Context(0)
+    ctx.set_scene(sc)
+    rays, off = ctx.generate_rays_batch(integ, cams, args.seed, spp)
+    n = rays.shape[1]
+    log(f"batch64: {len(cams)} cameras, {n} paths (resolutions {sorted(set(c.camera.x_resolution for c in cams))})")
+
+    del rays
+    torch.cuda.empty_cache()
+
+    def frame(i):
+        r_, _ = ctx.generate_rays_batch(integ, cams, args.seed + i, spp)
+        hits, _ = ctx.trace_batch(integ, sc, r_, want_steps=False)
+        st = ctx.stats()
+        del r_, hits
+        return st
+
+    wall, per = time_frames(frame, args.steps, args.warmup)
+    st = per[-1]
+    res = {"metric": "batched uneven workload rays/s (64 cameras, one persistent launch per frame)",
+           "value": n * args.steps / wall, "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32" if prec != M.PREC_FP64 else "f64", "data": "synthetic (seeded cameras and rays)",
+           "config": {"workload": f"batch64: {len(cams)} cameras, a~U[0,0.99], incl~U[10,89] deg, r~U[20,100] M, "
+                                  f"res in {{128,256,512,1024}}^2, {spp} spp, {n} paths"},
+           "trace_ms_per_frame": st.ms, "geodesic_steps_per_frame": st.n_steps, "rkf45_trials_per_frame": st.n_trials}
     ctx.close()
     print(json.dumps(res), flush=True)
 

@@ -273,6 +273,29 @@ MGK_API e_magik_result_types mgk_trace_rays_final(mgk_context_handle ctx, const 
                                                   uint64_t n_rays, const uint32_t* d_queue, mgk_hit_t* d_hits, double* d_final,
                                                   void* stream);
 
+/* Batched cameras in one persistent launch (several scenes' viewpoints with
+ * their own spin, resolution and pose).  mgk_generate_rays_batch writes the
+ * rays of camera k to [offsets[k], offsets[k + 1]) (offsets: n_cam + 1
+ * host entries, filled in), each camera's block laid out as in
+ * mgk_generate_rays, as 11 doubles per ray: the 10 components above plus the
+ * camera's spin of the equations.  integ supplies everything but the spin
+ * (precision MIXED or FP64).  mgk_trace_rays_batch integrates such rays with
+ * each ray's own spin, the escape sphere centred on each ray's origin and the
+ * disk from each ray's ISCO to scene->disk_r_out. */
+typedef struct mgk_batch_camera_t {
+    mgk_camera_t camera;
+    float spin;                  /* dimensionless a as given by the user */
+    uint32_t reserved;
+} mgk_batch_camera_t;
+
+MGK_API e_magik_result_types mgk_generate_rays_batch(mgk_context_handle ctx, const mgk_integrator_t* integ,
+                                                     const mgk_batch_camera_t* cameras, uint32_t n_cameras, uint64_t seed,
+                                                     uint32_t spp, double* d_rays, uint64_t ray_stride, uint64_t* offsets,
+                                                     void* stream);
+MGK_API e_magik_result_types mgk_trace_rays_batch(mgk_context_handle ctx, const mgk_integrator_t* integ, const mgk_scene_t* scene,
+                                                  const double* d_rays, uint64_t ray_stride, uint64_t n_rays, const uint32_t* d_queue,
+                                                  mgk_hit_t* d_hits, uint32_t* d_steps, void* stream);
+
 /* Statistics of the last mgk_trace_rays / mgk_render_frame (synchronises). */
 MGK_API e_magik_result_types mgk_last_trace_stats(mgk_context_handle ctx, mgk_trace_stats_t* out);
 

@@ -134,6 +134,29 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
     raise ValueError(f"unknown workload {name}")
 
 
+def batch_workload(seed: int = 4, n_cam: int = 64, resolutions=(128, 256, 512, 1024), spp: int = 8,
+                   precision: int = PREC_MIXED, fov_deg: float = 30.0):
+    """The batched uneven workload: n_cam seeded Boyer-Lindquist cameras with
+    spin a ~ U[0, 0.99], inclination ~ U[10, 89] deg, r_obs ~ U[20, 100] M and
+    a square resolution drawn from `resolutions`, spp samples each; a thin disk
+    from each camera's ISCO to 30 M and the 1e5-star field.  Returns
+    (integrator, scene, spectral, batch cameras, spp)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    cams = (abi.BatchCamera * n_cam)()
+    for k in range(n_cam):
+        a = float(rng.uniform(0.0, 0.99))
+        th = float(rng.uniform(10.0, 89.0))
+        r = float(rng.uniform(20.0, 100.0))
+        res = int(rng.choice(resolutions))
+        cams[k].camera = camera_bl(r, th, a, fov_deg, res, res, camera_id=k)
+        cams[k].spin = a
+    integ = integrator(metric=KERR, precision=precision, spin=0.0)
+    sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_STARS, disk_r_in=0.0, disk_r_out=30.0,
+               disk_T_in=1e4, n_star=100000, star_seed=0x5EED, star_scale=1.0, star_radius_rad=1e-3, star_T=6500.0)
+    return integ, sc, spectral(n_hero=4), cams, spp
+
+
 class Context:
     """One mgk_context on a device; device buffers are torch tensors."""
 
@@ -199,6 +222,31 @@ class Context:
                                                 C.c_void_p(hits.data_ptr()), C.c_void_p(final.data_ptr()),
                                                 self._stream()), "mgk_trace_rays_final")
         return hits, final
+
+    def generate_rays_batch(self, integ, cams, seed: int, spp: int):
+        """Rays of every batch camera: ((11, n) float64 tensor, offsets (n_cam + 1))."""
+        import numpy as np
+        import torch
+        n = sum(c.camera.x_resolution * c.camera.y_resolution for c in cams) * spp
+        rays = torch.empty((11, n), dtype=torch.float64, device=f"cuda:{self.device}")
+        off = (C.c_uint64 * (len(cams) + 1))()
+        abi.check(self.lib.mgk_generate_rays_batch(self.h, C.byref(integ), cams, len(cams), seed, spp,
+                                                   C.c_void_p(rays.data_ptr()), n, off, self._stream()),
+                  "mgk_generate_rays_batch")
+        return rays, np.array(off[:], dtype=np.uint64)
+
+    def trace_batch(self, integ, sc, rays, n: int | None = None, queue=None, want_steps: bool = True):
+        import torch
+        stride = rays.shape[1]
+        n = stride if n is None else n
+        hits = torch.zeros((stride, 4), dtype=torch.int32, device=rays.device)
+        steps = torch.zeros(stride, dtype=torch.int32, device=rays.device) if want_steps else None
+        abi.check(self.lib.mgk_trace_rays_batch(self.h, C.byref(integ), C.byref(sc), C.c_void_p(rays.data_ptr()), stride, n,
+                                                C.c_void_p(queue.data_ptr() if queue is not None else None),
+                                                C.c_void_p(hits.data_ptr()),
+                                                C.c_void_p(steps.data_ptr() if steps is not None else None),
+                                                self._stream()), "mgk_trace_rays_batch")
+        return hits, steps
 
     def shade(self, sc, spec, cam, seed: int, sample_begin: int, spp: int, hits):
         import torch

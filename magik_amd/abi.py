@@ -95,6 +95,10 @@ class Hit(C.Structure):  # mgk_hit_t
     _fields_ = [("cls", C.c_uint32), ("g", C.c_float), ("a0", C.c_float), ("a1", C.c_float)]
 
 
+class BatchCamera(C.Structure):  # mgk_batch_camera_t
+    _fields_ = [("camera", Camera), ("spin", C.c_float), ("reserved", C.c_uint32)]
+
+
 class TraceStats(C.Structure):  # mgk_trace_stats_t
     _fields_ = [("n_rays", C.c_uint64), ("n_steps", C.c_uint64), ("n_trials", C.c_uint64),
                 ("n_trials_fp64", C.c_uint64), ("ms", C.c_double)]
@@ -143,6 +147,10 @@ SIGNATURES = {
                             C.c_uint64, _P, _P, _P, _P]),
     "mgk_trace_rays_final": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(C.c_float), _P, C.c_uint64,
                                   C.c_uint64, _P, _P, _P, _P]),
+    "mgk_generate_rays_batch": (_R, [_P, C.POINTER(Integrator), C.POINTER(BatchCamera), C.c_uint32, C.c_uint64,
+                                     C.c_uint32, _P, C.c_uint64, C.POINTER(C.c_uint64), _P]),
+    "mgk_trace_rays_batch": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), _P, C.c_uint64, C.c_uint64, _P, _P, _P,
+                                  _P]),
     "mgk_last_trace_stats": (_R, [_P, C.POINTER(TraceStats)]),
     "mgk_shade": (_R, [_P, C.POINTER(Scene), C.POINTER(Spectral), C.POINTER(Camera), C.c_uint64, C.c_uint32,
                        C.c_uint32, _P, _P, _P]),

@@ -81,6 +81,7 @@ struct mgk_context {
     float band_rho = -1.0f;
     // frame buffers
     DeviceBuffer rays, hits;
+    DeviceBuffer batch_params;   // per-camera ray-generation parameters + ray offsets
     // zone-split integration: parked states and the two hand-off queues
     DeviceBuffer zone_state[2], zone_steps[2], zone_q0, zone_q1;
     int last_passes = 0;
@@ -591,6 +592,7 @@ MGK_API e_magik_result_types mgk_context_destroy(mgk_context_handle c) {
     c->cell_items.release();
     c->rays.release();
     c->hits.release();
+    c->batch_params.release();
     for (int k = 0; k < 2; ++k) {
         c->zone_state[k].release();
         c->zone_steps[k].release();
@@ -679,6 +681,61 @@ e_magik_result_types mgk_trace_rays_per_ray(mgk_context_handle c, const mgk_inte
 }
 
 extern "C" {
+
+MGK_API e_magik_result_types mgk_generate_rays_batch(mgk_context_handle c, const mgk_integrator_t* in, const mgk_batch_camera_t* cams,
+                                                     uint32_t n_cam, uint64_t seed, uint32_t spp, double* d_rays, uint64_t stride,
+                                                     uint64_t* offsets, void* stream) {
+    if (!c || !cams || !offsets || (n_cam > 0 && !d_rays)) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = check_integrator(in);
+    if (r != MAGIK_SUCCESS) return r;
+    if (in->precision == MGK_PRECISION_FP32_REFERENCE || spp == 0 || n_cam > kMaxBatchCameras) return MGK_ERROR_INVALID_ARGUMENT;
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    std::vector<RayGenParams> P(n_cam);
+    std::vector<uint64_t> off(size_t(n_cam) + 1, 0);
+    for (uint32_t k = 0; k < n_cam; ++k) {
+        mgk_integrator_t ik = *in;
+        ik.spin = cams[k].spin;
+        const uint64_t nk = uint64_t(cams[k].camera.x_resolution) * cams[k].camera.y_resolution * spp;
+        off[k + 1] = off[k] + nk;
+        if ((r = fill_raygen(&ik, &cams[k].camera, seed, 0, spp, d_rays, stride, P[k])) != MAGIK_SUCCESS) return r;
+        P[k].per_ray_spin = 1;
+    }
+    for (uint32_t k = 0; k <= n_cam; ++k) offsets[k] = off[k];
+    if (off[n_cam] > stride) return MGK_ERROR_CAPACITY_EXCEEDED;
+    if (n_cam == 0 || off[n_cam] == 0) return MAGIK_SUCCESS;
+    hipStream_t s = hipStream_t(stream);
+    HIP_TRY(c->batch_params.reserve(P.size() * sizeof(RayGenParams) + off.size() * sizeof(uint64_t)));
+    RayGenParams* d_p = reinterpret_cast<RayGenParams*>(c->batch_params.ptr);
+    uint64_t* d_off = reinterpret_cast<uint64_t*>(d_p + n_cam);
+    HIP_TRY(hipMemcpyAsync(d_p, P.data(), P.size() * sizeof(RayGenParams), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_generate_rays_batch(d_p, d_off, n_cam, off[n_cam], s));
+    HIP_TRY(hipStreamSynchronize(s));   // the host staging vectors go out of scope
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_trace_rays_batch(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc,
+                                                  const double* d_rays, uint64_t stride, uint64_t n, const uint32_t* d_queue,
+                                                  mgk_hit_t* d_hits, uint32_t* d_steps, void* stream) {
+    if (!c || !sc || !d_hits || (n > 0 && !d_rays)) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = check_integrator(in);
+    if (r != MAGIK_SUCCESS) return r;
+    if (in->precision == MGK_PRECISION_FP32_REFERENCE) return MGK_ERROR_INVALID_ARGUMENT;
+    if (!d_queue && n > stride) return MGK_ERROR_CAPACITY_EXCEEDED;
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    const float origin[3] = {0.0f, 0.0f, 0.0f};
+    TraceParams P;
+    fill_trace_params(in, sc, origin, P);
+    P.rays = d_rays;
+    P.stride = stride;
+    P.n = n;
+    P.queue = d_queue;
+    P.hits = d_hits;
+    P.steps_out = d_steps;
+    P.per_ray = 1;
+    P.disk_isco = 1;
+    return do_trace(c, P, hipStream_t(stream));
+}
 
 MGK_API e_magik_result_types mgk_last_trace_stats(mgk_context_handle c, mgk_trace_stats_t* out) {
     if (!c || !out) return MAGIK_ERROR_INVALID_POINTER;

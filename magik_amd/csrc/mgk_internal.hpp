@@ -24,7 +24,10 @@ struct RayGenParams {
     uint32_t sample_begin, spp;
     void* rays;
     uint64_t stride;
+    int per_ray_spin;            // also write the spin of the equations as component 10
 };
+
+constexpr uint32_t kMaxBatchCameras = 4096;
 
 struct TraceParams {
     int metric, precision;
@@ -121,6 +124,7 @@ struct StarView {
 };
 
 hipError_t launch_generate_rays(const RayGenParams& p, hipStream_t s);
+hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t* d_offsets, uint32_t n_cam, uint64_t n, hipStream_t s);
 hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s);
 hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s);

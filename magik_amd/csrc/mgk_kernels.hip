@@ -819,7 +819,7 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
 // Ray generation + initial conditions.
 // ---------------------------------------------------------------------------
 template <int METRIC, class S>
-__device__ __forceinline__ void gen_one(const RayGenParams& P, uint64_t i) {
+__device__ __forceinline__ void gen_one(const RayGenParams& P, uint64_t i, uint64_t o) {
 #pragma clang fp contract(off)
     const uint32_t npix = P.W * P.H;
     const uint32_t pixel = uint32_t(i % npix);
@@ -854,16 +854,35 @@ __device__ __forceinline__ void gen_one(const RayGenParams& P, uint64_t i) {
     V4<S> pm = lower(g, pup);
     S* out = reinterpret_cast<S*>(P.rays);
     const uint64_t st = P.stride;
-    out[0 * st + i] = qx; out[1 * st + i] = qy; out[2 * st + i] = qz; out[3 * st + i] = qt;
-    out[4 * st + i] = pm.x; out[5 * st + i] = pm.y; out[6 * st + i] = pm.z; out[7 * st + i] = pm.t;
-    out[8 * st + i] = METRIC == MINKOWSKI ? S(1000) : S(1);
-    out[9 * st + i] = dot4(uobs, pm);
+    out[0 * st + o] = qx; out[1 * st + o] = qy; out[2 * st + o] = qz; out[3 * st + o] = qt;
+    out[4 * st + o] = pm.x; out[5 * st + o] = pm.y; out[6 * st + o] = pm.z; out[7 * st + o] = pm.t;
+    out[8 * st + o] = METRIC == MINKOWSKI ? S(1000) : S(1);
+    out[9 * st + o] = dot4(uobs, pm);
+    if (P.per_ray_spin) out[10 * st + o] = a;   // the spin of the equations, per ray (batched cameras)
 }
 
 template <int METRIC, class S> __global__ void __launch_bounds__(256) k_generate_rays(RayGenParams P) {
     const uint64_t n = uint64_t(P.W) * P.H * P.spp;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-        gen_one<METRIC, S>(P, i);
+        gen_one<METRIC, S>(P, i, i);
+}
+
+// Batched cameras: ray i belongs to camera c with offsets[c] <= i < offsets[c + 1]
+// (at most kMaxBatchCameras); each camera's rays are laid out as a single
+// camera's, sample-major, with the camera's own spin in component 10.
+__global__ void __launch_bounds__(256) k_generate_rays_batch(const RayGenParams* cams, const uint64_t* offsets, uint32_t n_cam,
+                                                             uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
+        uint32_t lo = 0, hi = n_cam;   // last c with offsets[c] <= i
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (offsets[mid] <= i) lo = mid; else hi = mid;
+        }
+        const RayGenParams& P = cams[lo];
+        if (P.metric == KERR) gen_one<KERR, double>(P, i - offsets[lo], i);
+        else if (P.metric == SCHWARZSCHILD) gen_one<SCHWARZSCHILD, double>(P, i - offsets[lo], i);
+        else gen_one<MINKOWSKI, double>(P, i - offsets[lo], i);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1099,6 +1118,14 @@ hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool nea
         else hipLaunchKernelGGL((k_trace_zone<M, false>), dim3(grid), dim3(256), 0, s, p, z);
         return hipGetLastError();
     });
+}
+
+hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t* d_offsets, uint32_t n_cam, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + 255) / 256;
+    if (blocks > 65536ull * 16) blocks = 65536ull * 16;
+    hipLaunchKernelGGL(k_generate_rays_batch, dim3(unsigned(blocks)), dim3(256), 0, s, d_cams, d_offsets, n_cam, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s) {
