@@ -20,7 +20,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++20", "-O3", "--offload-arch=gfx950", "-fPIC", "-shared",
     "-fvisibility=hidden",
-    # packed FP32 (v_pk_*_f32) stays off: see DESIGN.md, "Compiler settings"
+    # packed FP32 (v_pk_*_f32) stays off: forms of it have produced silently wrong results
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 

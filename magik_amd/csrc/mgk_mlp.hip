@@ -7,7 +7,7 @@
 //                    fused epilogues (bias + SiLU -> z, h; SiLU' -> dz, g).
 //                    Used for the training forward, backward-data and
 //                    weight-gradient products.
-//   k_mlp_infer    : the whole network for 256 rays per workgroup, computed
+//   k_mlp_infer    : the whole network for 128 rays per workgroup, computed
 //                    transposed (Z^T = W^T H^T: weights are the MFMA A
 //                    operand, rays the columns) so that a layer's FP32
 //                    accumulators, after bias + SiLU, are already the next
@@ -207,17 +207,27 @@ __global__ void k_adam(float* w, float* m, float* v, const float* g, int64_t n, 
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: 4 waves x 2 ray tiles x 32 rays per workgroup.
+// Fused inference: 4 waves x MGK_INFER_TILES ray tiles x 32 rays per workgroup.
 // ---------------------------------------------------------------------------
+// One 32-ray tile per wave and two workgroups per CU (one workgroup's SiLU
+// epilogue overlaps the other's matrix work), hidden weights staged two
+// k-steps at a time: measured fastest of 1-2 tiles x 1-2 workgroups x 1-4
+// k-steps per stage (32.1 ms for a 1080p x 32 frame, 825 TFLOP/s).
+#ifndef MGK_INFER_TILES
+#define MGK_INFER_TILES 1
+#define MGK_INFER_OCC 2
+#endif
+#ifndef MGK_INFER_STAGE
+#define MGK_INFER_STAGE 2
+#endif
 constexpr int kInferWaves = 4;
-constexpr int kInferTiles = 2;                            // 32-ray tiles per wave
-constexpr int kInferRays = kInferWaves * kInferTiles * 32;
 constexpr int kStepFrags = kMlpHidden / 32;               // A fragments per hidden k-step
 
 __device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 // bias + SiLU of one layer's accumulators into the next layer's B operand:
 // H[c][2 nt + h] holds elements i = 8 h .. 8 h + 7 of accumulator tile nt.
+template <int kInferTiles>
 __device__ __forceinline__ void infer_epilogue(const f32x16 (&acc)[kInferTiles][kStepFrags], const float* bias, int g,
                                                bf16x8 (&H)[kInferTiles][2 * kStepFrags]) {
 #pragma unroll
@@ -232,16 +242,23 @@ __device__ __forceinline__ void infer_epilogue(const f32x16 (&acc)[kInferTiles][
     }
 }
 
-__global__ void __launch_bounds__(256, 1) k_mlp_infer(InferArgs a) {
-    __shared__ bf16x8 wbuf[2][kStepFrags * 64];          // one hidden k-step, double buffered (16 KB)
+// TILES 32-ray tiles per wave (each staged weight fragment serves TILES
+// MFMAs); WG_PER_CU workgroups per CU (with two, one workgroup's SiLU
+// epilogue overlaps the other's matrix work).
+template <int kInferTiles, int WG_PER_CU>
+__global__ void __launch_bounds__(256, WG_PER_CU) k_mlp_infer(InferArgs a) {
+    constexpr int kInferRays = kInferWaves * kInferTiles * 32;
+    constexpr int kStage = MGK_INFER_STAGE;               // hidden k-steps per LDS stage
+    constexpr int kStageFrags = kStage * kStepFrags * 64;  // bf16x8 per stage
+    __shared__ bf16x8 wbuf[2][kStageFrags];               // double buffered (kStage x 8 KB each)
     __shared__ float bias[4 * kMlpHidden + kMlpOutPad];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 5, r = lane & 31;
     for (int k = tid; k < 4 * kMlpHidden; k += 256) bias[k] = a.b[k / kMlpHidden][k % kMlpHidden];
     if (tid < kMlpOutPad) bias[4 * kMlpHidden + tid] = a.b[kMlpLayers - 1][tid];
     {
         const bf16x8* F1 = reinterpret_cast<const bf16x8*>(a.Wf[1]);
-        wbuf[0][tid] = F1[tid];
-        wbuf[0][tid + 256] = F1[tid + 256];
+#pragma unroll
+        for (int k = 0; k < 2 * kStage; ++k) wbuf[0][tid + 256 * k] = F1[tid + 256 * k];
     }
     const int64_t ray0 = int64_t(blockIdx.x) * kInferRays + w * (kInferTiles * 32);
     bf16x8 H[kInferTiles][2 * kStepFrags];
@@ -286,26 +303,30 @@ __global__ void __launch_bounds__(256, 1) k_mlp_infer(InferArgs a) {
 #pragma unroll
                 for (int i = 0; i < 16; ++i) acc[c][nt][i] = 0.0f;
 #pragma unroll
-        for (int ks = 0; ks < 2 * kStepFrags; ++ks) {
-            const bool last = ks == 2 * kStepFrags - 1;
+        for (int st = 0; st < 2 * kStepFrags / kStage; ++st) {
+            const bool last = st == 2 * kStepFrags / kStage - 1;
             const bool has_next = !last || l + 1 < kMlpLayers - 1;
-            bf16x8 n0, n1;
+            bf16x8 nx[2 * kStage];
             if (has_next) {
                 const bf16x8* Fn = last ? reinterpret_cast<const bf16x8*>(a.Wf[l + 1])
-                                        : reinterpret_cast<const bf16x8*>(a.Wf[l]) + (ks + 1) * (kStepFrags * 64);
-                n0 = Fn[tid];
-                n1 = Fn[tid + 256];
+                                        : reinterpret_cast<const bf16x8*>(a.Wf[l]) + (st + 1) * kStageFrags;
+#pragma unroll
+                for (int k = 0; k < 2 * kStage; ++k) nx[k] = Fn[tid + 256 * k];
             }
 #pragma unroll
-            for (int nt = 0; nt < kStepFrags; ++nt) {
-                const bf16x8 A = wbuf[ks & 1][nt * 64 + lane];
+            for (int kk = 0; kk < kStage; ++kk) {
+                const int ks = st * kStage + kk;
 #pragma unroll
-                for (int c = 0; c < kInferTiles; ++c)
-                    acc[c][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, H[c][ks], acc[c][nt], 0, 0, 0);
+                for (int nt = 0; nt < kStepFrags; ++nt) {
+                    const bf16x8 A = wbuf[st & 1][(kk * kStepFrags + nt) * 64 + lane];
+#pragma unroll
+                    for (int c = 0; c < kInferTiles; ++c)
+                        acc[c][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, H[c][ks], acc[c][nt], 0, 0, 0);
+                }
             }
             if (has_next) {
-                wbuf[(ks + 1) & 1][tid] = n0;
-                wbuf[(ks + 1) & 1][tid + 256] = n1;
+#pragma unroll
+                for (int k = 0; k < 2 * kStage; ++k) wbuf[(st + 1) & 1][tid + 256 * k] = nx[k];
             }
             __syncthreads();
         }
@@ -380,7 +401,9 @@ hipError_t launch_adam(float* w, float* m, float* v, const float* g, int64_t n, 
 }
 hipError_t launch_infer(const InferArgs& a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_mlp_infer, dim3(unsigned((a.n + kInferRays - 1) / kInferRays)), dim3(256), 0, s, a);
+    constexpr int kTiles = MGK_INFER_TILES, kOcc = MGK_INFER_OCC;
+    constexpr int kRays = kInferWaves * kTiles * 32;
+    hipLaunchKernelGGL((k_mlp_infer<kTiles, kOcc>), dim3(unsigned((a.n + kRays - 1) / kRays)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
