@@ -45,6 +45,10 @@ ERROR_BAR = 1e-3
 FLOP_PER_TRIAL = {2: 1428, 1: 1272, 0: 0}
 # MLP forward, 5 -> 4 x 256 SiLU -> 6 outputs: 2 (5*256 + 3*256*256 + 256*6)
 FLOP_PER_INFERENCE = 398848
+# HBM bytes moved by the integration of one Kerr 1080p x 32 frame (all zone
+# passes): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors being what
+# the same counters read for a copy kernel of known size (bench.py --copy-probe).
+TRACE_TRAFFIC_BYTES = {"kerr_disk_1080p": (2802924 + 1091326) * 2.0 * 1024 + (3813746 + 679575) * 1024.0}
 
 
 def log(msg):
@@ -297,7 +301,9 @@ def main():
         "surrogate": sur_res,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "frac_of_unpacked_peak": achieved / PEAK_FP32_UNPACKED_TFLOPS,
-                     "traffic": None, "kernel": "k_trace_zone (far + near passes of one frame)",
+                     "traffic": TRACE_TRAFFIC_BYTES.get(args.workload) if prec == M.PREC_MIXED else None,
+                     "traffic_unit": "bytes per frame (all zone passes)",
+                     "kernel": "k_trace_zone (far + near passes of one frame)",
                      "note": "integration kernel of the plain path: FP32 vector ALU (no MFMA on this path); "
                              f"algorithmic FLOPs = {FLOP_PER_TRIAL[integ.metric_type]} per RKF45 trial x trials "
                              "per frame / HIP-event time of the frame's integration on the library stream"},
