@@ -183,6 +183,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--train-rays", type=int, default=1 << 24)
     ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="only time the inference kernel over one 1080p x 32 frame of feature rows (counter runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a copy kernel over BYTES three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -202,6 +204,24 @@ def main():
 
     import torch
     import magik_amd as M
+
+    if args.mlp_only:
+        n = 1920 * 1080 * 32
+        sur = M.Surrogate(0, seed=args.seed)
+        xf = torch.zeros((n, 16), dtype=torch.float32, device="cuda:0")
+        xf[:, :5].uniform_(-1.0, 1.0)
+        sur.forward(xf)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(3):
+            sur.forward(xf)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        print(json.dumps({"kernel": "k_mlp_infer", "rows": n, "ms": ms,
+                          "tflops": FLOP_PER_INFERENCE * n / (ms * 1e-3) / 1e12}), flush=True)
+        return
 
     prec = {"mixed": M.PREC_MIXED, "fp32_reference": M.PREC_FP32_REFERENCE, "fp64": M.PREC_FP64}[args.precision]
     if args.workload == "batch64":

@@ -101,13 +101,17 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
     }
 }
 
-// Column sums (bias gradients): out[n] = sum_m X[m, n], FP32 atomics per block.
+// Column sums (bias gradients): out[n] = sum_m X[m, n]; each block sums
+// kColsumRows rows (coalesced along n), one FP32 atomic per column and block.
+constexpr int kColsumRows = 64;
 __global__ void __launch_bounds__(256) k_colsum(const float* X, int M, int N, float* out) {
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= N) return;
-    const int r0 = blockIdx.y * 512, r1 = min(M, r0 + 512);
     float s = 0.0f;
-    for (int m = r0; m < r1; ++m) s += X[int64_t(m) * N + n];
+    for (int r0 = blockIdx.y * kColsumRows; r0 < M; r0 += gridDim.y * kColsumRows) {
+        const int r1 = min(M, r0 + kColsumRows);
+        for (int m = r0; m < r1; ++m) s += X[int64_t(m) * N + n];
+    }
     atomicAdd(&out[n], s);
 }
 
@@ -365,7 +369,8 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s) {
     return hipGetLastError();
 }
 hipError_t launch_colsum(const float* X, int M, int N, float* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256, (M + 511) / 512), dim3(256), 0, s, X, M, N, out);
+    const int blocks_y = std::min((M + kColsumRows - 1) / kColsumRows, 4096);
+    hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256, blocks_y), dim3(256), 0, s, X, M, N, out);
     return hipGetLastError();
 }
 hipError_t launch_zero(float* p, int64_t n, hipStream_t s) {
