@@ -57,8 +57,9 @@ struct TraceParams {
     int disk_isco;               // per-ray rays: the disk starts at the ISCO of the ray's spin
 };
 
-// Doubles parked per handed-off ray: q (4), p (4), h, u.p, L0, Q0.
-constexpr int kParked = 12;
+// Doubles parked per handed-off ray: q (4), p (4), h, u.p, L0, Q0, and the
+// step controller's memory (last accepted h and ln error).
+constexpr int kParked = 14;
 
 // One pass of the zone-split mixed-precision integrator.
 struct ZoneParams {

@@ -675,6 +675,7 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
     St<S> y{};
     S h = S(0), contraction = S(0);
     S L0 = S(0), Q0 = S(0);   // the ray's axial angular momentum and Carter constant
+    float h_acc = 0.0f, lnerr_acc = 0.0f;   // last accepted trial: h and ln(error); h_acc = 0: none yet
     Lane<S> ln;
     lane_constants<METRIC, S>(P, S(P.spin), ln);
     ln.cx = S(P.cam[0]); ln.cy = S(P.cam[1]); ln.cz = S(P.cam[2]);
@@ -708,6 +709,7 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
                 st[4 * sd + pos] = y.px; st[5 * sd + pos] = y.py; st[6 * sd + pos] = y.pz; st[7 * sd + pos] = y.pt;
                 st[8 * sd + pos] = h;
                 st[9 * sd + pos] = contraction; st[10 * sd + pos] = L0; st[11 * sd + pos] = Q0;
+                st[12 * sd + pos] = double(h_acc); st[13 * sd + pos] = double(lnerr_acc);
                 Z.steps_out[pos] = n_step;
                 handoff = false;
                 active = false;
@@ -752,9 +754,13 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
                         const bool ok = METRIC != MINKOWSKI && invariants<METRIC>(y, double(ln.a), v);
                         L0 = ok ? v.L : S(__builtin_nan(""));
                         Q0 = ok ? v.Q : S(__builtin_nan(""));
+                        h_acc = 0.0f;
+                        lnerr_acc = 0.0f;
                     } else {
                         L0 = src[10 * sd + at];
                         Q0 = src[11 * sd + at];
+                        h_acc = float(src[12 * sd + at]);
+                        lnerr_acc = float(src[13 * sd + at]);
                     }
                     n_sub = 0;
                     active = true;
@@ -778,18 +784,31 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
         }
         ++trials_acc;
         const double rs = rms > 1e-30 ? rms : 1e-30;
-        // The reference's step factor 0.9 err^-1/5 (internal_math.h:2190-2195),
-        // limited to [0.2, 5] and to 1 right after a rejected trial (Hairer,
-        // Norsett & Wanner, Solving ODEs I, II.4).  Unlimited growth after
-        // easy steps overshoots; the cap makes this frame's integration 20 %
-        // faster at the same tolerance.  The FP32-reference kernel keeps the
-        // reference's unlimited factor.
-        const float fac = fmaxf(fminf(0.9f * __expf(-0.2f * __logf(float(rs))), n_sub > 0 ? 1.0f : 5.0f), 0.2f);
+        // Step factor: the reference's 0.9 err^-1/5 (internal_math.h:2190-2195).
+        // After an accepted trial it is also bounded by Gustafsson's
+        // predictive factor 0.9 (h / h_acc) (err_acc / err^2)^1/5 (Gustafsson,
+        // ACM TOMS 20 (1994); Hairer & Wanner, Solving ODEs II, IV.8), which
+        // extrapolates the growth of the error constant from the last two
+        // accepted trials.  A ray falling toward the hole meets an error
+        // constant that grows several-fold per step; the plain factor then
+        // over-proposes and every step is rejected once.  The factor is
+        // limited to [0.2, 5], and to 1 right after a rejection (Hairer,
+        // Norsett & Wanner, Solving ODEs I, II.4).  The FP32-reference kernel
+        // keeps the reference's controller.
+        const float lr = __logf(float(rs));
+        const bool accept = (rs < 1.0) || (n_sub + 1 >= P.n_max);
+        float fac = 0.9f * __expf(-0.2f * lr);
+        if (accept && h_acc > 0.0f) fac = fminf(fac, 0.9f * (float(h) / h_acc) * __expf(0.2f * lnerr_acc - 0.4f * lr));
+        fac = fmaxf(fminf(fac, n_sub > 0 ? 1.0f : 5.0f), 0.2f);
+        if (accept) {
+            h_acc = float(h);
+            lnerr_acc = fmaxf(lr, -4.6051702f);   // ln 1e-2: a tiny error says little about the trend
+        }
         double hn = h * double(fac);
         if (hn > double(P.h_max)) hn = double(P.h_max);
         h = hn;
         ++n_sub;
-        if (!((rs < 1.0) || (n_sub >= P.n_max))) continue;
+        if (!accept) continue;
 
         // ---- step complete ----
         ++n_step;
