@@ -41,6 +41,20 @@ template <bool FAST> __device__ __forceinline__ double rcp(double x) {
         return 1.0 / x;
     }
 }
+// Reciprocal square root: v_rsq_f32 (1 ulp), or for FP64 the v_rsq_f64 seed
+// with one Newton step (relative error e -> 1.5 e^2).
+template <bool FAST> __device__ __forceinline__ float rsq(float x) {
+    if constexpr (FAST) return __builtin_amdgcn_rsqf(x);
+    else return 1.0f / __fsqrt_rn(x);
+}
+template <bool FAST> __device__ __forceinline__ double rsq(double x) {
+    if constexpr (FAST) {
+        const double y = __builtin_amdgcn_rsq(x);
+        return y * fma(-0.5 * x * y, y, 1.5);
+    } else {
+        return 1.0 / __dsqrt_rn(x);
+    }
+}
 template <bool FAST> __device__ __forceinline__ float sqrt_(float x) {
     if constexpr (FAST) return __builtin_amdgcn_sqrtf(x);
     else return __fsqrt_rn(x);
@@ -116,60 +130,72 @@ __device__ __forceinline__ Rhs<S> rhs(S x, S y, S z, S px, S py, S pz, S pt, S a
         o.cx = o.cy = o.cz = o.ct = o.dpx = o.dpy = o.dpz = S(0);
         return o;
     } else {
-        S a_sq, w_sq, sigma;
+        // r from r^4 - (rho^2 - a^2) r^2 - a^2 y^2 = 0: 2 r^2 = w + sigma with
+        // w = rho^2 - a^2, sigma = sqrt(w^2 + 4 a^2 y^2).  The same quartic
+        // gives r^4 + a^2 y^2 = r^2 sigma, hence
+        //   f = 2 r^3 / (r^4 + a^2 y^2) = 2 r / sigma,
+        //   dr/dx = x r / sigma, dr/dz = z r / sigma, dr/dy = y (r^2 + a^2) / (r sigma),
+        // and with A = x p_x + z p_z, B = x p_z - z p_x, d = r^2 + a^2, the
+        // spatial k.p = (r A + a B) / d + p_y y / r, whose gradient at fixed p is
+        //   p.d_i k = dr_i [(A - 2 r u) / d - p_y y / r^2] + (r dA_i + a dB_i) / d + delta_iy p_y / r
+        // with u = (r A + a B) / d.  Three hardware seeds (two rsq, one rcp).
+        const S y_sq = y * y;
+        S a_sq, sigma, inv_sigma, r_sq;
         if constexpr (METRIC == KERR) {
             a_sq = a * a;
-            w_sq = (x * x + y * y + z * z) - a_sq;
-            sigma = sqrt_<FAST>(S(4) * a_sq * (y * y) + w_sq * w_sq);
+            const S w = (x * x + y_sq + z * z) - a_sq;
+            const S sig_sq = S(4) * a_sq * y_sq + w * w;
+            inv_sigma = rsq<FAST>(sig_sq);
+            sigma = sig_sq * inv_sigma;
+            r_sq = S(0.5) * (w + sigma);
         } else {
             a_sq = S(0);
-            w_sq = (x * x + y * y + z * z);
-            sigma = w_sq;
+            r_sq = x * x + y_sq + z * z;
         }
-        S r = sqrt_<FAST>(S(0.5) * (w_sq + sigma));
-        S r_sq = r * r, r_cu = r_sq * r;
-        S inv_r = rcp<FAST>(r);
-        S inv_rsq = inv_r * inv_r;
-        S inv_sigma = rcp<FAST>(sigma);
-        S scale_xz = S(0.5) * inv_r * (S(1) + w_sq * inv_sigma);
-        S scale_y = S(0.5) * inv_r * (S(1) + (w_sq + S(2) * a_sq) * inv_sigma);
-        S drx = x * scale_xz, dry = y * scale_y, drz = z * scale_xz;
-        S d = r_sq + a_sq;
-        S inv_d = rcp<FAST>(d);
-        S inv_dsq = inv_d * inv_d;
-        S kx_num = r * x - a * z;
-        S kz_num = r * z + a * x;
-        S kx = kx_num * inv_d, kz = kz_num * inv_d, ky = y * inv_r;
-        S f_den = r_sq * r_sq + a_sq * (y * y);
-        S inv_f_den = rcp<FAST>(f_den);
-        S inv_f_den_sq = inv_f_den * inv_f_den;
+        const S inv_r = rsq<FAST>(r_sq);
+        const S r = r_sq * inv_r;
+        const S inv_rsq = inv_r * inv_r;
+        if constexpr (METRIC != KERR) { sigma = r_sq; inv_sigma = inv_rsq; }
+        const S d = r_sq + a_sq;
+        const S inv_d = METRIC == KERR ? rcp<FAST>(d) : inv_rsq;
+        const S s_xz = r * inv_sigma, s_y = d * inv_r * inv_sigma;
+        const S drx = x * s_xz, dry = y * s_y, drz = z * s_xz;
+        const S inv_sig_sq = inv_sigma * inv_sigma;
         S f, dfs, dfy_extra;
         if constexpr (METRIC == KERR) {
-            f = S(2) * r_cu * inv_f_den;
-            dfs = S(2) * (S(3) * a_sq * r_sq * (y * y) - r_cu * r_cu) * inv_f_den_sq;
-            dfy_extra = S(4) * a_sq * r_cu * y * inv_f_den_sq;
+            f = S(2) * r * inv_sigma;
+            dfs = S(2) * (S(3) * a_sq * y_sq - r_sq * r_sq) * (inv_rsq * inv_sig_sq);
+            dfy_extra = S(4) * a_sq * y * (inv_r * inv_sig_sq);
         } else {
-            f = mass_sign * S(2) * r_cu * inv_f_den;
-            dfs = mass_sign * S(2) * -(r_cu * r_cu) * inv_f_den_sq;
+            f = mass_sign * S(2) * inv_r;
+            dfs = mass_sign * S(-2) * inv_rsq;
             dfy_extra = S(0);
         }
-        S dfx = dfs * drx, dfy = dfs * dry - dfy_extra, dfz = dfs * drz;
-        S two_r = S(2) * r;
-        S dkx_dx = ((x * drx + r) * d - two_r * drx * kx_num) * inv_dsq;
-        S dkx_dy = ((x * dry) * d - two_r * dry * kx_num) * inv_dsq;
-        S dkx_dz = ((x * drz - a) * d - two_r * drz * kx_num) * inv_dsq;
-        S dky_dx = -y * drx * inv_rsq;
-        S dky_dy = (r - y * dry) * inv_rsq;
-        S dky_dz = -y * drz * inv_rsq;
-        S dkz_dx = ((a + z * drx) * d - two_r * drx * kz_num) * inv_dsq;
-        S dkz_dy = ((z * dry) * d - two_r * dry * kz_num) * inv_dsq;
-        S dkz_dz = ((r + z * drz) * d - two_r * drz * kz_num) * inv_dsq;
-        S kTp = px * kx + py * ky + pz * kz - pt;
-        S f_kTp = f * kTp;
-        S half_kTp_sq = S(0.5) * kTp * kTp;
-        S bx = px * dkx_dx + py * dky_dx + pz * dkz_dx;
-        S by = px * dkx_dy + py * dky_dy + pz * dkz_dy;
-        S bz = px * dkx_dz + py * dky_dz + pz * dkz_dz;
+        const S dfx = dfs * drx, dfy = dfs * dry - dfy_extra, dfz = dfs * drz;
+        const S A = x * px + z * pz;
+        const S ky = y * inv_r;
+        S u, kx, kz, ex, ez;
+        if constexpr (METRIC == KERR) {
+            const S B = x * pz - z * px;
+            u = (r * A + a * B) * inv_d;
+            kx = (r * x - a * z) * inv_d;
+            kz = (r * z + a * x) * inv_d;
+            ex = (r * px + a * pz) * inv_d;
+            ez = (r * pz - a * px) * inv_d;
+        } else {
+            u = A * inv_r;
+            kx = x * inv_r;
+            kz = z * inv_r;
+            ex = px * inv_r;
+            ez = pz * inv_r;
+        }
+        const S kTp = (u + py * ky) - pt;
+        const S Hc = (A - S(2) * r * u) * inv_d - py * ky * inv_r;
+        const S bx = drx * Hc + ex;
+        const S by = dry * Hc + py * inv_r;
+        const S bz = drz * Hc + ez;
+        const S f_kTp = f * kTp;
+        const S half_kTp_sq = S(0.5) * kTp * kTp;
         o.cx = f_kTp * kx;
         o.cy = f_kTp * ky;
         o.cz = f_kTp * kz;
