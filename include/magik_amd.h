@@ -258,7 +258,8 @@ MGK_API e_magik_result_types mgk_generate_rays(mgk_context_handle ctx, const mgk
 /* Persistent geodesic integration (replaces kernel_advance_geodesic,
  * kernel_intersect, kernel_enforce_ray_limits and their queues).  Rays are
  * pulled from [0, n_rays), or from d_queue[0..n_rays) when d_queue is not
- * NULL; hit i (or d_queue[i]'s slot) receives the result.  d_steps may be NULL. */
+ * NULL (entries below ray_stride and below 0xFFFFFFFF); hit i (or d_queue[i]'s
+ * slot) receives the result.  d_steps may be NULL. */
 MGK_API e_magik_result_types mgk_trace_rays(mgk_context_handle ctx, const mgk_integrator_t* integ, const mgk_scene_t* scene,
                                             const float camera_position[3], const void* d_rays, uint64_t ray_stride,
                                             uint64_t n_rays, const uint32_t* d_queue, mgk_hit_t* d_hits,

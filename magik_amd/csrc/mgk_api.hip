@@ -183,7 +183,12 @@ int trace_grid(mgk_context* c, const TraceParams& P) {
 // Mixed precision: alternate the far-zone (FP32) and near-zone (FP64) passes
 // until no ray is handed over.  Reads one queue count per pass (a 8-byte copy).
 e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t s) {
-    const uint64_t cap = std::max<uint64_t>(P.n, 1);   // a pass hands off at most the rays it was given
+    if (P.n >= uint64_t(kHole)) return MGK_ERROR_CAPACITY_EXCEEDED;   // ray ids travel as uint32 below kHole
+    // A pass hands off each of its rays at most once; slots are reserved kOutBlock
+    // per wave, so the unused tails of the waves' last blocks add at most
+    // (waves of the largest launch) * kOutBlock.
+    const uint64_t max_waves = uint64_t(c->n_cu) * 4u * 4u;
+    const uint64_t cap = P.n + max_waves * kOutBlock;
     for (int k = 0; k < 2; ++k) {
         HIP_TRY(c->zone_state[k].reserve(cap * kParked * sizeof(double)));
         HIP_TRY(c->zone_steps[k].reserve(cap * sizeof(uint32_t)));
@@ -217,7 +222,9 @@ e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t 
         HIP_TRY(hipGetLastError());
         TraceParams Q = P;
         Q.n = n_in;
-        HIP_TRY(launch_trace_zone(Q, Z, near || Z.no_handoff, trace_grid(c, Q), s));
+        const int grid = trace_grid(c, Q);
+        if (uint64_t(grid) * 4u > max_waves) return MGK_ERROR_CAPACITY_EXCEEDED;   // the slot bound above
+        HIP_TRY(launch_trace_zone(Q, Z, near || Z.no_handoff, grid, s));
         ++pass;
         unsigned long long next = 0;
         HIP_TRY(hipMemcpyAsync(&next, cnt[(pass - 1) & 1], sizeof(next), hipMemcpyDeviceToHost, s));

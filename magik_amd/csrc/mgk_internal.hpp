@@ -61,11 +61,18 @@ struct TraceParams {
 // step controller's memory (last accepted h and ln error).
 constexpr int kParked = 14;
 
+// Hand-off queue slots are reserved kOutBlock at a time per wave: one atomic
+// per block of hand-offs instead of one per wave and iteration.  Slots of a
+// wave's last block that stay unused hold kHole and are skipped by the next
+// pass; a pass reserves at most n_in + (waves of the launch) * kOutBlock slots.
+constexpr uint32_t kOutBlock = 256;
+constexpr uint32_t kHole = 0xFFFFFFFFu;
+
 // One pass of the zone-split mixed-precision integrator.
 struct ZoneParams {
     int from_ic;                     // load rays from the IC buffer (first pass) instead of the parked state
     int no_handoff;                  // integrate to termination regardless of the zone
-    const uint32_t* in_queue;        // ray ids of this pass (NULL: 0 .. n_in-1)
+    const uint32_t* in_queue;        // ray ids of this pass (NULL: 0 .. n_in-1); kHole entries are skipped
     uint64_t n_in;
     uint32_t* out_queue;             // rays handed to the other zone
     unsigned long long* out_count;

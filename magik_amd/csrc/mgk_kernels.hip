@@ -685,6 +685,8 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
     uint64_t steps_acc = 0, trials_acc = 0, fp64_acc = 0;
     uint64_t chunk_next = 0, chunk_end = 0;
     bool exhausted = false;
+    uint64_t ob_base = 0;          // the wave's current block of out-queue slots
+    uint32_t ob_used = kOutBlock;  // slots of it taken (kOutBlock: no block yet)
     // hysteresis: enter the near zone at rho^2 < zone, leave it at rho^2 > 1.21 zone
     auto in_zone = [&](const St<S>& q) {
         const S r2 = q.qx * q.qx + q.qy * q.qy + q.qz * q.qz;
@@ -695,13 +697,19 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
         // ---- park the rays that crossed the zone boundary (wave-uniform) ----
         const uint64_t ho = __ballot(handoff);
         if (ho != 0) {
-            unsigned long long base = 0;
-            const int leader = __ffsll((unsigned long long)ho) - 1;
-            if (int(lane) == leader) base = atomicAdd(Z.out_count, (unsigned long long)__popcll(ho));
-            base = __shfl(base, leader);
+            // slots first from the wave's current block, the rest from a new one
+            const uint32_t k = uint32_t(__popcll(ho));
+            const uint32_t avail = kOutBlock - ob_used;
+            unsigned long long nb = 0;
+            if (k > avail) {
+                const int leader = __ffsll((unsigned long long)ho) - 1;
+                if (int(lane) == leader) nb = atomicAdd(Z.out_count, (unsigned long long)kOutBlock);
+                nb = __shfl(nb, leader);
+            }
             if (handoff) {
                 // parked at the out-queue slot: the next pass reads it back contiguously
-                const uint64_t pos = base + uint64_t(__popcll(ho & lt_mask));
+                const uint32_t rank = uint32_t(__popcll(ho & lt_mask));
+                const uint64_t pos = rank < avail ? ob_base + ob_used + rank : uint64_t(nb) + (rank - avail);
                 Z.out_queue[pos] = uint32_t(ray);
                 double* st = Z.state_out;
                 const uint64_t sd = Z.cap;
@@ -713,6 +721,12 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
                 Z.steps_out[pos] = n_step;
                 handoff = false;
                 active = false;
+            }
+            if (k > avail) {
+                ob_base = nb;
+                ob_used = k - avail;
+            } else {
+                ob_used += k;
             }
         }
         // ---- refill idle lanes ----
@@ -731,9 +745,10 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
             const uint32_t take = cnt < avail ? cnt : avail;
             if (!active) {
                 const uint32_t rank = uint32_t(__popcll(idle & lt_mask));
-                if (rank < take) {
-                    const uint64_t slot = chunk_next + rank;
-                    ray = Z.in_queue ? uint64_t(Z.in_queue[slot]) : slot;
+                const uint64_t slot = chunk_next + rank;
+                const uint32_t qv = (rank < take && Z.in_queue) ? Z.in_queue[slot] : 0u;
+                if (rank < take && qv != kHole) {
+                    ray = Z.in_queue ? uint64_t(qv) : slot;
                     // initial conditions by ray id; parked states by queue slot
                     const double* src = Z.from_ic ? reinterpret_cast<const double*>(P.rays) : Z.state_in;
                     const uint64_t sd = Z.from_ic ? P.stride : Z.cap, at = Z.from_ic ? ray : slot;
@@ -828,6 +843,9 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
             handoff = !Z.no_handoff && (in_zone(y) != NEAR);
         }
     }
+
+    // the unused tail of the wave's last block: holes for the next pass
+    for (uint32_t i = ob_used + lane; i < kOutBlock; i += 64) Z.out_queue[ob_base + i] = kHole;
 
     for (int off = 32; off > 0; off >>= 1) {
         steps_acc += __shfl_xor(steps_acc, off);
