@@ -81,7 +81,9 @@ struct mgk_context {
     float band_rho = -1.0f;
     // frame buffers
     DeviceBuffer rays, hits;
-    DeviceBuffer batch_params;   // per-camera ray-generation parameters + ray offsets
+    DeviceBuffer batch_params;   // per-camera ray-generation parameters, ray offsets and IC constants
+    DeviceBuffer ic_camera;      // IC constants of the last single-camera generation
+    DeviceBuffer shade_resp;     // tabulated spectral response of the last shading call
     // zone-split integration: parked states and the two hand-off queues
     DeviceBuffer zone_state[2], zone_steps[2], zone_q0, zone_q1;
     int last_passes = 0;
@@ -521,6 +523,8 @@ e_magik_result_types fill_shade_params(mgk_context* c, const mgk_scene_t* sc, co
     float thr = P.hero_span / float(sp->n_hero);
     P.irr_scale = thr * (area / npx) * sp->iso * 2.0f * kPi / ((360.0f * sp->fps) / sp->shutter_angle);
     bin_weights(sp, P.weights);
+    HIP_TRY(c->shade_resp.reserve(size_t(kRespN) * sizeof(float4)));
+    P.resp = reinterpret_cast<const float4*>(c->shade_resp.ptr);
     return MAGIK_SUCCESS;
 }
 
@@ -600,6 +604,8 @@ MGK_API e_magik_result_types mgk_context_destroy(mgk_context_handle c) {
     c->rays.release();
     c->hits.release();
     c->batch_params.release();
+    c->ic_camera.release();
+    c->shade_resp.release();
     for (int k = 0; k < 2; ++k) {
         c->zone_state[k].release();
         c->zone_steps[k].release();
@@ -625,7 +631,8 @@ MGK_API e_magik_result_types mgk_generate_rays(mgk_context_handle c, const mgk_i
     if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
     RayGenParams P;
     if ((r = fill_raygen(in, cam, seed, sample_begin, spp, d_rays, stride, P)) != MAGIK_SUCCESS) return r;
-    HIP_TRY(launch_generate_rays(P, hipStream_t(stream)));
+    HIP_TRY(c->ic_camera.reserve(sizeof(IcCamera)));
+    HIP_TRY(launch_generate_rays(P, reinterpret_cast<IcCamera*>(c->ic_camera.ptr), hipStream_t(stream)));
     return MAGIK_SUCCESS;
 }
 
@@ -711,12 +718,15 @@ MGK_API e_magik_result_types mgk_generate_rays_batch(mgk_context_handle c, const
     if (off[n_cam] > stride) return MGK_ERROR_CAPACITY_EXCEEDED;
     if (n_cam == 0 || off[n_cam] == 0) return MAGIK_SUCCESS;
     hipStream_t s = hipStream_t(stream);
-    HIP_TRY(c->batch_params.reserve(P.size() * sizeof(RayGenParams) + off.size() * sizeof(uint64_t)));
+    // layout: RayGenParams[n_cam] | offsets[n_cam + 1] | IcCamera[n_cam] (256-byte aligned)
+    const size_t ic_at = (P.size() * sizeof(RayGenParams) + off.size() * sizeof(uint64_t) + 255) & ~size_t(255);
+    HIP_TRY(c->batch_params.reserve(ic_at + size_t(n_cam) * sizeof(IcCamera)));
     RayGenParams* d_p = reinterpret_cast<RayGenParams*>(c->batch_params.ptr);
     uint64_t* d_off = reinterpret_cast<uint64_t*>(d_p + n_cam);
+    IcCamera* d_ic = reinterpret_cast<IcCamera*>(static_cast<char*>(c->batch_params.ptr) + ic_at);
     HIP_TRY(hipMemcpyAsync(d_p, P.data(), P.size() * sizeof(RayGenParams), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_off, off.data(), off.size() * sizeof(uint64_t), hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_generate_rays_batch(d_p, d_off, n_cam, off[n_cam], s));
+    HIP_TRY(launch_generate_rays_batch(d_p, d_off, n_cam, off[n_cam], d_ic, s));
     HIP_TRY(hipStreamSynchronize(s));   // the host staging vectors go out of scope
     return MAGIK_SUCCESS;
 }
@@ -790,7 +800,8 @@ MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle c, const mgk_in
     hipStream_t s = hipStream_t(stream);
     RayGenParams G;
     if ((r = fill_raygen(in, cam, seed, 0, spp, c->rays.ptr, n, G)) != MAGIK_SUCCESS) return r;
-    HIP_TRY(launch_generate_rays(G, s));
+    HIP_TRY(c->ic_camera.reserve(sizeof(IcCamera)));
+    HIP_TRY(launch_generate_rays(G, reinterpret_cast<IcCamera*>(c->ic_camera.ptr), s));
     TraceParams P;
     fill_trace_params(in, sc, cam->position, P);
     P.rays = c->rays.ptr;

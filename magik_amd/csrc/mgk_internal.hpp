@@ -10,6 +10,7 @@
 namespace mgk {
 
 constexpr int kMaxBins = 64;
+constexpr int kRespN = 8192;   // samples of the tabulated spectral response
 
 struct RayGenParams {
     int metric, precision;
@@ -28,6 +29,15 @@ struct RayGenParams {
 };
 
 constexpr uint32_t kMaxBatchCameras = 4096;
+
+// FP64 initial conditions shared by all rays of one pinhole camera: every ray
+// starts at the lens point, so the position, the metric there, the observer
+// and the camera tetrad are per camera and a ray only adds its direction.
+struct IcCamera {
+    double q[4];      // x, y, z, t (dimensionless)
+    double M[4][4];   // covariant momentum p_mu = sum_k M[mu][k] l_k, mu = x, y, z, t; l = (d_x, d_y, d_z, |d|) / L
+    double w[4];      // u_obs . p = sum_k w[k] l_k
+};
 
 struct TraceParams {
     int metric, precision;
@@ -104,6 +114,7 @@ struct ShadeParams {
     float bin_min, bin_width, sigma;
     float irr_scale;             // radiance_to_irradiance factor per unit L * throughput
     float weights[3 * kMaxBins];
+    const float4* resp;          // spectral response R(lambda), kRespN samples over [hero_min, hero_min + hero_span]
 };
 
 struct GeodesicStepParams {
@@ -131,8 +142,10 @@ struct StarView {
     float rho;
 };
 
-hipError_t launch_generate_rays(const RayGenParams& p, hipStream_t s);
-hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t* d_offsets, uint32_t n_cam, uint64_t n, hipStream_t s);
+// d_ic: device space for the per-camera constants (one IcCamera per camera)
+hipError_t launch_generate_rays(const RayGenParams& p, IcCamera* d_ic, hipStream_t s);
+hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t* d_offsets, uint32_t n_cam, uint64_t n, IcCamera* d_ic,
+                                      hipStream_t s);
 hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s);
 hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s);

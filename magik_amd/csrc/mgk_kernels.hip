@@ -905,34 +905,115 @@ __device__ __forceinline__ void gen_one(const RayGenParams& P, uint64_t i, uint6
     if (P.per_ray_spin) out[10 * st + o] = a;   // the spin of the equations, per ray (batched cameras)
 }
 
-template <int METRIC, class S> __global__ void __launch_bounds__(256) k_generate_rays(RayGenParams P) {
+// FP32 reference arithmetic: every ray builds its own initial conditions in
+// the reference's order.
+template <int METRIC> __global__ void __launch_bounds__(256) k_generate_rays_ref(RayGenParams P) {
     const uint64_t n = uint64_t(P.W) * P.H * P.spp;
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-        gen_one<METRIC, S>(P, i, i);
+        gen_one<METRIC, float>(P, i, i);
+}
+
+// FP64: the per-camera part of gen_one, once per camera.  tetrad_apply is
+// linear in its local components, so the camera tetrad lowered with g is the
+// matrix M of p_mu = M l, and u_obs . p = (u_obs^T M) l.
+template <int METRIC> __device__ void ic_camera(const RayGenParams& P, IcCamera& C) {
+#pragma clang fp contract(off)
+    using S = double;
+    const S f_m = S(P.focal_mm) / S(1000);
+    const float* m = P.orientation;
+    const S ox = (S(m[0]) * S(0) + S(m[1]) * S(0) + S(m[2]) * -f_m) + S(P.position[0]);
+    const S oy = (S(m[3]) * S(0) + S(m[4]) * S(0) + S(m[5]) * -f_m) + S(P.position[1]);
+    const S oz = (S(m[6]) * S(0) + S(m[7]) * S(0) + S(m[8]) * -f_m) + S(P.position[2]);
+    const S L = S(P.length_scale), a = S(P.spin), ms = S(P.mass_sign);
+    const S qx = ox / L, qy = oy / L, qz = oz / L;
+    C.q[0] = qx; C.q[1] = qy; C.q[2] = qz; C.q[3] = S(1) / (L / S(kC));
+    const Sym4<S> g = metric_g<METRIC, S>(qx, qy, qz, a, ms, false);
+    const V4<S> uref = reference_observer<METRIC, S>(qx, qy, qz, a, ms);
+    S wx = S(P.observer_velocity[0]) / S(kC), wy = S(P.observer_velocity[1]) / S(kC), wz = S(P.observer_velocity[2]) / S(kC);
+    if (P.time_reversed) { wx = -wx; wy = -wy; wz = -wz; }
+    const V4<S> X{S(1), S(0), S(0), S(0)}, Y{S(0), S(1), S(0), S(0)}, Z{S(0), S(0), S(1), S(0)};
+    const V4<S> uobs = tetrad_apply(g, X, Y, Z, uref, V4<S>{wx, wy, wz, sqrt_<false>((wx * wx + wy * wy + wz * wz) - S(-1))});
+    const V4<S> CX{S(m[0]), S(m[3]), S(m[6]), S(0)}, CY{S(m[1]), S(m[4]), S(m[7]), S(0)}, CZ{S(m[2]), S(m[5]), S(m[8]), S(0)};
+    for (int k = 0; k < 4; ++k) {
+        const V4<S> e{S(k == 0), S(k == 1), S(k == 2), S(k == 3)};
+        const V4<S> pm = lower(g, tetrad_apply(g, CX, CY, CZ, uobs, e));
+        C.M[0][k] = pm.x; C.M[1][k] = pm.y; C.M[2][k] = pm.z; C.M[3][k] = pm.t;
+        C.w[k] = dot4(uobs, pm);
+    }
+}
+
+__device__ void ic_camera_any(const RayGenParams& P, IcCamera& C) {
+    if (P.metric == KERR) ic_camera<KERR>(P, C);
+    else if (P.metric == SCHWARZSCHILD) ic_camera<SCHWARZSCHILD>(P, C);
+    else ic_camera<MINKOWSKI>(P, C);
+}
+
+__global__ void k_ic_camera(RayGenParams P, IcCamera* out) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ic_camera_any(P, *out);
+}
+
+__global__ void __launch_bounds__(64) k_ic_cameras(const RayGenParams* cams, uint32_t n_cam, IcCamera* out) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n_cam) ic_camera_any(cams[c], out[c]);
+}
+
+// The per-ray part: sensor point and direction (device_sensor_utilities.cuh:35-80,
+// thin_lens.cuh:60-76), then p = M l and u_obs . p = w . l.
+__device__ __forceinline__ void gen_one_fp64(const RayGenParams& P, const IcCamera& C, uint64_t i, uint64_t o) {
+#pragma clang fp contract(off)
+    using S = double;
+    const uint32_t npix = P.W * P.H;
+    const uint32_t pixel = uint32_t(i % npix);
+    const uint32_t sample = P.sample_begin + uint32_t(i / npix);
+    const uint32_t px = pixel % P.W, py = pixel / P.W;
+    float u[4];
+    path_uniforms(P.seed, P.camera_id, pixel, sample, 0, u);
+    S sx, sy;
+    sensor_point<S>(P.W, P.H, P.sensor_width_mm, px, py, u[0], u[1], sx, sy);
+    const S f_m = S(P.focal_mm) / S(1000);
+    S dx = S(0) - sx, dy = S(0) - sy, dz = -f_m - S(0);
+    const S dl = sqrt_<false>(dx * dx + dy * dy + dz * dz);
+    dx = dx / dl; dy = dy / dl; dz = dz / dl;
+    const S L = S(P.length_scale);
+    const S lx = dx / L, ly = dy / L, lz = dz / L;
+    const S lt = sqrt_<false>((lx * lx + ly * ly + lz * lz) - S(0));
+    S* out = reinterpret_cast<S*>(P.rays);
+    const uint64_t st = P.stride;
+    out[0 * st + o] = C.q[0]; out[1 * st + o] = C.q[1]; out[2 * st + o] = C.q[2]; out[3 * st + o] = C.q[3];
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+        out[(4 + mu) * st + o] = ((C.M[mu][0] * lx + C.M[mu][1] * ly) + C.M[mu][2] * lz) + C.M[mu][3] * lt;
+    out[8 * st + o] = P.metric == MINKOWSKI ? S(1000) : S(1);
+    out[9 * st + o] = ((C.w[0] * lx + C.w[1] * ly) + C.w[2] * lz) + C.w[3] * lt;
+    if (P.per_ray_spin) out[10 * st + o] = S(P.spin);   // the spin of the equations, per ray (batched cameras)
+}
+
+__global__ void __launch_bounds__(256) k_generate_rays_fp64(RayGenParams P, const IcCamera* __restrict__ ic) {
+    const uint64_t n = uint64_t(P.W) * P.H * P.spp;
+    const IcCamera C = *ic;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        gen_one_fp64(P, C, i, i);
 }
 
 // Batched cameras: ray i belongs to camera c with offsets[c] <= i < offsets[c + 1]
 // (at most kMaxBatchCameras); each camera's rays are laid out as a single
 // camera's, sample-major, with the camera's own spin in component 10.
 __global__ void __launch_bounds__(256) k_generate_rays_batch(const RayGenParams* cams, const uint64_t* offsets, uint32_t n_cam,
-                                                             uint64_t n) {
+                                                             uint64_t n, const IcCamera* ic) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x) {
         uint32_t lo = 0, hi = n_cam;   // last c with offsets[c] <= i
         while (hi - lo > 1) {
             const uint32_t mid = (lo + hi) >> 1;
             if (offsets[mid] <= i) lo = mid; else hi = mid;
         }
-        const RayGenParams& P = cams[lo];
-        if (P.metric == KERR) gen_one<KERR, double>(P, i - offsets[lo], i);
-        else if (P.metric == SCHWARZSCHILD) gen_one<SCHWARZSCHILD, double>(P, i - offsets[lo], i);
-        else gen_one<MINKOWSKI, double>(P, i - offsets[lo], i);
+        gen_one_fp64(cams[lo], ic[lo], i - offsets[lo], i);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Shading: one thread per pixel, bins in LDS (bin-major, thread-minor).
+// Shading: one thread per pixel.
 // ---------------------------------------------------------------------------
-constexpr int kShadeBlock = 128;
+constexpr int kShadeBlock = 256;
 
 __device__ __forceinline__ float star_lookup(const ShadeParams& P, float c, float ph) {
     float sn = sqrtf(fmaxf(0.0f, 1.0f - c * c));
@@ -949,79 +1030,92 @@ __device__ __forceinline__ float star_lookup(const ShadeParams& P, float c, floa
     return acc;
 }
 
+// Spectral response of the bins to one unit of irradiance observed at lambda,
+//   R(lambda) = sum_i W_i (Phi(e_{i+1} - lambda) - Phi(e_i - lambda)),
+// with bin edges e_i, the bins' CMF weights W_i and the normal CDF Phi of
+// width sigma: the Gaussian redistribution of kernel_redistribute_radiance.cu:55-92
+// followed by the splat of kernel_splat_to_renderbuffer.cu:37-57.  The splat is
+// linear in the bins, so a path adds E R(lambda) to its pixel and no per-pixel
+// bin array is needed.  Tabulated in FP64 at kRespN points over the hero range;
+// linear interpolation between them errs by about (step^2 / 8) |R''|, near
+// 3e-6 of R for 0.05 nm steps and sigma = 10 nm.
+__global__ void __launch_bounds__(256) k_shade_response(ShadeParams P, float4* out) {
+    const int j = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= kRespN) return;
+    const double lam = double(P.hero_min) + double(P.hero_span) * double(j) / double(kRespN - 1);
+    const double k = 1.0 / (double(P.sigma) * 1.4142135623730951);
+    double X = 0.0, Y = 0.0, Z = 0.0;
+    double prev = 0.5 * (1.0 + erf((double(P.bin_min) - lam) * k));
+    for (uint32_t i = 0; i < P.n_bin; ++i) {
+        const double e1 = double(P.bin_min) + double(i + 1) * double(P.bin_width);
+        const double cur = 0.5 * (1.0 + erf((e1 - lam) * k));
+        const double d = cur - prev;
+        X += d * double(P.weights[3 * i]);
+        Y += d * double(P.weights[3 * i + 1]);
+        Z += d * double(P.weights[3 * i + 2]);
+        prev = cur;
+    }
+    out[j] = make_float4(float(X), float(Y), float(Z), 0.0f);
+}
+
+// One thread per pixel: the pixel's spp paths, n_hero wavelengths each.
 __global__ void __launch_bounds__(kShadeBlock) k_shade(ShadeParams P) {
-    __shared__ float bins[kMaxBins * kShadeBlock];
     const uint32_t npix = P.W * P.H;
     const uint32_t pixel = blockIdx.x * kShadeBlock + threadIdx.x;
-    for (uint32_t i = 0; i < P.n_bin; ++i) bins[i * kShadeBlock + threadIdx.x] = 0.0f;
-    if (pixel < npix) {
-        const float inv_sqrt2_sigma = 1.0f / (P.sigma * 1.41421356237f);
-        for (uint32_t s = 0; s < P.spp; ++s) {
-            mgk_hit_t h = P.hits[uint64_t(s) * npix + pixel];
-            if (h.cls != MGK_HIT_DISK && h.cls != MGK_HIT_SPHERE && h.cls != MGK_HIT_ESCAPE) continue;
-            float u[4];
-            path_uniforms(P.seed, P.camera_id, pixel, P.sample_begin + s, 0, u);
-            const float g = h.g, g5 = g * g * g * g * g;
-            float emit_scale = 0.0f, T = 0.0f, star = 0.0f;
-            if (h.cls == MGK_HIT_DISK) {
-                T = P.disk_T_in * powf(h.a0 / P.disk_r_in, -0.75f);
-                emit_scale = 1.0f;
-            } else if (h.cls == MGK_HIT_SPHERE) {
-                float th = acosf(fmaxf(-1.0f, fminf(1.0f, h.a0)));
-                float ph = h.a1 + kPi;
-                uint32_t n = P.sphere_checker;
-                uint32_t ci = min(n - 1, uint32_t(th / kPi * float(n)));
-                uint32_t cj = min(n - 1, uint32_t(ph / (2.0f * kPi) * float(n)));
-                T = P.sphere_T;
-                emit_scale = ((ci + cj) & 1u) ? 0.0f : 1.0f;
+    if (pixel >= npix) return;
+    const float resp_scale = P.hero_span > 0.0f ? float(kRespN - 1) / P.hero_span : 0.0f;
+    float X = 0.0f, Yv = 0.0f, Zv = 0.0f;
+    for (uint32_t s = 0; s < P.spp; ++s) {
+        mgk_hit_t h = P.hits[uint64_t(s) * npix + pixel];
+        if (h.cls != MGK_HIT_DISK && h.cls != MGK_HIT_SPHERE && h.cls != MGK_HIT_ESCAPE) continue;
+        float u[4];
+        path_uniforms(P.seed, P.camera_id, pixel, P.sample_begin + s, 0, u);
+        const float g = h.g, g5 = g * g * g * g * g;
+        float emit_scale = 0.0f, T = 0.0f, star = 0.0f;
+        if (h.cls == MGK_HIT_DISK) {
+            T = P.disk_T_in * powf(h.a0 / P.disk_r_in, -0.75f);
+            emit_scale = 1.0f;
+        } else if (h.cls == MGK_HIT_SPHERE) {
+            float th = acosf(fmaxf(-1.0f, fminf(1.0f, h.a0)));
+            float ph = h.a1 + kPi;
+            uint32_t n = P.sphere_checker;
+            uint32_t ci = min(n - 1, uint32_t(th / kPi * float(n)));
+            uint32_t cj = min(n - 1, uint32_t(ph / (2.0f * kPi) * float(n)));
+            T = P.sphere_T;
+            emit_scale = ((ci + cj) & 1u) ? 0.0f : 1.0f;
+        } else {
+            if (P.flags & MGK_SCENE_STARS) star = star_lookup(P, h.a0, h.a1);
+        }
+        for (uint32_t k = 0; k < P.n_hero; ++k) {
+            float f = u[2] + float(k) / float(P.n_hero);
+            f -= floorf(f);
+            const float lam = P.hero_min + P.hero_span * f;
+            const float le = lam * g;
+            float L;
+            if (h.cls == MGK_HIT_ESCAPE) {
+                L = 0.0f;
+                if (P.flags & MGK_SCENE_BACKGROUND) L += P.background_scale * planck(le, P.background_T);
+                if (star > 0.0f) L += star * planck(le, P.star_T);
             } else {
-                if (P.flags & MGK_SCENE_STARS) star = star_lookup(P, h.a0, h.a1);
+                L = emit_scale > 0.0f ? planck(le, T) : 0.0f;
             }
-            for (uint32_t k = 0; k < P.n_hero; ++k) {
-                float f = u[2] + float(k) / float(P.n_hero);
-                f -= floorf(f);
-                const float lam = P.hero_min + P.hero_span * f;
-                const float le = lam * g;
-                float L;
-                if (h.cls == MGK_HIT_ESCAPE) {
-                    L = 0.0f;
-                    if (P.flags & MGK_SCENE_BACKGROUND) L += P.background_scale * planck(le, P.background_T);
-                    if (star > 0.0f) L += star * planck(le, P.star_T);
-                } else {
-                    L = emit_scale > 0.0f ? planck(le, T) : 0.0f;
-                }
-                L *= g5;
-                if (isnan(L) || isinf(L) || L < 0.0f) L = 0.0f;
-                if (!(L > 0.0f)) continue;
-                const float E = L * P.irr_scale;
-                // Only bins with an edge inside lambda +- 8 sigma: beyond that the
-                // FP32 normal CDF is exactly 0 or 1 (erf saturates at 3.92), so the
-                // bins outside receive exactly zero and are skipped.
-                const float w8 = 8.0f * P.sigma;
-                const int i0 = max(0, int(floorf((lam - w8 - P.bin_min) / P.bin_width)));
-                const int i1 = min(int(P.n_bin) - 1, int(floorf((lam + w8 - P.bin_min) / P.bin_width)));
-                float x1 = P.bin_min + float(i0) * P.bin_width;
-                float prev = 0.5f * (1.0f + erff((x1 - lam) * inv_sqrt2_sigma));
-                for (int i = i0; i <= i1; ++i) {
-                    x1 = P.bin_min + float(i + 1) * P.bin_width;
-                    float cur = 0.5f * (1.0f + erff((x1 - lam) * inv_sqrt2_sigma));
-                    bins[i * kShadeBlock + threadIdx.x] += E * (cur - prev);
-                    prev = cur;
-                }
-            }
+            L *= g5;
+            if (isnan(L) || isinf(L) || L < 0.0f) L = 0.0f;
+            if (!(L > 0.0f)) continue;
+            const float E = L * P.irr_scale;
+            const float t = fmaxf(0.0f, (lam - P.hero_min) * resp_scale);
+            const int i = min(int(t), kRespN - 2);
+            const float w = t - float(i);
+            const float4 r0 = P.resp[i], r1 = P.resp[i + 1];
+            X += E * (r0.x + w * (r1.x - r0.x));
+            Yv += E * (r0.y + w * (r1.y - r0.y));
+            Zv += E * (r0.z + w * (r1.z - r0.z));
         }
-        float X = 0.0f, Yv = 0.0f, Zv = 0.0f;
-        for (uint32_t i = 0; i < P.n_bin; ++i) {
-            float b = bins[i * kShadeBlock + threadIdx.x];
-            X += b * P.weights[3 * i];
-            Yv += b * P.weights[3 * i + 1];
-            Zv += b * P.weights[3 * i + 2];
-        }
-        const float inv = 1.0f / float(P.spp);
-        P.xyz[3 * uint64_t(pixel) + 0] = X * inv;
-        P.xyz[3 * uint64_t(pixel) + 1] = Yv * inv;
-        P.xyz[3 * uint64_t(pixel) + 2] = Zv * inv;
     }
+    const float inv = 1.0f / float(P.spp);
+    P.xyz[3 * uint64_t(pixel) + 0] = X * inv;
+    P.xyz[3 * uint64_t(pixel) + 1] = Yv * inv;
+    P.xyz[3 * uint64_t(pixel) + 2] = Zv * inv;
 }
 
 // ---------------------------------------------------------------------------
@@ -1122,17 +1216,19 @@ template <template <int> class K, class Fn> static hipError_t dispatch_metric(in
 }
 template <int> struct tag_t {};
 
-hipError_t launch_generate_rays(const RayGenParams& p, hipStream_t s) {
+hipError_t launch_generate_rays(const RayGenParams& p, IcCamera* d_ic, hipStream_t s) {
     const uint64_t n = uint64_t(p.W) * p.H * p.spp;
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 65536ull * 16) blocks = 65536ull * 16;
+    if (p.precision != MGK_PRECISION_FP32_REFERENCE) {
+        hipLaunchKernelGGL(k_ic_camera, dim3(1), dim3(64), 0, s, p, d_ic);
+        hipLaunchKernelGGL(k_generate_rays_fp64, dim3(unsigned(blocks)), dim3(256), 0, s, p, d_ic);
+        return hipGetLastError();
+    }
     return dispatch_metric<tag_t>(p.metric, [&](auto m) -> hipError_t {
         constexpr int M = decltype(m)::value;
-        if (p.precision == MGK_PRECISION_FP32_REFERENCE)
-            hipLaunchKernelGGL((k_generate_rays<M, float>), dim3(unsigned(blocks)), dim3(256), 0, s, p);
-        else
-            hipLaunchKernelGGL((k_generate_rays<M, double>), dim3(unsigned(blocks)), dim3(256), 0, s, p);
+        hipLaunchKernelGGL((k_generate_rays_ref<M>), dim3(unsigned(blocks)), dim3(256), 0, s, p);
         return hipGetLastError();
     });
 }
@@ -1164,17 +1260,21 @@ hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool nea
     });
 }
 
-hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t* d_offsets, uint32_t n_cam, uint64_t n, hipStream_t s) {
-    if (n == 0) return hipSuccess;
+hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t* d_offsets, uint32_t n_cam, uint64_t n, IcCamera* d_ic,
+                                      hipStream_t s) {
+    if (n == 0 || n_cam == 0) return hipSuccess;
     uint64_t blocks = (n + 255) / 256;
     if (blocks > 65536ull * 16) blocks = 65536ull * 16;
-    hipLaunchKernelGGL(k_generate_rays_batch, dim3(unsigned(blocks)), dim3(256), 0, s, d_cams, d_offsets, n_cam, n);
+    hipLaunchKernelGGL(k_ic_cameras, dim3((n_cam + 63) / 64), dim3(64), 0, s, d_cams, n_cam, d_ic);
+    hipLaunchKernelGGL(k_generate_rays_batch, dim3(unsigned(blocks)), dim3(256), 0, s, d_cams, d_offsets, n_cam, n,
+                       static_cast<const IcCamera*>(d_ic));
     return hipGetLastError();
 }
 
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s) {
     const uint32_t npix = p.W * p.H;
     if (npix == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_shade_response, dim3((kRespN + 255) / 256), dim3(256), 0, s, p, const_cast<float4*>(p.resp));
     hipLaunchKernelGGL(k_shade, dim3((npix + kShadeBlock - 1) / kShadeBlock), dim3(kShadeBlock), 0, s, p);
     return hipGetLastError();
 }
