@@ -1,6 +1,6 @@
 """Benchmark: Kerr a=0.9 1920x1080x32 spp frames on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kerr_disk_1080p]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kerr_disk_1080p|kerr_disk_4k|batch64]
 
 A step is one full frame through the C-ABI of synthetic seeded rays, timed on
 two paths:
@@ -283,7 +283,7 @@ def main():
 
     # ---- surrogate path ----
     sur_res, mlp_roof = None, None
-    if not args.no_surrogate and args.workload == "kerr_disk_1080p" and prec != M.PREC_FP32_REFERENCE:
+    if not args.no_surrogate and args.workload in ("kerr_disk_1080p", "kerr_disk_4k") and prec != M.PREC_FP32_REFERENCE:
         sur_res, mlp_roof = bench_surrogate(M, ctx, args, integ, sc, spec, cam, spp, n_paths, out)
 
     if sur_res:
@@ -312,9 +312,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if prec != M.PREC_FP64 else "f64",
-        "data": "synthetic (seeded Philox rays, Kerr a=0.9 disk + 1e5-star field scene; surrogate trained on "
-                "seeded FP64-traced rays)",
-        "config": {"workload": f"{args.workload}: Kerr a=0.9, r_obs=50M, theta=80 deg, FOV 25 deg, "
+        "data": f"synthetic (seeded Philox rays, Kerr a={integ.spin:.2f} disk + 1e5-star field scene; surrogate "
+                "trained on seeded FP64-traced rays)",
+        "config": {"workload": f"{args.workload}: Kerr a={integ.spin:.2f}, r_obs=50M, theta=80 deg, FOV 25 deg, "
                                f"{W}x{H}x{spp} spp, 4 hero wavelengths, tol 1e-7, cap 1e4",
                    "paths_per_frame": n_paths, "value_path": best[0], "error_bar": ERROR_BAR},
         "plain": plain,

@@ -101,6 +101,9 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
     kerr_disk_1080p   : a=0.9, 1920x1080x32, r=50M, theta=80 deg, 25 deg; disk
                         r_ISCO-30M, same T law; 4 stratified hero lambdas;
                         same stars; tol 1e-7, cap 1e4.
+    kerr_disk_4k      : the learned-surrogate frame: a=0.99, 3840x2160x16,
+                        otherwise as kerr_disk_1080p (camera r=50M, theta=80
+                        deg, 25 deg; disk r_ISCO(0.99)-30M).
     """
     if name == "minkowski_sphere":
         W, H, spp = W or 256, H or 256, spp or 4
@@ -121,9 +124,10 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
                    star_radius_rad=1e-3, star_T=6500.0)
         spec = spectral(n_hero=1)
         return integ, sc, spec, cam, spp
-    if name == "kerr_disk_1080p":
-        W, H, spp = W or 1920, H or 1080, spp or 32
-        a = 0.9
+    if name in ("kerr_disk_1080p", "kerr_disk_4k"):
+        big = name == "kerr_disk_4k"
+        W, H, spp = W or (3840 if big else 1920), H or (2160 if big else 1080), spp or (16 if big else 32)
+        a = 0.99 if big else 0.9
         integ = integrator(metric=KERR, precision=PREC_MIXED if precision is None else precision, spin=a)
         cam = camera_bl(50.0, 80.0, a, 25.0, W, H)
         sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_STARS, disk_r_in=r_isco(a),

@@ -518,6 +518,7 @@ template <class S> __device__ __forceinline__ S load_c(const void* rays, uint64_
 }
 
 constexpr uint32_t kChunk = 64;   // ray indices taken from the device counter per wave refill
+constexpr int kRefillMin = 8;     // zone kernels: idle lanes that trigger a refill
 
 // Persistent integrator.  T is the storage / state type: float for the FP32
 // reference path, double for the mixed and FP64 paths.
@@ -729,8 +730,11 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
                 ob_used += k;
             }
         }
-        // ---- refill idle lanes ----
+        // ---- refill idle lanes, once at least kRefillMin are idle: the refill
+        // code runs only on the idle lanes, so refilling one or two at a time
+        // spends most of its issue slots on masked lanes ----
         uint64_t idle = __ballot(!active);
+        if (__popcll(idle) < kRefillMin && idle != ~0ull) idle = 0;
         while (idle != 0 && !exhausted) {
             if (chunk_next >= chunk_end) {
                 unsigned long long base = 0;

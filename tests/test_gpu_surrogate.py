@@ -121,13 +121,13 @@ def test_train_step_loss_and_adam(sur):
         fresh.close()
 
 
-def kerr_camera_ranges(width=0.0):
-    return M.surrogate_ranges(spin=(A - width, A + width), r=(R_OBS - 0.5, R_OBS + 0.5), theta=(TH_OBS - 0.5, TH_OBS + 0.5),
+def kerr_camera_ranges(width=0.0, a=A):
+    return M.surrogate_ranges(spin=(a - width, a + width), r=(R_OBS - 0.5, R_OBS + 0.5), theta=(TH_OBS - 0.5, TH_OBS + 0.5),
                               screen=12.0, disk_r_max=30.0)
 
 
-def training_scene():
-    return M.scene(celestial_radius_m=1000.0 * M.length_scale(), flags=M.SCENE_DISK, disk_r_in=M.r_isco(A),
+def training_scene(a=A):
+    return M.scene(celestial_radius_m=1000.0 * M.length_scale(), flags=M.SCENE_DISK, disk_r_in=M.r_isco(a),
                    disk_r_out=30.0)
 
 
@@ -232,3 +232,46 @@ def test_surrogate_judge_and_gain(gpu_ctx, trained):
     assert st["judge_class_agree"] >= 0.99 * st["n_judge"], st
     assert st["judge_n_disk"] > 0 and st["judge_mean_rel_r"] < 0.05 and st["judge_mean_rel_g"] < 0.05, st
     assert np.isfinite(xyz.cpu().numpy()).all()
+
+
+@pytest.fixture(scope="module")
+def trained_099(gpu_ctx):
+    """The learned-surrogate configuration's spin (a = 0.99): a network for
+    one camera of the kerr_disk_4k workload, trained briefly on the GPU."""
+    import torch
+    s = M.Surrogate(0, seed=17, ranges=kerr_camera_ranges(a=0.99))
+    integ = M.integrator(metric=M.KERR, precision=M.PREC_FP64, spin=0.99)
+    _, x, t, _ = s.generate_data(gpu_ctx, integ, training_scene(0.99), seed=23, n=1 << 18)
+    hist = s.train(x, t, batch=4096, epochs=12, lr=2e-3, lr_final=5e-5)
+    del x, t
+    torch.cuda.empty_cache()
+    yield s, hist
+    s.close()
+
+
+def test_surrogate_spin_099_frame(gpu_ctx, trained_099):
+    """The kerr_disk_4k workload (a = 0.99) at a reduced size: the network
+    answers most paths, the RKF45 judge agrees on the class and bounds the
+    disk-radius and redshift errors, and with every path sent to RKF45 the
+    frame equals the plain render."""
+    import torch
+    s, hist = trained_099
+    assert hist[-1, 3] > 0.9, hist
+    integ, sc, spec, cam, _ = M.workload("kerr_disk_4k", W=192, H=108, spp=2)
+    assert abs(integ.spin - 0.99) < 1e-6 and abs(sc.disk_r_in - M.r_isco(0.99)) < 1e-5
+    gpu_ctx.set_scene(sc)
+    plain = gpu_ctx.render_frame(integ, sc, spec, cam, seed=5, spp=2)
+    plain_trials = gpu_ctx.stats().n_trials
+    xyz, st = M.render_frame_surrogate(gpu_ctx, s, integ, sc, spec, cam, seed=5, spp=2, fb=M.fallback(judge_stride=7))
+    torch.cuda.synchronize()
+    n = st["n_paths"]
+    assert st["n_fallback"] < 0.5 * n, st
+    assert st["n_trials"] < 0.6 * plain_trials, (st["n_trials"], plain_trials)
+    assert st["judge_class_agree"] >= 0.98 * st["n_judge"], st
+    assert st["judge_n_disk"] > 0 and st["judge_mean_rel_r"] < 0.05 and st["judge_mean_rel_g"] < 0.05, st
+    assert np.isfinite(xyz.cpu().numpy()).all()
+    xyz_all, st_all = M.render_frame_surrogate(gpu_ctx, s, integ, sc, spec, cam, seed=5, spp=2,
+                                               fb=M.fallback(min_confidence=1.5, judge_stride=0))
+    torch.cuda.synchronize()
+    assert st_all["n_fallback"] == n
+    assert torch.equal(xyz_all, plain)
