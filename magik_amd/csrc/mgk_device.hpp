@@ -161,14 +161,16 @@ __device__ __forceinline__ Rhs<S> rhs(S x, S y, S z, S px, S py, S pz, S pt, S a
         const S s_xz = r * inv_sigma, s_y = d * inv_r * inv_sigma;
         const S drx = x * s_xz, dry = y * s_y, drz = z * s_xz;
         const S inv_sig_sq = inv_sigma * inv_sigma;
-        S f, dfs, dfy_extra;
+        // half_f = f / 2 and the halved gradient of f: the factor 1/2 of the
+        // Hamiltonian's (k.p - p_t)^2 term is folded in here
+        S half_f, dfs, dfy_extra;
         if constexpr (METRIC == KERR) {
-            f = S(2) * r * inv_sigma;
-            dfs = S(2) * (S(3) * a_sq * y_sq - r_sq * r_sq) * (inv_rsq * inv_sig_sq);
-            dfy_extra = S(4) * a_sq * y * (inv_r * inv_sig_sq);
+            half_f = s_xz;   // r / sigma
+            dfs = (S(3) * a_sq * y_sq - r_sq * r_sq) * (inv_rsq * inv_sig_sq);
+            dfy_extra = S(2) * a_sq * y * (inv_r * inv_sig_sq);
         } else {
-            f = mass_sign * S(2) * inv_r;
-            dfs = mass_sign * S(-2) * inv_rsq;
+            half_f = mass_sign * inv_r;
+            dfs = -mass_sign * inv_rsq;
             dfy_extra = S(0);
         }
         const S dfx = dfs * drx, dfy = dfs * dry - dfy_extra, dfz = dfs * drz;
@@ -194,15 +196,15 @@ __device__ __forceinline__ Rhs<S> rhs(S x, S y, S z, S px, S py, S pz, S pt, S a
         const S bx = drx * Hc + ex;
         const S by = dry * Hc + py * inv_r;
         const S bz = drz * Hc + ez;
-        const S f_kTp = f * kTp;
-        const S half_kTp_sq = S(0.5) * kTp * kTp;
+        const S f_kTp = S(2) * (half_f * kTp);
+        const S kTp_sq = kTp * kTp;
         o.cx = f_kTp * kx;
         o.cy = f_kTp * ky;
         o.cz = f_kTp * kz;
         o.ct = -f_kTp;
-        o.dpx = half_kTp_sq * dfx + f_kTp * bx;
-        o.dpy = half_kTp_sq * dfy + f_kTp * by;
-        o.dpz = half_kTp_sq * dfz + f_kTp * bz;
+        o.dpx = kTp_sq * dfx + f_kTp * bx;
+        o.dpy = kTp_sq * dfy + f_kTp * by;
+        o.dpz = kTp_sq * dfz + f_kTp * bz;
         return o;
     }
 }
