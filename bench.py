@@ -38,6 +38,7 @@ METRIC = "Kerr rays/s (1080p, a=0.9): MLP surrogate vs plain RKF45 at ≤1e-3 ra
 PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak (vendor figure, counts packed FP32)
 PEAK_FP32_UNPACKED_TFLOPS = 78.6  # the same without packed FP32 (this build compiles packed FP32 off)
 PEAK_BF16_TFLOPS = 2500.0         # MI355X dense bf16 MFMA peak
+PEAK_HBM_TBS = 8.0                # MI355X HBM3E peak
 ERROR_BAR = 1e-3
 # Algorithmic operations of one reference RKF45 trial (6 Kerr right-hand sides,
 # stage sums, error norm, controller), counted by oracle/opcount.cpp:
@@ -281,6 +282,9 @@ def main():
             plain["fp32_reference_radiance_err_vs_fp64"] = radiance_error(ctx, M, it32, sc, spec, cam, spp, args.seed)
             log(f"  radiance error {plain['fp32_reference_radiance_err_vs_fp64']}")
 
+    # ---- display side of the splat over the last frame ----
+    display_res = bench_display(M, ctx, out)
+
     # ---- surrogate path ----
     sur_res, mlp_roof = None, None
     if not args.no_surrogate and args.workload in ("kerr_disk_1080p", "kerr_disk_4k") and prec != M.PREC_FP32_REFERENCE:
@@ -328,12 +332,45 @@ def main():
                              f"algorithmic FLOPs = {FLOP_PER_TRIAL[integ.metric_type]} per RKF45 trial x trials "
                              "per frame / HIP-event time of the frame's integration on the library stream"},
         "roofline_mlp": mlp_roof,
+        "display": display_res,
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(M, integ, sc, cam, spp, args.seed)
         log(f"cpu baseline {res['cpu_baseline']['value']:.3e} rays/s")
     ctx.close()
     print(json.dumps(res), flush=True)
+
+
+def bench_display(M, ctx, frame, reps=20):
+    """mgk_display_transform over one rendered frame (XYZ, H x W x 3 float32),
+    per view transform: the mean launch time from HIP events on the stream the
+    kernel runs on, and the algorithmic HBM rate (12 bytes read + 12 written
+    per pixel) against the 8 TB/s peak.  The timed launches are queued behind a
+    short spin kernel, so that they run back to back and the events see kernel
+    time rather than the host's call overhead."""
+    import torch
+    n = frame.numel() // 3
+    dst = torch.empty_like(frame)
+    names = {M.VIEW_NONE: "none", M.VIEW_REINHARD: "reinhard_simple",
+             M.VIEW_REINHARD_EXTENDED: "reinhard_extended", M.VIEW_GT7: "gt7_jzazbz",
+             M.VIEW_AGX: "agx", M.VIEW_KHRONOS_PBR_NEUTRAL: "khronos_pbr_neutral"}
+    res = {"pixels": n, "bytes_per_pixel": 24, "display": "Rec.709 (1/2.4 power)", "views": {}}
+    for view, name in names.items():
+        disp = M.display(view_transform=view)
+        M.display_transform(ctx, disp, frame, out=dst)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda._sleep(5_000_000)   # ~2 ms at 2.4 GHz: time for the host to queue the launches
+        e0.record()
+        for _ in range(reps):
+            M.display_transform(ctx, disp, frame, out=dst)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = 24.0 * n / (ms * 1e-3) / 1e9
+        res["views"][name] = {"ms": ms, "achieved_GB_s": gbs, "frac_of_hbm_peak": gbs / (PEAK_HBM_TBS * 1e3)}
+    log("display: " + ", ".join(f"{k} {v['ms'] * 1e3:.1f} us" for k, v in res["views"].items()))
+    return res
 
 
 def bench_batch(M, args, prec):

@@ -115,6 +115,51 @@ MGK_API e_magik_result_types magik_trace_geodesic(magik_trace_geodesic_container
                                                   magik_trace_geodesic_metric_parameters_t metric_parameters,
                                                   magik_trace_geodesic_integrator_parameters_t integrator_parameters);
 
+/* Colour pipeline enums (values as magik.h:330-383). */
+typedef enum e_magik_scene_colorspace_types {
+    MAGIK_SCENE_COLORSPACE_CIE_XYZ = 0,
+    MAGIK_SCENE_COLORSPACE_CIE_RGB = 1,
+    MAGIK_SCENE_COLORSPACE_LINEAR_REC709 = 2,
+    MAGIK_SCENE_COLORSPACE_LINEAR_REC2020 = 3,
+    MAGIK_SCENE_COLORSPACE_ACES_CG = 4,
+    MAGIK_SCENE_COLORSPACE_ADOBE_RGB_1998 = 5
+} e_magik_scene_colorspace_types;
+
+typedef enum e_magik_display_colorspace_types {
+    MAGIK_DISPLAY_COLORSPACE_SRGB = 0,
+    MAGIK_DISPLAY_COLORSPACE_REC709 = 1,
+    MAGIK_DISPLAY_COLORSPACE_REC2100_PQ = 2,
+    MAGIK_DISPLAY_COLORSPACE_REC2100_HLG = 3,
+    MAGIK_DISPLAY_COLORSPACE_DCI_P3_THEATRICAL_WHITE_POINT = 4,
+    MAGIK_DISPLAY_COLORSPACE_DISPLAY_P3 = 5,
+    MAGIK_DISPLAY_COLORSPACE_ADOBE_RGB = 6
+} e_magik_display_colorspace_types;
+
+typedef enum e_magik_view_transform_types {
+    MAGIK_VIEW_TRANSFORM_NONE = 0,
+    MAGIK_VIEW_TRANSFORM_REINHARD_SIMPLE = 1,
+    MAGIK_VIEW_TRANSFORM_REINHARD_EXTENDED = 2,
+    MAGIK_VIEW_TRANSFORM_GRAN_TURISMO_7 = 3,
+    MAGIK_VIEW_TRANSFORM_AGX = 4,
+    MAGIK_VIEW_TRANSFORM_KHRONOS_PBR_NEUTRAL = 5
+} e_magik_view_transform_types;
+
+typedef enum e_magik_gran_turismo_7_mode_types {
+    MAGIK_GRAN_TURISMO_7_MODE_ICTCP = 0,
+    MAGIK_GRAN_TURISMO_7_MODE_JZAZBZ = 1
+} e_magik_gran_turismo_7_mode_types;
+
+typedef enum e_magik_agx_look_types { MAGIK_AGX_LOOK_NONE = 0, MAGIK_AGX_LOOK_GOLDEN = 1, MAGIK_AGX_LOOK_PUNCHY = 2 } e_magik_agx_look_types;
+
+typedef enum e_magik_output_format_types {
+    MAGIK_OUTPUT_FORMAT_UNORM8 = 0,
+    MAGIK_OUTPUT_FORMAT_UNORM10 = 1,
+    MAGIK_OUTPUT_FORMAT_UNORM12 = 2,
+    MAGIK_OUTPUT_FORMAT_UNORM16 = 3,
+    MAGIK_OUTPUT_FORMAT_FLOAT16 = 4,
+    MAGIK_OUTPUT_FORMAT_FLOAT32 = 5
+} e_magik_output_format_types;
+
 /* magik.h:262 (version query). */
 MGK_API e_magik_result_types magik_get_version(uint32_t* major, uint32_t* minor, uint32_t* revision, const char** as_char);
 
@@ -320,6 +365,34 @@ MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle ctx, const mgk_
  * :29-68; color_pipeline.cuh:21-48): bin CMF weights (n_bin x 3, XYZ) and the
  * white-balanced XYZ -> linear Rec.709 matrix (row-major). */
 MGK_API e_magik_result_types mgk_color_weights(const mgk_spectral_t* spec, float* weights_xyz, float* xyz_to_scene);
+
+/* Display settings: the colour part of the reference's stage
+ * (color_settings.h:9-30, with its defaults in the comments). */
+typedef struct mgk_display_t {
+    uint32_t scene_colorspace;          /* e_magik_scene_colorspace_types (LINEAR_REC709) */
+    uint32_t view_transform;            /* e_magik_view_transform_types (GRAN_TURISMO_7) */
+    uint32_t display_colorspace;        /* e_magik_display_colorspace_types (REC709) */
+    uint32_t output_format;             /* e_magik_output_format_types (FLOAT32); values stay FP32 */
+    float white_point_K;                /* white balance: blackbody (5600) */
+    float tint;                         /* ... shifted along the locus normal (0) */
+    float reinhard_white_luminance;     /* (4) */
+    float khronos_start_compression;    /* (0.8) */
+    float khronos_desaturation;         /* (0.15) */
+    uint32_t gt7_mode;                  /* e_magik_gran_turismo_7_mode_types (JZAZBZ) */
+    float gt7_sdr_paper_white;          /* cd/m^2 (250) */
+    float gt7_reference_luminance;      /* cd/m^2 per framebuffer unit (100) */
+    float gt7_jzazbz_exponent_scale;    /* (1.7) */
+    uint32_t agx_look;                  /* e_magik_agx_look_types (GOLDEN) */
+} mgk_display_t;
+
+/* The display side of the splat (kernel_splat_to_renderbuffer.cu:37-57,
+ * device_color_utilities.cuh:56-72 with color_pipeline.cuh:21-48): per pixel,
+ * linear XYZ (as mgk_shade writes it) -> white-balanced scene colour space ->
+ * view transform -> display primaries (Bradford adaptation between the white
+ * points) -> display transfer function.  d_xyz and d_out hold 3 floats per
+ * pixel and may be the same buffer. */
+MGK_API e_magik_result_types mgk_display_transform(mgk_context_handle ctx, const mgk_display_t* display, const float* d_xyz,
+                                                   uint64_t n_pixels, float* d_out, void* stream);
 
 /* Places a camera at Boyer-Lindquist (r, theta) of a Kerr black hole with spin
  * a (as given by the user), in Kerr-Schild Cartesian coordinates with the spin

@@ -296,6 +296,47 @@ def color_weights(spec):
     return W, M.reshape(3, 3)
 
 
+# Colour pipeline enums (magik.h:330-383)
+SCENE_CIE_XYZ, SCENE_CIE_RGB, SCENE_REC709, SCENE_REC2020, SCENE_ACES_CG, SCENE_ADOBE_RGB = range(6)
+DISPLAY_SRGB, DISPLAY_REC709, DISPLAY_PQ, DISPLAY_HLG, DISPLAY_DCI_P3, DISPLAY_P3, DISPLAY_ADOBE_RGB = range(7)
+VIEW_NONE, VIEW_REINHARD, VIEW_REINHARD_EXTENDED, VIEW_GT7, VIEW_AGX, VIEW_KHRONOS_PBR_NEUTRAL = range(6)
+GT7_ICTCP, GT7_JZAZBZ = 0, 1
+AGX_LOOK_NONE, AGX_LOOK_GOLDEN, AGX_LOOK_PUNCHY = 0, 1, 2
+
+
+def display(**kw) -> abi.Display:
+    """Display settings with the reference's colour defaults (color_settings.h:9-30)."""
+    d = abi.Display(scene_colorspace=SCENE_REC709, view_transform=VIEW_GT7, display_colorspace=DISPLAY_REC709,
+                    output_format=5, white_point_K=5600.0, tint=0.0, reinhard_white_luminance=4.0,
+                    khronos_start_compression=0.8, khronos_desaturation=0.15, gt7_mode=GT7_JZAZBZ,
+                    gt7_sdr_paper_white=250.0, gt7_reference_luminance=100.0, gt7_jzazbz_exponent_scale=1.7,
+                    agx_look=AGX_LOOK_GOLDEN)
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d
+
+
+def display_transform(ctx, disp, xyz, out=None):
+    """XYZ (..., 3) float32 on the GPU -> display-encoded colour (same shape).
+    `out` may be `xyz` itself (in place) or a disjoint contiguous float32 tensor."""
+    import torch
+    if xyz.dtype != torch.float32 or not xyz.is_cuda or xyz.shape[-1] != 3:
+        raise ValueError("display_transform: xyz must be a float32 CUDA tensor of shape (..., 3)")
+    if out is xyz:
+        if not xyz.is_contiguous():
+            raise ValueError("display_transform: in-place output needs a contiguous tensor")
+    else:
+        xyz = xyz.contiguous()
+        if out is None:
+            out = torch.empty_like(xyz)
+    if out.dtype != torch.float32 or not out.is_contiguous() or out.numel() != xyz.numel():
+        raise ValueError("display_transform: out must be a contiguous float32 tensor with as many elements as xyz")
+    n = xyz.numel() // 3
+    abi.check(ctx.lib.mgk_display_transform(ctx.h, C.byref(disp), C.c_void_p(xyz.data_ptr()), n, C.c_void_p(out.data_ptr()),
+                                            Context._stream()), "mgk_display_transform")
+    return out
+
+
 # ---------------------------------------------------------------------------
 # Surrogate (learned path)
 # ---------------------------------------------------------------------------

@@ -91,6 +91,14 @@ class Spectral(C.Structure):  # mgk_spectral_t
                 ("reserved", C.c_uint32)]
 
 
+class Display(C.Structure):  # mgk_display_t
+    _fields_ = [("scene_colorspace", C.c_uint32), ("view_transform", C.c_uint32), ("display_colorspace", C.c_uint32),
+                ("output_format", C.c_uint32), ("white_point_K", C.c_float), ("tint", C.c_float),
+                ("reinhard_white_luminance", C.c_float), ("khronos_start_compression", C.c_float),
+                ("khronos_desaturation", C.c_float), ("gt7_mode", C.c_uint32), ("gt7_sdr_paper_white", C.c_float),
+                ("gt7_reference_luminance", C.c_float), ("gt7_jzazbz_exponent_scale", C.c_float), ("agx_look", C.c_uint32)]
+
+
 class Hit(C.Structure):  # mgk_hit_t
     _fields_ = [("cls", C.c_uint32), ("g", C.c_float), ("a0", C.c_float), ("a1", C.c_float)]
 
@@ -157,6 +165,7 @@ SIGNATURES = {
     "mgk_render_frame": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(Spectral), C.POINTER(Camera),
                               C.c_uint64, C.c_uint32, _P, _P]),
     "mgk_color_weights": (_R, [C.POINTER(Spectral), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mgk_display_transform": (_R, [_P, C.POINTER(Display), _P, C.c_uint64, _P, _P]),
     "mgk_camera_boyer_lindquist": (_R, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32,
                                         C.c_uint32, C.POINTER(Camera)]),
     "mgk_surrogate_create": (_R, [C.c_int, C.c_uint64, C.POINTER(SurrogateRanges), C.POINTER(_P)]),

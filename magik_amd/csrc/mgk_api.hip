@@ -17,6 +17,7 @@
 
 #include "magik_amd.h"
 #include "mgk_device.hpp"
+#include "mgk_display.hpp"
 #include "mgk_internal.hpp"
 #include "mgk_surrogate.hpp"
 
@@ -87,6 +88,11 @@ struct mgk_context {
     // zone-split integration: parked states and the two hand-off queues
     DeviceBuffer zone_state[2], zone_steps[2], zone_q0, zone_q1;
     int last_passes = 0;
+    // display parameters derived from the last display settings (the blackbody
+    // white point alone is ~1e3 Planck evaluations): reused while the settings repeat
+    mgk_display_t disp_key{};
+    disp::Params disp_params{};
+    bool disp_valid = false;
 };
 
 namespace {
@@ -318,31 +324,33 @@ void bin_weights(const mgk_spectral_t* sp, float* W) {
 f3 xy_to_XYZ(float x, float y) { return {x / y, 1.0f, (1.0f - x - y) / y}; }
 
 // Blackbody white point shifted by tint along the locus normal (host_color_utilities.cuh:29-57).
+// In double: the locus direction is the difference of the uv points at T and
+// T + 1 K, about 1e-5 of uv, which FP32 resolves to only ~1e-2.
 f3 white_point_xyz(float T, float tint) {
-    f3 a{0, 0, 0}, b{0, 0, 0};
-    float w = 360.0f, dw = (830.0f - 360.0f) / float(471 - 1);
+    double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+    const double h = 6.62607015e-34, c = 299792458.0, k = 1.380649e-23;
     for (uint32_t i = 0; i < 471; ++i) {
-        f3 c = cmf_at(w);
-        float p0 = planck_host(w, T), p1 = planck_host(w, T + 1);
-        a = {a.x + c.x * p0, a.y + c.y * p0, a.z + c.z * p0};
-        b = {b.x + c.x * p1, b.y + c.y * p1, b.z + c.z * p1};
-        w += dw;
+        const double lam = (360.0 + double(i)) * 1e-9;
+        const double p0 = 1.0 / (std::pow(lam, 5) * std::expm1(h * c / (lam * k * double(T))));
+        const double p1 = 1.0 / (std::pow(lam, 5) * std::expm1(h * c / (lam * k * (double(T) + 1.0))));
+        for (int j = 0; j < 3; ++j) {
+            a[j] += double(kCie1931[i][j]) * p0;
+            b[j] += double(kCie1931[i][j]) * p1;
+        }
     }
-    a = {a.x / a.y, a.y / a.y, a.z / a.y};
-    b = {b.x / b.y, b.y / b.y, b.z / b.y};
-    auto uv = [](f3 v) {
-        float d = v.x + 15.0f * v.y + 3.0f * v.z;
-        return std::pair<float, float>{4.0f * v.x / d, 6.0f * v.y / d};
+    auto uv = [](const double* v) {
+        const double d = v[0] + 15.0 * v[1] + 3.0 * v[2];
+        return std::pair<double, double>{4.0 * v[0] / d, 6.0 * v[1] / d};
     };
-    auto u0 = uv(a), u1 = uv(b);
-    float nx = u1.first - u0.first, ny = u1.second - u0.second;
-    float nl = std::sqrt(nx * nx + ny * ny);
+    const auto u0 = uv(a), u1 = uv(b);
+    double nx = u1.first - u0.first, ny = u1.second - u0.second;
+    const double nl = std::sqrt(nx * nx + ny * ny);
     nx /= nl; ny /= nl;
-    float tx = -ny, ty = nx;
-    if (ty < 0.0f) { tx = -tx; ty = -ty; }
-    float up = u0.first + tx * tint, vp = u0.second + ty * tint;
-    float den = 2.0f * up - 8.0f * vp + 4.0f;
-    return xy_to_XYZ(3.0f * up / den, 2.0f * vp / den);
+    double tx = -ny, ty = nx;
+    if (ty < 0.0) { tx = -tx; ty = -ty; }
+    const double up = u0.first + tx * double(tint), vp = u0.second + ty * double(tint);
+    const double den = 2.0 * up - 8.0 * vp + 4.0;
+    return xy_to_XYZ(float(3.0 * up / den), float(2.0 * vp / den));
 }
 
 // Bradford chromatic adaptation (host_color_utilities.cuh:59-68).
@@ -361,6 +369,69 @@ m3 xyz_to_scene(float white_K) {
     f3 src = white_point_xyz(white_K, 0.0f);
     f3 d65 = xy_to_XYZ(0.31272f, 0.32903f);
     return mul(to709, bradford(src, d65));
+}
+
+// Display pipeline parameters (color_pipeline.cuh:21-48 and the view
+// transforms' make_params, view_transform.cuh): white balance into the scene
+// space, the scene -> display matrix, the working-space conversions of the view
+// transforms (convert_scene_colorspace, host_color_utilities.cuh:70-73).
+disp::M3 to_disp(const m3& a) {
+    disp::M3 r;
+    std::memcpy(r.m, a.m, sizeof(r.m));
+    return r;
+}
+m3 from_disp(const disp::M3& a) {
+    m3 r;
+    std::memcpy(r.m, a.m, sizeof(r.m));
+    return r;
+}
+m3 space_convert(const disp::Space& from, const disp::Space& to) {
+    return mul(mul(from_disp(to.from_xyz), bradford(xy_to_XYZ(from.wx, from.wy), xy_to_XYZ(to.wx, to.wy))), from_disp(from.to_xyz));
+}
+
+e_magik_result_types display_params(const mgk_display_t* d, disp::Params& P) {
+    std::memset(&P, 0, sizeof(P));
+    disp::Space sc, dc, work;
+    if (!disp::scene_space(d->scene_colorspace, sc) || !disp::display_space(d->display_colorspace, dc)) return MAGIK_UNKNOWN_ENUM_TYPE;
+    if (d->view_transform > MAGIK_VIEW_TRANSFORM_KHRONOS_PBR_NEUTRAL || d->gt7_mode > MAGIK_GRAN_TURISMO_7_MODE_JZAZBZ ||
+        d->agx_look > MAGIK_AGX_LOOK_PUNCHY || d->output_format > MAGIK_OUTPUT_FORMAT_FLOAT32)
+        return MAGIK_UNKNOWN_ENUM_TYPE;
+    if (!(d->white_point_K > 0.0f) || !std::isfinite(d->tint)) return MGK_ERROR_INVALID_ARGUMENT;
+    P.view = d->view_transform;
+    P.display = d->display_colorspace;
+    P.xyz_to_scene = to_disp(mul(from_disp(sc.from_xyz), bradford(white_point_xyz(d->white_point_K, d->tint), xy_to_XYZ(sc.wx, sc.wy))));
+    P.scene_to_display = to_disp(space_convert(sc, dc));
+    P.lum = {sc.to_xyz.m[3], sc.to_xyz.m[4], sc.to_xyz.m[5]};
+    switch (d->view_transform) {
+        case MAGIK_VIEW_TRANSFORM_REINHARD_EXTENDED:
+            if (!(d->reinhard_white_luminance > 0.0f)) return MGK_ERROR_INVALID_ARGUMENT;
+            P.reinhard_white = d->reinhard_white_luminance;
+            break;
+        case MAGIK_VIEW_TRANSFORM_KHRONOS_PBR_NEUTRAL:
+            disp::scene_space(MAGIK_SCENE_COLORSPACE_LINEAR_REC709, work);
+            P.to_work = to_disp(space_convert(sc, work));
+            P.from_work = to_disp(space_convert(work, sc));
+            P.khr_start = d->khronos_start_compression;
+            P.khr_desat = d->khronos_desaturation;
+            break;
+        case MAGIK_VIEW_TRANSFORM_GRAN_TURISMO_7:
+            if (!(d->gt7_sdr_paper_white > 0.0f) || !(d->gt7_reference_luminance > 0.0f) || !(d->gt7_jzazbz_exponent_scale > 0.0f))
+                return MGK_ERROR_INVALID_ARGUMENT;
+            disp::scene_space(MAGIK_SCENE_COLORSPACE_LINEAR_REC2020, work);
+            P.to_work = to_disp(space_convert(sc, work));
+            P.from_work = to_disp(space_convert(work, sc));
+            P.gt7 = disp::gt7_setup(d->gt7_mode, d->gt7_sdr_paper_white, d->gt7_reference_luminance, d->gt7_jzazbz_exponent_scale);
+            break;
+        case MAGIK_VIEW_TRANSFORM_AGX:
+            disp::scene_space(MAGIK_SCENE_COLORSPACE_LINEAR_REC2020, work);
+            P.to_work = to_disp(space_convert(sc, work));
+            P.from_work = to_disp(space_convert(work, sc));
+            P.agx_look = d->agx_look;
+            break;
+        default:
+            break;
+    }
+    return MAGIK_SUCCESS;
 }
 
 // ---------------------------------------------------------------------------
@@ -846,6 +917,25 @@ MGK_API e_magik_result_types mgk_color_weights(const mgk_spectral_t* sp, float* 
         m3 m = xyz_to_scene(sp->white_point_K > 0.0f ? sp->white_point_K : 5600.0f);
         std::memcpy(xyz_to_scene_out, m.m, sizeof(m.m));
     }
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_display_transform(mgk_context_handle c, const mgk_display_t* d, const float* d_xyz, uint64_t n,
+                                                   float* d_out, void* stream) {
+    if (!c || !d || (n > 0 && (!d_xyz || !d_out))) return MAGIK_ERROR_INVALID_POINTER;
+    if (n > (uint64_t(1) << 60) / 12) return MGK_ERROR_INVALID_ARGUMENT;
+    // in place (d_out == d_xyz) or disjoint: a partial overlap would race between pixels
+    const uintptr_t a = uintptr_t(d_xyz), b = uintptr_t(d_out), bytes = uintptr_t(n) * 12u;
+    if (a != b && a < b + bytes && b < a + bytes) return MGK_ERROR_INVALID_ARGUMENT;
+    e_magik_result_types r;
+    if (!c->disp_valid || std::memcmp(&c->disp_key, d, sizeof(*d)) != 0) {
+        c->disp_valid = false;
+        if ((r = display_params(d, c->disp_params)) != MAGIK_SUCCESS) return r;
+        c->disp_key = *d;
+        c->disp_valid = true;
+    }
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    HIP_TRY(launch_display(c->disp_params, d_xyz, n, d_out, hipStream_t(stream)));
     return MAGIK_SUCCESS;
 }
 

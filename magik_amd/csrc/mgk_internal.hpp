@@ -142,6 +142,9 @@ struct StarView {
     float rho;
 };
 
+namespace disp { struct Params; }
+hipError_t launch_display(const disp::Params& p, const float* xyz, uint64_t n, float* out, hipStream_t s);
+
 // d_ic: device space for the per-camera constants (one IcCamera per camera)
 hipError_t launch_generate_rays(const RayGenParams& p, IcCamera* d_ic, hipStream_t s);
 hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t* d_offsets, uint32_t n_cam, uint64_t n, IcCamera* d_ic,

@@ -60,6 +60,8 @@ _REF_SIG = {
     "ref_sample_cmf": (None, [C.c_float, _F]),
     "ref_cmf_table": (C.c_uint, [_F]),
     "ref_layout": (None, [C.POINTER(C.c_size_t)]),
+    "ref_scene_colorspace": (C.c_int, [C.c_uint, _F, _F, _F]),
+    "ref_display_colorspace": (C.c_int, [C.c_uint, _F, _F, _F, _F, _F, C.c_uint, _F]),
 }
 
 _lib = None

@@ -1,12 +1,14 @@
 // Harness around the reference's own host-compilable headers, built only when
 // the reference sources are present (see oracle/Makefile, target `ref`).
-// Compiles internal_math.h, magik.h and color_matching_functions.cuh as they
-// lie in the reference tree -- no stand-in headers -- and exposes their
-// arithmetic through extern "C" so tests can pin the oracle against it.
+// Compiles internal_math.h, magik.h, color_matching_functions.cuh and
+// colorspaces.cuh as they lie in the reference tree -- no stand-in headers --
+// and exposes their arithmetic through extern "C" so tests can pin the oracle
+// against it.
 // TEST INFRASTRUCTURE ONLY; output goes to oracle/_ref/.
 #include "magik.h"
 #include "internal_math.h"
 #include "color_matching_functions.cuh"
+#include "colorspaces.cuh"
 #include "oracle.hpp"
 
 #include <cstddef>
@@ -47,6 +49,33 @@ hstate oscillator(hstate s) {
 }
 
 // The oracle's Kerr right-hand side in the harness' state type.
+void put(const magik::mat3f& m, float* out) {
+    const float v[9] = {m.m11, m.m12, m.m13, m.m21, m.m22, m.m23, m.m31, m.m32, m.m33};
+    for (int i = 0; i < 9; ++i) out[i] = v[i];
+}
+
+template <class S>
+void scene_space(float* to, float* from, float* wp) {
+    put(S::colorspace_to_XYZ, to);
+    put(S::XYZ_to_colorspace, from);
+    wp[0] = S::white_point_cie_xy.x;
+    wp[1] = S::white_point_cie_xy.y;
+}
+
+template <class D>
+void display_space(float* to, float* from, float* wp, float* peak) {
+    scene_space<D>(to, from, wp);
+    *peak = D::peak_luminance;
+}
+
+template <class D>
+void encode_all(const float* in, unsigned n, float* out) {
+    for (unsigned i = 0; i < n; ++i) {
+        const magik::vec3f v = D::encode(magik::vec3f{in[3 * i], in[3 * i + 1], in[3 * i + 2]});
+        out[3 * i] = v.x; out[3 * i + 1] = v.y; out[3 * i + 2] = v.z;
+    }
+}
+
 hstate kerr(hstate s, float a) {
     oracle::State<float> o{{s.q.x, s.q.y, s.q.z, s.q.t}, {s.p.x, s.p.y, s.p.z, s.p.t}};
     oracle::State<float> r = oracle::kerr_rhs(o, a);
@@ -94,6 +123,43 @@ unsigned ref_cmf_table(float* out) {
         out[3 * i] = C::values[i].x; out[3 * i + 1] = C::values[i].y; out[3 * i + 2] = C::values[i].z;
     }
     return C::n_entries;
+}
+
+// Scene colour space `id` (colorspaces.cuh:24-63): RGB -> XYZ and XYZ -> RGB
+// (row-major) and the white point's CIE xy.  Returns 0 for an unknown id.
+int ref_scene_colorspace(unsigned id, float* to, float* from, float* wp) {
+    using namespace magik;
+    switch (id) {
+        case MAGIK_SCENE_COLORSPACE_CIE_XYZ: scene_space<generic_scene_colorspace_t<MAGIK_SCENE_COLORSPACE_CIE_XYZ>>(to, from, wp); return 1;
+        case MAGIK_SCENE_COLORSPACE_CIE_RGB: scene_space<generic_scene_colorspace_t<MAGIK_SCENE_COLORSPACE_CIE_RGB>>(to, from, wp); return 1;
+        case MAGIK_SCENE_COLORSPACE_LINEAR_REC709: scene_space<generic_scene_colorspace_t<MAGIK_SCENE_COLORSPACE_LINEAR_REC709>>(to, from, wp); return 1;
+        case MAGIK_SCENE_COLORSPACE_LINEAR_REC2020: scene_space<generic_scene_colorspace_t<MAGIK_SCENE_COLORSPACE_LINEAR_REC2020>>(to, from, wp); return 1;
+        case MAGIK_SCENE_COLORSPACE_ACES_CG: scene_space<generic_scene_colorspace_t<MAGIK_SCENE_COLORSPACE_ACES_CG>>(to, from, wp); return 1;
+        case MAGIK_SCENE_COLORSPACE_ADOBE_RGB_1998: scene_space<generic_scene_colorspace_t<MAGIK_SCENE_COLORSPACE_ADOBE_RGB_1998>>(to, from, wp); return 1;
+        default: return 0;
+    }
+}
+
+// Display colour space `id` (colorspaces.cuh:90-338): matrices, white point,
+// peak luminance, and its encode applied to n RGB triples (in -> out).
+int ref_display_colorspace(unsigned id, float* to, float* from, float* wp, float* peak, const float* in, unsigned n, float* out) {
+    using namespace magik;
+    switch (id) {
+#define MGK_REF_DISPLAY(ID)                                                   \
+    case ID:                                                                  \
+        display_space<generic_display_colorspace_t<ID>>(to, from, wp, peak); \
+        encode_all<generic_display_colorspace_t<ID>>(in, n, out);           \
+        return 1;
+        MGK_REF_DISPLAY(MAGIK_DISPLAY_COLORSPACE_SRGB)
+        MGK_REF_DISPLAY(MAGIK_DISPLAY_COLORSPACE_REC709)
+        MGK_REF_DISPLAY(MAGIK_DISPLAY_COLORSPACE_REC2100_PQ)
+        MGK_REF_DISPLAY(MAGIK_DISPLAY_COLORSPACE_REC2100_HLG)
+        MGK_REF_DISPLAY(MAGIK_DISPLAY_COLORSPACE_DCI_P3_THEATRICAL_WHITE_POINT)
+        MGK_REF_DISPLAY(MAGIK_DISPLAY_COLORSPACE_DISPLAY_P3)
+        MGK_REF_DISPLAY(MAGIK_DISPLAY_COLORSPACE_ADOBE_RGB)
+#undef MGK_REF_DISPLAY
+        default: return 0;
+    }
 }
 
 // sizeof / offsetof of the magik.h structs the library re-declares.

@@ -1,8 +1,9 @@
 """Writes tests/golden/ref_vectors.json from the reference's own headers.
 
 Runs in the development container only, where /root/reference exists: it
-builds oracle/_ref/libmagik_ref.so (internal_math.h, magik.h and
-color_matching_functions.cuh compiled as they are, see oracle/Makefile) and
+builds oracle/_ref/libmagik_ref.so (internal_math.h, magik.h,
+color_matching_functions.cuh and colorspaces.cuh compiled as they are, see
+oracle/Makefile) and
 records input/output vectors of the reference arithmetic that the oracle
 restates.  The JSON holds data only (inputs, expected outputs).
 
@@ -31,7 +32,8 @@ def main():
     R = O.ref()
     assert R is not None, "reference harness not built (needs /root/reference)"
     rng = np.random.default_rng(20261203)
-    out = {"source": "reference headers via oracle/_ref (internal_math.h, magik.h, color_matching_functions.cuh)"}
+    out = {"source": "reference headers via oracle/_ref (internal_math.h, magik.h, color_matching_functions.cuh, "
+                     "colorspaces.cuh)"}
 
     # RKF45 driver (internal_math.h:2160-2205) on the harmonic oscillator
     osc = []
@@ -96,6 +98,29 @@ def main():
     tab = np.zeros((471, 3), dtype=np.float32)
     R.ref_cmf_table(O.fptr(tab))
     out["cmf_table_sum_bits"] = [f32bits(v) for v in tab.sum(axis=0, dtype=np.float64).astype(np.float32)]
+
+    # colour spaces and display encodes (colorspaces.cuh:24-338)
+    scene = []
+    for sid in range(6):
+        to, frm, wp = (C.c_float * 9)(), (C.c_float * 9)(), (C.c_float * 2)()
+        assert R.ref_scene_colorspace(sid, to, frm, wp) == 1
+        scene.append({"id": sid, "to_xyz_bits": [f32bits(v) for v in to], "from_xyz_bits": [f32bits(v) for v in frm],
+                      "white_xy_bits": [f32bits(v) for v in wp]})
+    out["scene_colorspaces"] = scene
+    rng_c = np.random.default_rng(20261204)
+    probe = np.array([-0.1, 0.0, 1e-7, 1e-4, 1e-3, 0.0031308, 0.004, 0.01, 0.05, 1 / 12, 0.1, 0.18, 0.5, 0.9, 1.0, 1.5, 10.0,
+                      100.0], dtype=np.float32)
+    rgb = np.concatenate([np.repeat(probe[:, None], 3, axis=1), rng_c.choice(probe, size=(30, 3)),
+                          rng_c.uniform(0, 1.2, size=(30, 3))]).astype(np.float32)
+    display = []
+    for did in range(7):
+        to, frm, wp, peak = (C.c_float * 9)(), (C.c_float * 9)(), (C.c_float * 2)(), C.c_float(0)
+        enc = np.zeros_like(rgb)
+        assert R.ref_display_colorspace(did, to, frm, wp, C.byref(peak), O.fptr(rgb), len(rgb), O.fptr(enc)) == 1
+        display.append({"id": did, "to_xyz_bits": [f32bits(v) for v in to], "from_xyz_bits": [f32bits(v) for v in frm],
+                        "white_xy_bits": [f32bits(v) for v in wp], "peak_nits": peak.value,
+                        "encode_in": rgb.tolist(), "encode_out_bits": [[f32bits(v) for v in r] for r in enc]})
+    out["display_colorspaces"] = display
 
     # struct layouts and result codes of magik.h
     lay = (C.c_size_t * 14)()
