@@ -1,6 +1,6 @@
 """Benchmark: Kerr a=0.9 1920x1080x32 spp frames on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kerr_disk_1080p|kerr_disk_4k|batch64]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kerr_disk_1080p|kerr_disk_4k|batch64|box_led]
 
 A step is one full frame through the C-ABI of synthetic seeded rays, timed on
 two paths:
@@ -129,6 +129,30 @@ def radiance_error(ctx, M, integ, sc, spec, cam, spp, seed, test_hits=None, n_su
     return res
 
 
+def frame_error(ctx, M, integ, sc, spec, cam, spp, seed, frame):
+    """Box scenes: paths that reflect around the hole are chaotic at the
+    reference's tolerance, so the frame is compared with the FP64 frame of the
+    same seed in aggregate: the relative difference of the mean Y, and of the
+    mean Y over 32 x 32-pixel blocks (mean absolute, relative to the frame's
+    mean)."""
+    import torch
+    it64 = M.integrator()
+    C.memmove(C.byref(it64), C.byref(integ), C.sizeof(integ))
+    it64.precision = M.PREC_FP64
+    ref = ctx.render_frame(it64, sc, spec, cam, seed, spp)
+    torch.cuda.synchronize()
+    y, yr = frame[..., 1].double(), ref[..., 1].double()
+    H, W = y.shape
+    b = 32
+    hb, wb = H // b, W // b
+    blk = y[:hb * b, :wb * b].reshape(hb, b, wb, b).mean(dim=(1, 3))
+    blk_r = yr[:hb * b, :wb * b].reshape(hb, b, wb, b).mean(dim=(1, 3))
+    m = float(yr.mean())
+    return {"mean_Y_rel_diff": float(abs(y.mean() - yr.mean()) / max(m, 1e-30)),
+            "block32_mean_abs_rel_diff": float((blk - blk_r).abs().mean() / max(m, 1e-30)),
+            "note": "aggregate comparison with the FP64 frame of the same seed (whole paths are chaotic)"}
+
+
 def cpu_baseline(M, integ, sc, cam, spp, seed, budget_s=12.0):
     """Oracle FP32 restatement of the reference integrator on the host cores."""
     from oracle import oracle as O
@@ -139,20 +163,32 @@ def cpu_baseline(M, integ, sc, cam, spp, seed, budget_s=12.0):
     npx = cam.x_resolution * cam.y_resolution
     rng = np.random.default_rng(seed)
 
+    boxes = bool(sc.flags & M.SCENE_BOXES)
+
     def run(n):
-        idx = rng.choice(npx * spp, size=n, replace=False)
+        idx = np.sort(rng.choice(npx * spp, size=n, replace=False))
         rays = O.generate_rays(O.F32, it, cam, seed, (idx % npx).astype(np.uint32), (idx // npx).astype(np.uint32))
         t = time.perf_counter()
-        _, steps, trials = O.trace_rays(O.F32, it, sc, list(cam.position), rays, threads=threads)
+        if boxes:
+            # whole paths with their reflections; the sample's paths keep their own Philox streams
+            # only approximately (ray i of the sample stands for a path of the frame)
+            _, steps, _ = O.trace_paths(O.F32, it, sc, cam, seed, 0, rays, threads=threads)
+            trials = steps
+        else:
+            _, steps, trials = O.trace_rays(O.F32, it, sc, list(cam.position), rays, threads=threads)
         return time.perf_counter() - t, int(steps.sum()), int(trials.sum())
 
     dt, _, _ = run(1 << 14)
     n = int(min(1 << 25, max(1 << 16, (1 << 14) * budget_s / max(dt, 1e-3))))
     dt, steps, trials = run(n)
-    return {"value": n / dt, "unit": "rays/s", "cores": threads, "kind": "port",
-            "sample": f"{n} seeded rays of the same 1920x1080x32 Kerr frame, FP32 reference integrator "
-                      f"(oracle/oracle.hpp) with OpenMP over rays, {dt:.1f} s",
-            "steps_per_s": steps / dt, "trials_per_s": trials / dt}
+    what = "whole paths (reflections included)" if boxes else "rays"
+    res = {"value": n / dt, "unit": "rays/s", "cores": threads, "kind": "port",
+           "sample": f"{n} seeded {what} of the same {cam.x_resolution}x{cam.y_resolution}x{spp} frame, FP32 "
+                     f"reference integrator (oracle/oracle.hpp) with OpenMP over rays, {dt:.1f} s",
+           "steps_per_s": steps / dt}
+    if not boxes:
+        res["trials_per_s"] = trials / dt
+    return res
 
 
 def time_frames(render, steps, warmup, sync_stats=None):
@@ -260,10 +296,16 @@ def main():
     plain = {"rays_per_s": plain_rate, "ms_per_frame": wall / args.steps * 1e3,
              "geodesic_steps_per_s": acc["steps"] / wall, "rkf45_trials_per_s": acc["trials"] / wall,
              "fp64_trial_fraction": acc["fp64"] / max(acc["trials"], 1), "trace_ms_per_frame": trace_ms}
-    if not args.no_error:
+    boxes = bool(sc.flags & M.SCENE_BOXES)
+    if boxes:
+        plain["reflections_per_path"] = sum(st.n_reflections for st in per) / (n_paths * args.steps)
+    if not args.no_error and boxes:
+        plain["frame_err_vs_fp64"] = frame_error(ctx, M, integ, sc, spec, cam, spp, args.seed, out)
+        log(f"  frame error {plain['frame_err_vs_fp64']}")
+    elif not args.no_error:
         plain["radiance_err_vs_fp64"] = radiance_error(ctx, M, integ, sc, spec, cam, spp, args.seed)
         log(f"  radiance error {plain['radiance_err_vs_fp64']}")
-    if prec == M.PREC_MIXED:
+    if prec == M.PREC_MIXED and not boxes:
         # the plain path in the reference's FP32 arithmetic (the integrator the
         # reference ships), for the speed ratios
         it32 = M.integrator()
@@ -316,10 +358,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if prec != M.PREC_FP64 else "f64",
-        "data": f"synthetic (seeded Philox rays, Kerr a={integ.spin:.2f} disk + 1e5-star field scene; surrogate "
-                "trained on seeded FP64-traced rays)",
-        "config": {"workload": f"{args.workload}: Kerr a={integ.spin:.2f}, r_obs=50M, theta=80 deg, FOV 25 deg, "
-                               f"{W}x{H}x{spp} spp, 4 hero wavelengths, tol 1e-7, cap 1e4",
+        "data": (f"synthetic (seeded Philox rays, Kerr a={integ.spin:.3f} hole inside a diffuse box with an LED "
+                 "panel, the reference's default stage; seeded Lambertian reflections)") if boxes else
+                (f"synthetic (seeded Philox rays, Kerr a={integ.spin:.2f} disk + 1e5-star field scene; surrogate "
+                 "trained on seeded FP64-traced rays)"),
+        "config": {"workload": (f"{args.workload}: Kerr a={integ.spin:.3f}, M=6.73e26 kg, 40x20x40 m diffuse box, "
+                                f"20x20 m LED panel, camera (2,0,9) m, {W}x{H}x{spp} spp, 4 hero wavelengths, "
+                                f"tol 1e-3/1e-10, step limit 512, {sc.max_reflections} reflections") if boxes else
+                               (f"{args.workload}: Kerr a={integ.spin:.2f}, r_obs=50M, theta=80 deg, FOV 25 deg, "
+                                f"{W}x{H}x{spp} spp, 4 hero wavelengths, tol 1e-7, cap 1e4"),
                    "paths_per_frame": n_paths, "value_path": best[0], "error_bar": ERROR_BAR},
         "plain": plain,
         "surrogate": sur_res,

@@ -189,7 +189,12 @@ typedef enum mgk_hit_class_t {
     MGK_HIT_ESCAPE = 2,
     MGK_HIT_DISK = 3,
     MGK_HIT_SPHERE = 4,
-    MGK_HIT_STEP_LIMIT = 5
+    MGK_HIT_STEP_LIMIT = 5,
+    /* diffuse box matter still hit after the last allowed reflection
+     * (kernel_enforce_ray_limits.cu:48-50, n_reflected > limit): no radiance */
+    MGK_HIT_BOX_DIFFUSE = 6,
+    /* illuminant box matter (matter_examples.cuh:63-76) */
+    MGK_HIT_BOX_EMITTER = 7
 } mgk_hit_class_t;
 
 /* Integrator / spacetime settings (relativity_settings.h:11-25,
@@ -227,8 +232,26 @@ enum {
     MGK_SCENE_DISK = 1u,
     MGK_SCENE_SPHERE = 2u,
     MGK_SCENE_STARS = 4u,
-    MGK_SCENE_BACKGROUND = 8u
+    MGK_SCENE_BACKGROUND = 8u,
+    MGK_SCENE_BOXES = 16u
 };
+
+/* Matter of a box: the reference's registered matter slots 1 and 2
+ * (kernel_bxdf.cu:126, matter_examples.cuh:36-76). */
+typedef enum mgk_matter_t {
+    MGK_MATTER_DIFFUSE = 1,      /* specular on faces with an x normal, Lambertian on the others */
+    MGK_MATTER_ILLUMINANT = 2    /* white LED spectrum x led_scale */
+} mgk_matter_t;
+
+#define MGK_MAX_BOX 4
+
+/* An axis-aligned box in SI metres (stage.h:19-22; traceables.cuh:17-74). */
+typedef struct mgk_box_t {
+    float min[3];
+    float max[3];
+    uint32_t matter;             /* mgk_matter_t */
+    uint32_t reserved;
+} mgk_box_t;
 
 /* Traceables and emitters. */
 typedef struct mgk_scene_t {
@@ -247,6 +270,13 @@ typedef struct mgk_scene_t {
     float star_radius_rad;
     uint32_t n_star;
     uint64_t star_seed;
+    /* Boxes (MGK_SCENE_BOXES), tested in order after the disk and the sphere;
+     * a later box wins only when strictly closer (kernel_intersect.cu:83-105). */
+    uint32_t n_box;
+    uint32_t max_reflections;    /* render_settings.h:17 n_reflected_limit (10) */
+    float led_scale;             /* illuminant radiance scale (matter_examples.cuh:70: 7e5) */
+    uint32_t reserved2;
+    mgk_box_t box[MGK_MAX_BOX];
 } mgk_scene_t;
 
 /* Spectral accumulation (color_settings.h:11-14, optical_system_settings.h:22-27). */
@@ -277,7 +307,8 @@ typedef struct mgk_trace_stats_t {
     uint64_t n_steps;            /* accepted (or exhausted) steps */
     uint64_t n_trials;           /* RKF45 trials */
     uint64_t n_trials_fp64;      /* trials run entirely in FP64 */
-    double ms;                   /* device time of the integration kernel */
+    double ms;                   /* device time of the integration kernel(s) */
+    uint64_t n_reflections;      /* reflections off diffuse box matter (paths re-traced) */
 } mgk_trace_stats_t;
 
 typedef struct mgk_context* mgk_context_handle;
@@ -311,13 +342,30 @@ MGK_API e_magik_result_types mgk_trace_rays(mgk_context_handle ctx, const mgk_in
                                             uint32_t* d_steps, void* stream);
 
 /* mgk_trace_rays that also writes each ray's state at termination (the end
- * of its last step): d_final[c * ray_stride + i] for c = 0..7 = x, y, z, t,
- * p_x, p_y, p_z, p_t (doubles, dimensionless).  For endpoint-direction and
- * invariant-drift measurements against the FP64 path. */
+ * of its last step; for box matter, the chord point of the hit with the
+ * step's end momentum): d_final[c * ray_stride + i] for c = 0..7 = x, y,
+ * z, t, p_x, p_y, p_z, p_t (doubles, dimensionless).  For endpoint-direction
+ * and invariant-drift measurements against the FP64 path. */
 MGK_API e_magik_result_types mgk_trace_rays_final(mgk_context_handle ctx, const mgk_integrator_t* integ, const mgk_scene_t* scene,
                                                   const float camera_position[3], const void* d_rays, uint64_t ray_stride,
                                                   uint64_t n_rays, const uint32_t* d_queue, mgk_hit_t* d_hits, double* d_final,
                                                   void* stream);
+
+/* Whole paths of a camera's rays (W x H x spp, ray i = sample i / (W H) of
+ * pixel i % (W H), as mgk_generate_rays lays them out): trace, and at diffuse
+ * box matter reflect and trace again, up to scene->max_reflections times
+ * (kernel_bxdf.cu:69-142 with bxdf_utilities.cuh:56-151, matter_examples.cuh:36-60,
+ * brdf.cuh:12-29; the limit of kernel_enforce_ray_limits.cu:48-50).  The
+ * step count runs on across reflections.  Reflection k draws its uniforms from
+ * Philox block 1 + k of (seed, camera_id, pixel, sample_begin + sample).
+ * d_rays (FP32 for the FP32 reference precision, else FP64) is overwritten in
+ * place with the reflected rays' initial conditions.  d_hits receives each
+ * path's last hit; d_steps (optional) its total steps; d_final (optional,
+ * 8 x ray_stride doubles) its terminal state as mgk_trace_rays_final writes it. */
+MGK_API e_magik_result_types mgk_trace_paths(mgk_context_handle ctx, const mgk_integrator_t* integ, const mgk_scene_t* scene,
+                                             const mgk_camera_t* camera, uint64_t seed, uint32_t sample_begin, uint32_t spp,
+                                             void* d_rays, uint64_t ray_stride, mgk_hit_t* d_hits, uint32_t* d_steps,
+                                             double* d_final, void* stream);
 
 /* Batched cameras in one persistent launch (several scenes' viewpoints with
  * their own spin, resolution and pose).  mgk_generate_rays_batch writes the

@@ -12,9 +12,9 @@ import ctypes as C
 import math
 
 from . import abi
-from .abi import (HIT_DISK, HIT_ESCAPE, HIT_HORIZON, HIT_NONE, HIT_SPHERE, HIT_STEP_LIMIT, KERR, MINKOWSKI,
-                  PREC_FP32_REFERENCE, PREC_FP64, PREC_MIXED, SCHWARZSCHILD, SCENE_BACKGROUND, SCENE_DISK,
-                  SCENE_SPHERE, SCENE_STARS)
+from .abi import (HIT_BOX_DIFFUSE, HIT_BOX_EMITTER, HIT_DISK, HIT_ESCAPE, HIT_HORIZON, HIT_NONE, HIT_SPHERE,
+                  HIT_STEP_LIMIT, KERR, MATTER_DIFFUSE, MATTER_ILLUMINANT, MINKOWSKI, PREC_FP32_REFERENCE, PREC_FP64,
+                  PREC_MIXED, SCHWARZSCHILD, SCENE_BACKGROUND, SCENE_BOXES, SCENE_DISK, SCENE_SPHERE, SCENE_STARS)
 
 DEFAULT_MASS_KG = 6.7329546075e26       # relativity_settings.h:21,24
 G, C_LIGHT = 6.67430e-11, 299792458.0
@@ -72,13 +72,23 @@ def camera_pinhole(position, orientation, fov_deg: float, W: int, H: int, camera
 
 
 def scene(**kw) -> abi.Scene:
+    """A scene; `boxes` is a list of (min xyz, max xyz, matter) in SI metres."""
     s = abi.Scene(celestial_radius_m=1000.0, flags=0, disk_r_in=6.0, disk_r_out=20.0, disk_T_in=1e4,
                   sphere_radius=1.0, sphere_T=6500.0, sphere_checker=16, background_scale=0.0, background_T=6500.0,
-                  star_scale=1.0, star_T=6500.0, star_radius_rad=1e-3, n_star=0, star_seed=0)
+                  star_scale=1.0, star_T=6500.0, star_radius_rad=1e-3, n_star=0, star_seed=0, n_box=0,
+                  max_reflections=10, led_scale=7e5)
     for k, v in kw.items():
         if k == "sphere_center":
             for i in range(3):
                 s.sphere_center[i] = v[i]
+        elif k == "boxes":
+            if len(v) > abi.MAX_BOX:
+                raise ValueError(f"at most {abi.MAX_BOX} boxes")
+            s.n_box = len(v)
+            for b, (lo, hi, matter) in enumerate(v):
+                for i in range(3):
+                    s.box[b].min[i], s.box[b].max[i] = lo[i], hi[i]
+                s.box[b].matter = matter
         else:
             setattr(s, k, v)
     return s
@@ -104,6 +114,10 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
     kerr_disk_4k      : the learned-surrogate frame: a=0.99, 3840x2160x16,
                         otherwise as kerr_disk_1080p (camera r=50M, theta=80
                         deg, 25 deg; disk r_ISCO(0.99)-30M).
+    box_led           : the reference's default stage: Kerr a=0.999, M=6.73e26
+                        kg, a diffuse 40x20x40 m box with a 20x20 m LED panel
+                        at its ceiling, camera at (2,0,9) m, 2195x1860x8, tol
+                        1e-3 / 1e-10, step limit 512, 10 reflections.
     """
     if name == "minkowski_sphere":
         W, H, spp = W or 256, H or 256, spp or 4
@@ -133,6 +147,25 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
         sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_STARS, disk_r_in=r_isco(a),
                    disk_r_out=30.0, disk_T_in=1e4, n_star=100000, star_seed=0x5EED, star_scale=1.0,
                    star_radius_rad=1e-3, star_T=6500.0)
+        spec = spectral(n_hero=4)
+        return integ, sc, spec, cam, spp
+    if name == "box_led":
+        # the reference's default stage (stage.h:19-22, relativity_settings.h:11-25,
+        # optical_system_settings.h:20-34, render_settings.h:16-17): a Kerr hole of
+        # 6.73e26 kg (M = 0.5 m), a = 0.999, inside a 40 x 20 x 40 m diffuse box
+        # with a 20 x 20 m LED panel at its ceiling; camera at (2, 0, 9) m
+        W, H, spp = W or 2195, H or 1860, spp or 8
+        integ = integrator(metric=KERR, precision=PREC_MIXED if precision is None else precision, spin=0.999,
+                           abs_tol=1e-3, rel_tol=1e-10, step_limit=512)
+        cam = abi.Camera()
+        cam.position[0], cam.position[1], cam.position[2] = 2.0, 0.0, 9.0
+        for i, v in enumerate((1, 0, 0, 0, 1, 0, 0, 0, 1)):
+            cam.orientation[i] = v
+        cam.focal_length_mm, cam.sensor_width_mm = 12.0, 24.0
+        cam.x_resolution, cam.y_resolution = W, H
+        sc = scene(celestial_radius_m=1000.0, flags=SCENE_BOXES, max_reflections=10, led_scale=7e5,
+                   boxes=[((-20.0, -10.0, -20.0), (20.0, 10.0, 20.0), MATTER_DIFFUSE),
+                          ((-10.0, 9.5, -10.0), (10.0, 10.5, 10.0), MATTER_ILLUMINANT)])
         spec = spectral(n_hero=4)
         return integ, sc, spec, cam, spp
     raise ValueError(f"unknown workload {name}")
@@ -226,6 +259,22 @@ class Context:
                                                 C.c_void_p(hits.data_ptr()), C.c_void_p(final.data_ptr()),
                                                 self._stream()), "mgk_trace_rays_final")
         return hits, final
+
+    def trace_paths(self, integ, sc, cam, seed: int, rays, spp: int, sample_begin: int = 0, want_final: bool = False):
+        """Whole paths of a camera's rays with reflections off diffuse boxes
+        (mgk_trace_paths); `rays` is overwritten for the reflected paths.
+        Returns (hits, steps), and the terminal states (8, stride) with want_final."""
+        import torch
+        stride = rays.shape[1]
+        hits = torch.zeros((stride, 4), dtype=torch.int32, device=rays.device)
+        steps = torch.zeros(stride, dtype=torch.int32, device=rays.device)
+        final = torch.zeros((8, stride), dtype=torch.float64, device=rays.device) if want_final else None
+        abi.check(self.lib.mgk_trace_paths(self.h, C.byref(integ), C.byref(sc), C.byref(cam), seed, sample_begin, spp,
+                                           C.c_void_p(rays.data_ptr()), stride, C.c_void_p(hits.data_ptr()),
+                                           C.c_void_p(steps.data_ptr()),
+                                           C.c_void_p(final.data_ptr() if want_final else None), self._stream()),
+                  "mgk_trace_paths")
+        return (hits, steps, final) if want_final else (hits, steps)
 
     def generate_rays_batch(self, integ, cams, seed: int, spp: int):
         """Rays of every batch camera: ((11, n) float64 tensor, offsets (n_cam + 1))."""

@@ -29,10 +29,12 @@ MASSIVE, MASSLESS = 0, 1
 
 PREC_FP32_REFERENCE, PREC_MIXED, PREC_FP64 = 0, 1, 2
 
-HIT_NONE, HIT_HORIZON, HIT_ESCAPE, HIT_DISK, HIT_SPHERE, HIT_STEP_LIMIT = range(6)
-HIT_NAMES = {0: "none", 1: "horizon", 2: "escape", 3: "disk", 4: "sphere", 5: "step_limit"}
+HIT_NONE, HIT_HORIZON, HIT_ESCAPE, HIT_DISK, HIT_SPHERE, HIT_STEP_LIMIT, HIT_BOX_DIFFUSE, HIT_BOX_EMITTER = range(8)
+HIT_NAMES = {0: "none", 1: "horizon", 2: "escape", 3: "disk", 4: "sphere", 5: "step_limit", 6: "box_diffuse",
+             7: "box_emitter"}
 
-SCENE_DISK, SCENE_SPHERE, SCENE_STARS, SCENE_BACKGROUND = 1, 2, 4, 8
+SCENE_DISK, SCENE_SPHERE, SCENE_STARS, SCENE_BACKGROUND, SCENE_BOXES = 1, 2, 4, 8, 16
+MATTER_DIFFUSE, MATTER_ILLUMINANT = 1, 2
 
 
 class MetricParameters(C.Structure):  # magik_trace_geodesic_metric_parameters_t
@@ -75,13 +77,21 @@ class Camera(C.Structure):  # mgk_camera_t
                 ("observer_velocity", C.c_float * 3), ("camera_id", C.c_uint32)]
 
 
+MAX_BOX = 4
+
+
+class Box(C.Structure):  # mgk_box_t
+    _fields_ = [("min", C.c_float * 3), ("max", C.c_float * 3), ("matter", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class Scene(C.Structure):  # mgk_scene_t
     _fields_ = [("celestial_radius_m", C.c_float), ("flags", C.c_uint32), ("disk_r_in", C.c_float),
                 ("disk_r_out", C.c_float), ("disk_T_in", C.c_float), ("sphere_center", C.c_float * 3),
                 ("sphere_radius", C.c_float), ("sphere_T", C.c_float), ("sphere_checker", C.c_uint32),
                 ("background_scale", C.c_float), ("background_T", C.c_float), ("star_scale", C.c_float),
                 ("star_T", C.c_float), ("star_radius_rad", C.c_float), ("n_star", C.c_uint32),
-                ("star_seed", C.c_uint64)]
+                ("star_seed", C.c_uint64), ("n_box", C.c_uint32), ("max_reflections", C.c_uint32),
+                ("led_scale", C.c_float), ("reserved2", C.c_uint32), ("box", Box * MAX_BOX)]
 
 
 class Spectral(C.Structure):  # mgk_spectral_t
@@ -109,7 +119,7 @@ class BatchCamera(C.Structure):  # mgk_batch_camera_t
 
 class TraceStats(C.Structure):  # mgk_trace_stats_t
     _fields_ = [("n_rays", C.c_uint64), ("n_steps", C.c_uint64), ("n_trials", C.c_uint64),
-                ("n_trials_fp64", C.c_uint64), ("ms", C.c_double)]
+                ("n_trials_fp64", C.c_uint64), ("ms", C.c_double), ("n_reflections", C.c_uint64)]
 
 
 class SurrogateRanges(C.Structure):  # mgk_surrogate_ranges_t
@@ -155,6 +165,8 @@ SIGNATURES = {
                             C.c_uint64, _P, _P, _P, _P]),
     "mgk_trace_rays_final": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(C.c_float), _P, C.c_uint64,
                                   C.c_uint64, _P, _P, _P, _P]),
+    "mgk_trace_paths": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(Camera), C.c_uint64, C.c_uint32,
+                             C.c_uint32, _P, C.c_uint64, _P, _P, _P, _P]),
     "mgk_generate_rays_batch": (_R, [_P, C.POINTER(Integrator), C.POINTER(BatchCamera), C.c_uint32, C.c_uint64,
                                      C.c_uint32, _P, C.c_uint64, C.POINTER(C.c_uint64), _P]),
     "mgk_trace_rays_batch": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), _P, C.c_uint64, C.c_uint64, _P, _P, _P,

@@ -88,6 +88,9 @@ struct mgk_context {
     // zone-split integration: parked states and the two hand-off queues
     DeviceBuffer zone_state[2], zone_steps[2], zone_q0, zone_q1;
     int last_passes = 0;
+    // paths continued after reflections: terminal states, step counts, reflected-ray queues
+    DeviceBuffer path_state, path_steps, refl_q[2];
+    uint64_t last_reflections = 0;
     // display parameters derived from the last display settings (the blackbody
     // white point alone is ~1e3 Planck evaluations): reused while the settings repeat
     mgk_display_t disp_key{};
@@ -174,6 +177,33 @@ void fill_trace_params(const mgk_integrator_t* in, const mgk_scene_t* sc, const 
     P.disk_r_out = sc->disk_r_out;
     for (int k = 0; k < 3; ++k) P.sphere_c[k] = sc->sphere_center[k];
     P.sphere_r = sc->sphere_radius;
+    if (sc->flags & MGK_SCENE_BOXES) {
+        P.n_box = sc->n_box < uint32_t(MGK_MAX_BOX) ? sc->n_box : uint32_t(MGK_MAX_BOX);
+        for (uint32_t b = 0; b < P.n_box; ++b) {
+            for (int k = 0; k < 3; ++k) {
+                P.box_min[b][k] = sc->box[b].min[k];
+                P.box_max[b][k] = sc->box[b].max[k];
+            }
+            P.box_matter[b] = sc->box[b].matter;
+        }
+    }
+}
+
+e_magik_result_types check_scene(const mgk_scene_t* sc) {
+    if (!sc) return MAGIK_ERROR_INVALID_POINTER;
+    if (sc->flags & MGK_SCENE_BOXES) {
+        if (sc->n_box > uint32_t(MGK_MAX_BOX)) return MGK_ERROR_INVALID_ARGUMENT;
+        for (uint32_t b = 0; b < sc->n_box; ++b)
+            if (sc->box[b].matter != MGK_MATTER_DIFFUSE && sc->box[b].matter != MGK_MATTER_ILLUMINANT) return MGK_ERROR_INVALID_ARGUMENT;
+    }
+    return MAGIK_SUCCESS;
+}
+
+bool has_diffuse_box(const TraceParams& P) {
+    if (!(P.flags & MGK_SCENE_BOXES)) return false;
+    for (uint32_t b = 0; b < P.n_box; ++b)
+        if (P.box_matter[b] == MGK_MATTER_DIFFUSE) return true;
+    return false;
 }
 
 int trace_grid(mgk_context* c, const TraceParams& P) {
@@ -245,18 +275,89 @@ e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t 
     return MAGIK_SUCCESS;
 }
 
-e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s) {
+// One trace of P.n rays (of P.queue when given): the zone passes, or one
+// persistent launch.
+e_magik_result_types trace_once(mgk_context* c, TraceParams& P, hipStream_t s) {
+    if (P.n == 0) return MAGIK_SUCCESS;
+    if (P.precision == MGK_PRECISION_MIXED && P.metric != MINKOWSKI) return do_trace_zones(c, P, s);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c->d_counters, 1);   // head
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(launch_trace(P, trace_grid(c, P), s));
+    return MAGIK_SUCCESS;
+}
+
+// The Philox stream of a camera's paths, for the reflections' uniforms.
+struct PathInfo {
+    uint64_t seed;
+    uint32_t camera_id, npix, sample_begin;
+};
+
+// Trace, then (with `paths` and a scene with diffuse boxes) reflect the paths
+// that ended on diffuse matter and trace them again, up to max_reflections
+// times (kernel_bxdf.cu:69-95, kernel_enforce_ray_limits.cu:48-50).  Each
+// round reads one count back (an 8-byte copy).  Reflected rays get new initial
+// conditions in place in P.rays.
+e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, const PathInfo* paths = nullptr,
+                              uint32_t max_reflections = 0) {
     P.head = c->d_counters;
     P.stats = c->d_counters + 1;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c->d_counters, 4);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev0, s));
-    if (P.n > 0) {
-        if (P.precision == MGK_PRECISION_MIXED && P.metric != MINKOWSKI) {
-            e_magik_result_types r = do_trace_zones(c, P, s);
-            if (r != MAGIK_SUCCESS) return r;
-        } else {
-            HIP_TRY(launch_trace(P, trace_grid(c, P), s));
+    c->last_reflections = 0;
+    const bool reflect = paths && max_reflections > 0 && P.n > 0 && has_diffuse_box(P);
+    if (reflect) {
+        if (P.n >= uint64_t(kHole)) return MGK_ERROR_CAPACITY_EXCEEDED;
+        HIP_TRY(c->refl_q[0].reserve(P.n * sizeof(uint32_t)));
+        HIP_TRY(c->refl_q[1].reserve(P.n * sizeof(uint32_t)));
+        if (!P.final_state) {
+            HIP_TRY(c->path_state.reserve(P.stride * 8 * sizeof(double)));
+            P.final_state = reinterpret_cast<double*>(c->path_state.ptr);
+        }
+        if (!P.steps_out) {
+            HIP_TRY(c->path_steps.reserve(P.stride * sizeof(uint32_t)));
+            P.steps_out = reinterpret_cast<uint32_t*>(c->path_steps.ptr);
+        }
+    }
+    e_magik_result_types r = trace_once(c, P, s);
+    if (r != MAGIK_SUCCESS) return r;
+    if (reflect) {
+        const uint32_t* in = P.queue;
+        uint64_t n_in = P.n;
+        for (uint32_t k = 0; k < max_reflections; ++k) {
+            ReflectParams R;
+            std::memset(&R, 0, sizeof(R));
+            R.metric = P.metric;
+            R.hits = P.hits;
+            R.state = P.final_state;
+            R.rays = const_cast<void*>(P.rays);
+            R.stride = P.stride;
+            R.queue_in = in;
+            R.n_in = n_in;
+            R.queue_out = reinterpret_cast<uint32_t*>(c->refl_q[k & 1].ptr);
+            R.count = c->d_counters + 6;
+            R.seed = paths->seed;
+            R.camera_id = paths->camera_id;
+            R.npix = paths->npix;
+            R.sample_begin = paths->sample_begin;
+            R.block = 1 + k;
+            R.spin = P.spin;
+            R.mass_sign = P.mass_sign;
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, R.count, 1);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(launch_reflect(R, P.precision == MGK_PRECISION_FP32_REFERENCE, s));
+            unsigned long long cnt = 0;
+            HIP_TRY(hipMemcpyAsync(&cnt, R.count, sizeof(cnt), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (cnt == 0) break;
+            c->last_reflections += cnt;
+            TraceParams Q = P;
+            Q.queue = R.queue_out;
+            Q.n = cnt;
+            Q.steps_in = P.steps_out;
+            if ((r = trace_once(c, Q, s)) != MAGIK_SUCCESS) return r;
+            in = R.queue_out;
+            n_in = cnt;
         }
     }
     HIP_TRY(hipEventRecord(c->ev1, s));
@@ -593,6 +694,7 @@ e_magik_result_types fill_shade_params(mgk_context* c, const mgk_scene_t* sc, co
     float npx = float(cam->x_resolution * cam->y_resolution);
     float thr = P.hero_span / float(sp->n_hero);
     P.irr_scale = thr * (area / npx) * sp->iso * 2.0f * kPi / ((360.0f * sp->fps) / sp->shutter_angle);
+    P.led_scale = sc->led_scale;
     bin_weights(sp, P.weights);
     HIP_TRY(c->shade_resp.reserve(size_t(kRespN) * sizeof(float4)));
     P.resp = reinterpret_cast<const float4*>(c->shade_resp.ptr);
@@ -713,6 +815,7 @@ MGK_API e_magik_result_types mgk_trace_rays(mgk_context_handle c, const mgk_inte
     if (!c || !sc || !cam_pos || !d_hits || (n > 0 && !d_rays)) return MAGIK_ERROR_INVALID_POINTER;
     e_magik_result_types r = check_integrator(in);
     if (r != MAGIK_SUCCESS) return r;
+    if ((r = check_scene(sc)) != MAGIK_SUCCESS) return r;
     if (!d_queue && n > stride) return MGK_ERROR_CAPACITY_EXCEEDED;
     if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
     TraceParams P;
@@ -732,6 +835,7 @@ MGK_API e_magik_result_types mgk_trace_rays_final(mgk_context_handle c, const mg
     if (!c || !sc || !cam_pos || !d_hits || !d_final || (n > 0 && !d_rays)) return MAGIK_ERROR_INVALID_POINTER;
     e_magik_result_types r = check_integrator(in);
     if (r != MAGIK_SUCCESS) return r;
+    if ((r = check_scene(sc)) != MAGIK_SUCCESS) return r;
     if (!d_queue && n > stride) return MGK_ERROR_CAPACITY_EXCEEDED;
     if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
     TraceParams P;
@@ -743,6 +847,29 @@ MGK_API e_magik_result_types mgk_trace_rays_final(mgk_context_handle c, const mg
     P.hits = d_hits;
     P.final_state = d_final;
     return do_trace(c, P, hipStream_t(stream));
+}
+
+MGK_API e_magik_result_types mgk_trace_paths(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc,
+                                             const mgk_camera_t* cam, uint64_t seed, uint32_t sample_begin, uint32_t spp, void* d_rays,
+                                             uint64_t stride, mgk_hit_t* d_hits, uint32_t* d_steps, double* d_final, void* stream) {
+    if (!c || !sc || !cam || !d_hits || !d_rays) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = check_integrator(in);
+    if (r != MAGIK_SUCCESS) return r;
+    if ((r = check_scene(sc)) != MAGIK_SUCCESS) return r;
+    if (spp == 0) return MGK_ERROR_INVALID_ARGUMENT;
+    const uint64_t n = uint64_t(cam->x_resolution) * cam->y_resolution * spp;
+    if (n > stride) return MGK_ERROR_CAPACITY_EXCEEDED;
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    TraceParams P;
+    fill_trace_params(in, sc, cam->position, P);
+    P.rays = d_rays;
+    P.stride = stride;
+    P.n = n;
+    P.hits = d_hits;
+    P.steps_out = d_steps;
+    P.final_state = d_final;
+    const PathInfo paths{seed, cam->camera_id, cam->x_resolution * cam->y_resolution, sample_begin};
+    return do_trace(c, P, hipStream_t(stream), &paths, sc->max_reflections);
 }
 
 }  // extern "C"
@@ -841,6 +968,7 @@ MGK_API e_magik_result_types mgk_last_trace_stats(mgk_context_handle c, mgk_trac
     out->n_trials = h[2];
     out->n_trials_fp64 = h[3];
     out->ms = double(ms);
+    out->n_reflections = c->last_reflections;
     return MAGIK_SUCCESS;
 }
 
@@ -863,6 +991,7 @@ MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle c, const mgk_in
     if (!c || !sc || !sp || !cam || !d_xyz) return MAGIK_ERROR_INVALID_POINTER;
     e_magik_result_types r = check_integrator(in);
     if (r != MAGIK_SUCCESS) return r;
+    if ((r = check_scene(sc)) != MAGIK_SUCCESS) return r;
     if (spp == 0) return MGK_ERROR_INVALID_ARGUMENT;
     if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
     const uint64_t n = uint64_t(cam->x_resolution) * cam->y_resolution * spp;
@@ -879,7 +1008,8 @@ MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle c, const mgk_in
     P.stride = n;
     P.n = n;
     P.hits = reinterpret_cast<mgk_hit_t*>(c->hits.ptr);
-    if ((r = do_trace(c, P, s)) != MAGIK_SUCCESS) return r;
+    const PathInfo paths{seed, cam->camera_id, cam->x_resolution * cam->y_resolution, 0};
+    if ((r = do_trace(c, P, s, &paths, sc->max_reflections)) != MAGIK_SUCCESS) return r;
     ShadeParams S;
     if ((r = fill_shade_params(c, sc, sp, cam, seed, 0, spp, reinterpret_cast<mgk_hit_t*>(c->hits.ptr), d_xyz, S)) != MAGIK_SUCCESS)
         return r;

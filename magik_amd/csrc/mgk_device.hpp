@@ -434,6 +434,14 @@ __device__ __forceinline__ float planck(float lambda_nm, float T) {
     return (t1 / t2) * (1.0f / t3);
 }
 
+// White LED spectrum: five Gaussians exp(-(x - mu)^2 / (2 w^2)) k
+// (internal_math.h:2441-2445, :2501-2509).
+__device__ __forceinline__ float white_led(float x) {
+    auto gd = [](float x, float mu, float w, float k) { return expf(-0.5f * (((x - mu) * (x - mu)) / (w * w))) * k; };
+    return gd(x, 577.0f, 50.7937003968f, 157.0f) + gd(x, 451.0f, 7.07106781187f, 200.0f) + gd(x, 455.0f, 14.1421356237f, 50.0f) +
+           gd(x, 528.0f, 13.7840487521f, 20.0f) + gd(x, 690.0f, 44.72135955f, 10.0f);
+}
+
 }  // namespace mgk
 
 namespace mgk {

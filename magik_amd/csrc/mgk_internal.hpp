@@ -46,7 +46,9 @@ struct TraceParams {
     const uint32_t* queue;
     mgk_hit_t* hits;
     uint32_t* steps_out;
-    double* final_state;         // optional: the state (q, p) at termination, 8 x stride doubles
+    const uint32_t* steps_in;    // optional: step counts a ray starts from (paths continued after a reflection)
+    double* final_state;         // optional: the state (q, p) at termination, 8 x stride doubles; for a
+                                 // box hit, the chord point of the hit and the step's end momentum
     unsigned long long* head;    // work counter (zeroed by the launcher)
     unsigned long long* stats;   // [0] steps [1] trials [2] fp64 trials
     float spin, mass_sign, length_scale;
@@ -65,6 +67,27 @@ struct TraceParams {
     int per_ray;                 // spin in ray component 10; escape sphere about each ray's origin
     float horizon_scalar;
     int disk_isco;               // per-ray rays: the disk starts at the ISCO of the ray's spin
+    uint32_t n_box;              // axis-aligned boxes (SI metres) and their matter
+    float box_min[MGK_MAX_BOX][3], box_max[MGK_MAX_BOX][3];
+    uint32_t box_matter[MGK_MAX_BOX];
+};
+
+// Reflection of the paths that ended on diffuse box matter: new initial
+// conditions in place in the ray buffer, and the reflected ray ids appended
+// to a queue for the next trace.
+struct ReflectParams {
+    int metric;
+    const mgk_hit_t* hits;
+    const double* state;             // the trace's final_state (hit point, momentum), 8 x stride
+    void* rays;                      // 10 x stride, float (FP32 reference path) or double
+    uint64_t stride;
+    const uint32_t* queue_in;        // the rays just traced (NULL: 0 .. n_in-1)
+    uint64_t n_in;
+    uint32_t* queue_out;
+    unsigned long long* count;
+    uint64_t seed;
+    uint32_t camera_id, npix, sample_begin, block;   // Philox stream of reflection k: block 1 + k
+    float spin, mass_sign;
 };
 
 // Doubles parked per handed-off ray: q (4), p (4), h, u.p, L0, Q0, and the
@@ -113,6 +136,7 @@ struct ShadeParams {
     uint32_t n_bin;
     float bin_min, bin_width, sigma;
     float irr_scale;             // radiance_to_irradiance factor per unit L * throughput
+    float led_scale;             // illuminant boxes: led_scale x white LED spectrum
     float weights[3 * kMaxBins];
     const float4* resp;          // spectral response R(lambda), kRespN samples over [hero_min, hero_min + hero_span]
 };
@@ -152,6 +176,7 @@ hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t
 hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s);
 hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s);
+hipError_t launch_reflect(const ReflectParams& p, bool fp32, hipStream_t s);
 hipError_t launch_geodesic_steps(const GeodesicStepParams& p, hipStream_t s);
 
 }  // namespace mgk

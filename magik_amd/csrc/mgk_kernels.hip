@@ -386,11 +386,23 @@ template <int METRIC, class S> __device__ __forceinline__ void lane_constants(co
 // integration loop ~45 VGPRs.
 template <int METRIC, class S>
 __device__ __forceinline__ void hit_record(const St<S>& y0, const St<S>& y1, uint32_t cls, S best, S r_hit, S contraction,
-                                        const TraceParams& P, S a, mgk_hit_t& rec) {
+                                        const TraceParams& P, S a, int box_n, mgk_hit_t& rec) {
 #pragma clang fp contract(off)
     rec.cls = cls;
     rec.g = 0.0f; rec.a0 = 0.0f; rec.a1 = 0.0f;
-    if (cls == MGK_HIT_DISK || cls == MGK_HIT_SPHERE) {
+    if (cls == MGK_HIT_BOX_DIFFUSE || cls == MGK_HIT_BOX_EMITTER) {
+        // a0: the face normal's code, a1: the chord parameter (kernel_intersect.cu:125-132)
+        rec.a0 = float(box_n);
+        rec.a1 = float(best);
+        if (cls == MGK_HIT_BOX_EMITTER) {
+            // matter at rest: the reference observer at the hit point, the momentum of the step's end
+            const S hx = y0.qx + best * (y1.qx - y0.qx), hy = y0.qy + best * (y1.qy - y0.qy), hz = y0.qz + best * (y1.qz - y0.qz);
+            const V4<S> u = reference_observer<METRIC, S>(hx, hy, hz, a, S(P.mass_sign));
+            S g = contraction / dot4(u, V4<S>{y1.px, y1.py, y1.pz, y1.pt});
+            if (isnan(g) || isinf(g)) g = S(1e-10f);
+            rec.g = float(g);
+        }
+    } else if (cls == MGK_HIT_DISK || cls == MGK_HIT_SPHERE) {
         V4<S> ph{y0.px + best * (y1.px - y0.px), y0.py + best * (y1.py - y0.py), y0.pz + best * (y1.pz - y0.pz),
                  y0.pt + best * (y1.pt - y0.pt)};
         S hx = y0.qx + best * (y1.qx - y0.qx), hy = y0.qy + best * (y1.qy - y0.qy), hz = y0.qz + best * (y1.qz - y0.qz);
@@ -436,14 +448,50 @@ __device__ __forceinline__ void hit_record(const St<S>& y0, const St<S>& y1, uin
 // and horizon threshold tests run in FP32; only rays within ~1e-7 of a
 // threshold can see a different step, and the hit itself is still recorded
 // from the FP64 state.
+// Slab test of a chord against an axis-aligned box (traceables.cuh:17-74): o
+// the chord's start, d its unit direction, inv_len 1 / its length (SI
+// metres).  The hit parameter is in chord units; the normal, along the face's
+// axis and against the chord, comes back as a code (axis, + 3 if negative).
+template <class T>
+__device__ __forceinline__ bool box_chord(T ox, T oy, T oz, T dx, T dy, T dz, T inv_len, const float* bmin, const float* bmax, T& t_out,
+                                          int& code) {
+#pragma clang fp contract(off)
+    const T ix = T(1) / dx, iy = T(1) / dy, iz = T(1) / dz;
+    const bool sx = ix < T(0), sy = iy < T(0), sz = iz < T(0);
+    T tmin = (T(sx ? bmax[0] : bmin[0]) - ox) * ix, tmax = (T(sx ? bmin[0] : bmax[0]) - ox) * ix;
+    const T tymin = (T(sy ? bmax[1] : bmin[1]) - oy) * iy, tymax = (T(sy ? bmin[1] : bmax[1]) - oy) * iy;
+    if ((tmin > tymax) || (tymin > tmax)) return false;
+    int amin = 0, amax = 0;
+    if (tymin > tmin) { tmin = tymin; amin = 1; }
+    if (tymax < tmax) { tmax = tymax; amax = 1; }
+    const T tzmin = (T(sz ? bmax[2] : bmin[2]) - oz) * iz, tzmax = (T(sz ? bmin[2] : bmax[2]) - oz) * iz;
+    if ((tmin > tzmax) || (tzmin > tmax)) return false;
+    if (tzmin > tmin) { tmin = tzmin; amin = 2; }
+    if (tzmax < tmax) { tmax = tzmax; amax = 2; }
+    if (tmax < T(0)) return false;
+    const bool inside = tmin < T(0);
+    const T t = (inside ? tmax : tmin) * inv_len;
+    if (t > T(1)) return false;
+    const int axis = inside ? amax : amin;
+    const bool neg_axis = axis == 0 ? sx : (axis == 1 ? sy : sz);
+    T ns = neg_axis ? T(1) : T(-1);
+    if (inside) ns = -ns;
+    const T dcomp = axis == 0 ? dx : (axis == 1 ? dy : dz);
+    if (ns * dcomp > T(0)) ns = -ns;
+    t_out = t;
+    code = axis + (ns < T(0) ? 3 : 0);
+    return true;
+}
+
 template <int METRIC, class S, bool FAST_TESTS = false>
 __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, uint32_t n_step, S contraction, const TraceParams& P,
-                                             const Lane<S>& ln, mgk_hit_t& rec) {
+                                             const Lane<S>& ln, mgk_hit_t& rec, S& t_hit) {
 #pragma clang fp contract(off)
     const S a = METRIC == KERR ? ln.a : S(0);
     S best = S(1e30f);
     uint32_t cls = MGK_HIT_NONE;
     S r_hit = S(0);
+    int box_n = 0;
     if (P.flags & MGK_SCENE_DISK) {
         S ya = y0.qy, yb = y1.qy;
         if ((ya > S(0) && yb <= S(0)) || (ya < S(0) && yb >= S(0))) {
@@ -467,6 +515,25 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
             S t = (-B - sd) / A;
             if (t < S(0)) t = (-B + sd) / A;
             if (t >= S(0) && t <= S(1) && t < best) { best = t; cls = MGK_HIT_SPHERE; }
+        }
+    }
+    if (P.flags & MGK_SCENE_BOXES) {
+        // the chord in SI metres (kernel_intersect.cu:63-70); FP32 in the zone kernels
+        using T = typename std::conditional<FAST_TESTS, float, S>::type;
+        const T L = T(P.length_scale);
+        const T ox = T(y0.qx) * L, oy = T(y0.qy) * L, oz = T(y0.qz) * L;
+        T dx = T(y1.qx) * L - ox, dy = T(y1.qy) * L - oy, dz = T(y1.qz) * L - oz;
+        const T len = sqrt_<false>(dx * dx + dy * dy + dz * dz);
+        dx = dx / len; dy = dy / len; dz = dz / len;
+        const T inv_len = T(1) / len;
+        for (uint32_t b = 0; b < P.n_box; ++b) {
+            T t;
+            int code;
+            if (box_chord<T>(ox, oy, oz, dx, dy, dz, inv_len, P.box_min[b], P.box_max[b], t, code) && S(t) < best) {
+                best = S(t);
+                cls = P.box_matter[b] == MGK_MATTER_DIFFUSE ? MGK_HIT_BOX_DIFFUSE : MGK_HIT_BOX_EMITTER;
+                box_n = code;
+            }
         }
     }
     if constexpr (FAST_TESTS) {
@@ -502,7 +569,8 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
     }
     if (cls == MGK_HIT_NONE && n_step > P.step_limit) cls = MGK_HIT_STEP_LIMIT;
     if (cls == MGK_HIT_NONE) return cls;
-    hit_record<METRIC, S>(y0, y1, cls, best, r_hit, contraction, P, a, rec);
+    hit_record<METRIC, S>(y0, y1, cls, best, r_hit, contraction, P, a, box_n, rec);
+    t_hit = best;
     return cls;
 }
 
@@ -512,6 +580,24 @@ template <class S> __device__ __forceinline__ void write_final(const TraceParams
     f[0 * sd + ray] = double(y.qx); f[1 * sd + ray] = double(y.qy); f[2 * sd + ray] = double(y.qz); f[3 * sd + ray] = double(y.qt);
     f[4 * sd + ray] = double(y.px); f[5 * sd + ray] = double(y.py); f[6 * sd + ray] = double(y.pz); f[7 * sd + ray] = double(y.pt);
 }
+
+// Terminal state of a path: the step's end, or for box matter the chord point
+// of the hit with the step's end momentum (kernel_intersect.cu:129-131).
+template <class S>
+__device__ __forceinline__ void write_terminal(const TraceParams& P, uint64_t ray, uint32_t cls, const St<S>& y0, const St<S>& y1, S t) {
+#pragma clang fp contract(off)
+    if (cls != MGK_HIT_BOX_DIFFUSE && cls != MGK_HIT_BOX_EMITTER) {
+        write_final(P, ray, y1);
+        return;
+    }
+    St<S> h = y1;
+    h.qx = y0.qx + t * (y1.qx - y0.qx);
+    h.qy = y0.qy + t * (y1.qy - y0.qy);
+    h.qz = y0.qz + t * (y1.qz - y0.qz);
+    h.qt = y0.qt + t * (y1.qt - y0.qt);
+    write_final(P, ray, h);
+}
+
 
 template <class S> __device__ __forceinline__ S load_c(const void* rays, uint64_t stride, int c, uint64_t i) {
     return reinterpret_cast<const S*>(rays)[uint64_t(c) * stride + i];
@@ -577,7 +663,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
                         ln.cx = y.qx * L; ln.cy = y.qy * L; ln.cz = y.qz * L;   // escape sphere about the ray's origin
                     }
                     active = true;
-                    n_step = 0;
+                    n_step = P.steps_in ? P.steps_in[ray] : 0u;
                     n_sub = 0;
                 }
             }
@@ -635,11 +721,12 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
         ++steps_acc;
         if constexpr (PREC == MGK_PRECISION_MIXED) renormalise_null<METRIC>(y1, double(ln.a), double(P.mass_sign));
         mgk_hit_t rec;
-        uint32_t cls = classify<METRIC, S>(y, y1, n_step, contraction, P, ln, rec);
+        S t_hit = S(0);
+        uint32_t cls = classify<METRIC, S>(y, y1, n_step, contraction, P, ln, rec, t_hit);
         if (cls != MGK_HIT_NONE) {
             P.hits[ray] = rec;
             if (P.steps_out) P.steps_out[ray] = n_step;
-            if (P.final_state) write_final(P, ray, y1);
+            if (P.final_state) write_terminal(P, ray, cls, y, y1, t_hit);
             active = false;
         } else {
             y = y1;
@@ -759,7 +846,7 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
                     y.qx = src[0 * sd + at]; y.qy = src[1 * sd + at]; y.qz = src[2 * sd + at];
                     y.qt = src[3 * sd + at]; y.px = src[4 * sd + at]; y.py = src[5 * sd + at];
                     y.pz = src[6 * sd + at]; y.pt = src[7 * sd + at]; h = src[8 * sd + at];
-                    n_step = Z.from_ic ? 0u : Z.steps_in[slot];
+                    n_step = Z.from_ic ? (P.steps_in ? P.steps_in[ray] : 0u) : Z.steps_in[slot];
                     contraction = src[9 * sd + at];
                     if (P.per_ray) {
                         lane_constants<METRIC, S>(P, METRIC == KERR ? load_c<S>(P.rays, P.stride, 10, ray) : S(0), ln);
@@ -836,11 +923,12 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
         if (isnan(Q0)) renormalise_null<METRIC>(y1, double(ln.a), double(P.mass_sign));
         else renormalise_constants<METRIC>(y1, double(ln.a), double(P.mass_sign), L0, Q0);
         mgk_hit_t rec;
-        const uint32_t cls = classify<METRIC, S, true>(y, y1, n_step, contraction, P, ln, rec);
+        S t_hit = S(0);
+        const uint32_t cls = classify<METRIC, S, true>(y, y1, n_step, contraction, P, ln, rec, t_hit);
         if (cls != MGK_HIT_NONE) {
             P.hits[ray] = rec;
             if (P.steps_out) P.steps_out[ray] = n_step;
-            if (P.final_state) write_final(P, ray, y1);
+            if (P.final_state) write_terminal(P, ray, cls, y, y1, t_hit);
             active = false;
         } else {
             y = y1;
@@ -1071,7 +1159,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(ShadeParams P) {
     float X = 0.0f, Yv = 0.0f, Zv = 0.0f;
     for (uint32_t s = 0; s < P.spp; ++s) {
         mgk_hit_t h = P.hits[uint64_t(s) * npix + pixel];
-        if (h.cls != MGK_HIT_DISK && h.cls != MGK_HIT_SPHERE && h.cls != MGK_HIT_ESCAPE) continue;
+        if (h.cls != MGK_HIT_DISK && h.cls != MGK_HIT_SPHERE && h.cls != MGK_HIT_ESCAPE && h.cls != MGK_HIT_BOX_EMITTER) continue;
         float u[4];
         path_uniforms(P.seed, P.camera_id, pixel, P.sample_begin + s, 0, u);
         const float g = h.g, g5 = g * g * g * g * g;
@@ -1100,6 +1188,8 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(ShadeParams P) {
                 L = 0.0f;
                 if (P.flags & MGK_SCENE_BACKGROUND) L += P.background_scale * planck(le, P.background_T);
                 if (star > 0.0f) L += star * planck(le, P.star_T);
+            } else if (h.cls == MGK_HIT_BOX_EMITTER) {
+                L = P.led_scale * white_led(le);
             } else {
                 L = emit_scale > 0.0f ? planck(le, T) : 0.0f;
             }
@@ -1120,6 +1210,148 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(ShadeParams P) {
     P.xyz[3 * uint64_t(pixel) + 0] = X * inv;
     P.xyz[3 * uint64_t(pixel) + 1] = Yv * inv;
     P.xyz[3 * uint64_t(pixel) + 2] = Zv * inv;
+}
+
+// ---------------------------------------------------------------------------
+// Reflection off diffuse box matter (kernel_bxdf.cu:69-95 with
+// bxdf_utilities.cuh:56-151, matter_examples.cuh:36-60, brdf.cuh:12-29).
+// ---------------------------------------------------------------------------
+// Self-intersection offset of one float coordinate along n (Waechter & Binder,
+// Ray Tracing Gems ch. 6, as bxdf_utilities.cuh:112-134 uses it): int(1024 n)
+// units in the last place, or 2^-14 n within 1/32 of the origin.
+__device__ __forceinline__ float offset_coord(float x, float n) {
+    const int of = int(1024.0f * n);
+    const float p = __int_as_float(__float_as_int(x) + ((x < 0.0f) ? -of : of));
+    return fabsf(x) < (1.0f / 32.0f) ? x + (4.0f / 65536.0f) * n : p;
+}
+
+// Spatial legs of the Gram-Schmidt tetrad of tetrad_apply (order z, x, y).
+template <class S>
+__device__ __forceinline__ void tetrad_legs(const Sym4<S>& g, V4<S> u, V4<S>& ex, V4<S>& ey, V4<S>& ez) {
+#pragma clang fp contract(off)
+    const V4<S> X{S(1), S(0), S(0), S(0)}, Y{S(0), S(1), S(0), S(0)}, Z{S(0), S(0), S(1), S(0)};
+    ez = axpy(inner(g, Z, u), u, Z);
+    ez = divs(ez, sqrt_<false>(inner(g, ez, ez)));
+    ex = axpy(-inner(g, X, ez), ez, axpy(inner(g, X, u), u, X));
+    ex = divs(ex, sqrt_<false>(inner(g, ex, ex)));
+    ey = axpy(-inner(g, Y, ex), ex, axpy(-inner(g, Y, ez), ez, axpy(inner(g, Y, u), u, Y)));
+    ey = divs(ey, sqrt_<false>(inner(g, ey, ey)));
+}
+
+template <class S> __device__ __forceinline__ void norm3(S& x, S& y, S& z) {
+#pragma clang fp contract(off)
+    const S l = sqrt_<false>(x * x + y * y + z * z);
+    x = x / l; y = y / l; z = z / l;
+}
+
+// One thread per traced ray; rays that ended on diffuse matter get new
+// initial conditions (in place, components 0-7 and 9; the step size of
+// component 8 is kept) and are appended to the out-queue, one atomic per wave.
+// The matter is at rest: its four-velocity is the reference observer's.  The
+// incoming direction and the face normal are taken into that observer's
+// frame; faces with an x normal reflect specularly, the others send
+// (p + n) / |p + n| for p uniform on the sphere, cosine-weighted about the
+// facing normal n.  The new u.p carries the redshift of the path so far.
+template <int METRIC, class S> __global__ void __launch_bounds__(256) k_reflect(ReflectParams P) {
+#pragma clang fp contract(off)
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    uint64_t ray = 0;
+    bool go = false;
+    if (i < P.n_in) {
+        ray = P.queue_in ? uint64_t(P.queue_in[i]) : i;
+        go = P.hits[ray].cls == MGK_HIT_BOX_DIFFUSE;
+    }
+    if (go) {
+        const uint64_t sd = P.stride;
+        const S qx = S(P.state[0 * sd + ray]), qy = S(P.state[1 * sd + ray]), qz = S(P.state[2 * sd + ray]);
+        const S qt = S(P.state[3 * sd + ray]);
+        const V4<S> pin{S(P.state[4 * sd + ray]), S(P.state[5 * sd + ray]), S(P.state[6 * sd + ray]), S(P.state[7 * sd + ray])};
+        S* rays = reinterpret_cast<S*>(P.rays);
+        const S contraction = rays[9 * sd + ray];
+        const S a = S(P.spin), ms = S(P.mass_sign);
+        const Sym4<S> g = metric_g<METRIC, S>(qx, qy, qz, a, ms, false);
+        const Sym4<S> gi = metric_g<METRIC, S>(qx, qy, qz, a, ms, true);
+        const V4<S> u = reference_observer<METRIC, S>(qx, qy, qz, a, ms);
+        V4<S> ex, ey, ez;
+        tetrad_legs(g, u, ex, ey, ez);
+        // incoming direction in the matter's frame (bxdf_utilities.cuh:92-96)
+        const V4<S> pup = lower(gi, pin);
+        S dx = inner(g, ex, pup), dy = inner(g, ey, pup), dz = inner(g, ez, pup);
+        norm3(dx, dy, dz);
+        S z = contraction / dot4(u, pin);
+        if (isnan(z) || isinf(z)) z = S(1e-10f);
+        // geometric normal into the same frame, turned against the incoming direction (:101-104)
+        const int code = int(P.hits[ray].a0);
+        const int axis = code % 3;
+        const S ns = code >= 3 ? S(-1) : S(1);
+        const V4<S> ng{axis == 0 ? ns : S(0), axis == 1 ? ns : S(0), axis == 2 ? ns : S(0), S(0)};
+        const V4<S> nup = lower(gi, ng);
+        S nx = inner(g, ex, nup), ny = inner(g, ey, nup), nz = inner(g, ez, nup);
+        norm3(nx, ny, nz);
+        if (nx * dx + ny * dy + nz * dz > S(0)) { nx = -nx; ny = -ny; nz = -nz; }
+        S ox, oy, oz;
+        if (axis == 0) {
+            const S k = S(2) * (dx * nx + dy * ny + dz * nz);
+            ox = dx - nx * k; oy = dy - ny * k; oz = dz - nz * k;
+        } else {
+            float uu[4];
+            const uint32_t pixel = uint32_t(ray % P.npix), sample = P.sample_begin + uint32_t(ray / P.npix);
+            path_uniforms(P.seed, P.camera_id, pixel, sample, P.block, uu);
+            const S th = acos(S(1) - S(2) * S(uu[0])), ph = S(2) * S(kPi) * S(uu[1]);
+            const S st = sin(th);
+            ox = cos(ph) * st + nx; oy = sin(ph) * st + ny; oz = cos(th) + nz;
+        }
+        norm3(ox, oy, oz);
+        // outgoing momentum from the frame (relativity_utilities.cuh:186-195)
+        const S lt = sqrt_<false>((ox * ox + oy * oy + oz * oz) - S(0));
+        V4<S> vup = scale(u, lt);
+        vup = axpy(ox, ex, vup);
+        vup = axpy(oy, ey, vup);
+        vup = axpy(oz, ez, vup);
+        const V4<S> pout = lower(g, vup);
+        // offset along the global facing normal (the box normal already opposes the chord)
+        const float fx = float(qx), fy = float(qy), fz = float(qz);
+        rays[0 * sd + ray] = qx + S(offset_coord(fx, float(ng.x)) - fx);
+        rays[1 * sd + ray] = qy + S(offset_coord(fy, float(ng.y)) - fy);
+        rays[2 * sd + ray] = qz + S(offset_coord(fz, float(ng.z)) - fz);
+        rays[3 * sd + ray] = qt;
+        rays[4 * sd + ray] = pout.x; rays[5 * sd + ray] = pout.y; rays[6 * sd + ray] = pout.z; rays[7 * sd + ray] = pout.t;
+        rays[9 * sd + ray] = dot4(u, pout) * z;
+    }
+    // append the reflected rays: one atomic per wave
+    const uint64_t m = __ballot(go);
+    if (m == 0) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned long long base = 0;
+    if (int(lane) == leader) base = atomicAdd(P.count, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    if (go) {
+        const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        P.queue_out[base + uint64_t(__popcll(m & lt_mask))] = uint32_t(ray);
+    }
+}
+
+hipError_t launch_reflect(const ReflectParams& p, bool fp32, hipStream_t s) {
+    if (p.n_in == 0) return hipSuccess;
+    const dim3 g{unsigned((p.n_in + 255) / 256)}, b{256u};
+    if (g.x == 0 || (p.n_in + 255) / 256 > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    switch (p.metric) {
+        case KERR:
+            if (fp32) hipLaunchKernelGGL((k_reflect<KERR, float>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_reflect<KERR, double>), g, b, 0, s, p);
+            break;
+        case SCHWARZSCHILD:
+            if (fp32) hipLaunchKernelGGL((k_reflect<SCHWARZSCHILD, float>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_reflect<SCHWARZSCHILD, double>), g, b, 0, s, p);
+            break;
+        case MINKOWSKI:
+            if (fp32) hipLaunchKernelGGL((k_reflect<MINKOWSKI, float>), g, b, 0, s, p);
+            else hipLaunchKernelGGL((k_reflect<MINKOWSKI, double>), g, b, 0, s, p);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------

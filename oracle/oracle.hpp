@@ -531,7 +531,11 @@ inline RayIC<R> initial_conditions(const Camera<R>& cam, const MetricParams<R>& 
 // traceables.cuh:17-86; the thin disk, emissive sphere and Keplerian emitter
 // are this build's additions for the benchmark scenes.
 // ---------------------------------------------------------------------------
-enum HitClass : uint32_t { HIT_NONE = 0, HIT_HORIZON = 1, HIT_ESCAPE = 2, HIT_DISK = 3, HIT_SPHERE = 4, HIT_STEP_LIMIT = 5 };
+enum HitClass : uint32_t {
+    HIT_NONE = 0, HIT_HORIZON = 1, HIT_ESCAPE = 2, HIT_DISK = 3, HIT_SPHERE = 4, HIT_STEP_LIMIT = 5,
+    HIT_BOX_DIFFUSE = 6, HIT_BOX_EMITTER = 7
+};
+enum Matter : uint32_t { MATTER_DIFFUSE = 1, MATTER_ILLUMINANT = 2 };   // kernel_bxdf.cu:126 slots 1 and 2
 
 template <class R> struct Scene {
     R celestial_radius_m = R(1000);   // measured from the camera origin, SI metres
@@ -543,36 +547,130 @@ template <class R> struct Scene {
     uint32_t sphere_checker = 16;
     R background_scale = R(0), background_T = R(6500);
     uint32_t step_limit = 512;
+    // axis-aligned boxes in SI metres (stage.h:19-22), tested in order
+    struct Box { V3<R> mn, mx; uint32_t matter; };
+    uint32_t n_box = 0;
+    Box box[4];
+    uint32_t max_reflections = 10;   // render_settings.h:17
+    R led_scale = R(7e5f);           // matter_examples.cuh:70
 };
 
 template <class R> struct Hit {
     uint32_t cls = HIT_NONE;
     R g = R(0), a0 = R(0), a1 = R(0);
     uint32_t steps = 0, trials = 0;
+    State<R> at{};   // box hits: the chord point of the hit and the step's end momentum
 };
 
 // Slab test of the chord against an AABB, hit parameter in chord units
-// (traceables.cuh:17-74).  Only the parameter is needed on this path.
+// (traceables.cuh:17-74).  The normal (an axis, against the chord direction)
+// comes back as a code: axis 0-2, plus 3 when it points along the negative axis.
 template <class R>
-inline bool aabb_hit(R* t_out, V3<R> dir, V3<R> pos, R inv_chord, V3<R> bmin, V3<R> bmax) {
+inline bool aabb_hit(R* t_out, V3<R> dir, V3<R> pos, R inv_chord, V3<R> bmin, V3<R> bmax, int* normal_code = nullptr) {
     V3<R> b[2] = {bmin, bmax};
     V3<R> inv{R(1) / dir.x, R(1) / dir.y, R(1) / dir.z};
     int sx = inv.x < 0, sy = inv.y < 0, sz = inv.z < 0;
+    int axis_min = 0, axis_max = 0;
     R tmin = (b[sx].x - pos.x) * inv.x, tmax = (b[1 - sx].x - pos.x) * inv.x;
     R tymin = (b[sy].y - pos.y) * inv.y, tymax = (b[1 - sy].y - pos.y) * inv.y;
     if ((tmin > tymax) || (tymin > tmax)) return false;
-    if (tymin > tmin) tmin = tymin;
-    if (tymax < tmax) tmax = tymax;
+    if (tymin > tmin) { tmin = tymin; axis_min = 1; }
+    if (tymax < tmax) { tmax = tymax; axis_max = 1; }
     R tzmin = (b[sz].z - pos.z) * inv.z, tzmax = (b[1 - sz].z - pos.z) * inv.z;
     if ((tmin > tzmax) || (tzmin > tmax)) return false;
-    if (tzmin > tmin) tmin = tzmin;
-    if (tzmax < tmax) tmax = tzmax;
+    if (tzmin > tmin) { tmin = tzmin; axis_min = 2; }
+    if (tzmax < tmax) { tmax = tzmax; axis_max = 2; }
     if (tmax < R(0)) return false;
     bool inside = tmin < R(0);
     R t = (inside ? tmax : tmin) * inv_chord;
     if (t > R(1)) return false;
     *t_out = t;
+    if (normal_code) {
+        const int axis = inside ? axis_max : axis_min;
+        const int sgn[3] = {sx, sy, sz};
+        R ns = sgn[axis] ? R(1) : R(-1);
+        if (inside) ns = -ns;
+        const R d = axis == 0 ? dir.x : (axis == 1 ? dir.y : dir.z);
+        if (ns * d > R(0)) ns = -ns;   // against the chord direction
+        *normal_code = axis + (ns < R(0) ? 3 : 0);
+    }
     return true;
+}
+
+template <class R> inline V3<R> normal_of(int code) {
+    const R s = code >= 3 ? R(-1) : R(1);
+    const int axis = code % 3;
+    return {axis == 0 ? s : R(0), axis == 1 ? s : R(0), axis == 2 ? s : R(0)};
+}
+
+// White LED spectrum: five Gaussians exp(-(x - mu)^2 / (2 w^2)) * scale
+// (internal_math.h:2441-2445, :2501-2509).
+template <class R> inline R white_led(R lambda_nm) {
+    auto gd = [](R x, R mu, R w, R k) { return std::exp(R(-0.5) * (((x - mu) * (x - mu)) / (w * w))) * k; };
+    R f0 = gd(lambda_nm, R(577.0), R(50.7937003968), R(157.0));
+    R f1 = gd(lambda_nm, R(451.0), R(7.07106781187), R(200.0));
+    R f2 = gd(lambda_nm, R(455.0), R(14.1421356237), R(50.0));
+    R f3 = gd(lambda_nm, R(528.0), R(13.7840487521), R(20.0));
+    R f4 = gd(lambda_nm, R(690.0), R(44.72135955), R(10.0));
+    return f0 + f1 + f2 + f3 + f4;
+}
+
+// Self-intersection offset along the normal n (Wächter & Binder, "A Fast and
+// Robust Method for Avoiding Self-Intersection", Ray Tracing Gems ch. 6, as
+// used by bxdf_utilities.cuh:112-134): a move of int(1024 n) units in the last
+// place of each float coordinate, or 2^-14 n near the origin.  Evaluated on the
+// float coordinates; in double the same float displacement is added.
+inline float offset_coord(float x, float n) {
+    const float int_scale = 256.0f * 4.0f, float_scale = (1.0f / 65536.0f) * 4.0f, origin = 1.0f / 32.0f;
+    const int of = int(int_scale * n);
+    int32_t xi;
+    std::memcpy(&xi, &x, 4);
+    xi += (x < 0) ? -of : of;
+    float p;
+    std::memcpy(&p, &xi, 4);
+    return std::fabs(x) < origin ? x + float_scale * n : p;
+}
+template <class R> inline V4<R> offset_geodesic(V4<R> q, V3<R> n) {
+    const float fx = float(q.x), fy = float(q.y), fz = float(q.z);
+    return {q.x + R(offset_coord(fx, float(n.x)) - fx), q.y + R(offset_coord(fy, float(n.y)) - fy),
+            q.z + R(offset_coord(fz, float(n.z)) - fz), q.t};
+}
+
+// Reflection off diffuse box matter at rest (kernel_bxdf.cu:69-95 with
+// bxdf_utilities.cuh:56-110, :136-151 and matter_examples.cuh:36-60,
+// brdf.cuh:12-29).  `at` holds the hit point and the step's end momentum
+// (covariant).  The rest-frame incoming direction and geometric normal come
+// from the reference observer's tetrad; faces with an x normal reflect
+// specularly, the others send (p + n) / |p + n| for p uniform on the sphere
+// (cosine-weighted about the facing normal n).  Returns the new state (offset
+// position, outgoing covariant momentum) and the new u.p contraction, which
+// carries the redshift accumulated so far.
+template <class R>
+inline void reflect_diffuse(const MetricParams<R>& p, const State<R>& at, R contraction, int normal_code, R u0, R u1, State<R>* out,
+                            R* contraction_out) {
+    const Seeds<R> seeds;
+    const V4<R> q = at.q;
+    const V4<R> u = rest_frame_four_velocity(q, seeds, V3<R>{R(0), R(0), R(0)}, p);
+    const Sym4<R> ginv = compute_g(q, p, true);
+    const V4<R> prf = global_to_observer_rest_frame(q, seeds, contract(ginv, at.p), u, p);
+    const V3<R> din = norm3(V3<R>{prf.x, prf.y, prf.z});
+    const R z = redshift_factor(contraction, u, at.p);
+    const V3<R> ng = normal_of<R>(normal_code);
+    const V4<R> nrf = global_to_observer_rest_frame(q, seeds, contract(ginv, V4<R>{ng.x, ng.y, ng.z, R(0)}), u, p);
+    V3<R> nr = norm3(V3<R>{nrf.x, nrf.y, nrf.z});
+    if (dot3(nr, din) > R(0)) nr = -nr;
+    V3<R> dout;
+    if (ng.x * ng.x > R(0)) {
+        dout = norm3(din - (R(2) * nr * dot3(din, nr)));
+    } else {
+        const R th = std::acos(R(1) - R(2) * u0), ph = R(2) * R(kPi) * u1;
+        const V3<R> s{std::cos(ph) * std::sin(th), std::sin(ph) * std::sin(th), std::cos(th)};
+        dout = norm3(s + nr);
+    }
+    const V4<R> pout = direction_to_momentum(q, seeds, dout, u, R(0), p);
+    out->q = offset_geodesic(q, ng);
+    out->p = pout;
+    *contraction_out = dot4(u, pout) * z;
 }
 
 // Thin equatorial disk y = 0, r_in <= r <= r_out: chord crossing of the plane.
@@ -639,12 +737,12 @@ template <class R> inline R planck(R lambda_nm, R T) {
 // (closest of disk / sphere / celestial sphere at t = 1; else horizon),
 // then the step limit of kernel_enforce_ray_limits.cu:37-64 (n_step > limit).
 template <class R>
-inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scene<R>& sc, V3<R> cam_origin) {
+inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scene<R>& sc, V3<R> cam_origin, uint32_t n_step0 = 0) {
     Hit<R> hit;
     State<R> y0 = ic.y;
     R h = ic.h, h_taken = 0;
     R L = length_scale(p);
-    uint32_t n_step = 0, n_trial = 0;
+    uint32_t n_step = n_step0, n_trial = 0;
     for (;;) {
         uint32_t tr = 0;
         State<R> y1 = solve_step(y0, p, &h, &h_taken, &tr);
@@ -658,6 +756,22 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
         }
         if (sc.sphere && sphere_hit(&t, y0.q, y1.q, sc.sphere_center, sc.sphere_radius)) {
             if (t < best) { best = t; cls = HIT_SPHERE; }
+        }
+        int box_normal = 0;
+        if (sc.n_box > 0) {
+            // kernel_intersect.cu:63-105: the chord in SI metres
+            const V3<R> o = V3<R>{y0.q.x, y0.q.y, y0.q.z} * L, e = V3<R>{y1.q.x, y1.q.y, y1.q.z} * L;
+            const V3<R> d = e - o;
+            const R len = sqrt_(len2_3(d));
+            const V3<R> dir = d / len;
+            for (uint32_t b = 0; b < sc.n_box; ++b) {
+                int nc = 0;
+                if (aabb_hit(&t, dir, o, R(1) / len, sc.box[b].mn, sc.box[b].mx, &nc) && t < best) {
+                    best = t;
+                    cls = sc.box[b].matter == MATTER_DIFFUSE ? HIT_BOX_DIFFUSE : HIT_BOX_EMITTER;
+                    box_normal = nc;
+                }
+            }
         }
         {
             V3<R> x1 = V3<R>{y1.q.x, y1.q.y, y1.q.z} * L - cam_origin;
@@ -692,6 +806,13 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
                     hit.a0 = n.y;
                     hit.a1 = std::atan2(n.z, n.x);
                 }
+            } else if (cls == HIT_BOX_DIFFUSE || cls == HIT_BOX_EMITTER) {
+                // the hit point on the chord, the momentum of the step's end (kernel_intersect.cu:129-131)
+                hit.at.q = y0.q + best * (y1.q - y0.q);
+                hit.at.p = y1.p;
+                hit.a0 = R(box_normal);
+                hit.a1 = best;
+                if (cls == HIT_BOX_EMITTER) hit.g = redshift_factor(ic.contraction, reference_observer(hit.at.q, p), y1.p);
             } else if (cls == HIT_ESCAPE) {
                 V3<R> d = norm3(V3<R>{y1.q.x - y0.q.x, y1.q.y - y0.q.y, y1.q.z - y0.q.z});
                 hit.a0 = d.y;
@@ -703,6 +824,15 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
         y0 = y1;
     }
 }
+
+// A whole path: trace, and at diffuse box matter reflect and trace again, up to
+// max_reflections reflections (kernel_enforce_ray_limits.cu:48-50); the step
+// count runs on across reflections.  Reflection k (0-based) draws its two
+// uniforms from Philox block 1 + k of the path.  The step size restarts from
+// the initial conditions' h after each reflection.
+template <class R>
+inline Hit<R> trace_path(const RayIC<R>& ic, const MetricParams<R>& p, const Scene<R>& sc, V3<R> cam_origin, uint64_t seed,
+                         uint32_t cam_id, uint32_t pixel, uint32_t sample, uint32_t* n_reflect = nullptr);
 
 // ---------------------------------------------------------------------------
 // Spectral pipeline.
@@ -791,6 +921,27 @@ inline void path_uniforms(uint64_t seed, uint32_t camera, uint32_t pixel, uint32
     uint32_t o[4];
     philox4x32_10(ctr, key, o);
     for (int i = 0; i < 4; ++i) u[i] = u01(o[i]);
+}
+
+template <class R>
+inline Hit<R> trace_path(const RayIC<R>& ic, const MetricParams<R>& p, const Scene<R>& sc, V3<R> cam_origin, uint64_t seed,
+                         uint32_t cam_id, uint32_t pixel, uint32_t sample, uint32_t* n_reflect) {
+    RayIC<R> cur = ic;
+    uint32_t steps = 0, trials = 0;
+    for (uint32_t k = 0;; ++k) {
+        Hit<R> h = trace_ray(cur, p, sc, cam_origin, steps);
+        steps = h.steps;
+        trials += h.trials;
+        if (h.cls != HIT_BOX_DIFFUSE || k >= sc.max_reflections) {
+            h.trials = trials;
+            if (n_reflect) *n_reflect = k;
+            return h;
+        }
+        float u[4];
+        path_uniforms(seed, cam_id, pixel, sample, 1 + k, u);
+        reflect_diffuse(p, h.at, cur.contraction, int(h.a0), R(u[0]), R(u[1]), &cur.y, &cur.contraction);
+        cur.h = ic.h;
+    }
 }
 
 // Star catalogue: n stars, direction uniform on the sphere (cos(theta) about +y

@@ -47,6 +47,13 @@ _SIG = {
     "orc_rotation": (None, [C.c_float, C.c_float, C.c_float, _F]),
     "orc_sample_cmf": (None, [_F, C.c_uint32, C.c_float, _F]),
     "orc_layout": (None, [C.POINTER(C.c_size_t)]),
+    "orc_trace_paths": (C.c_int, [C.c_int, C.POINTER(abi.Integrator), C.POINTER(abi.Scene), C.POINTER(abi.Camera), C.c_uint64,
+                                  C.c_uint32, _D, C.c_uint64, C.c_void_p, _U, _U, _D, C.c_int]),
+    "orc_aabb_hit": (C.c_int, [C.c_int, _D, _D, C.c_double, _D, _D, _D, C.POINTER(C.c_int)]),
+    "orc_offset_geodesic": (None, [_D, C.c_int, _D]),
+    "orc_reflect_diffuse": (C.c_int, [C.c_int, C.POINTER(abi.Integrator), _D, C.c_double, C.c_int, C.c_double, C.c_double,
+                                      _D, _D]),
+    "orc_white_led": (C.c_double, [C.c_int, C.c_double]),
 }
 
 _REF_SIG = {
@@ -165,6 +172,53 @@ def trace_rays(prec: int, integ, scene, cam_pos, rays: np.ndarray, threads: int 
                                    hits.ctypes.data_as(C.c_void_p), uptr(steps), uptr(trials), threads),
               "orc_trace_rays")
     return hits, steps, trials
+
+
+def trace_paths(prec: int, integ, scene, cam, seed: int, sample_begin: int, rays: np.ndarray, threads: int = 0,
+                want_at: bool = False):
+    """Whole paths with reflections off diffuse boxes; returns (hits, steps,
+    reflections) and, with want_at, the (n, 8) hit point and momentum of the
+    paths that ended on a box."""
+    rays = np.ascontiguousarray(rays, dtype=np.float64)
+    n = rays.shape[0]
+    hits = np.zeros(n, dtype=HIT_DTYPE)
+    steps = np.zeros(n, dtype=np.uint32)
+    refl = np.zeros(n, dtype=np.uint32)
+    at = np.zeros((n, 8)) if want_at else None
+    abi.check(lib().orc_trace_paths(prec, C.byref(integ), C.byref(scene), C.byref(cam), seed, sample_begin, dptr(rays), n,
+                                    hits.ctypes.data_as(C.c_void_p), uptr(steps), uptr(refl),
+                                    dptr(at) if want_at else None, threads), "orc_trace_paths")
+    return (hits, steps, refl, at) if want_at else (hits, steps, refl)
+
+
+def aabb_hit(prec: int, origin, direction, inv_chord: float, bmin, bmax):
+    """(hit, t, normal code) of the slab test."""
+    o, d = np.ascontiguousarray(origin, dtype=np.float64), np.ascontiguousarray(direction, dtype=np.float64)
+    lo, hi = np.ascontiguousarray(bmin, dtype=np.float64), np.ascontiguousarray(bmax, dtype=np.float64)
+    t, nc = C.c_double(0), C.c_int(0)
+    ok = lib().orc_aabb_hit(prec, dptr(o), dptr(d), inv_chord, dptr(lo), dptr(hi), C.byref(t), C.byref(nc))
+    return bool(ok), t.value, nc.value
+
+
+def offset_geodesic(q, normal_code: int) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    out = np.zeros(4)
+    lib().orc_offset_geodesic(dptr(q), normal_code, dptr(out))
+    return out
+
+
+def reflect_diffuse(prec: int, integ, at, contraction: float, normal_code: int, u0: float, u1: float):
+    """(new state (8), new contraction) of one reflection off diffuse matter."""
+    at = np.ascontiguousarray(at, dtype=np.float64)
+    out = np.zeros(8)
+    c = C.c_double(0)
+    abi.check(lib().orc_reflect_diffuse(prec, C.byref(integ), dptr(at), contraction, normal_code, u0, u1, dptr(out),
+                                        C.byref(c)), "orc_reflect_diffuse")
+    return out, c.value
+
+
+def white_led(prec: int, lam_nm: float) -> float:
+    return lib().orc_white_led(prec, lam_nm)
 
 
 def shade(scene, spec, cam, seed: int, sample_begin: int, spp: int, pixel_ids, hits: np.ndarray,

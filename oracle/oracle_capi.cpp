@@ -36,6 +36,16 @@ template <class R> Scene<R> scene_from(const mgk_scene_t* s, uint32_t step_limit
     sc.background_scale = R(s->background_scale);
     sc.background_T = R(s->background_T);
     sc.step_limit = step_limit;
+    if (s->flags & MGK_SCENE_BOXES) {
+        sc.n_box = s->n_box < 4 ? s->n_box : 4;
+        for (uint32_t b = 0; b < sc.n_box; ++b) {
+            sc.box[b].mn = {R(s->box[b].min[0]), R(s->box[b].min[1]), R(s->box[b].min[2])};
+            sc.box[b].mx = {R(s->box[b].max[0]), R(s->box[b].max[1]), R(s->box[b].max[2])};
+            sc.box[b].matter = s->box[b].matter;
+        }
+    }
+    sc.max_reflections = s->max_reflections;
+    sc.led_scale = R(s->led_scale);
     return sc;
 }
 
@@ -109,6 +119,8 @@ double emission(const mgk_scene_t* s, const StarIndex* stars, const mgk_hit_t& h
             uint32_t cj = std::min(n - 1, uint32_t(ph / (2.0 * double(kPi)) * n));
             return ((ci + cj) & 1u) ? 0.0 : planck<double>(lam_emit, double(s->sphere_T));
         }
+        case MGK_HIT_BOX_EMITTER:
+            return double(s->led_scale) * white_led<double>(lam_emit);
         case MGK_HIT_ESCAPE: {
             double v = 0;
             if (s->flags & MGK_SCENE_BACKGROUND) v += double(s->background_scale) * planck<double>(lam_emit, double(s->background_T));
@@ -237,6 +249,89 @@ int orc_trace_rays(int prec, const mgk_integrator_t* integ, const mgk_scene_t* s
     };
     if (prec == 0) run(0.0f); else run(0.0);
     return MAGIK_SUCCESS;
+}
+
+// Whole paths with reflections off diffuse boxes (trace_path): rays as above,
+// ray i being sample sample_begin + i / (W H) of pixel i % (W H) of `cam`,
+// whose seed and camera id key the reflections' uniforms.  reflections[i]
+// (optional) receives the number of reflections of path i, at[8 i ..] (optional)
+// the hit point and momentum of a path that ended on a box.
+int orc_trace_paths(int prec, const mgk_integrator_t* integ, const mgk_scene_t* scene, const mgk_camera_t* cam, uint64_t seed,
+                    uint32_t sample_begin, const double* rays, uint64_t n, mgk_hit_t* hits, uint32_t* steps, uint32_t* reflections,
+                    double* at, int threads) {
+    if (!integ || !scene || !cam || !rays || !hits) return MAGIK_ERROR_INVALID_POINTER;
+    if (threads > 0) omp_set_num_threads(threads);
+    const uint64_t npix = uint64_t(cam->x_resolution) * cam->y_resolution;
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        MetricParams<R> p = params_from<R>(integ);
+        Scene<R> sc = scene_from<R>(scene, integ->step_limit);
+        V3<R> o{R(cam->position[0]), R(cam->position[1]), R(cam->position[2])};
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t i = 0; i < int64_t(n); ++i) {
+            const double* ray = rays + 10 * i;
+            RayIC<R> ic;
+            ic.y = load_state<R>(ray);
+            ic.h = R(ray[8]);
+            ic.contraction = R(ray[9]);
+            uint32_t k = 0;
+            Hit<R> h = trace_path(ic, p, sc, o, seed, cam->camera_id, uint32_t(uint64_t(i) % npix),
+                                  sample_begin + uint32_t(uint64_t(i) / npix), &k);
+            hits[i].cls = h.cls;
+            hits[i].g = float(h.g);
+            hits[i].a0 = float(h.a0);
+            hits[i].a1 = float(h.a1);
+            if (steps) steps[i] = h.steps;
+            if (reflections) reflections[i] = k;
+            if (at) store_state(h.at, at + 8 * i);
+        }
+    };
+    if (prec == 0) run(0.0f); else run(0.0);
+    return MAGIK_SUCCESS;
+}
+
+// The box pieces on their own, for the tests: the slab test of a chord
+// (origin, unit direction, 1 / chord length) with its normal code, the
+// self-intersection offset, one reflection, and the white LED spectrum.
+int orc_aabb_hit(int prec, const double o[3], const double d[3], double inv_chord, const double bmin[3], const double bmax[3],
+                 double* t, int* normal_code) {
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        R tt = 0;
+        int nc = 0;
+        bool ok = aabb_hit<R>(&tt, V3<R>{R(d[0]), R(d[1]), R(d[2])}, V3<R>{R(o[0]), R(o[1]), R(o[2])}, R(inv_chord),
+                              V3<R>{R(bmin[0]), R(bmin[1]), R(bmin[2])}, V3<R>{R(bmax[0]), R(bmax[1]), R(bmax[2])}, &nc);
+        *t = double(tt);
+        *normal_code = nc;
+        return ok ? 1 : 0;
+    };
+    return prec == 0 ? run(0.0f) : run(0.0);
+}
+
+void orc_offset_geodesic(const double q[4], int normal_code, double out[4]) {
+    V4<double> r = offset_geodesic(V4<double>{q[0], q[1], q[2], q[3]}, normal_of<double>(normal_code));
+    out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.t;
+}
+
+int orc_reflect_diffuse(int prec, const mgk_integrator_t* integ, const double at[8], double contraction, int normal_code, double u0,
+                        double u1, double out[8], double* contraction_out) {
+    if (!integ || !at || !out || !contraction_out) return MAGIK_ERROR_INVALID_POINTER;
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        MetricParams<R> p = params_from<R>(integ);
+        State<R> y;
+        R c = 0;
+        reflect_diffuse<R>(p, load_state<R>(at), R(contraction), normal_code, R(u0), R(u1), &y, &c);
+        store_state(y, out);
+        *contraction_out = double(c);
+    };
+    if (prec == 0) run(0.0f); else run(0.0);
+    return MAGIK_SUCCESS;
+}
+
+double orc_white_led(int prec, double lam_nm) {
+    if (prec == 0) return double(white_led<float>(float(lam_nm)));
+    return white_led<double>(lam_nm);
 }
 
 // Bin CMF weights (host_color_utilities.cuh:145-178) from a CIE table (n x 3).

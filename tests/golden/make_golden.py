@@ -99,6 +99,10 @@ def main():
     R.ref_cmf_table(O.fptr(tab))
     out["cmf_table_sum_bits"] = [f32bits(v) for v in tab.sum(axis=0, dtype=np.float64).astype(np.float32)]
 
+    # white LED spectrum of the illuminant matter (internal_math.h:2501-2509)
+    lam_led = np.concatenate([np.linspace(300.0, 900.0, 61), np.random.default_rng(20261205).uniform(360, 830, 20)])
+    out["white_led"] = [{"lam": float(l), "f32_bits": f32bits(R.ref_white_led(float(l)))} for l in lam_led]
+
     # colour spaces and display encodes (colorspaces.cuh:24-338)
     scene = []
     for sid in range(6):
