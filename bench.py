@@ -222,6 +222,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--mlp-only", action="store_true",
                     help="only time the inference kernel over one 1080p x 32 frame of feature rows (counter runs)")
+    ap.add_argument("--manager", action="store_true",
+                    help="only time the render manager (command queue + worker) against direct one-sample renders")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a copy kernel over BYTES three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -241,6 +243,10 @@ def main():
 
     import torch
     import magik_amd as M
+
+    if args.manager:
+        bench_manager(M, args)
+        return
 
     if args.mlp_only:
         n = 1920 * 1080 * 32
@@ -385,6 +391,55 @@ def main():
         res["cpu_baseline"] = cpu_baseline(M, integ, sc, cam, spp, args.seed)
         log(f"cpu baseline {res['cpu_baseline']['value']:.3e} rays/s")
     ctx.close()
+    print(json.dumps(res), flush=True)
+
+
+def bench_manager(M, args, window_s=6.0):
+    """The render manager at the stage's defaults (the box scene, 2195 x 1860,
+    one sample per worker iteration, GT7 display into the "test" AOV): samples
+    per second from its statistics over a time window, against the same
+    one-sample frames rendered and display-transformed directly."""
+    import torch
+    m = M.Manager(0)
+    try:
+        m.push(M.Manager.aov_command("add_aov", "test", 3))
+        while not m.dispatch():
+            time.sleep(0.001)
+        t0 = time.time()
+        while m.stats()[0].x_sample < 3 and time.time() - t0 < 120:
+            time.sleep(0.01)
+        s0, t0 = m.stats()[0].x_sample, time.perf_counter()
+        time.sleep(window_s)
+        st = m.stats()[0]
+        s1, t1 = st.x_sample, time.perf_counter()
+        err = m.error()
+    finally:
+        m.close()
+    mgr_rate = (s1 - s0) / (t1 - t0)
+    # direct: the same frames through the library, one sample per call
+    integ, sc, _, cam, _ = M.workload("box_led", precision=M.PREC_MIXED)
+    spec = M.spectral(n_hero=4, hero_min=360.0, hero_max=830.0, n_bin=24, bin_min=360.0, bin_max=830.0)
+    ctx = M.Context(0)
+    ctx.set_scene(sc)
+    disp = M.display()
+    xyz = torch.zeros((cam.y_resolution, cam.x_resolution, 3), dtype=torch.float32, device="cuda:0")
+    ctx.render_frame(integ, sc, spec, cam, 0, 1, xyz)
+    torch.cuda.synchronize()
+    k = 5
+    t = time.perf_counter()
+    for i in range(k):
+        ctx.render_frame(integ, sc, spec, cam, i, 1, xyz)
+        M.display_transform(ctx, disp, xyz, out=xyz)
+    torch.cuda.synchronize()
+    direct_rate = k / (time.perf_counter() - t)
+    ctx.close()
+    W, H = cam.x_resolution, cam.y_resolution
+    res = {"metric": "render manager samples/s (box scene, 2195x1860, 1 sample per iteration)", "value": mgr_rate,
+           "unit": "samples/s", "direct_samples_per_s": direct_rate, "manager_overhead_frac": 1 - mgr_rate / direct_rate,
+           "manager_paths_per_s": mgr_rate * W * H, "latest_iteration_ms": st.latest_render_loop_iteration_time_seconds * 1e3,
+           "latest_sample_ms": st.latest_wavefront_sample_time_seconds * 1e3, "samples": int(s1 - s0),
+           "window_s": t1 - t0, "error": int(err)}
+    log(f"manager {mgr_rate:.2f} samples/s vs direct {direct_rate:.2f} samples/s")
     print(json.dumps(res), flush=True)
 
 

@@ -48,7 +48,20 @@ typedef enum e_magik_result_types {
     MAGIK_ERROR_HOST_TRACE_ALLOCATED_STEPS_AND_CAPACITY_MISMATCH = 102,
     MAGIK_ERROR_DEVICE_OUT_OF_MEMORY = 200,
     MAGIK_ERROR_DEVICE_ALLOCATION_FAILED = 201,
+    MAGIK_ERROR_INVALID_ID = 400,
+    MAGIK_ERROR_PROVIDED_ID_NOT_FOUND = 401,
+    MAGIK_ERROR_COMMAND_BUFFER_OVERFLOW = 402,
+    MAGIK_ERROR_INVALID_COMMAND = 403,
+    MAGIK_ERROR_PACKED_COMMAND_NOT_ALLIGNED = 404,
+    MAGIK_ERROR_COMMAND_DROPPED = 405,
+    MAGIK_ERROR_COMMAND_BUFFER_ALLOCATION_FAILED = 406,
+    MAGIK_ERROR_COMMAND_SIZE_NOT_A_MULTIPLE_OF_4 = 407,
+    MAGIK_ERROR_OUT_OF_BOUNDS_COMMAND_BUFFER_READ = 408,
     MAGIK_ERROR_UNKNOWN_METRIC = 707,
+    MAGIK_ERROR_AOV_INCORRECT_EXTRACT_CALL = 900,
+    MAGIK_ERROR_AOV_ALLOCATED_BEFORE_INITIALIZATION = 901,
+    MAGIK_ERROR_AOV_SWAPCHAIN_NOT_INITALIZED = 902,
+    MAGIK_ERROR_GL_LOADER_FAILED = 1000,
     MAGIK_ERROR_UNKNOWN = 10000,
     /* Additions of this library (outside the reference's ranges). */
     MGK_ERROR_INVALID_ARGUMENT = 20000,

@@ -324,6 +324,74 @@ class Context:
         return st
 
 
+class Manager:
+    """A render manager (include/magik_manager.h): the command queue system
+    and the worker thread that renders the stage progressively."""
+
+    def __init__(self, device: int = 0, display=abi.MAGIK_DISPLAY_SWAPCHAIN, n_chunk: int = 4096, drop_overflow=False):
+        self.lib = abi.load()
+        d = abi.ManagerDescriptor(display_type=display, user_device_id=device, user_stream=None,
+                                  cqs_n_reserved_chunk=n_chunk, cqs_drop_overflow=drop_overflow)
+        h = C.c_void_p()
+        abi.check(self.lib.magik_render_manager_create(C.byref(h), d), "magik_render_manager_create")
+        self.h = h
+        fb = C.c_void_p()
+        abi.check(self.lib.magik_configure_framebuffer_collection(C.byref(fb), abi.FB_CONFIG_HOST),
+                  "magik_configure_framebuffer_collection")
+        self.fb = fb
+
+    def close(self):
+        if self.h:
+            self.lib.magik_destroy_framebuffer_collection(self.fb)
+            self.lib.magik_render_manager_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def command(name: str, *fields):
+        """A command as 4-byte words: its type, then the fields (float or int)."""
+        import struct
+        words = [abi.CMD[name]]
+        for f in fields:
+            words.append(struct.unpack("<I", struct.pack("<f", f))[0] if isinstance(f, float) else int(f))
+        return (C.c_uint32 * len(words))(*words)
+
+    @staticmethod
+    def aov_command(name: str, aov: str, channels: int = 3):
+        c = abi.AovCommand(embedded_type=abi.CMD[name], length_of_name=len(aov), name=aov.encode(), channels=channels)
+        return c
+
+    def push(self, cmd) -> int:
+        return self.lib.magik_cqs_push_command(self.h, C.cast(C.pointer(cmd), C.c_void_p))
+
+    def dispatch(self) -> bool:
+        ok = C.c_bool(False)
+        abi.check(self.lib.magik_cqs_dispatch_command_buffer(self.h, C.byref(ok)), "magik_cqs_dispatch_command_buffer")
+        return ok.value
+
+    def error(self) -> int:
+        return self.lib.magik_render_manager_fetch_error(self.h)
+
+    def stats(self):
+        st, new = abi.RendererStatistics(), C.c_bool(False)
+        abi.check(self.lib.magik_fetch_renderer_statistics(self.h, C.byref(new), C.byref(st)), "magik_fetch_renderer_statistics")
+        return st, new.value
+
+    def fetch(self) -> tuple[int, bool]:
+        new = C.c_bool(False)
+        r = self.lib.magik_framebuffer_collection_fetch(self.h, C.byref(new), self.fb)
+        return r, new.value
+
+    def extract(self, aov: str = "test"):
+        """(H, W, channels) float32 copy of an AOV of the last fetch, or None."""
+        import numpy as np
+        c = abi.HostContainer()
+        abi.check(self.lib.magik_extract_config_host(self.h, C.byref(c), self.fb, aov.encode()), "magik_extract_config_host")
+        if not c.h_data:
+            return None
+        n = c.x_resolution * c.y_resolution * c.channels
+        return np.ctypeslib.as_array(c.h_data, shape=(n,)).copy().reshape(c.y_resolution, c.x_resolution, c.channels)
+
+
 def hits_to_numpy(hits):
     """(n, 4) int32 device tensor -> structured numpy array cls/g/a0/a1."""
     import numpy as np

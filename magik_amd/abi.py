@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libmagik_amd.so")
 # e_magik_result_types (include/magik_amd.h)
 MAGIK_SUCCESS = 0
 MAGIK_ERROR_INVALID_POINTER = 2
+MAGIK_UNKNOWN_ENUM_TYPE = 3
 MAGIK_INVALID_CUDA_DEVICE = 4
 MAGIK_ERROR_HOST_TRACE_ALLOCATED_STEPS_AND_CAPACITY_MISMATCH = 102
 MAGIK_ERROR_DEVICE_ALLOCATION_FAILED = 201
@@ -150,6 +151,43 @@ class SurrogateStats(C.Structure):  # mgk_surrogate_stats_t
 
 
 # Every symbol include/magik_amd.h declares, with its ctypes signature.
+class ManagerDescriptor(C.Structure):  # magik_manager_descriptor_t (include/magik_manager.h)
+    _fields_ = [("display_type", C.c_int32), ("user_device_id", C.c_uint32), ("user_stream", C.c_void_p),
+                ("cqs_n_reserved_chunk", C.c_uint32), ("cqs_drop_overflow", C.c_bool)]
+
+
+class RendererStatistics(C.Structure):  # magik_renderer_statistics_t
+    _fields_ = [("x_sample", C.c_size_t), ("total_render_time_seconds", C.c_double),
+                ("latest_render_loop_iteration_time_seconds", C.c_double),
+                ("total_wavefront_sample_time_seconds", C.c_double), ("latest_wavefront_sample_time_seconds", C.c_double),
+                ("latest_wavefront_history_length", C.c_size_t)] + \
+               [(f"h_latest_{q}_queue_count_history", C.c_void_p)
+                for q in ("advance_A", "advance_B", "intersect", "bxdf", "redistribute", "splat")] + \
+               [("total_framebuffer_swapchain_size", C.c_size_t), ("total_wavefront_buffer_size", C.c_size_t)]
+
+
+class HostContainer(C.Structure):  # magik_framebuffer_container_config_host_t
+    _fields_ = [("x_resolution", C.c_uint32), ("y_resolution", C.c_uint32), ("channels", C.c_uint32),
+                ("size_of_data", C.c_size_t), ("h_data", C.POINTER(C.c_float))]
+
+
+class AovCommand(C.Structure):  # magik_command_add_aov_t (remove: without channels)
+    _fields_ = [("embedded_type", C.c_uint32), ("length_of_name", C.c_uint32), ("name", C.c_char * 512),
+                ("channels", C.c_uint32)]
+
+
+MAGIK_DISPLAY_SWAPCHAIN, MAGIK_DISPLAY_HEADLESS = 0, 1
+FB_CONFIG_HOST, FB_CONFIG_CUDA, FB_CONFIG_OPENGL, FB_CONFIG_VULKAN = 0, 1, 2, 3
+CMD = {"render": 1000, "pause": 1001, "clear": 1002, "add_aov": 1003, "remove_aov": 1004, "printf": 2000,
+       "minkowski": 4000, "schwarzschild": 4001, "kerr": 4002, "abs_tol": 4100, "rel_tol": 4101, "max_substeps": 4102,
+       "velocity": 4200, "celestial_radius": 4300, "schwarzschild_scalar": 4400, "schwarzschild_mass": 4401,
+       "kerr_scalar": 4402, "kerr_mass": 4403, "kerr_spin": 4404, "active_camera": 5000, "camera_position": 5001,
+       "camera_orientation": 5002, "camera_resolution": 5003, "n_bin": 6000, "wl_max": 6001, "wl_min": 6002,
+       "sigma": 6003, "cmf": 6100, "scene_colorspace": 6101, "view_transform": 6102, "display_colorspace": 6103,
+       "output_format": 6104, "white_point": 6200, "tint": 6201, "reinhard_white": 6300, "khronos_start": 6301,
+       "khronos_desaturation": 6302, "gt7_mode": 6304, "gt7_paper_white": 6305, "gt7_reference_luminance": 6306,
+       "gt7_jzazbz_scale": 6407, "agx_look": 6308, "step_limit": 7000, "reflected_limit": 7001}
+
 _P = C.c_void_p
 _R = C.c_int32
 SIGNATURES = {
@@ -195,6 +233,17 @@ SIGNATURES = {
     "mgk_render_frame_surrogate": (_R, [_P, _P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(Spectral),
                                         C.POINTER(Camera), C.c_uint64, C.c_uint32, C.POINTER(Fallback), _P, _P,
                                         C.POINTER(SurrogateStats), _P]),
+    "magik_render_manager_create": (_R, [C.POINTER(_P), ManagerDescriptor]),
+    "magik_render_manager_destroy": (_R, [_P]),
+    "magik_render_manager_fetch_error": (_R, [_P]),
+    "magik_cqs_push_command": (_R, [_P, _P]),
+    "magik_cqs_dispatch_command_buffer": (_R, [_P, C.POINTER(C.c_bool)]),
+    "magik_fetch_renderer_statistics": (_R, [_P, C.POINTER(C.c_bool), C.POINTER(RendererStatistics)]),
+    "magik_destroy_renderer_statistics": (_R, [C.POINTER(RendererStatistics)]),
+    "magik_configure_framebuffer_collection": (_R, [C.POINTER(_P), C.c_int32]),
+    "magik_framebuffer_collection_fetch": (_R, [_P, C.POINTER(C.c_bool), _P]),
+    "magik_extract_config_host": (_R, [_P, C.POINTER(HostContainer), _P, C.c_char_p]),
+    "magik_destroy_framebuffer_collection": (_R, [_P]),
 }
 
 _lib = None

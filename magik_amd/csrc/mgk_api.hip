@@ -988,6 +988,16 @@ MGK_API e_magik_result_types mgk_shade(mgk_context_handle c, const mgk_scene_t* 
 MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc,
                                               const mgk_spectral_t* sp, const mgk_camera_t* cam, uint64_t seed, uint32_t spp,
                                               float* d_xyz, void* stream) {
+    return mgk::render_samples(c, in, sc, sp, cam, seed, 0, spp, d_xyz, hipStream_t(stream));
+}
+
+}  // extern "C"
+
+// Samples [sample_begin, sample_begin + spp) of a frame: generate + trace
+// (whole paths) + shade, XYZ / spp per pixel.
+e_magik_result_types mgk::render_samples(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc,
+                                         const mgk_spectral_t* sp, const mgk_camera_t* cam, uint64_t seed, uint32_t sample_begin,
+                                         uint32_t spp, float* d_xyz, hipStream_t s) {
     if (!c || !sc || !sp || !cam || !d_xyz) return MAGIK_ERROR_INVALID_POINTER;
     e_magik_result_types r = check_integrator(in);
     if (r != MAGIK_SUCCESS) return r;
@@ -997,9 +1007,8 @@ MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle c, const mgk_in
     const uint64_t n = uint64_t(cam->x_resolution) * cam->y_resolution * spp;
     HIP_TRY(c->rays.reserve(n * 10 * ray_elem(in)));
     HIP_TRY(c->hits.reserve(n * sizeof(mgk_hit_t)));
-    hipStream_t s = hipStream_t(stream);
     RayGenParams G;
-    if ((r = fill_raygen(in, cam, seed, 0, spp, c->rays.ptr, n, G)) != MAGIK_SUCCESS) return r;
+    if ((r = fill_raygen(in, cam, seed, sample_begin, spp, c->rays.ptr, n, G)) != MAGIK_SUCCESS) return r;
     HIP_TRY(c->ic_camera.reserve(sizeof(IcCamera)));
     HIP_TRY(launch_generate_rays(G, reinterpret_cast<IcCamera*>(c->ic_camera.ptr), s));
     TraceParams P;
@@ -1008,16 +1017,15 @@ MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle c, const mgk_in
     P.stride = n;
     P.n = n;
     P.hits = reinterpret_cast<mgk_hit_t*>(c->hits.ptr);
-    const PathInfo paths{seed, cam->camera_id, cam->x_resolution * cam->y_resolution, 0};
+    const PathInfo paths{seed, cam->camera_id, cam->x_resolution * cam->y_resolution, sample_begin};
     if ((r = do_trace(c, P, s, &paths, sc->max_reflections)) != MAGIK_SUCCESS) return r;
     ShadeParams S;
-    if ((r = fill_shade_params(c, sc, sp, cam, seed, 0, spp, reinterpret_cast<mgk_hit_t*>(c->hits.ptr), d_xyz, S)) != MAGIK_SUCCESS)
+    if ((r = fill_shade_params(c, sc, sp, cam, seed, sample_begin, spp, reinterpret_cast<mgk_hit_t*>(c->hits.ptr), d_xyz, S)) !=
+        MAGIK_SUCCESS)
         return r;
     HIP_TRY(launch_shade(S, s));
     return MAGIK_SUCCESS;
 }
-
-}  // extern "C"
 
 e_magik_result_types mgk_star_band_view(mgk_context_handle c, const mgk_scene_t* sc, float margin, StarView* out) {
     if (!c || !sc || !out) return MAGIK_ERROR_INVALID_POINTER;
@@ -1136,14 +1144,8 @@ MGK_API e_magik_result_types magik_trace_geodesic(magik_trace_geodesic_container
     P.position[0] = ic.spatial_position_x;
     P.position[1] = ic.spatial_position_y;
     P.position[2] = ic.spatial_position_z;
-    // rotate(x_rot, y_rot, z_rot) = m_x(z_rot) m_y(y_rot) m_z(x_rot) (internal_math.h:1784-1810)
-    {
-        float ca = std::cos(ic.x_rotation), sa = std::sin(ic.x_rotation), cb = std::cos(ic.y_rotation), sb = std::sin(ic.y_rotation);
-        float cg = std::cos(ic.z_rotation), sg = std::sin(ic.z_rotation);
-        m3 mz{{ca, -sa, 0, sa, ca, 0, 0, 0, 1}}, my{{cb, 0, sb, 0, 1, 0, -sb, 0, cb}}, mx{{1, 0, 0, 0, cg, -sg, 0, sg, cg}};
-        m3 m = mul(mul(mx, my), mz);
-        std::memcpy(P.rot, m.m, sizeof(P.rot));
-    }
+    // rotate(x_rot, y_rot, z_rot) = m_x(z_rot) m_y(y_rot) m_z(x_rot) (host_trace_geodesic.cu:17)
+    mgk::rotate_matrix(ic.x_rotation, ic.y_rotation, ic.z_rotation, P.rot);
     P.lpv[0] = ic.local_proper_velocity_x;
     P.lpv[1] = ic.local_proper_velocity_y;
     P.lpv[2] = ic.local_proper_velocity_z;

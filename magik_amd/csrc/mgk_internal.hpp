@@ -2,6 +2,8 @@
 // C-ABI layer (mgk_api.hip).
 #pragma once
 
+#include <cmath>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -177,6 +179,23 @@ hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s);
 hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s);
 hipError_t launch_reflect(const ReflectParams& p, bool fp32, hipStream_t s);
+
+// Samples [sample_begin, sample_begin + spp) of a frame into d_xyz (XYZ / spp
+// per pixel): the body of mgk_render_frame.
+e_magik_result_types render_samples(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc, const mgk_spectral_t* sp,
+                                    const mgk_camera_t* cam, uint64_t seed, uint32_t sample_begin, uint32_t spp, float* d_xyz,
+                                    hipStream_t s);
+
+// mat3 rotate(a, b, g) of internal_math.h:1784-1810: m_x(g) m_y(b) m_z(a), row-major.
+inline void rotate_matrix(float a, float b, float g, float out[9]) {
+    const float ca = std::cos(a), sa = std::sin(a), cb = std::cos(b), sb = std::sin(b), cg = std::cos(g), sg = std::sin(g);
+    const float mz[9] = {ca, -sa, 0, sa, ca, 0, 0, 0, 1}, my[9] = {cb, 0, sb, 0, 1, 0, -sb, 0, cb}, mx[9] = {1, 0, 0, 0, cg, -sg, 0, sg, cg};
+    float t[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) t[3 * i + j] = mx[3 * i] * my[j] + mx[3 * i + 1] * my[3 + j] + mx[3 * i + 2] * my[6 + j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) out[3 * i + j] = t[3 * i] * mz[j] + t[3 * i + 1] * mz[3 + j] + t[3 * i + 2] * mz[6 + j];
+}
 hipError_t launch_geodesic_steps(const GeodesicStepParams& p, hipStream_t s);
 
 }  // namespace mgk

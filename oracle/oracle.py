@@ -54,6 +54,7 @@ _SIG = {
     "orc_reflect_diffuse": (C.c_int, [C.c_int, C.POINTER(abi.Integrator), _D, C.c_double, C.c_int, C.c_double, C.c_double,
                                       _D, _D]),
     "orc_white_led": (C.c_double, [C.c_int, C.c_double]),
+    "orc_manager_layout": (C.c_size_t, [C.POINTER(C.c_size_t)]),
 }
 
 _REF_SIG = {
@@ -67,6 +68,7 @@ _REF_SIG = {
     "ref_sample_cmf": (None, [C.c_float, _F]),
     "ref_cmf_table": (C.c_uint, [_F]),
     "ref_layout": (None, [C.POINTER(C.c_size_t)]),
+    "ref_manager_layout": (C.c_size_t, [C.POINTER(C.c_size_t)]),
     "ref_scene_colorspace": (C.c_int, [C.c_uint, _F, _F, _F]),
     "ref_display_colorspace": (C.c_int, [C.c_uint, _F, _F, _F, _F, _F, C.c_uint, _F]),
 }

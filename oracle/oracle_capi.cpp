@@ -5,6 +5,7 @@
 // hand the same settings to the oracle and to the HIP library.
 #include "oracle.hpp"
 #include "magik_amd.h"
+#include "magik_manager.h"
 
 #include <omp.h>
 #include <cstddef>
@@ -487,6 +488,17 @@ void orc_layout(size_t* out) {
     out[i++] = offsetof(magik_trace_geodesic_container_t, steps);
     out[i++] = (size_t)MAGIK_ERROR_HOST_TRACE_ALLOCATED_STEPS_AND_CAPACITY_MISMATCH;
     out[i++] = (size_t)MAGIK_ERROR_UNKNOWN_METRIC;
+}
+
+// The render-manager API's layout list (manager_layout.inc); returns its length.
+size_t orc_manager_layout(size_t* out) {
+    size_t i = 0;
+#define L(x) out[i++] = (size_t)(x);
+#define CMD(T, V) L(sizeof(T)) L(V)
+#include "manager_layout.inc"
+#undef L
+#undef CMD
+    return i;
 }
 
 // Star catalogue exactly as the oracle generates it (n x 4: dx dy dz brightness).

@@ -162,6 +162,17 @@ int ref_display_colorspace(unsigned id, float* to, float* from, float* wp, float
     }
 }
 
+// The render-manager API's layout list (manager_layout.inc); returns its length.
+size_t ref_manager_layout(size_t* out) {
+    size_t i = 0;
+#define L(x) out[i++] = (size_t)(x);
+#define CMD(T, V) L(sizeof(T)) L(V)
+#include "manager_layout.inc"
+#undef L
+#undef CMD
+    return i;
+}
+
 // sizeof / offsetof of the magik.h structs the library re-declares.
 void ref_layout(size_t* out) {
     size_t i = 0;

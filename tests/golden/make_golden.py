@@ -33,7 +33,7 @@ def main():
     assert R is not None, "reference harness not built (needs /root/reference)"
     rng = np.random.default_rng(20261203)
     out = {"source": "reference headers via oracle/_ref (internal_math.h, magik.h, color_matching_functions.cuh, "
-                     "colorspaces.cuh)"}
+                     "colorspaces.cuh; magik.h's manager declarations)"}
 
     # RKF45 driver (internal_math.h:2160-2205) on the harmonic oscillator
     osc = []
@@ -125,6 +125,11 @@ def main():
                         "white_xy_bits": [f32bits(v) for v in wp], "peak_nits": peak.value,
                         "encode_in": rgb.tolist(), "encode_out_bits": [[f32bits(v) for v in r] for r in enc]})
     out["display_colorspaces"] = display
+
+    # the render-manager API: sizes, offsets and enum values (oracle/manager_layout.inc)
+    man = (C.c_size_t * 256)()
+    k = R.ref_manager_layout(man)
+    out["manager_layout"] = list(man[:k])
 
     # struct layouts and result codes of magik.h
     lay = (C.c_size_t * 14)()

@@ -1,5 +1,5 @@
 """The C-ABI library without a GPU: it loads, exports every entry point
-include/magik_amd.h declares, and its host-side logic (colour pipeline,
+include/*.h declares, and its host-side logic (colour pipeline,
 camera placement, argument validation) matches the oracle.  No kernel runs."""
 import ctypes as C
 import os
@@ -12,11 +12,11 @@ import magik_amd as M
 from magik_amd import abi
 from oracle import oracle as O
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "magik_amd.h")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
 
 
 def declared_symbols():
-    text = open(HEADER).read()
+    text = "".join(open(os.path.join(INCLUDE, f)).read() for f in sorted(os.listdir(INCLUDE)) if f.endswith(".h"))
     return sorted(set(re.findall(r"MGK_API\s+[\w\s\*]+?\b(\w+)\s*\(", text)))
 
 
