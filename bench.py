@@ -1,6 +1,6 @@
 """Benchmark: Kerr a=0.9 1920x1080x32 spp frames on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kerr_disk_1080p|kerr_disk_4k|batch64|box_led]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload kerr_disk_1080p|kerr_disk_4k|batch64|box_led|box_mesh]
 
 A step is one full frame through the C-ABI of synthetic seeded rays, timed on
 two paths:
@@ -164,6 +164,8 @@ def cpu_baseline(M, integ, sc, cam, spp, seed, budget_s=12.0):
     rng = np.random.default_rng(seed)
 
     boxes = bool(sc.flags & M.SCENE_BOXES)
+    if sc.flags & M.SCENE_MESHES:
+        O.set_meshes(*M.workload_meshes("box_mesh"))
 
     def run(n):
         idx = np.sort(rng.choice(npx * spp, size=n, replace=False))
@@ -275,6 +277,9 @@ def main():
     n_paths = W * H * spp
     ctx = M.Context(0)
     ctx.set_scene(sc)
+    meshes = M.workload_meshes(args.workload) if sc.flags & M.SCENE_MESHES else None
+    if meshes:
+        ctx.set_meshes(*meshes)
     out = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
     log(f"bench: {args.workload} {W}x{H}x{spp} = {n_paths} paths, precision {args.precision}")
 
@@ -365,12 +370,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if prec != M.PREC_FP64 else "f64",
         "data": (f"synthetic (seeded Philox rays, Kerr a={integ.spin:.3f} hole inside a diffuse box with an LED "
-                 "panel, the reference's default stage; seeded Lambertian reflections)") if boxes else
+                 "panel, the reference's default stage; seeded Lambertian reflections"
+                 + ("; procedural triangle meshes, Hermite chords)" if meshes else ")")) if boxes else
                 (f"synthetic (seeded Philox rays, Kerr a={integ.spin:.2f} disk + 1e5-star field scene; surrogate "
                  "trained on seeded FP64-traced rays)"),
         "config": {"workload": (f"{args.workload}: Kerr a={integ.spin:.3f}, M=6.73e26 kg, 40x20x40 m diffuse box, "
                                 f"20x20 m LED panel, camera (2,0,9) m, {W}x{H}x{spp} spp, 4 hero wavelengths, "
-                                f"tol 1e-3/1e-10, step limit 512, {sc.max_reflections} reflections") if boxes else
+                                f"tol 1e-3/1e-10, step limit 512, {sc.max_reflections} reflections"
+                                + (f", meshes: {sum(len(f) for _, f in meshes[0])} triangles in {len(meshes[0])} "
+                                   f"meshes, {len(meshes[1])} instances, Hermite chords" if meshes else "")) if boxes else
                                (f"{args.workload}: Kerr a={integ.spin:.2f}, r_obs=50M, theta=80 deg, FOV 25 deg, "
                                 f"{W}x{H}x{spp} spp, 4 hero wavelengths, tol 1e-7, cap 1e4"),
                    "paths_per_frame": n_paths, "value_path": best[0], "error_bar": ERROR_BAR},

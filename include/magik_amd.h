@@ -207,7 +207,11 @@ typedef enum mgk_hit_class_t {
      * (kernel_enforce_ray_limits.cu:48-50, n_reflected > limit): no radiance */
     MGK_HIT_BOX_DIFFUSE = 6,
     /* illuminant box matter (matter_examples.cuh:63-76) */
-    MGK_HIT_BOX_EMITTER = 7
+    MGK_HIT_BOX_EMITTER = 7,
+    /* triangle-mesh matter (MGK_SCENE_MESHES), diffuse and illuminant; the hit
+     * record's a0 is the triangle's index in its mesh, a1 the instance index */
+    MGK_HIT_MESH_DIFFUSE = 8,
+    MGK_HIT_MESH_EMITTER = 9
 } mgk_hit_class_t;
 
 /* Integrator / spacetime settings (relativity_settings.h:11-25,
@@ -246,8 +250,20 @@ enum {
     MGK_SCENE_SPHERE = 2u,
     MGK_SCENE_STARS = 4u,
     MGK_SCENE_BACKGROUND = 8u,
-    MGK_SCENE_BOXES = 16u
+    MGK_SCENE_BOXES = 16u,
+    MGK_SCENE_MESHES = 32u       /* the context's triangle meshes (mgk_set_meshes) */
 };
+
+/* How a completed step's chord meets triangle meshes.  AFFINE: the straight
+ * segment between the step's end points, as the reference's box test
+ * (kernel_intersect.cu:63-70).  HERMITE: the cubic Hermite curve through the
+ * end points with the geodesic's tangents dx/dlambda = g^{mu nu} p_nu there,
+ * the curved chord the reference reserves (wavefront.cpp:94-100).  Boxes keep
+ * the affine chord. */
+typedef enum mgk_chord_t {
+    MGK_CHORD_AFFINE = 0,
+    MGK_CHORD_HERMITE = 1
+} mgk_chord_t;
 
 /* Matter of a box: the reference's registered matter slots 1 and 2
  * (kernel_bxdf.cu:126, matter_examples.cuh:36-76). */
@@ -257,6 +273,28 @@ typedef enum mgk_matter_t {
 } mgk_matter_t;
 
 #define MGK_MAX_BOX 4
+
+/* A triangle mesh in object space, SI metres, in host memory.  Triangles are
+ * two-sided; their geometric normal is (v1 - v0) x (v2 - v0). */
+typedef struct mgk_mesh_t {
+    const float* vertices;       /* n_vertex x 3 */
+    uint32_t n_vertex;
+    uint32_t n_triangle;
+    const uint32_t* indices;     /* n_triangle x 3 vertex indices */
+} mgk_mesh_t;
+
+/* A placed mesh: its matter and the row-major 3 x 4 affine object -> world map
+ * (invertible). */
+typedef struct mgk_instance_t {
+    uint32_t mesh;               /* index into the meshes of mgk_set_meshes */
+    uint32_t matter;             /* mgk_matter_t */
+    float object_to_world[12];
+} mgk_instance_t;
+
+/* Triangles per mesh and over all meshes, and instances: hit records carry the
+ * triangle and instance indices as floats, exact below 2^24. */
+#define MGK_MAX_MESH_TRIANGLES 16777215u
+#define MGK_MAX_INSTANCES 16777215u
 
 /* An axis-aligned box in SI metres (stage.h:19-22; traceables.cuh:17-74). */
 typedef struct mgk_box_t {
@@ -288,7 +326,7 @@ typedef struct mgk_scene_t {
     uint32_t n_box;
     uint32_t max_reflections;    /* render_settings.h:17 n_reflected_limit (10) */
     float led_scale;             /* illuminant radiance scale (matter_examples.cuh:70: 7e5) */
-    uint32_t reserved2;
+    uint32_t chord;              /* mgk_chord_t for the meshes */
     mgk_box_t box[MGK_MAX_BOX];
 } mgk_scene_t;
 
@@ -333,6 +371,14 @@ MGK_API e_magik_result_types mgk_context_destroy(mgk_context_handle ctx);
 /* Uploads the scene's star catalogue (generated from star_seed) and its
  * sky-grid index.  Must precede tracing or shading of a scene with stars. */
 MGK_API e_magik_result_types mgk_set_scene(mgk_context_handle ctx, const mgk_scene_t* scene);
+
+/* Builds the two-level bounding volume hierarchy of the meshes (one bottom
+ * level per mesh, a top level over the instances' world bounds) and uploads
+ * it; scenes with MGK_SCENE_MESHES trace against it until the next call.
+ * n_mesh = 0 clears it.  Has no counterpart in the reference, whose top- and
+ * bottom-level intersection stages are placeholders (wavefront.cpp:94-100). */
+MGK_API e_magik_result_types mgk_set_meshes(mgk_context_handle ctx, const mgk_mesh_t* meshes, uint32_t n_mesh,
+                                            const mgk_instance_t* instances, uint32_t n_instance);
 
 /* Ray generation + initial conditions (kernel_generate_thin_lens_ray.cu:51-77,
  * kernel_generate_initial_conditions.cu:54-74) for every pixel and the samples

@@ -12,9 +12,11 @@ import ctypes as C
 import math
 
 from . import abi
-from .abi import (HIT_BOX_DIFFUSE, HIT_BOX_EMITTER, HIT_DISK, HIT_ESCAPE, HIT_HORIZON, HIT_NONE, HIT_SPHERE,
-                  HIT_STEP_LIMIT, KERR, MATTER_DIFFUSE, MATTER_ILLUMINANT, MINKOWSKI, PREC_FP32_REFERENCE, PREC_FP64,
-                  PREC_MIXED, SCHWARZSCHILD, SCENE_BACKGROUND, SCENE_BOXES, SCENE_DISK, SCENE_SPHERE, SCENE_STARS)
+from . import mesh
+from .abi import (CHORD_AFFINE, CHORD_HERMITE, HIT_BOX_DIFFUSE, HIT_BOX_EMITTER, HIT_DISK, HIT_ESCAPE, HIT_HORIZON,
+                  HIT_MESH_DIFFUSE, HIT_MESH_EMITTER, HIT_NONE, HIT_SPHERE, HIT_STEP_LIMIT, KERR, MATTER_DIFFUSE,
+                  MATTER_ILLUMINANT, MINKOWSKI, PREC_FP32_REFERENCE, PREC_FP64, PREC_MIXED, SCHWARZSCHILD,
+                  SCENE_BACKGROUND, SCENE_BOXES, SCENE_DISK, SCENE_MESHES, SCENE_SPHERE, SCENE_STARS)
 
 DEFAULT_MASS_KG = 6.7329546075e26       # relativity_settings.h:21,24
 G, C_LIGHT = 6.67430e-11, 299792458.0
@@ -118,6 +120,10 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
                         kg, a diffuse 40x20x40 m box with a 20x20 m LED panel
                         at its ceiling, camera at (2,0,9) m, 2195x1860x8, tol
                         1e-3 / 1e-10, step limit 512, 10 reflections.
+    box_mesh          : box_led plus triangle meshes (workload_meshes): two
+                        diffuse icospheres (20,480 triangles each), a diffuse
+                        tessellated plate (8,192) and an illuminant icosphere,
+                        traced with Hermite chords.
     """
     if name == "minkowski_sphere":
         W, H, spp = W or 256, H or 256, spp or 4
@@ -149,7 +155,7 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
                    star_radius_rad=1e-3, star_T=6500.0)
         spec = spectral(n_hero=4)
         return integ, sc, spec, cam, spp
-    if name == "box_led":
+    if name in ("box_led", "box_mesh"):
         # the reference's default stage (stage.h:19-22, relativity_settings.h:11-25,
         # optical_system_settings.h:20-34, render_settings.h:16-17): a Kerr hole of
         # 6.73e26 kg (M = 0.5 m), a = 0.999, inside a 40 x 20 x 40 m diffuse box
@@ -167,8 +173,23 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
                    boxes=[((-20.0, -10.0, -20.0), (20.0, 10.0, 20.0), MATTER_DIFFUSE),
                           ((-10.0, 9.5, -10.0), (10.0, 10.5, 10.0), MATTER_ILLUMINANT)])
         spec = spectral(n_hero=4)
+        if name == "box_mesh":
+            sc.flags |= SCENE_MESHES
+            sc.chord = CHORD_HERMITE
         return integ, sc, spec, cam, spp
     raise ValueError(f"unknown workload {name}")
+
+
+def workload_meshes(name: str = "box_mesh"):
+    """(meshes, instances) of a workload with SCENE_MESHES (Context.set_meshes)."""
+    if name != "box_mesh":
+        raise ValueError(f"workload {name} has no meshes")
+    meshes = [mesh.icosphere(5), mesh.grid_plane(1.0, 64)]
+    instances = [(0, MATTER_DIFFUSE, mesh.affine(1.5, (-4.0, -2.0, -3.0))),
+                 (0, MATTER_DIFFUSE, mesh.affine(2.0, (4.5, 1.5, -7.0))),
+                 (1, MATTER_DIFFUSE, mesh.affine((8.0, 1.0, 8.0), (0.0, -6.0, -5.0), rotate_y_deg=30.0)),
+                 (0, MATTER_ILLUMINANT, mesh.affine(0.8, (0.0, 4.0, -4.0)))]
+    return meshes, instances
 
 
 def batch_workload(seed: int = 4, n_cam: int = 64, resolutions=(128, 256, 512, 1024), spp: int = 8,
@@ -222,6 +243,13 @@ class Context:
 
     def set_scene(self, sc):
         abi.check(self.lib.mgk_set_scene(self.h, C.byref(sc)), "mgk_set_scene")
+
+    def set_meshes(self, meshes, instances):
+        """Builds and uploads the two-level hierarchy (mgk_set_meshes): meshes
+        = [(vertices (n, 3), indices (m, 3))], instances = [(mesh, matter,
+        3 x 4 object -> world)].  Empty lists clear it."""
+        m, inst, _keep = mesh.to_abi(meshes, instances)
+        abi.check(self.lib.mgk_set_meshes(self.h, m, len(meshes), inst, len(instances)), "mgk_set_meshes")
 
     def generate_rays(self, integ, cam, seed: int, sample_begin: int, spp: int):
         import torch

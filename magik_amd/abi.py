@@ -30,11 +30,14 @@ MASSIVE, MASSLESS = 0, 1
 
 PREC_FP32_REFERENCE, PREC_MIXED, PREC_FP64 = 0, 1, 2
 
-HIT_NONE, HIT_HORIZON, HIT_ESCAPE, HIT_DISK, HIT_SPHERE, HIT_STEP_LIMIT, HIT_BOX_DIFFUSE, HIT_BOX_EMITTER = range(8)
+(HIT_NONE, HIT_HORIZON, HIT_ESCAPE, HIT_DISK, HIT_SPHERE, HIT_STEP_LIMIT, HIT_BOX_DIFFUSE, HIT_BOX_EMITTER,
+ HIT_MESH_DIFFUSE, HIT_MESH_EMITTER) = range(10)
 HIT_NAMES = {0: "none", 1: "horizon", 2: "escape", 3: "disk", 4: "sphere", 5: "step_limit", 6: "box_diffuse",
-             7: "box_emitter"}
+             7: "box_emitter", 8: "mesh_diffuse", 9: "mesh_emitter"}
 
-SCENE_DISK, SCENE_SPHERE, SCENE_STARS, SCENE_BACKGROUND, SCENE_BOXES = 1, 2, 4, 8, 16
+SCENE_DISK, SCENE_SPHERE, SCENE_STARS, SCENE_BACKGROUND, SCENE_BOXES, SCENE_MESHES = 1, 2, 4, 8, 16, 32
+CHORD_AFFINE, CHORD_HERMITE = 0, 1
+MAX_MESH_TRIANGLES = MAX_INSTANCES = 16777215
 MATTER_DIFFUSE, MATTER_ILLUMINANT = 1, 2
 
 
@@ -92,7 +95,16 @@ class Scene(C.Structure):  # mgk_scene_t
                 ("background_scale", C.c_float), ("background_T", C.c_float), ("star_scale", C.c_float),
                 ("star_T", C.c_float), ("star_radius_rad", C.c_float), ("n_star", C.c_uint32),
                 ("star_seed", C.c_uint64), ("n_box", C.c_uint32), ("max_reflections", C.c_uint32),
-                ("led_scale", C.c_float), ("reserved2", C.c_uint32), ("box", Box * MAX_BOX)]
+                ("led_scale", C.c_float), ("chord", C.c_uint32), ("box", Box * MAX_BOX)]
+
+
+class Mesh(C.Structure):  # mgk_mesh_t
+    _fields_ = [("vertices", C.POINTER(C.c_float)), ("n_vertex", C.c_uint32), ("n_triangle", C.c_uint32),
+                ("indices", C.POINTER(C.c_uint32))]
+
+
+class Instance(C.Structure):  # mgk_instance_t
+    _fields_ = [("mesh", C.c_uint32), ("matter", C.c_uint32), ("object_to_world", C.c_float * 12)]
 
 
 class Spectral(C.Structure):  # mgk_spectral_t
@@ -197,6 +209,7 @@ SIGNATURES = {
     "mgk_context_create": (_R, [C.c_int, C.POINTER(_P)]),
     "mgk_context_destroy": (_R, [_P]),
     "mgk_set_scene": (_R, [_P, C.POINTER(Scene)]),
+    "mgk_set_meshes": (_R, [_P, C.POINTER(Mesh), C.c_uint32, C.POINTER(Instance), C.c_uint32]),
     "mgk_generate_rays": (_R, [_P, C.POINTER(Integrator), C.POINTER(Camera), C.c_uint64, C.c_uint32, C.c_uint32, _P,
                                C.c_uint64, _P]),
     "mgk_trace_rays": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(C.c_float), _P, C.c_uint64,

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmagik_amd.so")
 SOURCES = ["mgk_kernels.hip", "mgk_api.hip", "mgk_mlp.hip", "mgk_surrogate.hip", "mgk_surrogate_frame.hip", "mgk_display.hip",
-           "mgk_manager.hip"]
+           "mgk_manager.hip", "mgk_mesh.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-std=c++20", "-O3", "--offload-arch=gfx950", "-fPIC", "-shared",

@@ -96,6 +96,11 @@ struct mgk_context {
     mgk_display_t disp_key{};
     disp::Params disp_params{};
     bool disp_valid = false;
+    // triangle meshes (mgk_set_meshes): the two-level hierarchy
+    DeviceBuffer mesh_nodes, mesh_tris, mesh_normals, mesh_inst;
+    uint32_t mesh_root = 0;
+    float mesh_lo[3] = {0, 0, 0}, mesh_hi[3] = {0, 0, 0};
+    bool mesh_valid = false, mesh_diffuse = false;
 };
 
 namespace {
@@ -187,6 +192,7 @@ void fill_trace_params(const mgk_integrator_t* in, const mgk_scene_t* sc, const 
             P.box_matter[b] = sc->box[b].matter;
         }
     }
+    P.chord = sc->chord;
 }
 
 e_magik_result_types check_scene(const mgk_scene_t* sc) {
@@ -196,10 +202,12 @@ e_magik_result_types check_scene(const mgk_scene_t* sc) {
         for (uint32_t b = 0; b < sc->n_box; ++b)
             if (sc->box[b].matter != MGK_MATTER_DIFFUSE && sc->box[b].matter != MGK_MATTER_ILLUMINANT) return MGK_ERROR_INVALID_ARGUMENT;
     }
+    if (sc->chord != MGK_CHORD_AFFINE && sc->chord != MGK_CHORD_HERMITE) return MGK_ERROR_INVALID_ARGUMENT;
     return MAGIK_SUCCESS;
 }
 
-bool has_diffuse_box(const TraceParams& P) {
+bool has_diffuse_matter(const mgk_context* c, const TraceParams& P) {
+    if ((P.flags & MGK_SCENE_MESHES) && c->mesh_diffuse) return true;
     if (!(P.flags & MGK_SCENE_BOXES)) return false;
     for (uint32_t b = 0; b < P.n_box; ++b)
         if (P.box_matter[b] == MGK_MATTER_DIFFUSE) return true;
@@ -299,13 +307,21 @@ struct PathInfo {
 // conditions in place in P.rays.
 e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, const PathInfo* paths = nullptr,
                               uint32_t max_reflections = 0) {
+    if (P.flags & MGK_SCENE_MESHES) {
+        if (!c->mesh_valid) return MGK_ERROR_INVALID_ARGUMENT;   // no meshes set on this context
+        P.mesh_nodes = reinterpret_cast<const float4*>(c->mesh_nodes.ptr);
+        P.mesh_tris = reinterpret_cast<const float4*>(c->mesh_tris.ptr);
+        P.mesh_inst = reinterpret_cast<const float4*>(c->mesh_inst.ptr);
+        P.mesh_root = c->mesh_root;
+        for (int k = 0; k < 3; ++k) { P.mesh_lo[k] = c->mesh_lo[k]; P.mesh_hi[k] = c->mesh_hi[k]; }
+    }
     P.head = c->d_counters;
     P.stats = c->d_counters + 1;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c->d_counters, 4);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev0, s));
     c->last_reflections = 0;
-    const bool reflect = paths && max_reflections > 0 && P.n > 0 && has_diffuse_box(P);
+    const bool reflect = paths && max_reflections > 0 && P.n > 0 && has_diffuse_matter(c, P);
     if (reflect) {
         if (P.n >= uint64_t(kHole)) return MGK_ERROR_CAPACITY_EXCEEDED;
         HIP_TRY(c->refl_q[0].reserve(P.n * sizeof(uint32_t)));
@@ -343,6 +359,8 @@ e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, con
             R.block = 1 + k;
             R.spin = P.spin;
             R.mass_sign = P.mass_sign;
+            R.mesh_inst = P.mesh_inst;
+            R.mesh_normals = reinterpret_cast<const float4*>(c->mesh_normals.ptr);
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, R.count, 1);
             HIP_TRY(hipGetLastError());
             HIP_TRY(launch_reflect(R, P.precision == MGK_PRECISION_FP32_REFERENCE, s));
@@ -779,6 +797,10 @@ MGK_API e_magik_result_types mgk_context_destroy(mgk_context_handle c) {
     c->batch_params.release();
     c->ic_camera.release();
     c->shade_resp.release();
+    c->mesh_nodes.release();
+    c->mesh_tris.release();
+    c->mesh_normals.release();
+    c->mesh_inst.release();
     for (int k = 0; k < 2; ++k) {
         c->zone_state[k].release();
         c->zone_steps[k].release();
@@ -794,6 +816,31 @@ MGK_API e_magik_result_types mgk_set_scene(mgk_context_handle c, const mgk_scene
     e_magik_result_types r = set_device(c);
     if (r != MAGIK_SUCCESS) return r;
     return ensure_stars(c, sc);
+}
+
+MGK_API e_magik_result_types mgk_set_meshes(mgk_context_handle c, const mgk_mesh_t* meshes, uint32_t n_mesh,
+                                            const mgk_instance_t* instances, uint32_t n_instance) {
+    if (!c) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = set_device(c);
+    if (r != MAGIK_SUCCESS) return r;
+    MeshBuild b;
+    if ((r = build_meshes(meshes, n_mesh, instances, n_instance, b)) != MAGIK_SUCCESS) return r;
+    c->mesh_valid = false;
+    c->mesh_diffuse = false;
+    if (n_mesh == 0) return MAGIK_SUCCESS;
+    HIP_TRY(hipDeviceSynchronize());   // earlier traces may still read the old hierarchy
+    struct Up { DeviceBuffer* buf; const std::vector<float4>* v; } ups[4] = {
+        {&c->mesh_nodes, &b.nodes}, {&c->mesh_tris, &b.tris}, {&c->mesh_normals, &b.normals}, {&c->mesh_inst, &b.inst}};
+    for (const Up& u : ups) {
+        const size_t bytes = u.v->size() * sizeof(float4);
+        HIP_TRY(u.buf->reserve(bytes));
+        HIP_TRY(hipMemcpy(u.buf->ptr, u.v->data(), bytes, hipMemcpyHostToDevice));
+    }
+    c->mesh_root = b.root;
+    for (int k = 0; k < 3; ++k) { c->mesh_lo[k] = b.lo[k]; c->mesh_hi[k] = b.hi[k]; }
+    c->mesh_diffuse = b.any_diffuse;
+    c->mesh_valid = true;
+    return MAGIK_SUCCESS;
 }
 
 MGK_API e_magik_result_types mgk_generate_rays(mgk_context_handle c, const mgk_integrator_t* in, const mgk_camera_t* cam, uint64_t seed,

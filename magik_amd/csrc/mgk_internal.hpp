@@ -3,6 +3,7 @@
 #pragma once
 
 #include <cmath>
+#include <vector>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -41,6 +42,13 @@ struct IcCamera {
     double w[4];      // u_obs . p = sum_k w[k] l_k
 };
 
+// Triangle meshes (mgk_mesh.hpp): hierarchy sentinels and record sizes.
+constexpr uint32_t kBvhEnd = 0xFFFFFFFFu;   // `skip` past the last node of a hierarchy
+constexpr uint32_t kLeafShift = 24;         // leaf info = (count << 24) | first
+constexpr uint32_t kLeafMask = 0xFFFFFFu;
+constexpr int kInstF4 = 6;                  // float4 per instance record
+constexpr int kHermiteSegments = 4;         // straight sub-chords that follow a Hermite chord
+
 struct TraceParams {
     int metric, precision;
     const void* rays;
@@ -72,6 +80,13 @@ struct TraceParams {
     uint32_t n_box;              // axis-aligned boxes (SI metres) and their matter
     float box_min[MGK_MAX_BOX][3], box_max[MGK_MAX_BOX][3];
     uint32_t box_matter[MGK_MAX_BOX];
+    // triangle meshes (MGK_SCENE_MESHES): the two-level hierarchy, its world bounds, the chord mode
+    const float4* mesh_nodes;
+    const float4* mesh_tris;
+    const float4* mesh_inst;
+    uint32_t mesh_root;          // top-level root node
+    uint32_t chord;              // mgk_chord_t
+    float mesh_lo[3], mesh_hi[3];
 };
 
 // Reflection of the paths that ended on diffuse box matter: new initial
@@ -90,6 +105,8 @@ struct ReflectParams {
     uint64_t seed;
     uint32_t camera_id, npix, sample_begin, block;   // Philox stream of reflection k: block 1 + k
     float spin, mass_sign;
+    const float4* mesh_inst;         // mesh hits: instance records and object-space unit normals
+    const float4* mesh_normals;
 };
 
 // Doubles parked per handed-off ray: q (4), p (4), h, u.p, L0, Q0, and the
@@ -197,6 +214,19 @@ inline void rotate_matrix(float a, float b, float g, float out[9]) {
         for (int j = 0; j < 3; ++j) out[3 * i + j] = t[3 * i] * mz[j] + t[3 * i + 1] * mz[3 + j] + t[3 * i + 2] * mz[6 + j];
 }
 hipError_t launch_geodesic_steps(const GeodesicStepParams& p, hipStream_t s);
+
+
+// Host build of the mesh hierarchy (mgk_mesh.hip): nodes, triangle slots,
+// normals and instance records as laid out in mgk_mesh.hpp, all meshes'
+// bottom levels first, then the top level.
+struct MeshBuild {
+    std::vector<float4> nodes, tris, normals, inst;
+    uint32_t root = kBvhEnd;
+    float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    bool any_diffuse = false;
+};
+e_magik_result_types build_meshes(const mgk_mesh_t* meshes, uint32_t n_mesh, const mgk_instance_t* inst, uint32_t n_inst,
+                                  MeshBuild& out);
 
 }  // namespace mgk
 

@@ -21,6 +21,7 @@
 //   k_geodesic_steps : magik_trace_geodesic's per-step records for one ray.
 #include "mgk_device.hpp"
 #include "mgk_internal.hpp"
+#include "mgk_mesh.hpp"
 
 namespace mgk {
 
@@ -386,17 +387,26 @@ template <int METRIC, class S> __device__ __forceinline__ void lane_constants(co
 // integration loop ~45 VGPRs.
 template <int METRIC, class S>
 __device__ __forceinline__ void hit_record(const St<S>& y0, const St<S>& y1, uint32_t cls, S best, S r_hit, S contraction,
-                                        const TraceParams& P, S a, int box_n, mgk_hit_t& rec) {
+                                        const TraceParams& P, S a, int box_n, const S hq[3], uint32_t m_tri, uint32_t m_inst,
+                                        mgk_hit_t& rec) {
 #pragma clang fp contract(off)
     rec.cls = cls;
     rec.g = 0.0f; rec.a0 = 0.0f; rec.a1 = 0.0f;
-    if (cls == MGK_HIT_BOX_DIFFUSE || cls == MGK_HIT_BOX_EMITTER) {
-        // a0: the face normal's code, a1: the chord parameter (kernel_intersect.cu:125-132)
-        rec.a0 = float(box_n);
-        rec.a1 = float(best);
-        if (cls == MGK_HIT_BOX_EMITTER) {
-            // matter at rest: the reference observer at the hit point, the momentum of the step's end
-            const S hx = y0.qx + best * (y1.qx - y0.qx), hy = y0.qy + best * (y1.qy - y0.qy), hz = y0.qz + best * (y1.qz - y0.qz);
+    if (cls >= MGK_HIT_BOX_DIFFUSE) {
+        // boxes: a0 the face normal's code, a1 the chord parameter (kernel_intersect.cu:125-132);
+        // meshes: a0 the triangle's index in its mesh, a1 the instance
+        const bool mesh = cls >= MGK_HIT_MESH_DIFFUSE;
+        rec.a0 = mesh ? float(m_tri) : float(box_n);
+        rec.a1 = mesh ? float(m_inst) : float(best);
+        if (cls == MGK_HIT_BOX_EMITTER || cls == MGK_HIT_MESH_EMITTER) {
+            // matter at rest: the reference observer at the hit point (boxes: on the chord), the
+            // momentum of the step's end
+            S hx, hy, hz;
+            if (mesh) {
+                hx = hq[0]; hy = hq[1]; hz = hq[2];
+            } else {
+                hx = y0.qx + best * (y1.qx - y0.qx); hy = y0.qy + best * (y1.qy - y0.qy); hz = y0.qz + best * (y1.qz - y0.qz);
+            }
             const V4<S> u = reference_observer<METRIC, S>(hx, hy, hz, a, S(P.mass_sign));
             S g = contraction / dot4(u, V4<S>{y1.px, y1.py, y1.pz, y1.pt});
             if (isnan(g) || isinf(g)) g = S(1e-10f);
@@ -483,15 +493,16 @@ __device__ __forceinline__ bool box_chord(T ox, T oy, T oz, T dx, T dy, T dz, T 
     return true;
 }
 
-template <int METRIC, class S, bool FAST_TESTS = false>
+template <int METRIC, class S, bool FAST_TESTS = false, bool MESH = false>
 __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, uint32_t n_step, S contraction, const TraceParams& P,
-                                             const Lane<S>& ln, mgk_hit_t& rec, S& t_hit) {
+                                             const Lane<S>& ln, float dl, mgk_hit_t& rec, S& t_hit, S hq[3]) {
 #pragma clang fp contract(off)
     const S a = METRIC == KERR ? ln.a : S(0);
     S best = S(1e30f);
     uint32_t cls = MGK_HIT_NONE;
     S r_hit = S(0);
     int box_n = 0;
+    uint32_t m_tri = 0, m_inst = 0;
     if (P.flags & MGK_SCENE_DISK) {
         S ya = y0.qy, yb = y1.qy;
         if ((ya > S(0) && yb <= S(0)) || (ya < S(0) && yb >= S(0))) {
@@ -536,6 +547,33 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
             }
         }
     }
+    if constexpr (MESH) {
+        // the chord in SI metres; FP32 in the zone kernels
+        using T = typename std::conditional<FAST_TESTS, float, S>::type;
+        const T L = T(P.length_scale);
+        const T p0[3] = {T(y0.qx) * L, T(y0.qy) * L, T(y0.qz) * L}, p1[3] = {T(y1.qx) * L, T(y1.qy) * L, T(y1.qz) * L};
+        T D0[3] = {T(0), T(0), T(0)}, D1[3] = {T(0), T(0), T(0)};
+        const bool herm = P.chord == MGK_CHORD_HERMITE;
+        if (herm) {
+            // end tangents dx/dlambda = g^{mu nu} p_nu, times the step's parameter length, in metres
+            const T ms = T(P.mass_sign), at = T(a), k = T(dl) * L;
+            const V4<T> v0 = lower(metric_g<METRIC, T>(T(y0.qx), T(y0.qy), T(y0.qz), at, ms, true),
+                                   V4<T>{T(y0.px), T(y0.py), T(y0.pz), T(y0.pt)});
+            const V4<T> v1 = lower(metric_g<METRIC, T>(T(y1.qx), T(y1.qy), T(y1.qz), at, ms, true),
+                                   V4<T>{T(y1.px), T(y1.py), T(y1.pz), T(y1.pt)});
+            D0[0] = v0.x * k; D0[1] = v0.y * k; D0[2] = v0.z * k;
+            D1[0] = v1.x * k; D1[1] = v1.y * k; D1[2] = v1.z * k;
+        }
+        MeshHit<T> mh;
+        if (mesh_chord<T>(P, p0, p1, D0, D1, herm, T(min_(best, S(1))), mh)) {
+            best = S(mh.s);
+            cls = mh.matter == MGK_MATTER_DIFFUSE ? MGK_HIT_MESH_DIFFUSE : MGK_HIT_MESH_EMITTER;
+            m_tri = mh.tri;
+            m_inst = mh.inst;
+            const S iL = S(1) / S(P.length_scale);
+            hq[0] = S(mh.x[0]) * iL; hq[1] = S(mh.x[1]) * iL; hq[2] = S(mh.x[2]) * iL;
+        }
+    }
     if constexpr (FAST_TESTS) {
         const float L = P.length_scale;
         const float ex = float(y1.qx) * L - float(ln.cx), ey = float(y1.qy) * L - float(ln.cy), ez = float(y1.qz) * L - float(ln.cz);
@@ -569,7 +607,7 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
     }
     if (cls == MGK_HIT_NONE && n_step > P.step_limit) cls = MGK_HIT_STEP_LIMIT;
     if (cls == MGK_HIT_NONE) return cls;
-    hit_record<METRIC, S>(y0, y1, cls, best, r_hit, contraction, P, a, box_n, rec);
+    hit_record<METRIC, S>(y0, y1, cls, best, r_hit, contraction, P, a, box_n, hq, m_tri, m_inst, rec);
     t_hit = best;
     return cls;
 }
@@ -581,19 +619,27 @@ template <class S> __device__ __forceinline__ void write_final(const TraceParams
     f[4 * sd + ray] = double(y.px); f[5 * sd + ray] = double(y.py); f[6 * sd + ray] = double(y.pz); f[7 * sd + ray] = double(y.pt);
 }
 
-// Terminal state of a path: the step's end, or for box matter the chord point
-// of the hit with the step's end momentum (kernel_intersect.cu:129-131).
+// Terminal state of a path: the step's end, or for box and mesh matter the hit
+// point (on the chord, or on the Hermite curve) with the step's end momentum
+// (kernel_intersect.cu:129-131).
 template <class S>
-__device__ __forceinline__ void write_terminal(const TraceParams& P, uint64_t ray, uint32_t cls, const St<S>& y0, const St<S>& y1, S t) {
+__device__ __forceinline__ void write_terminal(const TraceParams& P, uint64_t ray, uint32_t cls, const St<S>& y0, const St<S>& y1, S t,
+                                               const S hq[3]) {
 #pragma clang fp contract(off)
-    if (cls != MGK_HIT_BOX_DIFFUSE && cls != MGK_HIT_BOX_EMITTER) {
+    if (cls < MGK_HIT_BOX_DIFFUSE) {
         write_final(P, ray, y1);
         return;
     }
     St<S> h = y1;
-    h.qx = y0.qx + t * (y1.qx - y0.qx);
-    h.qy = y0.qy + t * (y1.qy - y0.qy);
-    h.qz = y0.qz + t * (y1.qz - y0.qz);
+    if (cls >= MGK_HIT_MESH_DIFFUSE) {
+        h.qx = hq[0];
+        h.qy = hq[1];
+        h.qz = hq[2];
+    } else {
+        h.qx = y0.qx + t * (y1.qx - y0.qx);
+        h.qy = y0.qy + t * (y1.qy - y0.qy);
+        h.qz = y0.qz + t * (y1.qz - y0.qz);
+    }
     h.qt = y0.qt + t * (y1.qt - y0.qt);
     write_final(P, ray, h);
 }
@@ -615,7 +661,9 @@ constexpr uint32_t kRenormEvery = 4;
 
 // Persistent integrator.  T is the storage / state type: float for the FP32
 // reference path, double for the mixed and FP64 paths.
-template <int METRIC, int PREC>
+// MESH: instantiation for scenes with triangle meshes (the mesh test stays out
+// of the register allocation of the others).
+template <int METRIC, int PREC, bool MESH>
 __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
     using S = typename std::conditional<PREC == MGK_PRECISION_FP32_REFERENCE, float, double>::type;
     const uint32_t lane = __lane_id();
@@ -682,6 +730,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
 
         // ---- one trial ----
         St<S> y1;
+        const float dl = float(h);   // parameter length of this trial's step (Hermite chords)
         bool step_done;
         if constexpr (METRIC == MINKOWSKI) {
             // explicit Euler with the stored h (metric_minkowski.cuh:49-53)
@@ -729,11 +778,12 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
         if constexpr (PREC == MGK_PRECISION_MIXED) renormalise_null<METRIC>(y1, double(ln.a), double(P.mass_sign));
         mgk_hit_t rec;
         S t_hit = S(0);
-        uint32_t cls = classify<METRIC, S>(y, y1, n_step, contraction, P, ln, rec, t_hit);
+        S hq[3];
+        uint32_t cls = classify<METRIC, S, false, MESH>(y, y1, n_step, contraction, P, ln, dl, rec, t_hit, hq);
         if (cls != MGK_HIT_NONE) {
             P.hits[ray] = rec;
             if (P.steps_out) P.steps_out[ray] = n_step;
-            if (P.final_state) write_terminal(P, ray, cls, y, y1, t_hit);
+            if (P.final_state) write_terminal(P, ray, cls, y, y1, t_hit, hq);
             active = false;
         } else {
             y = y1;
@@ -761,7 +811,7 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
 // alternates the two until both queues are empty.  A ray's arithmetic thus
 // depends on its own position only, never on the rays sharing its wave.
 // ---------------------------------------------------------------------------
-template <int METRIC, bool NEAR>
+template <int METRIC, bool NEAR, bool MESH>
 __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P, ZoneParams Z) {
     using S = double;
     const uint32_t lane = __lane_id();
@@ -891,6 +941,7 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
         // ---- one trial ----
         St<S> y1;
         double rms;
+        const float dl = float(h);   // parameter length of this trial's step (Hermite chords)
         if constexpr (NEAR) {
             trial_split<METRIC, double, true>(y, h, float(ln.a), P.mass_sign, P, y1, rms);
             ++fp64_acc;
@@ -935,11 +986,12 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
         }
         mgk_hit_t rec;
         S t_hit = S(0);
-        const uint32_t cls = classify<METRIC, S, true>(y, y1, n_step, contraction, P, ln, rec, t_hit);
+        S hq[3];
+        const uint32_t cls = classify<METRIC, S, true, MESH>(y, y1, n_step, contraction, P, ln, dl, rec, t_hit, hq);
         if (cls != MGK_HIT_NONE) {
             P.hits[ray] = rec;
             if (P.steps_out) P.steps_out[ray] = n_step;
-            if (P.final_state) write_terminal(P, ray, cls, y, y1, t_hit);
+            if (P.final_state) write_terminal(P, ray, cls, y, y1, t_hit, hq);
             active = false;
         } else {
             y = y1;
@@ -1170,7 +1222,9 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(ShadeParams P) {
     float X = 0.0f, Yv = 0.0f, Zv = 0.0f;
     for (uint32_t s = 0; s < P.spp; ++s) {
         mgk_hit_t h = P.hits[uint64_t(s) * npix + pixel];
-        if (h.cls != MGK_HIT_DISK && h.cls != MGK_HIT_SPHERE && h.cls != MGK_HIT_ESCAPE && h.cls != MGK_HIT_BOX_EMITTER) continue;
+        if (h.cls != MGK_HIT_DISK && h.cls != MGK_HIT_SPHERE && h.cls != MGK_HIT_ESCAPE && h.cls != MGK_HIT_BOX_EMITTER &&
+            h.cls != MGK_HIT_MESH_EMITTER)
+            continue;
         float u[4];
         path_uniforms(P.seed, P.camera_id, pixel, P.sample_begin + s, 0, u);
         const float g = h.g, g5 = g * g * g * g * g;
@@ -1199,7 +1253,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(ShadeParams P) {
                 L = 0.0f;
                 if (P.flags & MGK_SCENE_BACKGROUND) L += P.background_scale * planck(le, P.background_T);
                 if (star > 0.0f) L += star * planck(le, P.star_T);
-            } else if (h.cls == MGK_HIT_BOX_EMITTER) {
+            } else if (h.cls == MGK_HIT_BOX_EMITTER || h.cls == MGK_HIT_MESH_EMITTER) {
                 L = P.led_scale * white_led(le);
             } else {
                 L = emit_scale > 0.0f ? planck(le, T) : 0.0f;
@@ -1271,7 +1325,8 @@ template <int METRIC, class S> __global__ void __launch_bounds__(256) k_reflect(
     bool go = false;
     if (i < P.n_in) {
         ray = P.queue_in ? uint64_t(P.queue_in[i]) : i;
-        go = P.hits[ray].cls == MGK_HIT_BOX_DIFFUSE;
+        const uint32_t c = P.hits[ray].cls;
+        go = c == MGK_HIT_BOX_DIFFUSE || c == MGK_HIT_MESH_DIFFUSE;
     }
     if (go) {
         const uint64_t sd = P.stride;
@@ -1292,11 +1347,30 @@ template <int METRIC, class S> __global__ void __launch_bounds__(256) k_reflect(
         norm3(dx, dy, dz);
         S z = contraction / dot4(u, pin);
         if (isnan(z) || isinf(z)) z = S(1e-10f);
-        // geometric normal into the same frame, turned against the incoming direction (:101-104)
-        const int code = int(P.hits[ray].a0);
-        const int axis = code % 3;
-        const S ns = code >= 3 ? S(-1) : S(1);
-        const V4<S> ng{axis == 0 ? ns : S(0), axis == 1 ? ns : S(0), axis == 2 ? ns : S(0), S(0)};
+        // geometric normal into the same frame, turned against the incoming direction (:101-104).
+        // Boxes: the face's axis from the hit's code, against the chord already.  Meshes: the
+        // triangle's normal through the instance's normal map, turned against the incoming
+        // coordinate velocity; mesh matter is Lambertian on every face.
+        const mgk_hit_t hr = P.hits[ray];
+        int axis;
+        V4<S> ng;
+        if (hr.cls == MGK_HIT_MESH_DIFFUSE) {
+            const float4* I = P.mesh_inst + uint64_t(kInstF4) * uint32_t(hr.a1);
+            const float4 n3 = I[3], n4 = I[4], n5 = I[5];
+            const float4 no = P.mesh_normals[__float_as_uint(n4.w) + uint32_t(hr.a0)];
+            S wx = S(n3.x) * S(no.x) + S(n3.y) * S(no.y) + S(n3.z) * S(no.z);
+            S wy = S(n4.x) * S(no.x) + S(n4.y) * S(no.y) + S(n4.z) * S(no.z);
+            S wz = S(n5.x) * S(no.x) + S(n5.y) * S(no.y) + S(n5.z) * S(no.z);
+            norm3(wx, wy, wz);
+            if (wx * pup.x + wy * pup.y + wz * pup.z > S(0)) { wx = -wx; wy = -wy; wz = -wz; }
+            ng = V4<S>{wx, wy, wz, S(0)};
+            axis = -1;
+        } else {
+            const int code = int(hr.a0);
+            axis = code % 3;
+            const S ns = code >= 3 ? S(-1) : S(1);
+            ng = V4<S>{axis == 0 ? ns : S(0), axis == 1 ? ns : S(0), axis == 2 ? ns : S(0), S(0)};
+        }
         const V4<S> nup = lower(gi, ng);
         S nx = inner(g, ex, nup), ny = inner(g, ey, nup), nz = inner(g, ez, nup);
         norm3(nx, ny, nz);
@@ -1480,31 +1554,41 @@ hipError_t launch_generate_rays(const RayGenParams& p, IcCamera* d_ic, hipStream
     });
 }
 
-hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s) {
+template <bool MESH> hipError_t launch_trace_t(const TraceParams& p, int grid, hipStream_t s) {
     return dispatch_metric<tag_t>(p.metric, [&](auto m) -> hipError_t {
         constexpr int M = decltype(m)::value;
         switch (p.precision) {
             case MGK_PRECISION_FP32_REFERENCE:
-                hipLaunchKernelGGL((k_trace<M, MGK_PRECISION_FP32_REFERENCE>), dim3(grid), dim3(256), 0, s, p);
+                hipLaunchKernelGGL((k_trace<M, MGK_PRECISION_FP32_REFERENCE, MESH>), dim3(grid), dim3(256), 0, s, p);
                 break;
             case MGK_PRECISION_MIXED:
-                hipLaunchKernelGGL((k_trace<M, MGK_PRECISION_MIXED>), dim3(grid), dim3(256), 0, s, p);
+                hipLaunchKernelGGL((k_trace<M, MGK_PRECISION_MIXED, MESH>), dim3(grid), dim3(256), 0, s, p);
                 break;
             default:
-                hipLaunchKernelGGL((k_trace<M, MGK_PRECISION_FP64>), dim3(grid), dim3(256), 0, s, p);
+                hipLaunchKernelGGL((k_trace<M, MGK_PRECISION_FP64, MESH>), dim3(grid), dim3(256), 0, s, p);
                 break;
         }
         return hipGetLastError();
     });
 }
 
-hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s) {
+hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s) {
+    return (p.flags & MGK_SCENE_MESHES) ? launch_trace_t<true>(p, grid, s) : launch_trace_t<false>(p, grid, s);
+}
+
+template <bool MESH>
+hipError_t launch_trace_zone_t(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s) {
     return dispatch_metric<tag_t>(p.metric, [&](auto m) -> hipError_t {
         constexpr int M = decltype(m)::value;
-        if (near) hipLaunchKernelGGL((k_trace_zone<M, true>), dim3(grid), dim3(256), 0, s, p, z);
-        else hipLaunchKernelGGL((k_trace_zone<M, false>), dim3(grid), dim3(256), 0, s, p, z);
+        if (near) hipLaunchKernelGGL((k_trace_zone<M, true, MESH>), dim3(grid), dim3(256), 0, s, p, z);
+        else hipLaunchKernelGGL((k_trace_zone<M, false, MESH>), dim3(grid), dim3(256), 0, s, p, z);
         return hipGetLastError();
     });
+}
+
+hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s) {
+    return (p.flags & MGK_SCENE_MESHES) ? launch_trace_zone_t<true>(p, z, near, grid, s)
+                                        : launch_trace_zone_t<false>(p, z, near, grid, s);
 }
 
 hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t* d_offsets, uint32_t n_cam, uint64_t n, IcCamera* d_ic,

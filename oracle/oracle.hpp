@@ -533,9 +533,29 @@ inline RayIC<R> initial_conditions(const Camera<R>& cam, const MetricParams<R>& 
 // ---------------------------------------------------------------------------
 enum HitClass : uint32_t {
     HIT_NONE = 0, HIT_HORIZON = 1, HIT_ESCAPE = 2, HIT_DISK = 3, HIT_SPHERE = 4, HIT_STEP_LIMIT = 5,
-    HIT_BOX_DIFFUSE = 6, HIT_BOX_EMITTER = 7
+    HIT_BOX_DIFFUSE = 6, HIT_BOX_EMITTER = 7, HIT_MESH_DIFFUSE = 8, HIT_MESH_EMITTER = 9
 };
 enum Matter : uint32_t { MATTER_DIFFUSE = 1, MATTER_ILLUMINANT = 2 };   // kernel_bxdf.cu:126 slots 1 and 2
+
+// Triangle meshes placed by affine instances.  The reference reserves mesh
+// intersection (wavefront.cpp:94-100) without implementing it; this is the
+// restatement of the library's design: two-sided triangles (Moller &
+// Trumbore, JGT 1997) against the step's chord taken into each instance's
+// object space.  No hierarchy: every instance and triangle is tested, which
+// is what the library's top and bottom levels must reproduce.  The world ->
+// object maps, normal maps and triangles are held as the library stores them
+// (float), then widened to R.
+template <class R> struct MeshSet {
+    struct Tri { V3<R> v0, e1, e2, n; };   // object space; n the unit geometric normal
+    struct Inst {
+        uint32_t mesh, matter;
+        R w2o[12];                         // world -> object, row-major 3 x 4
+        R nm[9];                           // object -> world normals: the transposed linear part of w2o
+        V3<R> lo, hi;                      // object-space bounds of the mesh (a cull that drops no hit)
+    };
+    std::vector<std::vector<Tri>> mesh;
+    std::vector<Inst> inst;
+};
 
 template <class R> struct Scene {
     R celestial_radius_m = R(1000);   // measured from the camera origin, SI metres
@@ -553,6 +573,8 @@ template <class R> struct Scene {
     Box box[4];
     uint32_t max_reflections = 10;   // render_settings.h:17
     R led_scale = R(7e5f);           // matter_examples.cuh:70
+    const MeshSet<R>* meshes = nullptr;
+    bool hermite = false;            // Hermite chords for the meshes (else the straight chord)
 };
 
 template <class R> struct Hit {
@@ -644,10 +666,11 @@ template <class R> inline V4<R> offset_geodesic(V4<R> q, V3<R> n) {
 // specularly, the others send (p + n) / |p + n| for p uniform on the sphere
 // (cosine-weighted about the facing normal n).  Returns the new state (offset
 // position, outgoing covariant momentum) and the new u.p contraction, which
-// carries the redshift accumulated so far.
+// carries the redshift accumulated so far.  reflect_matter takes the global
+// geometric normal ng and the specular choice; mesh matter is Lambertian.
 template <class R>
-inline void reflect_diffuse(const MetricParams<R>& p, const State<R>& at, R contraction, int normal_code, R u0, R u1, State<R>* out,
-                            R* contraction_out) {
+inline void reflect_matter(const MetricParams<R>& p, const State<R>& at, R contraction, V3<R> ng, bool specular, R u0, R u1,
+                           State<R>* out, R* contraction_out) {
     const Seeds<R> seeds;
     const V4<R> q = at.q;
     const V4<R> u = rest_frame_four_velocity(q, seeds, V3<R>{R(0), R(0), R(0)}, p);
@@ -655,12 +678,11 @@ inline void reflect_diffuse(const MetricParams<R>& p, const State<R>& at, R cont
     const V4<R> prf = global_to_observer_rest_frame(q, seeds, contract(ginv, at.p), u, p);
     const V3<R> din = norm3(V3<R>{prf.x, prf.y, prf.z});
     const R z = redshift_factor(contraction, u, at.p);
-    const V3<R> ng = normal_of<R>(normal_code);
     const V4<R> nrf = global_to_observer_rest_frame(q, seeds, contract(ginv, V4<R>{ng.x, ng.y, ng.z, R(0)}), u, p);
     V3<R> nr = norm3(V3<R>{nrf.x, nrf.y, nrf.z});
     if (dot3(nr, din) > R(0)) nr = -nr;
     V3<R> dout;
-    if (ng.x * ng.x > R(0)) {
+    if (specular) {
         dout = norm3(din - (R(2) * nr * dot3(din, nr)));
     } else {
         const R th = std::acos(R(1) - R(2) * u0), ph = R(2) * R(kPi) * u1;
@@ -671,6 +693,14 @@ inline void reflect_diffuse(const MetricParams<R>& p, const State<R>& at, R cont
     out->q = offset_geodesic(q, ng);
     out->p = pout;
     *contraction_out = dot4(u, pout) * z;
+}
+
+// Box matter: the face normal from its code, specular on faces with an x normal.
+template <class R>
+inline void reflect_diffuse(const MetricParams<R>& p, const State<R>& at, R contraction, int normal_code, R u0, R u1, State<R>* out,
+                            R* contraction_out) {
+    const V3<R> ng = normal_of<R>(normal_code);
+    reflect_matter(p, at, contraction, ng, ng.x * ng.x > R(0), u0, u1, out, contraction_out);
 }
 
 // Thin equatorial disk y = 0, r_in <= r <= r_out: chord crossing of the plane.
@@ -732,6 +762,136 @@ template <class R> inline R planck(R lambda_nm, R T) {
     return (t1 / t2) * (R(1) / t3);
 }
 
+// Segment o + t d (d not normalised) against a two-sided triangle: the
+// Moller-Trumbore solution of o + t d = v0 + u e1 + v e2, accepted for
+// u, v >= 0, u + v <= 1 and 0 <= t < tmax.
+template <class R> inline bool seg_tri(V3<R> o, V3<R> d, V3<R> v0, V3<R> e1, V3<R> e2, R tmax, R* t_out) {
+    const V3<R> pv{d.y * e2.z - d.z * e2.y, d.z * e2.x - d.x * e2.z, d.x * e2.y - d.y * e2.x};
+    const R det = dot3(e1, pv);
+    if (det == R(0)) return false;
+    const R inv = R(1) / det;
+    const V3<R> tv{o.x - v0.x, o.y - v0.y, o.z - v0.z};
+    const R u = dot3(tv, pv) * inv;
+    if (u < R(0) || u > R(1)) return false;
+    const V3<R> qv{tv.y * e1.z - tv.z * e1.y, tv.z * e1.x - tv.x * e1.z, tv.x * e1.y - tv.y * e1.x};
+    const R v = dot3(d, qv) * inv;
+    if (v < R(0) || u + v > R(1)) return false;
+    const R t = dot3(e2, qv) * inv;
+    if (!(t >= R(0) && t < tmax)) return false;
+    *t_out = t;
+    return true;
+}
+
+template <class R> inline V3<R> affine_point(const R m[12], V3<R> x) {
+    return {m[0] * x.x + m[1] * x.y + m[2] * x.z + m[3], m[4] * x.x + m[5] * x.y + m[6] * x.z + m[7],
+            m[8] * x.x + m[9] * x.y + m[10] * x.z + m[11]};
+}
+template <class R> inline V3<R> affine_dir(const R m[12], V3<R> x) {
+    return {m[0] * x.x + m[1] * x.y + m[2] * x.z, m[4] * x.x + m[5] * x.y + m[6] * x.z, m[8] * x.x + m[9] * x.y + m[10] * x.z};
+}
+
+// Closest triangle on o + t d, t in [0, tb), over all instances; lowers tb.
+template <class R>
+inline bool mesh_segment(const MeshSet<R>& ms, V3<R> o, V3<R> d, R* tb, uint32_t* inst, uint32_t* tri) {
+    bool found = false;
+    for (uint32_t k = 0; k < ms.inst.size(); ++k) {
+        const auto& I = ms.inst[k];
+        const V3<R> oo = affine_point(I.w2o, o), dd = affine_dir(I.w2o, d);
+        // the segment against the mesh's object-space bounds: drops only triangles it cannot meet
+        R t0 = R(0), t1 = *tb;
+        const R oc[3] = {oo.x, oo.y, oo.z}, dc[3] = {dd.x, dd.y, dd.z}, lc[3] = {I.lo.x, I.lo.y, I.lo.z}, hc[3] = {I.hi.x, I.hi.y, I.hi.z};
+        bool miss = false;
+        for (int c = 0; c < 3 && !miss; ++c) {
+            const R pad = R(1e-4) * (hc[c] - lc[c]) + R(1e-6);
+            if (dc[c] == R(0)) { miss = oc[c] < lc[c] - pad || oc[c] > hc[c] + pad; continue; }
+            R a = (lc[c] - pad - oc[c]) / dc[c], b = (hc[c] + pad - oc[c]) / dc[c];
+            if (a > b) std::swap(a, b);
+            t0 = std::max(t0, a);
+            t1 = std::min(t1, b);
+            miss = t0 > t1;
+        }
+        if (miss) continue;
+        const auto& M = ms.mesh[I.mesh];
+        for (uint32_t j = 0; j < M.size(); ++j) {
+            R t;
+            if (seg_tri(oo, dd, M[j].v0, M[j].e1, M[j].e2, *tb, &t)) { *tb = t; *inst = k; *tri = j; found = true; }
+        }
+    }
+    return found;
+}
+
+template <class R> inline V3<R> hermite3(V3<R> p0, V3<R> D0, V3<R> p1, V3<R> D1, R s) {
+    const R s2 = s * s, s3 = s2 * s;
+    const R a = R(2) * s3 - R(3) * s2 + R(1), b = s3 - R(2) * s2 + s, c = R(3) * s2 - R(2) * s3, e = s3 - s2;
+    return {a * p0.x + b * D0.x + c * p1.x + e * D1.x, a * p0.y + b * D0.y + c * p1.y + e * D1.y, a * p0.z + b * D0.z + c * p1.z + e * D1.z};
+}
+template <class R> inline V3<R> hermite3_d(V3<R> p0, V3<R> D0, V3<R> p1, V3<R> D1, R s) {
+    const R s2 = s * s;
+    const R a = R(6) * s2 - R(6) * s, b = R(3) * s2 - R(4) * s + R(1), e = R(3) * s2 - R(2) * s;
+    return {a * (p0.x - p1.x) + b * D0.x + e * D1.x, a * (p0.y - p1.y) + b * D0.y + e * D1.y, a * (p0.z - p1.z) + b * D0.z + e * D1.z};
+}
+
+constexpr int kHermiteSegments = 4;
+
+// The chord of one step against the meshes, hits below s_max.  p0, p1 the end
+// points and D0, D1 the end tangents times the step's parameter length (world
+// SI).  Straight chord: one segment.  Hermite chord: kHermiteSegments straight
+// sub-chords through points of the cubic; the first that hits is refined by
+// three Newton steps on the triangle's plane (in object space), kept inside
+// the sub-chord's parameter range.
+template <class R>
+inline bool mesh_chord(const MeshSet<R>& ms, V3<R> p0, V3<R> p1, V3<R> D0, V3<R> D1, bool herm, R s_max, R* s_out, V3<R>* x_out,
+                       uint32_t* inst_out, uint32_t* tri_out) {
+    uint32_t inst = 0, tri = 0;
+    if (!herm) {
+        R tb = std::min(s_max, R(1));
+        if (!mesh_segment(ms, p0, p1 - p0, &tb, &inst, &tri)) return false;
+        *s_out = tb;
+        *x_out = p0 + tb * (p1 - p0);
+    } else {
+        V3<R> q = p0;
+        int k = 0;
+        R tb = 0;
+        bool found = false;
+        for (; k < kHermiteSegments; ++k) {
+            const R tmax = std::min(R(kHermiteSegments) * s_max - R(k), R(1));
+            if (!(tmax > R(0))) break;
+            const V3<R> e = k + 1 == kHermiteSegments ? p1 : hermite3(p0, D0, p1, D1, R(k + 1) / R(kHermiteSegments));
+            tb = tmax;
+            if (mesh_segment(ms, q, e - q, &tb, &inst, &tri)) { found = true; break; }
+            q = e;
+        }
+        if (!found) return false;
+        const auto& I = ms.inst[inst];
+        const auto& T = ms.mesh[I.mesh][tri];
+        const V3<R> n{T.e1.y * T.e2.z - T.e1.z * T.e2.y, T.e1.z * T.e2.x - T.e1.x * T.e2.z, T.e1.x * T.e2.y - T.e1.y * T.e2.x};
+        const R lo = R(k) / R(kHermiteSegments), hi = R(k + 1) / R(kHermiteSegments);
+        R s = (R(k) + tb) / R(kHermiteSegments);
+        for (int it = 0; it < 3; ++it) {
+            const R f = dot3(n, affine_point(I.w2o, hermite3(p0, D0, p1, D1, s)) - T.v0);
+            const R fd = dot3(n, affine_dir(I.w2o, hermite3_d(p0, D0, p1, D1, s)));
+            if (fd == R(0)) break;
+            s = std::min(std::max(s - f / fd, lo), hi);
+        }
+        *s_out = s;
+        *x_out = hermite3(p0, D0, p1, D1, s);
+    }
+    *inst_out = inst;
+    *tri_out = tri;
+    return true;
+}
+
+// World-space unit normal of a mesh hit, turned against the coordinate
+// velocity v of the arriving geodesic.
+template <class R> inline V3<R> mesh_normal(const MeshSet<R>& ms, uint32_t inst, uint32_t tri, V3<R> v) {
+    const auto& I = ms.inst[inst];
+    const V3<R> no = ms.mesh[I.mesh][tri].n;
+    V3<R> n = norm3(V3<R>{I.nm[0] * no.x + I.nm[1] * no.y + I.nm[2] * no.z, I.nm[3] * no.x + I.nm[4] * no.y + I.nm[5] * no.z,
+                          I.nm[6] * no.x + I.nm[7] * no.y + I.nm[8] * no.z});
+    if (dot3(n, v) > R(0)) n = -n;
+    return n;
+}
+
 // Trace one ray from its initial conditions to a termination class.
 // Per step: solve_step, then the chord tests of kernel_intersect.cu:55-138
 // (closest of disk / sphere / celestial sphere at t = 1; else horizon),
@@ -771,6 +931,24 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
                     cls = sc.box[b].matter == MATTER_DIFFUSE ? HIT_BOX_DIFFUSE : HIT_BOX_EMITTER;
                     box_normal = nc;
                 }
+            }
+        }
+        uint32_t m_inst = 0, m_tri = 0;
+        V3<R> m_x{};
+        if (sc.meshes) {
+            const V3<R> p0 = V3<R>{y0.q.x, y0.q.y, y0.q.z} * L, p1 = V3<R>{y1.q.x, y1.q.y, y1.q.z} * L;
+            V3<R> D0{}, D1{};
+            if (sc.hermite) {
+                // end tangents dx/dlambda = g^{mu nu} p_nu times the step's parameter length, in metres
+                const V4<R> v0 = contract(compute_g(y0.q, p, true), y0.p), v1 = contract(compute_g(y1.q, p, true), y1.p);
+                const R k = R(float(h_taken)) * L;
+                D0 = V3<R>{v0.x, v0.y, v0.z} * k;
+                D1 = V3<R>{v1.x, v1.y, v1.z} * k;
+            }
+            R s;
+            if (mesh_chord(*sc.meshes, p0, p1, D0, D1, sc.hermite, std::min(best, R(1)), &s, &m_x, &m_inst, &m_tri)) {
+                best = s;
+                cls = sc.meshes->inst[m_inst].matter == MATTER_DIFFUSE ? HIT_MESH_DIFFUSE : HIT_MESH_EMITTER;
             }
         }
         {
@@ -813,6 +991,13 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
                 hit.a0 = R(box_normal);
                 hit.a1 = best;
                 if (cls == HIT_BOX_EMITTER) hit.g = redshift_factor(ic.contraction, reference_observer(hit.at.q, p), y1.p);
+            } else if (cls == HIT_MESH_DIFFUSE || cls == HIT_MESH_EMITTER) {
+                // the hit point on the chord or the curve, the momentum of the step's end
+                hit.at.q = V4<R>{m_x.x / L, m_x.y / L, m_x.z / L, y0.q.t + best * (y1.q.t - y0.q.t)};
+                hit.at.p = y1.p;
+                hit.a0 = R(m_tri);
+                hit.a1 = R(m_inst);
+                if (cls == HIT_MESH_EMITTER) hit.g = redshift_factor(ic.contraction, reference_observer(hit.at.q, p), y1.p);
             } else if (cls == HIT_ESCAPE) {
                 V3<R> d = norm3(V3<R>{y1.q.x - y0.q.x, y1.q.y - y0.q.y, y1.q.z - y0.q.z});
                 hit.a0 = d.y;
@@ -932,14 +1117,20 @@ inline Hit<R> trace_path(const RayIC<R>& ic, const MetricParams<R>& p, const Sce
         Hit<R> h = trace_ray(cur, p, sc, cam_origin, steps);
         steps = h.steps;
         trials += h.trials;
-        if (h.cls != HIT_BOX_DIFFUSE || k >= sc.max_reflections) {
+        if ((h.cls != HIT_BOX_DIFFUSE && h.cls != HIT_MESH_DIFFUSE) || k >= sc.max_reflections) {
             h.trials = trials;
             if (n_reflect) *n_reflect = k;
             return h;
         }
         float u[4];
         path_uniforms(seed, cam_id, pixel, sample, 1 + k, u);
-        reflect_diffuse(p, h.at, cur.contraction, int(h.a0), R(u[0]), R(u[1]), &cur.y, &cur.contraction);
+        if (h.cls == HIT_BOX_DIFFUSE) {
+            reflect_diffuse(p, h.at, cur.contraction, int(h.a0), R(u[0]), R(u[1]), &cur.y, &cur.contraction);
+        } else {
+            const V4<R> v = contract(compute_g(h.at.q, p, true), h.at.p);
+            const V3<R> ng = mesh_normal(*sc.meshes, uint32_t(h.a1), uint32_t(h.a0), V3<R>{v.x, v.y, v.z});
+            reflect_matter(p, h.at, cur.contraction, ng, false, R(u[0]), R(u[1]), &cur.y, &cur.contraction);
+        }
         cur.h = ic.h;
     }
 }

@@ -55,6 +55,9 @@ _SIG = {
                                       _D, _D]),
     "orc_white_led": (C.c_double, [C.c_int, C.c_double]),
     "orc_manager_layout": (C.c_size_t, [C.POINTER(C.c_size_t)]),
+    "orc_set_meshes": (C.c_int, [C.POINTER(abi.Mesh), C.c_uint32, C.POINTER(abi.Instance), C.c_uint32]),
+    "orc_seg_tri": (C.c_int, [C.c_int, _D, _D, _D, _D, _D, C.c_double, _D]),
+    "orc_mesh_chord": (C.c_int, [C.c_int, _D, _D, _D, _D, C.c_int, C.c_double, _D, _U]),
 }
 
 _REF_SIG = {
@@ -217,6 +220,37 @@ def reflect_diffuse(prec: int, integ, at, contraction: float, normal_code: int, 
     abi.check(lib().orc_reflect_diffuse(prec, C.byref(integ), dptr(at), contraction, normal_code, u0, u1, dptr(out),
                                         C.byref(c)), "orc_reflect_diffuse")
     return out, c.value
+
+
+def set_meshes(meshes, instances) -> None:
+    """The oracle's meshes for scenes with SCENE_MESHES: the same inputs as
+    magik_amd.Context.set_meshes (lists of (vertices, indices) and of
+    (mesh, matter, 3 x 4 object -> world))."""
+    from magik_amd import mesh as mesh_mod
+    m, inst, keep = mesh_mod.to_abi(meshes, instances)
+    if lib().orc_set_meshes(m, len(meshes), inst, len(instances)) != 0:
+        raise ValueError("invalid meshes")
+    set_meshes._keep = keep
+
+
+def seg_tri(prec: int, o, d, v0, e1, e2, tmax: float = np.inf):
+    """Moller-Trumbore hit parameter of o + t d on a triangle, or None."""
+    arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (o, d, v0, e1, e2)]
+    t = np.zeros(1)
+    ok = lib().orc_seg_tri(prec, *[dptr(x) for x in arr], float(tmax), dptr(t))
+    return float(t[0]) if ok else None
+
+
+def mesh_chord(prec: int, p0, p1, D0=(0, 0, 0), D1=(0, 0, 0), hermite: bool = False, s_max: float = 1.0):
+    """The oracle's chord test against the meshes of set_meshes: (s, hit point,
+    instance, triangle) or None."""
+    arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (p0, p1, D0, D1)]
+    out = np.zeros(4)
+    it = np.zeros(2, dtype=np.uint32)
+    r = lib().orc_mesh_chord(prec, *[dptr(x) for x in arr], int(hermite), float(s_max), dptr(out), uptr(it))
+    if r < 0:
+        raise RuntimeError("no meshes set")
+    return (out[0], out[1:].copy(), int(it[0]), int(it[1])) if r else None
 
 
 def white_led(prec: int, lam_nm: float) -> float:

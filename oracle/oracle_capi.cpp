@@ -22,6 +22,70 @@ template <class R> MetricParams<R> params_from(const mgk_integrator_t* in) {
     return p;
 }
 
+// The meshes of orc_set_meshes, in both precisions.
+MeshSet<float> g_mesh_f;
+MeshSet<double> g_mesh_d;
+bool g_mesh_valid = false;
+template <class R> const MeshSet<R>* mesh_set();
+template <> const MeshSet<float>* mesh_set<float>() { return g_mesh_valid ? &g_mesh_f : nullptr; }
+template <> const MeshSet<double>* mesh_set<double>() { return g_mesh_valid ? &g_mesh_d : nullptr; }
+
+// Instance maps as the library stores them: the inverse of object_to_world
+// by cofactors in double, rounded to float.
+template <class R> bool fill_meshes(MeshSet<R>& ms, const mgk_mesh_t* meshes, uint32_t n_mesh, const mgk_instance_t* inst, uint32_t n_inst) {
+    ms = MeshSet<R>();
+    for (uint32_t m = 0; m < n_mesh; ++m) {
+        std::vector<typename MeshSet<R>::Tri> tris(meshes[m].n_triangle);
+        for (uint32_t t = 0; t < meshes[m].n_triangle; ++t) {
+            const float* v[3];
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t vi = meshes[m].indices[3 * t + k];
+                if (vi >= meshes[m].n_vertex) return false;
+                v[k] = meshes[m].vertices + 3 * vi;
+            }
+            auto& T = tris[t];
+            T.v0 = {R(v[0][0]), R(v[0][1]), R(v[0][2])};
+            // edges as floats, as stored
+            T.e1 = {R(v[1][0] - v[0][0]), R(v[1][1] - v[0][1]), R(v[1][2] - v[0][2])};
+            T.e2 = {R(v[2][0] - v[0][0]), R(v[2][1] - v[0][1]), R(v[2][2] - v[0][2])};
+            const double a[3] = {double(v[1][0]) - v[0][0], double(v[1][1]) - v[0][1], double(v[1][2]) - v[0][2]};
+            const double b[3] = {double(v[2][0]) - v[0][0], double(v[2][1]) - v[0][1], double(v[2][2]) - v[0][2]};
+            double n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+            const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            if (l > 0) { n[0] /= l; n[1] /= l; n[2] /= l; }
+            T.n = {R(float(n[0])), R(float(n[1])), R(float(n[2]))};
+        }
+        ms.mesh.push_back(std::move(tris));
+    }
+    for (uint32_t k = 0; k < n_inst; ++k) {
+        const float* m = inst[k].object_to_world;
+        if (inst[k].mesh >= n_mesh) return false;
+        const double a = m[0], b = m[1], c = m[2], d = m[4], e = m[5], f = m[6], g = m[8], h = m[9], i = m[10];
+        const double det = a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+        if (!(std::fabs(det) > 0)) return false;
+        const double r[9] = {(e * i - f * h) / det, (c * h - b * i) / det, (b * f - c * e) / det,
+                             (f * g - d * i) / det, (a * i - c * g) / det, (c * d - a * f) / det,
+                             (d * h - e * g) / det, (b * g - a * h) / det, (a * e - b * d) / det};
+        typename MeshSet<R>::Inst I;
+        I.mesh = inst[k].mesh;
+        I.matter = inst[k].matter;
+        for (int row = 0; row < 3; ++row) {
+            for (int col = 0; col < 3; ++col) I.w2o[4 * row + col] = R(float(r[3 * row + col]));
+            I.w2o[4 * row + 3] = R(float(-(r[3 * row] * m[3] + r[3 * row + 1] * m[7] + r[3 * row + 2] * m[11])));
+            for (int col = 0; col < 3; ++col) I.nm[3 * row + col] = R(float(r[3 * col + row]));
+        }
+        I.lo = {R(INFINITY), R(INFINITY), R(INFINITY)};
+        I.hi = {R(-INFINITY), R(-INFINITY), R(-INFINITY)};
+        for (uint32_t vi = 0; vi < meshes[I.mesh].n_vertex; ++vi) {
+            const float* v = meshes[I.mesh].vertices + 3 * vi;
+            I.lo = {std::min(I.lo.x, R(v[0])), std::min(I.lo.y, R(v[1])), std::min(I.lo.z, R(v[2]))};
+            I.hi = {std::max(I.hi.x, R(v[0])), std::max(I.hi.y, R(v[1])), std::max(I.hi.z, R(v[2]))};
+        }
+        ms.inst.push_back(I);
+    }
+    return true;
+}
+
 template <class R> Scene<R> scene_from(const mgk_scene_t* s, uint32_t step_limit) {
     Scene<R> sc;
     sc.celestial_radius_m = R(s->celestial_radius_m);
@@ -47,6 +111,8 @@ template <class R> Scene<R> scene_from(const mgk_scene_t* s, uint32_t step_limit
     }
     sc.max_reflections = s->max_reflections;
     sc.led_scale = R(s->led_scale);
+    if (s->flags & MGK_SCENE_MESHES) sc.meshes = mesh_set<R>();
+    sc.hermite = s->chord == MGK_CHORD_HERMITE;
     return sc;
 }
 
@@ -121,6 +187,7 @@ double emission(const mgk_scene_t* s, const StarIndex* stars, const mgk_hit_t& h
             return ((ci + cj) & 1u) ? 0.0 : planck<double>(lam_emit, double(s->sphere_T));
         }
         case MGK_HIT_BOX_EMITTER:
+        case MGK_HIT_MESH_EMITTER:
             return double(s->led_scale) * white_led<double>(lam_emit);
         case MGK_HIT_ESCAPE: {
             double v = 0;
@@ -328,6 +395,52 @@ int orc_reflect_diffuse(int prec, const mgk_integrator_t* integ, const double at
     };
     if (prec == 0) run(0.0f); else run(0.0);
     return MAGIK_SUCCESS;
+}
+
+// Meshes for the scenes with MGK_SCENE_MESHES (the library's mgk_set_meshes
+// inputs); n_mesh = 0 clears them.  Returns 0, or -1 for an invalid input.
+int orc_set_meshes(const mgk_mesh_t* meshes, uint32_t n_mesh, const mgk_instance_t* inst, uint32_t n_inst) {
+    g_mesh_valid = false;
+    if (n_mesh == 0) return 0;
+    if (!meshes || !inst || n_inst == 0) return -1;
+    if (!fill_meshes(g_mesh_f, meshes, n_mesh, inst, n_inst) || !fill_meshes(g_mesh_d, meshes, n_mesh, inst, n_inst)) return -1;
+    g_mesh_valid = true;
+    return 0;
+}
+
+// One segment against one triangle (test access to seg_tri).
+int orc_seg_tri(int prec, const double o[3], const double d[3], const double v0[3], const double e1[3], const double e2[3], double tmax,
+                double* t) {
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        auto v = [](const double* x) { return V3<R>{R(x[0]), R(x[1]), R(x[2])}; };
+        R tt = 0;
+        const bool ok = seg_tri<R>(v(o), v(d), v(v0), v(e1), v(e2), R(tmax), &tt);
+        *t = double(tt);
+        return ok ? 1 : 0;
+    };
+    return prec == 0 ? run(0.0f) : run(0.0);
+}
+
+// One chord against the meshes of orc_set_meshes (test access to mesh_chord):
+// out = {s, x, y, z}, inst_tri = {instance, triangle}.  Returns 1 on a hit.
+int orc_mesh_chord(int prec, const double p0[3], const double p1[3], const double D0[3], const double D1[3], int hermite, double s_max,
+                   double out[4], uint32_t inst_tri[2]) {
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        const MeshSet<R>* ms = mesh_set<R>();
+        if (!ms) return -1;
+        auto v = [](const double* x) { return V3<R>{R(x[0]), R(x[1]), R(x[2])}; };
+        R s = 0;
+        V3<R> x{};
+        uint32_t k = 0, j = 0;
+        if (!mesh_chord<R>(*ms, v(p0), v(p1), v(D0), v(D1), hermite != 0, R(s_max), &s, &x, &k, &j)) return 0;
+        out[0] = double(s); out[1] = double(x.x); out[2] = double(x.y); out[3] = double(x.z);
+        inst_tri[0] = k;
+        inst_tri[1] = j;
+        return 1;
+    };
+    return prec == 0 ? run(0.0f) : run(0.0);
 }
 
 double orc_white_led(int prec, double lam_nm) {
