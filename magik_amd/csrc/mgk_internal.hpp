@@ -48,6 +48,8 @@ constexpr uint32_t kLeafShift = 24;         // leaf info = (count << 24) | first
 constexpr uint32_t kLeafMask = 0xFFFFFFu;
 constexpr int kInstF4 = 6;                  // float4 per instance record
 constexpr int kHermiteSegments = 4;         // straight sub-chords that follow a Hermite chord
+constexpr int kHermiteFine = 8;             // finer sub-chords within the first coarse one that hits
+constexpr float kBaryEps = 1e-4f;           // barycentric slack of a Hermite crossing on its triangle
 
 struct TraceParams {
     int metric, precision;

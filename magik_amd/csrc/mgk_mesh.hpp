@@ -140,13 +140,45 @@ template <class T> struct MeshHit {
     uint32_t matter;
 };
 
+// The first hit along n straight sub-chords through the curve's points at
+// s_lo + j (s_hi - s_lo) / n, j = 0..n, accepted below the curve parameter
+// s_max.  Returns the sub-chord's index (-1: no hit), the hit's curve
+// parameter and its point on the sub-chord.
+template <class T>
+__device__ __forceinline__ int curve_pass(const TraceParams& P, const T p0[3], const T D0[3], const T p1[3], const T D1[3], T s_lo,
+                                          T s_hi, int n, T s_max, T& s_hit, T x[3], uint32_t& inst, uint32_t& slot) {
+#pragma clang fp contract(off)
+    const T ds = (s_hi - s_lo) / T(n);
+    T q[3];
+    for (int c = 0; c < 3; ++c) q[c] = s_lo == T(0) ? p0[c] : hermite(p0[c], D0[c], p1[c], D1[c], s_lo);
+    for (int j = 0; j < n; ++j) {
+        const T tmax = min_((s_max - s_lo) / ds - T(j), T(1));
+        if (!(tmax > T(0))) break;
+        const T s1 = s_lo + T(j + 1) * ds;
+        T e[3];
+        for (int c = 0; c < 3; ++c) e[c] = s1 >= T(1) ? p1[c] : hermite(p0[c], D0[c], p1[c], D1[c], s1);
+        const T d[3] = {e[0] - q[0], e[1] - q[1], e[2] - q[2]};
+        T tb = tmax;
+        if (mesh_segment<T>(P, q, d, tb, inst, slot)) {
+            s_hit = s_lo + (T(j) + tb) * ds;
+            for (int c = 0; c < 3; ++c) x[c] = q[c] + tb * d[c];
+            return j;
+        }
+        q[0] = e[0]; q[1] = e[1]; q[2] = e[2];
+    }
+    return -1;
+}
+
 // The chord of one completed step against the meshes.  p0, p1: end points
 // (world SI); for Hermite chords D0, D1 are the end tangents times the step's
 // parameter length.  Hits are accepted below s_max (the closest hit of the
 // other traceables, in chord units).  A Hermite chord is followed along
-// kHermiteSegments straight sub-chords through points of the curve; the first
-// sub-chord that hits a triangle is refined by Newton's method on that
-// triangle's plane, so the hit point lies on the cubic itself.
+// kHermiteSegments straight sub-chords through points of the curve, and the
+// first that hits along kHermiteFine finer ones.  Newton's method on the hit
+// triangle's plane then moves the hit onto the cubic itself; where the
+// crossing found that way leaves the triangle (the curve passes into a
+// neighbour the sub-chord missed), the sub-chord's hit is kept, so a hit
+// point always lies on the triangle its record names.
 template <class T>
 __device__ __forceinline__ bool mesh_chord(const TraceParams& P, const T p0[3], const T p1[3], const T D0[3], const T D1[3],
                                            bool hermite_chord, T s_max, MeshHit<T>& h) {
@@ -170,36 +202,37 @@ __device__ __forceinline__ bool mesh_chord(const TraceParams& P, const T p0[3], 
         h.s = tb;
         for (int c = 0; c < 3; ++c) h.x[c] = p0[c] + tb * d[c];
     } else {
-        T q[3] = {p0[0], p0[1], p0[2]};
-        int k = 0;
-        T tb = T(0);
-        bool found = false;
-        for (; k < kHermiteSegments; ++k) {
-            const T tmax = min_(T(kHermiteSegments) * s_max - T(k), T(1));
-            if (!(tmax > T(0))) break;
-            const T s1 = T(k + 1) / T(kHermiteSegments);
-            T e[3];
-            for (int c = 0; c < 3; ++c) e[c] = k + 1 == kHermiteSegments ? p1[c] : hermite(p0[c], D0[c], p1[c], D1[c], s1);
-            const T d[3] = {e[0] - q[0], e[1] - q[1], e[2] - q[2]};
-            tb = tmax;
-            if (mesh_segment<T>(P, q, d, tb, inst, slot)) { found = true; break; }
-            q[0] = e[0]; q[1] = e[1]; q[2] = e[2];
+        T s_hit, x[3];
+        const int k = curve_pass<T>(P, p0, D0, p1, D1, T(0), T(1), kHermiteSegments, s_max, s_hit, x, inst, slot);
+        if (k < 0) return false;
+        T lo = T(k) / T(kHermiteSegments), hi = T(k + 1) / T(kHermiteSegments);
+        {
+            T s2, x2[3];
+            uint32_t i2 = 0, sl2 = 0;
+            const int j = curve_pass<T>(P, p0, D0, p1, D1, lo, hi, kHermiteFine, s_max, s2, x2, i2, sl2);
+            if (j >= 0) {
+                const T ds = (hi - lo) / T(kHermiteFine);
+                lo = lo + T(j) * ds;
+                hi = lo + ds;
+                s_hit = s2;
+                x[0] = x2[0]; x[1] = x2[1]; x[2] = x2[2];
+                inst = i2;
+                slot = sl2;
+            }
         }
-        if (!found) return false;
         // Newton on the triangle's world plane n_w . x + c = 0, n_w = N (e1 x e2),
         // c = (e1 x e2) . (t - v0) with x_obj = R x + t: the plane in object space
         const float4* __restrict__ I = P.mesh_inst + uint64_t(kInstF4) * inst;
         const float4* __restrict__ tri = P.mesh_tris + 3u * uint64_t(slot);
         const float4 A = tri[0], E1 = tri[1], E2 = tri[2];
-        const T nx = T(E1.y) * T(E2.z) - T(E1.z) * T(E2.y), ny = T(E1.z) * T(E2.x) - T(E1.x) * T(E2.z),
-                nz = T(E1.x) * T(E2.y) - T(E1.y) * T(E2.x);
+        const T e1[3] = {T(E1.x), T(E1.y), T(E1.z)}, e2[3] = {T(E2.x), T(E2.y), T(E2.z)};
+        const T nx = e1[1] * e2[2] - e1[2] * e2[1], ny = e1[2] * e2[0] - e1[0] * e2[2], nz = e1[0] * e2[1] - e1[1] * e2[0];
         const float4 r0 = I[0], r1 = I[1], r2 = I[2];
         const T cc = nx * (T(r0.w) - T(A.x)) + ny * (T(r1.w) - T(A.y)) + nz * (T(r2.w) - T(A.z));
         const T wx = T(r0.x) * nx + T(r1.x) * ny + T(r2.x) * nz;
         const T wy = T(r0.y) * nx + T(r1.y) * ny + T(r2.y) * nz;
         const T wz = T(r0.z) * nx + T(r1.z) * ny + T(r2.z) * nz;
-        const T lo = T(k) / T(kHermiteSegments), hi = T(k + 1) / T(kHermiteSegments);
-        T s = (T(k) + tb) / T(kHermiteSegments);
+        T s = s_hit;
         for (int it = 0; it < 3; ++it) {
             const T f = wx * hermite(p0[0], D0[0], p1[0], D1[0], s) + wy * hermite(p0[1], D0[1], p1[1], D1[1], s) +
                         wz * hermite(p0[2], D0[2], p1[2], D1[2], s) + cc;
@@ -208,8 +241,25 @@ __device__ __forceinline__ bool mesh_chord(const TraceParams& P, const T p0[3], 
             if (fd == T(0)) break;
             s = min_(max_(s - f / fd, lo), hi);
         }
-        h.s = s;
-        for (int c = 0; c < 3; ++c) h.x[c] = hermite(p0[c], D0[c], p1[c], D1[c], s);
+        T xc[3];
+        for (int c = 0; c < 3; ++c) xc[c] = hermite(p0[c], D0[c], p1[c], D1[c], s);
+        // barycentric coordinates of the curve's crossing in object space
+        const T w[3] = {T(r0.x) * xc[0] + T(r0.y) * xc[1] + T(r0.z) * xc[2] + T(r0.w) - T(A.x),
+                        T(r1.x) * xc[0] + T(r1.y) * xc[1] + T(r1.z) * xc[2] + T(r1.w) - T(A.y),
+                        T(r2.x) * xc[0] + T(r2.y) * xc[1] + T(r2.z) * xc[2] + T(r2.w) - T(A.z)};
+        const T d00 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2], d01 = e1[0] * e2[0] + e1[1] * e2[1] + e1[2] * e2[2];
+        const T d11 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+        const T d20 = w[0] * e1[0] + w[1] * e1[1] + w[2] * e1[2], d21 = w[0] * e2[0] + w[1] * e2[1] + w[2] * e2[2];
+        const T den = d00 * d11 - d01 * d01;
+        const T bu = (d11 * d20 - d01 * d21) / den, bv = (d00 * d21 - d01 * d20) / den;
+        const T eps = T(kBaryEps);
+        if (bu >= -eps && bv >= -eps && bu + bv <= T(1) + eps && s < s_max) {
+            h.s = s;
+            h.x[0] = xc[0]; h.x[1] = xc[1]; h.x[2] = xc[2];
+        } else {
+            h.s = s_hit;
+            h.x[0] = x[0]; h.x[1] = x[1]; h.x[2] = x[2];
+        }
     }
     const float4* __restrict__ I = P.mesh_inst + uint64_t(kInstF4) * inst;
     h.inst = inst;

@@ -831,14 +831,40 @@ template <class R> inline V3<R> hermite3_d(V3<R> p0, V3<R> D0, V3<R> p1, V3<R> D
     return {a * (p0.x - p1.x) + b * D0.x + e * D1.x, a * (p0.y - p1.y) + b * D0.y + e * D1.y, a * (p0.z - p1.z) + b * D0.z + e * D1.z};
 }
 
-constexpr int kHermiteSegments = 4;
+constexpr int kHermiteSegments = 4, kHermiteFine = 8;
+constexpr double kBaryEps = 1e-4;
+
+// The first hit along n straight sub-chords through the curve's points at
+// s_lo + j (s_hi - s_lo) / n, below s_max: the sub-chord's index or -1, the
+// hit's curve parameter and its point on the sub-chord.
+template <class R>
+inline int curve_pass(const MeshSet<R>& ms, V3<R> p0, V3<R> D0, V3<R> p1, V3<R> D1, R s_lo, R s_hi, int n, R s_max, R* s_hit, V3<R>* x,
+                      uint32_t* inst, uint32_t* tri) {
+    const R ds = (s_hi - s_lo) / R(n);
+    V3<R> q = s_lo == R(0) ? p0 : hermite3(p0, D0, p1, D1, s_lo);
+    for (int j = 0; j < n; ++j) {
+        const R tmax = std::min((s_max - s_lo) / ds - R(j), R(1));
+        if (!(tmax > R(0))) break;
+        const R s1 = s_lo + R(j + 1) * ds;
+        const V3<R> e = s1 >= R(1) ? p1 : hermite3(p0, D0, p1, D1, s1);
+        R tb = tmax;
+        if (mesh_segment(ms, q, e - q, &tb, inst, tri)) {
+            *s_hit = s_lo + (R(j) + tb) * ds;
+            *x = q + tb * (e - q);
+            return j;
+        }
+        q = e;
+    }
+    return -1;
+}
 
 // The chord of one step against the meshes, hits below s_max.  p0, p1 the end
 // points and D0, D1 the end tangents times the step's parameter length (world
 // SI).  Straight chord: one segment.  Hermite chord: kHermiteSegments straight
-// sub-chords through points of the cubic; the first that hits is refined by
-// three Newton steps on the triangle's plane (in object space), kept inside
-// the sub-chord's parameter range.
+// sub-chords through points of the cubic, then kHermiteFine within the first
+// that hits; three Newton steps on the hit triangle's plane (in object space)
+// inside the fine sub-chord's parameter range move the hit onto the cubic,
+// kept only where that crossing lies on the triangle (else the sub-chord's hit).
 template <class R>
 inline bool mesh_chord(const MeshSet<R>& ms, V3<R> p0, V3<R> p1, V3<R> D0, V3<R> D1, bool herm, R s_max, R* s_out, V3<R>* x_out,
                        uint32_t* inst_out, uint32_t* tri_out) {
@@ -849,32 +875,49 @@ inline bool mesh_chord(const MeshSet<R>& ms, V3<R> p0, V3<R> p1, V3<R> D0, V3<R>
         *s_out = tb;
         *x_out = p0 + tb * (p1 - p0);
     } else {
-        V3<R> q = p0;
-        int k = 0;
-        R tb = 0;
-        bool found = false;
-        for (; k < kHermiteSegments; ++k) {
-            const R tmax = std::min(R(kHermiteSegments) * s_max - R(k), R(1));
-            if (!(tmax > R(0))) break;
-            const V3<R> e = k + 1 == kHermiteSegments ? p1 : hermite3(p0, D0, p1, D1, R(k + 1) / R(kHermiteSegments));
-            tb = tmax;
-            if (mesh_segment(ms, q, e - q, &tb, &inst, &tri)) { found = true; break; }
-            q = e;
+        R s_hit = 0;
+        V3<R> x{};
+        const int k = curve_pass(ms, p0, D0, p1, D1, R(0), R(1), kHermiteSegments, s_max, &s_hit, &x, &inst, &tri);
+        if (k < 0) return false;
+        R lo = R(k) / R(kHermiteSegments), hi = R(k + 1) / R(kHermiteSegments);
+        {
+            R s2 = 0;
+            V3<R> x2{};
+            uint32_t i2 = 0, t2 = 0;
+            const int j = curve_pass(ms, p0, D0, p1, D1, lo, hi, kHermiteFine, s_max, &s2, &x2, &i2, &t2);
+            if (j >= 0) {
+                const R ds = (hi - lo) / R(kHermiteFine);
+                lo = lo + R(j) * ds;
+                hi = lo + ds;
+                s_hit = s2;
+                x = x2;
+                inst = i2;
+                tri = t2;
+            }
         }
-        if (!found) return false;
         const auto& I = ms.inst[inst];
         const auto& T = ms.mesh[I.mesh][tri];
         const V3<R> n{T.e1.y * T.e2.z - T.e1.z * T.e2.y, T.e1.z * T.e2.x - T.e1.x * T.e2.z, T.e1.x * T.e2.y - T.e1.y * T.e2.x};
-        const R lo = R(k) / R(kHermiteSegments), hi = R(k + 1) / R(kHermiteSegments);
-        R s = (R(k) + tb) / R(kHermiteSegments);
+        R s = s_hit;
         for (int it = 0; it < 3; ++it) {
             const R f = dot3(n, affine_point(I.w2o, hermite3(p0, D0, p1, D1, s)) - T.v0);
             const R fd = dot3(n, affine_dir(I.w2o, hermite3_d(p0, D0, p1, D1, s)));
             if (fd == R(0)) break;
             s = std::min(std::max(s - f / fd, lo), hi);
         }
-        *s_out = s;
-        *x_out = hermite3(p0, D0, p1, D1, s);
+        const V3<R> xc = hermite3(p0, D0, p1, D1, s);
+        const V3<R> w = affine_point(I.w2o, xc) - T.v0;
+        const R d00 = dot3(T.e1, T.e1), d01 = dot3(T.e1, T.e2), d11 = dot3(T.e2, T.e2), d20 = dot3(w, T.e1), d21 = dot3(w, T.e2);
+        const R den = d00 * d11 - d01 * d01;
+        const R bu = (d11 * d20 - d01 * d21) / den, bv = (d00 * d21 - d01 * d20) / den;
+        const R eps = R(kBaryEps);
+        if (bu >= -eps && bv >= -eps && bu + bv <= R(1) + eps && s < s_max) {
+            *s_out = s;
+            *x_out = xc;
+        } else {
+            *s_out = s_hit;
+            *x_out = x;
+        }
     }
     *inst_out = inst;
     *tri_out = tri;
