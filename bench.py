@@ -215,6 +215,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="kerr_disk_1080p")
+    ap.add_argument("--chord", default=None, choices=["affine", "hermite"],
+                    help="mesh chord of a workload with meshes (default: the workload's)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp32_reference", "fp64"])
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -280,6 +282,8 @@ def main():
     meshes = M.workload_meshes(args.workload) if sc.flags & M.SCENE_MESHES else None
     if meshes:
         ctx.set_meshes(*meshes)
+        if args.chord:
+            sc.chord = M.CHORD_HERMITE if args.chord == "hermite" else M.CHORD_AFFINE
     out = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
     log(f"bench: {args.workload} {W}x{H}x{spp} = {n_paths} paths, precision {args.precision}")
 
@@ -371,14 +375,16 @@ def main():
         "dtype": "f32" if prec != M.PREC_FP64 else "f64",
         "data": (f"synthetic (seeded Philox rays, Kerr a={integ.spin:.3f} hole inside a diffuse box with an LED "
                  "panel, the reference's default stage; seeded Lambertian reflections"
-                 + ("; procedural triangle meshes, Hermite chords)" if meshes else ")")) if boxes else
+                 + ("; procedural triangle meshes)" if meshes else ")")) if boxes else
                 (f"synthetic (seeded Philox rays, Kerr a={integ.spin:.2f} disk + 1e5-star field scene; surrogate "
                  "trained on seeded FP64-traced rays)"),
         "config": {"workload": (f"{args.workload}: Kerr a={integ.spin:.3f}, M=6.73e26 kg, 40x20x40 m diffuse box, "
                                 f"20x20 m LED panel, camera (2,0,9) m, {W}x{H}x{spp} spp, 4 hero wavelengths, "
                                 f"tol 1e-3/1e-10, step limit 512, {sc.max_reflections} reflections"
                                 + (f", meshes: {sum(len(f) for _, f in meshes[0])} triangles in {len(meshes[0])} "
-                                   f"meshes, {len(meshes[1])} instances, Hermite chords" if meshes else "")) if boxes else
+                                   f"meshes, {len(meshes[1])} instances, "
+                                   f"{'Hermite' if sc.chord == M.CHORD_HERMITE else 'affine'} chords" if meshes else ""))
+                               if boxes else
                                (f"{args.workload}: Kerr a={integ.spin:.2f}, r_obs=50M, theta=80 deg, FOV 25 deg, "
                                 f"{W}x{H}x{spp} spp, 4 hero wavelengths, tol 1e-7, cap 1e4"),
                    "paths_per_frame": n_paths, "value_path": best[0], "error_bar": ERROR_BAR},
