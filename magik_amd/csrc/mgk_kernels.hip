@@ -1291,22 +1291,25 @@ __device__ __forceinline__ float offset_coord(float x, float n) {
 }
 
 // Spatial legs of the Gram-Schmidt tetrad of tetrad_apply (order z, x, y).
+// Each leg is normalised by one reciprocal square root and three multiplies:
+// FP64 division is a multi-instruction sequence, and the legs only need to be
+// unit to rounding.
 template <class S>
 __device__ __forceinline__ void tetrad_legs(const Sym4<S>& g, V4<S> u, V4<S>& ex, V4<S>& ey, V4<S>& ez) {
 #pragma clang fp contract(off)
     const V4<S> X{S(1), S(0), S(0), S(0)}, Y{S(0), S(1), S(0), S(0)}, Z{S(0), S(0), S(1), S(0)};
     ez = axpy(inner(g, Z, u), u, Z);
-    ez = divs(ez, sqrt_<false>(inner(g, ez, ez)));
+    ez = scale(ez, S(1) / sqrt_<false>(inner(g, ez, ez)));
     ex = axpy(-inner(g, X, ez), ez, axpy(inner(g, X, u), u, X));
-    ex = divs(ex, sqrt_<false>(inner(g, ex, ex)));
+    ex = scale(ex, S(1) / sqrt_<false>(inner(g, ex, ex)));
     ey = axpy(-inner(g, Y, ex), ex, axpy(-inner(g, Y, ez), ez, axpy(inner(g, Y, u), u, Y)));
-    ey = divs(ey, sqrt_<false>(inner(g, ey, ey)));
+    ey = scale(ey, S(1) / sqrt_<false>(inner(g, ey, ey)));
 }
 
 template <class S> __device__ __forceinline__ void norm3(S& x, S& y, S& z) {
 #pragma clang fp contract(off)
-    const S l = sqrt_<false>(x * x + y * y + z * z);
-    x = x / l; y = y / l; z = z / l;
+    const S il = S(1) / sqrt_<false>(x * x + y * y + z * z);
+    x = x * il; y = y * il; z = z * il;
 }
 
 // One thread per traced ray; rays that ended on diffuse matter get new
@@ -1383,9 +1386,12 @@ template <int METRIC, class S> __global__ void __launch_bounds__(256) k_reflect(
             float uu[4];
             const uint32_t pixel = uint32_t(ray % P.npix), sample = P.sample_begin + uint32_t(ray / P.npix);
             path_uniforms(P.seed, P.camera_id, pixel, sample, P.block, uu);
-            const S th = acos(S(1) - S(2) * S(uu[0])), ph = S(2) * S(kPi) * S(uu[1]);
-            const S st = sin(th);
-            ox = cos(ph) * st + nx; oy = sin(ph) * st + ny; oz = cos(th) + nz;
+            // p uniform on the sphere: cos(theta) = 1 - 2 u0, sin(theta) = 2 sqrt(u0 (1 - u0))
+            const S u0 = S(uu[0]), ct = S(1) - S(2) * u0, st = S(2) * sqrt_<false>(u0 * (S(1) - u0));
+            S sp, cp;
+            if constexpr (sizeof(S) == 4) sincosf(2.0f * kPi * uu[1], &sp, &cp);
+            else sincos(S(2) * S(kPi) * S(uu[1]), &sp, &cp);
+            ox = cp * st + nx; oy = sp * st + ny; oz = ct + nz;
         }
         norm3(ox, oy, oz);
         // outgoing momentum from the frame (relativity_utilities.cuh:186-195)
