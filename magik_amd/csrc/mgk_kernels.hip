@@ -655,9 +655,11 @@ constexpr int kRefillMin = 8;     // zone kernels: idle lanes that trigger a ref
 // complete in every kRenormEvery-th iteration of the wave.  The schedule is the
 // wave's, not the lane's, so the whole wave skips the block on the other
 // iterations; a lane is renormalised about every kRenormEvery steps.  On the
-// 1080p Kerr frame: integration 44.1 -> 38.0 ms, the same radiance error
-// (6.30e-4), drift of H / p^2 5e-8 (the FP64 path's own: 2e-7).
-constexpr uint32_t kRenormEvery = 4;
+// 1080p Kerr frame the integration takes 44.1 ms renormalising every step,
+// 38.0 every 4th iteration, 37.1 every 8th and 36.3 every 16th, all at the same
+// radiance error (6.33e-4).  At 16 the mean drift of H / p^2 (2.0e-7) equals
+// the all-FP64 path's own (1.9e-7), and that of L and Q stays below it.
+constexpr uint32_t kRenormEvery = 16;
 
 // Persistent integrator.  T is the storage / state type: float for the FP32
 // reference path, double for the mixed and FP64 paths.
