@@ -363,7 +363,7 @@ e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, con
             R.mesh_normals = reinterpret_cast<const float4*>(c->mesh_normals.ptr);
             hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, R.count, 1);
             HIP_TRY(hipGetLastError());
-            HIP_TRY(launch_reflect(R, P.precision == MGK_PRECISION_FP32_REFERENCE, s));
+            HIP_TRY(launch_reflect(R, P.precision, s));
             unsigned long long cnt = 0;
             HIP_TRY(hipMemcpyAsync(&cnt, R.count, sizeof(cnt), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));

@@ -197,7 +197,7 @@ hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t
 hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s);
 hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s);
-hipError_t launch_reflect(const ReflectParams& p, bool fp32, hipStream_t s);
+hipError_t launch_reflect(const ReflectParams& p, int precision, hipStream_t s);
 
 // Samples [sample_begin, sample_begin + spp) of a frame into d_xyz (XYZ / spp
 // per pixel): the body of mgk_render_frame.

@@ -1322,7 +1322,12 @@ template <class S> __device__ __forceinline__ void norm3(S& x, S& y, S& z) {
 // frame; faces with an x normal reflect specularly, the others send
 // (p + n) / |p + n| for p uniform on the sphere, cosine-weighted about the
 // facing normal n.  The new u.p carries the redshift of the path so far.
-template <int METRIC, class S> __global__ void __launch_bounds__(256) k_reflect(ReflectParams P) {
+// Threads per block of k_reflect: the reflected rays of a block take their
+// out-queue slots with one atomic per block (a single counter taken once per
+// wave had serialised ~0.5 M atomics per launch on one address).
+constexpr int kReflectBlock = 512;
+
+template <int METRIC, class Sst, class S> __global__ void __launch_bounds__(kReflectBlock) k_reflect(ReflectParams P) {
 #pragma clang fp contract(off)
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint32_t lane = __lane_id();
@@ -1335,11 +1340,11 @@ template <int METRIC, class S> __global__ void __launch_bounds__(256) k_reflect(
     }
     if (go) {
         const uint64_t sd = P.stride;
-        const S qx = S(P.state[0 * sd + ray]), qy = S(P.state[1 * sd + ray]), qz = S(P.state[2 * sd + ray]);
-        const S qt = S(P.state[3 * sd + ray]);
+        const double qxd = P.state[0 * sd + ray], qyd = P.state[1 * sd + ray], qzd = P.state[2 * sd + ray];
+        const S qx = S(qxd), qy = S(qyd), qz = S(qzd);
         const V4<S> pin{S(P.state[4 * sd + ray]), S(P.state[5 * sd + ray]), S(P.state[6 * sd + ray]), S(P.state[7 * sd + ray])};
-        S* rays = reinterpret_cast<S*>(P.rays);
-        const S contraction = rays[9 * sd + ray];
+        Sst* rays = reinterpret_cast<Sst*>(P.rays);
+        const S contraction = S(rays[9 * sd + ray]);
         const S a = S(P.spin), ms = S(P.mass_sign);
         const Sym4<S> g = metric_g<METRIC, S>(qx, qy, qz, a, ms, false);
         const Sym4<S> gi = metric_g<METRIC, S>(qx, qy, qz, a, ms, true);
@@ -1404,47 +1409,61 @@ template <int METRIC, class S> __global__ void __launch_bounds__(256) k_reflect(
         vup = axpy(oz, ez, vup);
         const V4<S> pout = lower(g, vup);
         // offset along the global facing normal (the box normal already opposes the chord)
-        const float fx = float(qx), fy = float(qy), fz = float(qz);
-        rays[0 * sd + ray] = qx + S(offset_coord(fx, float(ng.x)) - fx);
-        rays[1 * sd + ray] = qy + S(offset_coord(fy, float(ng.y)) - fy);
-        rays[2 * sd + ray] = qz + S(offset_coord(fz, float(ng.z)) - fz);
-        rays[3 * sd + ray] = qt;
-        rays[4 * sd + ray] = pout.x; rays[5 * sd + ray] = pout.y; rays[6 * sd + ray] = pout.z; rays[7 * sd + ray] = pout.t;
-        rays[9 * sd + ray] = dot4(u, pout) * z;
+        // the position keeps the storage precision: the float offset is added to the stored hit point
+        const float fx = float(qxd), fy = float(qyd), fz = float(qzd);
+        rays[0 * sd + ray] = Sst(qxd) + Sst(offset_coord(fx, float(ng.x)) - fx);
+        rays[1 * sd + ray] = Sst(qyd) + Sst(offset_coord(fy, float(ng.y)) - fy);
+        rays[2 * sd + ray] = Sst(qzd) + Sst(offset_coord(fz, float(ng.z)) - fz);
+        rays[3 * sd + ray] = Sst(P.state[3 * sd + ray]);
+        rays[4 * sd + ray] = Sst(pout.x); rays[5 * sd + ray] = Sst(pout.y); rays[6 * sd + ray] = Sst(pout.z);
+        rays[7 * sd + ray] = Sst(pout.t);
+        rays[9 * sd + ray] = Sst(dot4(u, pout) * z);
     }
-    // append the reflected rays: one atomic per wave
+    // append the reflected rays: per-wave counts in LDS, one atomic per block
+    __shared__ uint32_t wave_count[kReflectBlock / 64];
+    __shared__ unsigned long long block_base;
     const uint64_t m = __ballot(go);
-    if (m == 0) return;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    unsigned long long base = 0;
-    if (int(lane) == leader) base = atomicAdd(P.count, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader);
+    const uint32_t wave = threadIdx.x / 64u;
+    if (lane == 0) wave_count[wave] = uint32_t(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (uint32_t w = 0; w < kReflectBlock / 64; ++w) total += wave_count[w];
+        block_base = total ? atomicAdd(P.count, (unsigned long long)total) : 0ull;
+    }
+    __syncthreads();
     if (go) {
+        uint64_t off = block_base;
+        for (uint32_t w = 0; w < wave; ++w) off += wave_count[w];
         const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        P.queue_out[base + uint64_t(__popcll(m & lt_mask))] = uint32_t(ray);
+        P.queue_out[off + uint64_t(__popcll(m & lt_mask))] = uint32_t(ray);
     }
 }
 
-hipError_t launch_reflect(const ReflectParams& p, bool fp32, hipStream_t s) {
+// precision: MGK_PRECISION_* of the trace.  The frame of the reflection is
+// computed in FP32 on the FP32-reference and mixed paths (the reference's own
+// bxdf arithmetic is FP32, and the walls lie far from the FP64 zone), in FP64
+// on the FP64 path; initial conditions keep the ray buffer's precision.
+hipError_t launch_reflect(const ReflectParams& p, int precision, hipStream_t s) {
     if (p.n_in == 0) return hipSuccess;
-    const dim3 g{unsigned((p.n_in + 255) / 256)}, b{256u};
-    if (g.x == 0 || (p.n_in + 255) / 256 > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    const uint64_t blocks = (p.n_in + kReflectBlock - 1) / kReflectBlock;
+    if (blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    const dim3 g{unsigned(blocks)}, b{unsigned(kReflectBlock)};
+    auto go = [&](auto m) -> hipError_t {
+        constexpr int M = decltype(m)::value;
+        switch (precision) {
+            case MGK_PRECISION_FP32_REFERENCE: hipLaunchKernelGGL((k_reflect<M, float, float>), g, b, 0, s, p); break;
+            case MGK_PRECISION_MIXED: hipLaunchKernelGGL((k_reflect<M, double, float>), g, b, 0, s, p); break;
+            default: hipLaunchKernelGGL((k_reflect<M, double, double>), g, b, 0, s, p); break;
+        }
+        return hipGetLastError();
+    };
     switch (p.metric) {
-        case KERR:
-            if (fp32) hipLaunchKernelGGL((k_reflect<KERR, float>), g, b, 0, s, p);
-            else hipLaunchKernelGGL((k_reflect<KERR, double>), g, b, 0, s, p);
-            break;
-        case SCHWARZSCHILD:
-            if (fp32) hipLaunchKernelGGL((k_reflect<SCHWARZSCHILD, float>), g, b, 0, s, p);
-            else hipLaunchKernelGGL((k_reflect<SCHWARZSCHILD, double>), g, b, 0, s, p);
-            break;
-        case MINKOWSKI:
-            if (fp32) hipLaunchKernelGGL((k_reflect<MINKOWSKI, float>), g, b, 0, s, p);
-            else hipLaunchKernelGGL((k_reflect<MINKOWSKI, double>), g, b, 0, s, p);
-            break;
+        case KERR: return go(std::integral_constant<int, KERR>{});
+        case SCHWARZSCHILD: return go(std::integral_constant<int, SCHWARZSCHILD>{});
+        case MINKOWSKI: return go(std::integral_constant<int, MINKOWSKI>{});
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
