@@ -188,7 +188,8 @@ def test_mesh_frame_renders_and_is_deterministic(gpu_ctx):
     import torch
     meshes, inst = small_meshes()
     gpu_ctx.set_meshes(meshes, inst)
-    integ, sc, spec, cam, spp = setup(M.KERR, M.PREC_MIXED, M.CHORD_HERMITE, W=64, H=54, spp=2, max_reflections=10)
+    # FP64: every decision per lane, so whole paths repeat bit for bit
+    integ, sc, spec, cam, spp = setup(M.KERR, M.PREC_FP64, M.CHORD_HERMITE, W=64, H=54, spp=2, max_reflections=10)
     gpu_ctx.set_scene(sc)
     a = gpu_ctx.render_frame(integ, sc, spec, cam, SEED, spp)
     b = gpu_ctx.render_frame(integ, sc, spec, cam, SEED, spp)

@@ -120,6 +120,17 @@ def test_shade_matches_oracle(gpu_ctx, name):
     assert np.quantile(rel, 0.999) <= 1e-4, np.quantile(rel, 0.999)
 
 
+def reproducible(a, b, tol=1e-5, frac=2e-3):
+    """Two renders of the mixed path agree to FP32 rounding: the FP64
+    renormalisation runs on a wave-uniform schedule, so a ray's renormalisation
+    steps depend on which rays share its wave, which the dynamic work
+    distribution decides at run time.  Pixels with a sample on an outcome
+    boundary may differ by that sample."""
+    a, b = a.double().cpu(), b.double().cpu()
+    rel = (a - b).abs() / b.abs().clamp_min(1e-6 * float(b.abs().max()))
+    assert torch.isfinite(a).all() and float((rel > tol).double().mean()) <= frac, float((rel > tol).double().mean())
+
+
 def test_render_frame_matches_stages_and_is_deterministic(gpu_ctx):
     integ, sc, spec, cam, spp = setup("kerr_disk_1080p", M.PREC_MIXED)
     gpu_ctx.set_scene(sc)
@@ -129,9 +140,17 @@ def test_render_frame_matches_stages_and_is_deterministic(gpu_ctx):
     hits, _ = gpu_ctx.trace(integ, sc, list(cam.position), rays)
     c = gpu_ctx.shade(sc, spec, cam, SEED, 0, spp, hits)
     torch.cuda.synchronize()
-    assert torch.equal(a, b) and torch.equal(a, c)
+    reproducible(a, b)
+    reproducible(c, a)
     st = gpu_ctx.stats()
     assert st.n_rays == cam.x_resolution * cam.y_resolution * spp and st.n_trials >= st.n_steps > 0
+    # the FP32-reference and FP64 paths make every decision per lane: bit for bit
+    for prec in (M.PREC_FP32_REFERENCE, M.PREC_FP64):
+        integ, sc, spec, cam, spp = setup("kerr_disk_1080p", prec)
+        a = gpu_ctx.render_frame(integ, sc, spec, cam, SEED, spp).clone()
+        b = gpu_ctx.render_frame(integ, sc, spec, cam, SEED, spp).clone()
+        torch.cuda.synchronize()
+        assert torch.equal(a, b)
 
 
 def test_trace_queue_subset(gpu_ctx):
