@@ -41,6 +41,19 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
     if (i < n) p[i] = 0ull;
 }
 
+// The trace kernels' work counters (kHeads of them, kHeadStride apart).
+__global__ void k_zero_heads(unsigned long long* p) {
+    const uint32_t i = threadIdx.x;
+    if (i < mgk::kHeads) p[uint64_t(i) * mgk::kHeadStride] = 0ull;
+}
+
+// Counter block of a context, in u64: the work counters on their own 128-byte
+// lines, then the statistics, the two hand-off queue counts and the reflected
+// ray count, each line apart.
+constexpr uint64_t kCtrStats = uint64_t(mgk::kHeads) * mgk::kHeadStride;   // [0] steps [1] trials [2] fp64 trials
+constexpr uint64_t kCtrZone0 = kCtrStats + 16, kCtrZone1 = kCtrStats + 32, kCtrReflect = kCtrStats + 48;
+constexpr uint64_t kCtrTotal = kCtrStats + 64;
+
 struct DeviceBuffer {
     void* ptr = nullptr;
     size_t bytes = 0;
@@ -65,7 +78,7 @@ struct DeviceBuffer {
 struct mgk_context {
     int device = 0;
     int n_cu = 0;
-    unsigned long long* d_counters = nullptr;   // [0] head [1..3] stats
+    unsigned long long* d_counters = nullptr;   // work counters, statistics, queue counts (kCtr*)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     uint64_t last_n = 0;
@@ -242,7 +255,7 @@ e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t 
     HIP_TRY(c->zone_q0.reserve(cap * sizeof(uint32_t)));
     HIP_TRY(c->zone_q1.reserve(cap * sizeof(uint32_t)));
     uint32_t* q[2] = {reinterpret_cast<uint32_t*>(c->zone_q0.ptr), reinterpret_cast<uint32_t*>(c->zone_q1.ptr)};
-    unsigned long long* cnt[2] = {c->d_counters + 4, c->d_counters + 5};
+    unsigned long long* cnt[2] = {c->d_counters + kCtrZone0, c->d_counters + kCtrZone1};
     const uint32_t* caller_queue = P.queue;
     uint64_t n_in = P.n;
     bool near = false;
@@ -263,7 +276,7 @@ e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t 
         Z.steps_out = reinterpret_cast<uint32_t*>(c->zone_steps[pass & 1].ptr);
         Z.state_in = reinterpret_cast<const double*>(c->zone_state[(pass + 1) & 1].ptr);
         Z.steps_in = reinterpret_cast<const uint32_t*>(c->zone_steps[(pass + 1) & 1].ptr);
-        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c->d_counters, 1);   // head
+        hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(64), 0, s, c->d_counters);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, cnt[pass & 1], 1);
         HIP_TRY(hipGetLastError());
         TraceParams Q = P;
@@ -288,7 +301,7 @@ e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t 
 e_magik_result_types trace_once(mgk_context* c, TraceParams& P, hipStream_t s) {
     if (P.n == 0) return MAGIK_SUCCESS;
     if (P.precision == MGK_PRECISION_MIXED && P.metric != MINKOWSKI) return do_trace_zones(c, P, s);
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c->d_counters, 1);   // head
+    hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(64), 0, s, c->d_counters);
     HIP_TRY(hipGetLastError());
     HIP_TRY(launch_trace(P, trace_grid(c, P), s));
     return MAGIK_SUCCESS;
@@ -316,8 +329,9 @@ e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, con
         for (int k = 0; k < 3; ++k) { P.mesh_lo[k] = c->mesh_lo[k]; P.mesh_hi[k] = c->mesh_hi[k]; }
     }
     P.head = c->d_counters;
-    P.stats = c->d_counters + 1;
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, c->d_counters, 4);
+    P.stats = c->d_counters + kCtrStats;
+    hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(64), 0, s, c->d_counters);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, P.stats, 3);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev0, s));
     c->last_reflections = 0;
@@ -351,7 +365,7 @@ e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, con
             R.queue_in = in;
             R.n_in = n_in;
             R.queue_out = reinterpret_cast<uint32_t*>(c->refl_q[k & 1].ptr);
-            R.count = c->d_counters + 6;
+            R.count = c->d_counters + kCtrReflect;
             R.seed = paths->seed;
             R.camera_id = paths->camera_id;
             R.npix = paths->npix;
@@ -776,7 +790,7 @@ MGK_API e_magik_result_types mgk_context_create(int device, mgk_context_handle* 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MGK_ERROR_HIP);
     c->n_cu = prop.multiProcessorCount;
-    if (hipMalloc(&c->d_counters, 8 * sizeof(unsigned long long)) != hipSuccess) return fail(MAGIK_ERROR_DEVICE_ALLOCATION_FAILED);
+    if (hipMalloc(&c->d_counters, kCtrTotal * sizeof(unsigned long long)) != hipSuccess) return fail(MAGIK_ERROR_DEVICE_ALLOCATION_FAILED);
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(MGK_ERROR_HIP);
     *out = c;
     return MAGIK_SUCCESS;
@@ -1008,12 +1022,12 @@ MGK_API e_magik_result_types mgk_last_trace_stats(mgk_context_handle c, mgk_trac
     HIP_TRY(hipEventSynchronize(c->ev1));
     float ms = 0.0f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    unsigned long long h[4];
-    HIP_TRY(hipMemcpy(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost));
+    unsigned long long h[3];
+    HIP_TRY(hipMemcpy(h, c->d_counters + kCtrStats, sizeof(h), hipMemcpyDeviceToHost));
     out->n_rays = c->last_n;
-    out->n_steps = h[1];
-    out->n_trials = h[2];
-    out->n_trials_fp64 = h[3];
+    out->n_steps = h[0];
+    out->n_trials = h[1];
+    out->n_trials_fp64 = h[2];
     out->ms = double(ms);
     out->n_reflections = c->last_reflections;
     return MAGIK_SUCCESS;

@@ -47,6 +47,10 @@ constexpr uint32_t kBvhEnd = 0xFFFFFFFFu;   // `skip` past the last node of a hi
 constexpr uint32_t kLeafShift = 24;         // leaf info = (count << 24) | first
 constexpr uint32_t kLeafMask = 0xFFFFFFu;
 constexpr int kInstF4 = 6;                  // float4 per instance record
+// Work counters of the persistent trace kernels: kHeads of them, kHeadStride
+// u64 apart (128 bytes), at the start of the context's counter block.
+constexpr uint32_t kHeads = 8;
+constexpr uint32_t kHeadStride = 16;
 constexpr int kHermiteSegments = 4;         // straight sub-chords that follow a Hermite chord
 constexpr int kHermiteFine = 8;             // finer sub-chords within the first coarse one that hits
 constexpr float kBaryEps = 1e-4f;           // barycentric slack of a Hermite crossing on its triangle

@@ -656,6 +656,30 @@ template <class S> __device__ __forceinline__ S load_c(const void* rays, uint64_
 // path's wave-uniform renormalisation then makes a ray's last bits depend on
 // the launch it was traced in.
 constexpr uint32_t kChunk = 64;
+
+// Takes the next 64-ray chunk for a wave from the work counters.  Chunk g of
+// [0, n) belongs to counter g % kHeads, which hands its chunks out in order;
+// the counters sit on separate 128-byte lines, so the waves of a pass spread
+// their atomics over kHeads addresses instead of one.  A wave starts on the
+// counter of its workgroup's XCD (workgroups go to the XCDs round-robin) and
+// moves on when that counter's chunks are gone; after every counter has come
+// up empty it is done.  Wave-uniform; false when no chunk is left.
+__device__ __forceinline__ bool take_chunk(unsigned long long* heads, uint64_t n, uint32_t lane, uint32_t& hc, uint32_t& tried,
+                                           uint64_t& chunk_next, uint64_t& chunk_end) {
+    for (;;) {
+        unsigned long long k = 0;
+        if (lane == 0) k = atomicAdd(heads + uint64_t(kHeadStride) * hc, 1ull);
+        k = __shfl(k, 0);
+        const uint64_t base = (uint64_t(hc) + uint64_t(kHeads) * k) * kChunk;
+        if (base < n) {
+            chunk_next = base;
+            chunk_end = base + kChunk < n ? base + kChunk : n;
+            return true;
+        }
+        if (++tried >= kHeads) return false;
+        hc = hc + 1 == kHeads ? 0u : hc + 1;
+    }
+}
 constexpr int kRefillMin = 8;     // zone kernels: idle lanes that trigger a refill
 // Zone kernels: the FP64 renormalisation of H, L and Q runs on the steps that
 // complete in every kRenormEvery-th iteration of the wave.  The schedule is the
@@ -688,19 +712,16 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
     uint64_t steps_acc = 0, trials_acc = 0, fp64_acc = 0;
 
     uint64_t chunk_next = 0, chunk_end = 0;   // wave-uniform
+    uint32_t head_c = blockIdx.x % kHeads, head_tried = 0;   // wave-uniform: work counter in use, counters found empty
     bool exhausted = false;                   // wave-uniform
 
     for (;;) {
         // ---- refill idle lanes (wave-uniform control flow) ----
         uint64_t idle = __ballot(!active);
         while (idle != 0 && !exhausted) {
-            if (chunk_next >= chunk_end) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(P.head, (unsigned long long)kChunk);
-                base = __shfl(base, 0);
-                if (base >= P.n) { exhausted = true; break; }
-                chunk_next = base;
-                chunk_end = base + kChunk < P.n ? base + kChunk : P.n;
+            if (chunk_next >= chunk_end && !take_chunk(P.head, P.n, lane, head_c, head_tried, chunk_next, chunk_end)) {
+                exhausted = true;
+                break;
             }
             uint32_t avail = uint32_t(chunk_end - chunk_next);
             uint32_t cnt = uint32_t(__popcll(idle));
@@ -837,6 +858,7 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
     uint32_t n_step = 0, n_sub = 0;
     uint64_t steps_acc = 0, trials_acc = 0, fp64_acc = 0;
     uint64_t chunk_next = 0, chunk_end = 0;
+    uint32_t head_c = blockIdx.x % kHeads, head_tried = 0;   // wave-uniform: work counter in use, counters found empty
     bool exhausted = false;
     uint64_t ob_base = 0;          // the wave's current block of out-queue slots
     uint32_t ob_used = kOutBlock;  // slots of it taken (kOutBlock: no block yet)
@@ -890,13 +912,9 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
         uint64_t idle = __ballot(!active);
         if (__popcll(idle) < kRefillMin && idle != ~0ull) idle = 0;
         while (idle != 0 && !exhausted) {
-            if (chunk_next >= chunk_end) {
-                unsigned long long base = 0;
-                if (lane == 0) base = atomicAdd(P.head, (unsigned long long)kChunk);
-                base = __shfl(base, 0);
-                if (base >= Z.n_in) { exhausted = true; break; }
-                chunk_next = base;
-                chunk_end = base + kChunk < Z.n_in ? base + kChunk : Z.n_in;
+            if (chunk_next >= chunk_end && !take_chunk(P.head, Z.n_in, lane, head_c, head_tried, chunk_next, chunk_end)) {
+                exhausted = true;
+                break;
             }
             const uint32_t avail = uint32_t(chunk_end - chunk_next);
             const uint32_t cnt = uint32_t(__popcll(idle));
