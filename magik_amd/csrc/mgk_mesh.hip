@@ -17,7 +17,7 @@
 namespace mgk {
 namespace {
 
-constexpr uint32_t kLeafMax = 4;
+constexpr uint32_t kLeafMax = 8;   // the traversal is bound by its dependent node loads: on box_mesh, 8 beat 2, 4 and 16
 constexpr int kBins = 16;
 constexpr int kMaxSahDepth = 48;   // deeper: median splits only, which bound the depth by log2(n)
 
