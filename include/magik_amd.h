@@ -380,6 +380,22 @@ MGK_API e_magik_result_types mgk_set_scene(mgk_context_handle ctx, const mgk_sce
 MGK_API e_magik_result_types mgk_set_meshes(mgk_context_handle ctx, const mgk_mesh_t* meshes, uint32_t n_mesh,
                                             const mgk_instance_t* instances, uint32_t n_instance);
 
+/* The hierarchy mgk_set_meshes would build, built and checked on the host
+ * only (no device is touched): the same input checks and result codes, then
+ * a walk of every level that verifies the skip links move forward, children
+ * lie inside their parents and every triangle sits in exactly one leaf.
+ * Returns MGK_ERROR_INVALID_ARGUMENT if a check fails.  No counterpart in the
+ * reference; for callers that validate geometry before uploading it. */
+typedef struct mgk_mesh_hierarchy_info_t {
+    uint64_t n_nodes;            /* all levels */
+    uint64_t n_triangles;        /* triangle slots over all meshes */
+    uint32_t max_depth;          /* deepest bottom level */
+    uint32_t max_leaf;           /* most triangles in a leaf */
+    float world_min[3], world_max[3];
+} mgk_mesh_hierarchy_info_t;
+MGK_API e_magik_result_types mgk_mesh_hierarchy_info(const mgk_mesh_t* meshes, uint32_t n_mesh, const mgk_instance_t* instances,
+                                                     uint32_t n_instance, mgk_mesh_hierarchy_info_t* info);
+
 /* Ray generation + initial conditions (kernel_generate_thin_lens_ray.cu:51-77,
  * kernel_generate_initial_conditions.cu:54-74) for every pixel and the samples
  * [sample_begin, sample_begin + spp).  Ray i = s * W * H + py * W + px is

@@ -107,6 +107,11 @@ class Instance(C.Structure):  # mgk_instance_t
     _fields_ = [("mesh", C.c_uint32), ("matter", C.c_uint32), ("object_to_world", C.c_float * 12)]
 
 
+class MeshHierarchyInfo(C.Structure):  # mgk_mesh_hierarchy_info_t
+    _fields_ = [("n_nodes", C.c_uint64), ("n_triangles", C.c_uint64), ("max_depth", C.c_uint32), ("max_leaf", C.c_uint32),
+                ("world_min", C.c_float * 3), ("world_max", C.c_float * 3)]
+
+
 class Spectral(C.Structure):  # mgk_spectral_t
     _fields_ = [("n_hero", C.c_uint32), ("hero_min_nm", C.c_float), ("hero_max_nm", C.c_float),
                 ("n_bin", C.c_uint32), ("bin_min_nm", C.c_float), ("bin_max_nm", C.c_float), ("sigma_nm", C.c_float),
@@ -210,6 +215,8 @@ SIGNATURES = {
     "mgk_context_destroy": (_R, [_P]),
     "mgk_set_scene": (_R, [_P, C.POINTER(Scene)]),
     "mgk_set_meshes": (_R, [_P, C.POINTER(Mesh), C.c_uint32, C.POINTER(Instance), C.c_uint32]),
+    "mgk_mesh_hierarchy_info": (_R, [C.POINTER(Mesh), C.c_uint32, C.POINTER(Instance), C.c_uint32,
+                                     C.POINTER(MeshHierarchyInfo)]),
     "mgk_generate_rays": (_R, [_P, C.POINTER(Integrator), C.POINTER(Camera), C.c_uint64, C.c_uint32, C.c_uint32, _P,
                                C.c_uint64, _P]),
     "mgk_trace_rays": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(C.c_float), _P, C.c_uint64,

@@ -272,3 +272,77 @@ e_magik_result_types build_meshes(const mgk_mesh_t* meshes, uint32_t n_mesh, con
 }
 
 }  // namespace mgk
+
+namespace {
+
+// Walks one hierarchy from its root with the skip links, as the device does,
+// checking the layout; counts leaf entries per slot.
+bool walk(const mgk::MeshBuild& b, uint32_t root, std::vector<uint32_t>& seen, bool top, uint32_t n_inst, uint32_t& depth,
+          uint32_t& max_leaf) {
+    using namespace mgk;
+    auto u = [](float f) { uint32_t x; std::memcpy(&x, &f, 4); return x; };
+    const uint64_t n_nodes = b.nodes.size() / 2;
+    std::vector<uint32_t> ends;   // skip targets of the open interior nodes (for the depth)
+    uint32_t n = root;
+    uint64_t steps = 0;
+    while (n != kBvhEnd) {
+        if (n >= n_nodes || ++steps > n_nodes) return false;
+        while (!ends.empty() && n >= ends.back()) ends.pop_back();
+        depth = std::max(depth, uint32_t(ends.size()) + 1);
+        const float4 lo = b.nodes[2 * uint64_t(n)], hi = b.nodes[2 * uint64_t(n) + 1];
+        const uint32_t skip = u(lo.w), info = u(hi.w);
+        if (skip != kBvhEnd && skip <= n) return false;
+        if (info == 0) {
+            if (uint64_t(n) + 1 >= n_nodes) return false;
+            const float4 clo = b.nodes[2 * (uint64_t(n) + 1)], chi = b.nodes[2 * (uint64_t(n) + 1) + 1];
+            if (clo.x < lo.x || clo.y < lo.y || clo.z < lo.z || chi.x > hi.x || chi.y > hi.y || chi.z > hi.z) return false;
+            ends.push_back(skip == kBvhEnd ? uint32_t(n_nodes) : skip);
+            ++n;
+            continue;
+        }
+        const uint32_t first = info & kLeafMask, count = info >> kLeafShift;
+        if (count == 0 || (top && count != 1)) return false;
+        max_leaf = std::max(max_leaf, count);
+        for (uint32_t j = first; j < first + count; ++j) {
+            if (j >= seen.size() || (top && j >= n_inst)) return false;
+            ++seen[j];
+        }
+        n = skip;
+    }
+    return true;
+}
+
+}  // namespace
+
+MGK_API e_magik_result_types mgk_mesh_hierarchy_info(const mgk_mesh_t* meshes, uint32_t n_mesh, const mgk_instance_t* instances,
+                                                     uint32_t n_instance, mgk_mesh_hierarchy_info_t* info) {
+    using namespace mgk;
+    if (!info) return MAGIK_ERROR_INVALID_POINTER;
+    std::memset(info, 0, sizeof(*info));
+    MeshBuild b;
+    e_magik_result_types r = build_meshes(meshes, n_mesh, instances, n_instance, b);
+    if (r != MAGIK_SUCCESS || n_mesh == 0) return r;
+    auto u = [](float f) { uint32_t x; std::memcpy(&x, &f, 4); return x; };
+    const uint64_t n_slots = b.tris.size() / 3;
+    // bottom levels: each mesh's root from its first instance; every slot in exactly one leaf
+    std::vector<uint32_t> seen(n_slots, 0), roots;
+    for (uint32_t k = 0; k < n_instance; ++k) {
+        const uint32_t root = u(b.inst[uint64_t(kInstF4) * k + 3].w);
+        if (std::find(roots.begin(), roots.end(), root) == roots.end()) roots.push_back(root);
+    }
+    uint32_t depth = 0, max_leaf = 0, top_depth = 0, top_leaf = 0;
+    for (uint32_t root : roots)
+        if (!walk(b, root, seen, false, n_instance, depth, max_leaf)) return MGK_ERROR_INVALID_ARGUMENT;
+    for (uint32_t c : seen)
+        if (c != 1u) return MGK_ERROR_INVALID_ARGUMENT;
+    std::vector<uint32_t> seen_top(n_instance, 0);
+    if (!walk(b, b.root, seen_top, true, n_instance, top_depth, top_leaf)) return MGK_ERROR_INVALID_ARGUMENT;
+    for (uint32_t c : seen_top)
+        if (c != 1u) return MGK_ERROR_INVALID_ARGUMENT;
+    info->n_nodes = b.nodes.size() / 2;
+    info->n_triangles = n_slots;
+    info->max_depth = depth;
+    info->max_leaf = max_leaf;
+    for (int c = 0; c < 3; ++c) { info->world_min[c] = b.lo[c]; info->world_max[c] = b.hi[c]; }
+    return MAGIK_SUCCESS;
+}

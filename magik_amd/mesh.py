@@ -97,3 +97,12 @@ def to_abi(meshes, instances):
         for j, x in enumerate(np.asarray(M, np.float64).reshape(12)):
             i_arr[k].object_to_world[j] = x
     return m_arr, i_arr, keep
+
+
+def hierarchy_info(meshes, instances):
+    """(result code, MeshHierarchyInfo) of the hierarchy mgk_set_meshes would
+    build, built and checked on the host (mgk_mesh_hierarchy_info)."""
+    m, inst, _keep = to_abi(meshes, instances)
+    info = abi.MeshHierarchyInfo()
+    r = abi.load().mgk_mesh_hierarchy_info(m, len(meshes), inst, len(instances), C.byref(info))
+    return r, info

@@ -220,3 +220,33 @@ def test_paths_off_mesh_matter_keep_the_static_redshift():
             expect = np.sqrt((1 - 2 / r_e) / (1 - 2 / r_o))
             np.testing.assert_allclose(h["g"][em], expect, rtol=2e-6)
     O.set_meshes([], [])
+
+
+# ---- the library's hierarchy build, on the host (no device needed) ----
+def test_hierarchy_build_is_sound_on_the_host():
+    """mgk_mesh_hierarchy_info builds the hierarchy mgk_set_meshes would
+    upload and walks it as the device does: skip links forward, children
+    inside parents, every triangle in exactly one leaf, one instance per
+    top-level leaf.  Degenerate inputs (all triangles at one point, a single
+    triangle, 100k random triangles) and invalid ones are covered."""
+    from magik_amd import abi
+    rng = np.random.default_rng(9)
+    n = 100_000
+    c = rng.uniform(-10, 10, (n, 1, 3))
+    v = (c + 0.1 * rng.uniform(-1, 1, (n, 3, 3))).reshape(-1, 3).astype(np.float32)
+    f = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    point = (np.zeros((3, 3), np.float32), np.zeros((17, 3), np.uint32))
+    cases = [([(v, f)], n), ([mesh.icosphere(4)], 5120), ([point], 17), ([mesh.box_mesh((0, 0, 0), (1, 1, 1))], 12)]
+    for meshes, tris in cases:
+        inst = [(0, M.MATTER_DIFFUSE, mesh.affine(1.0, (3.0 * k, 0.0, 0.0), rotate_y_deg=10.0 * k)) for k in range(5)]
+        r, info = mesh.hierarchy_info(meshes, inst)
+        assert r == abi.MAGIK_SUCCESS, (r, tris)
+        assert info.n_triangles == tris and 1 <= info.max_leaf <= 8 and info.max_depth < 64
+        assert info.world_max[0] > info.world_min[0]
+    v1, f1 = mesh.box_mesh((0, 0, 0), (1, 1, 1))
+    good = (0, M.MATTER_DIFFUSE, mesh.affine())
+    for meshes, inst in (([(v1, np.where(f1 == 7, 8, f1))], [good]),                         # vertex index out of range
+                         ([(v1, f1)], [(0, 5, mesh.affine())]),                               # matter
+                         ([(v1, f1)], [(1, M.MATTER_DIFFUSE, mesh.affine())]),                # mesh index
+                         ([(v1, f1)], [(0, M.MATTER_DIFFUSE, mesh.affine((1.0, 0.0, 1.0)))])):  # singular map
+        assert mesh.hierarchy_info(meshes, inst)[0] == abi.MGK_ERROR_INVALID_ARGUMENT
