@@ -430,8 +430,10 @@ __device__ __forceinline__ float planck(float lambda_nm, float T) {
     const float t0 = kPlanck * kC;
     const float t1 = 2.0f * t0 * kC;
     float t2 = lm * lm * lm * lm * lm;
-    float t3 = expf(t0 / (lm * kBoltzmann * T)) - 1.0f;
-    return (t1 / t2) * (1.0f / t3);
+    // hardware reciprocals (1 ulp) for the two quotients: IEEE FP32 division is
+    // a multi-instruction sequence, and shading evaluates this per hero wavelength
+    float t3 = expf(t0 * __builtin_amdgcn_rcpf(lm * kBoltzmann * T)) - 1.0f;
+    return t1 * __builtin_amdgcn_rcpf(t2 * t3);
 }
 
 // White LED spectrum: five Gaussians exp(-(x - mu)^2 / (2 w^2)) k
