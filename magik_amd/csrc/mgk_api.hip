@@ -248,6 +248,7 @@ e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t 
     // (waves of the largest launch) * kOutBlock.
     const uint64_t max_waves = uint64_t(c->n_cu) * 4u * 4u;
     const uint64_t cap = P.n + max_waves * kOutBlock;
+    if (cap >= (1ull << 32)) return MGK_ERROR_CAPACITY_EXCEEDED;   // the kernels hold slot indices in 32 bits
     for (int k = 0; k < 2; ++k) {
         HIP_TRY(c->zone_state[k].reserve(cap * kParked * sizeof(double)));
         HIP_TRY(c->zone_steps[k].reserve(cap * sizeof(uint32_t)));

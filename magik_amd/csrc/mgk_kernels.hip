@@ -669,10 +669,8 @@ __device__ __forceinline__ bool take_chunk(unsigned long long* heads, uint64_t n
     for (;;) {
         unsigned long long k = 0;
         if (lane == 0) k = atomicAdd(heads + uint64_t(kHeadStride) * hc, 1ull);
-        // lane 0's count to every lane, then into a scalar register: the chunk
-        // bookkeeping is wave-uniform and need not occupy vector registers
-        const uint32_t k32 = __builtin_amdgcn_readfirstlane(uint32_t(__shfl(k, 0)));
-        const uint64_t base = (uint64_t(hc) + uint64_t(kHeads) * k32) * kChunk;
+        k = __shfl(k, 0);
+        const uint64_t base = (uint64_t(hc) + uint64_t(kHeads) * k) * kChunk;
         if (base < n) {
             chunk_next = base;
             chunk_end = base + kChunk < n ? base + kChunk : n;
