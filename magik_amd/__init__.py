@@ -363,6 +363,7 @@ class Manager:
         h = C.c_void_p()
         abi.check(self.lib.magik_render_manager_create(C.byref(h), d), "magik_render_manager_create")
         self.h = h
+        self._last_stats = abi.RendererStatistics()
         fb = C.c_void_p()
         abi.check(self.lib.magik_configure_framebuffer_collection(C.byref(fb), abi.FB_CONFIG_HOST),
                   "magik_configure_framebuffer_collection")
@@ -400,9 +401,14 @@ class Manager:
         return self.lib.magik_render_manager_fetch_error(self.h)
 
     def stats(self):
+        """(statistics, whether they are new).  The library fills the struct only
+        when the worker has published statistics since the last fetch; the last
+        statistics fetched are returned otherwise."""
         st, new = abi.RendererStatistics(), C.c_bool(False)
         abi.check(self.lib.magik_fetch_renderer_statistics(self.h, C.byref(new), C.byref(st)), "magik_fetch_renderer_statistics")
-        return st, new.value
+        if new.value:
+            self._last_stats = st
+        return self._last_stats, new.value
 
     def fetch(self) -> tuple[int, bool]:
         new = C.c_bool(False)
