@@ -99,8 +99,11 @@ def test_struct_layouts_match_magik_h():
     assert mine == GOLD["layout"]
 
 
-@pytest.mark.skipif(O.ref() is None, reason="reference tree not present (golden vectors cover this)")
 def test_oracle_against_live_reference_harness():
+    # Checked here, not at collection, so that a run without the harness (a GPU
+    # run) never loads it.
+    if not os.path.exists(O.REF_LIB):
+        pytest.skip("reference harness not built here (golden vectors cover this)")
     R = O.ref()
     rng = np.random.default_rng(5)
     for _ in range(200):
