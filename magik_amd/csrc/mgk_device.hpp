@@ -45,7 +45,7 @@ template <bool FAST> __device__ __forceinline__ double rcp(double x) {
 // with one Newton step (relative error e -> 1.5 e^2).
 template <bool FAST> __device__ __forceinline__ float rsq(float x) {
     if constexpr (FAST) return __builtin_amdgcn_rsqf(x);
-    else return 1.0f / __fsqrt_rn(x);
+    else return 1.0f / sqrtf(x);
 }
 template <bool FAST> __device__ __forceinline__ double rsq(double x) {
     if constexpr (FAST) {
@@ -55,9 +55,13 @@ template <bool FAST> __device__ __forceinline__ double rsq(double x) {
         return 1.0 / __dsqrt_rn(x);
     }
 }
+// The IEEE square root is sqrtf: on this toolchain it compiles to v_sqrt_f32
+// plus the one-ulp correction that rounds correctly, while __fsqrt_rn compiles
+// to the bare v_sqrt_f32 (1 ulp), which made FP32 reference trajectories part
+// from the CPU's within a step.
 template <bool FAST> __device__ __forceinline__ float sqrt_(float x) {
     if constexpr (FAST) return __builtin_amdgcn_sqrtf(x);
-    else return __fsqrt_rn(x);
+    else return sqrtf(x);
 }
 template <bool FAST> __device__ __forceinline__ double sqrt_(double x) {
     if constexpr (FAST) {

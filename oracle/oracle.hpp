@@ -35,7 +35,13 @@ constexpr float kBoltzmann = 1.380649E-23f;
 constexpr float kGravitational = 6.67430E-11f;
 
 template <class R> inline R sqrt_(R v) { return std::sqrt(v); }
+// The step controller's x^0.2 (internal_math.h:2193).  The reference calls the
+// platform powf, whose last bit is not specified; here the FP32 instance is the
+// correctly rounded power (the double result rounded once to float), which the
+// GPU's FP32 reference path computes the same way, so both sides take the same
+// step sizes.
 template <class R> inline R pow_(R a, R b) { return std::pow(a, b); }
+template <> inline float pow_<float>(float a, float b) { return float(std::pow(double(a), double(b))); }
 template <class R> inline R abs_(R v) { return std::fabs(v); }
 template <class R> inline R max_(R a, R b) { return std::fmax(a, b); }
 

@@ -58,6 +58,11 @@ _SIG = {
     "orc_set_meshes": (C.c_int, [C.POINTER(abi.Mesh), C.c_uint32, C.POINTER(abi.Instance), C.c_uint32]),
     "orc_seg_tri": (C.c_int, [C.c_int, _D, _D, _D, _D, _D, C.c_double, _D]),
     "orc_mesh_chord": (C.c_int, [C.c_int, _D, _D, _D, _D, C.c_int, C.c_double, _D, _U]),
+    "orc_rms_error": (C.c_double, [C.c_int, _D, _D, _D, C.c_double, C.c_double]),
+    "orc_sensor_ray": (None, [C.c_int, C.c_double, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double,
+                              C.c_double, _D]),
+    "orc_radiance_to_irradiance": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                                C.c_double, C.c_double, C.c_uint32, C.c_uint32]),
 }
 
 _REF_SIG = {
@@ -152,6 +157,23 @@ def solve_step(prec: int, integ, y, h: float):
     tr = C.c_uint32(0)
     lib().orc_solve_step(prec, C.byref(integ), dptr(y), C.byref(hh), C.byref(ht), dptr(y1), C.byref(tr))
     return y1, hh.value, ht.value, tr.value
+
+
+def rms_error(prec: int, e, y0, y1, abs_tol: float, rel_tol: float) -> float:
+    """The controller's error norm; states as (x, y, z, t, p_x, p_y, p_z, p_t)."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (e, y0, y1)]
+    return lib().orc_rms_error(prec, dptr(a[0]), dptr(a[1]), dptr(a[2]), abs_tol, rel_tol)
+
+
+def sensor_ray(prec: int, sensor_width_mm: float, focal_mm: float, W: int, H: int, px: int, py: int, ju: float, jv: float):
+    """(sensor point in metres, unit pinhole direction) of pixel (px, py)."""
+    o = np.zeros(6)
+    lib().orc_sensor_ray(prec, sensor_width_mm, focal_mm, W, H, px, py, ju, jv, dptr(o))
+    return o[:3], o[3:]
+
+
+def radiance_to_irradiance(prec: int, L, throughput, sensor_w_mm, sensor_h_mm, iso, fps, shutter, W: int, H: int) -> float:
+    return lib().orc_radiance_to_irradiance(prec, L, throughput, sensor_w_mm, sensor_h_mm, iso, fps, shutter, W, H)
 
 
 def generate_rays(prec: int, integ, cam, seed: int, pixels, samples) -> np.ndarray:

@@ -622,4 +622,36 @@ void orc_star_catalogue(uint64_t seed, uint32_t n, float scale, float* out) {
     }
 }
 
+// The three per-ray pieces that the end-to-end comparisons only see composed:
+// the error norm of the step controller, the sensor point of a pixel and its
+// pinhole ray, and the radiance -> sensor irradiance scale.
+double orc_rms_error(int prec, const double e[8], const double y0[8], const double y1[8], double abs_tol, double rel_tol) {
+    if (prec == 0)
+        return double(rms_error(load_state<float>(e), load_state<float>(y0), load_state<float>(y1), float(abs_tol), float(rel_tol)));
+    return rms_error(load_state<double>(e), load_state<double>(y0), load_state<double>(y1), abs_tol, rel_tol);
+}
+
+// Sensor point (metres) and the pinhole ray's unit direction of pixel (px, py)
+// with jitter uniforms (ju, jv): out = {sx, sy, sz, dx, dy, dz}.
+void orc_sensor_ray(int prec, double sensor_width_mm, double focal_mm, uint32_t W, uint32_t H, uint32_t px, uint32_t py, double ju,
+                    double jv, double out[6]) {
+    auto run = [&](auto zero) {
+        using R = decltype(zero);
+        V3<R> sp = sensor_position<R>(R(sensor_width_mm), W, H, px, py, R(ju), R(jv));
+        V3<R> d, o;
+        pinhole_ray<R>(sp, R(focal_mm), &d, &o);
+        out[0] = double(sp.x); out[1] = double(sp.y); out[2] = double(sp.z);
+        out[3] = double(d.x); out[4] = double(d.y); out[5] = double(d.z);
+    };
+    if (prec == 0) run(0.0f); else run(0.0);
+}
+
+double orc_radiance_to_irradiance(int prec, double L, double throughput, double sensor_w_mm, double sensor_h_mm, double iso,
+                                  double fps, double shutter, uint32_t W, uint32_t H) {
+    if (prec == 0)
+        return double(radiance_to_irradiance<float>(float(L), float(throughput), float(sensor_w_mm), float(sensor_h_mm), float(iso),
+                                                    float(fps), float(shutter), W, H));
+    return radiance_to_irradiance<double>(L, throughput, sensor_w_mm, sensor_h_mm, iso, fps, shutter, W, H);
+}
+
 }  // extern "C"

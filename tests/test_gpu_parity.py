@@ -1,16 +1,19 @@
 """HIP path vs the CPU oracle on identical seeded rays (through the C-ABI).
 
-Bars (floating point, so by tolerance):
+Bars:
   * ray generation: FP32 reference path within 4 ulp of the FP32 oracle,
     FP64 path within 1e-12 relative of the FP64 oracle;
-  * tracing: termination class identical on >= 99.9 % of rays; for rays of the
-    same class, redshift g and hit coordinate within 1e-3 relative at the
-    99.9 % quantile (FP32 reference vs FP32 oracle, mixed vs FP64 oracle) or
-    3e-7 (FP64 vs FP64: the FP32 rounding of the stored record);
+  * FP32 reference tracing vs the FP32 oracle: bit for bit (every hit record
+    and step count; magik_trace_geodesic's records likewise), and linear
+    radiance within 1e-3 relative per pixel on >= 99.9 % of pixels;
+  * mixed / FP64 tracing: termination class identical on >= 99.9 % of rays;
+    for rays of the same class, redshift g and hit coordinate within 2e-3
+    relative at the 99.9 % quantile (mixed vs FP64 oracle) or 3e-7 (FP64 vs
+    FP64: the FP32 rounding of the stored record);
   * shading: per-pixel XYZ within 1e-4 relative of the FP64 oracle shade of
     the same hits;
   * mixed-precision frame vs FP64 oracle frame: mean relative per-pixel Y
-    error <= 1e-3.
+    error <= 1e-3, and a floor under the fraction of pixels within 1e-3.
 """
 import ctypes as C
 
@@ -79,10 +82,15 @@ def compare_hits(gh, rh, rtol):
 
 @pytest.mark.parametrize("name", list(SIZES))
 def test_trace_fp32_reference_matches_fp32_oracle(gpu_ctx, name):
+    """Both sides round every operation as IEEE FP32 in the same order (divide
+    and square root correctly rounded, the controller's x^0.2 the correctly
+    rounded power), so the GPU's trajectories are the oracle's bit for bit: the
+    same hit record and step count for every ray."""
     gh, gs, rh, rs, _ = trace_both(gpu_ctx, name, M.PREC_FP32_REFERENCE, O.F32)
-    compare_hits(gh, rh, 1e-3)
-    # FP32 at tol 1e-7: the controller runs on rounding noise, so step counts drift
-    assert np.mean(np.abs(gs.astype(np.int64) - rs) <= 2) >= 0.9
+    compare_hits(gh, rh, 1e-6)
+    for f in ("cls", "g", "a0", "a1"):
+        assert np.array_equal(gh[f].view(np.uint32), rh[f].view(np.uint32)), (f, np.mean(gh[f] != rh[f]))
+    assert np.array_equal(gs.astype(np.int64), rs.astype(np.int64))
 
 
 @pytest.mark.parametrize("name", list(SIZES))
@@ -103,6 +111,53 @@ def test_mixed_matches_fp64_oracle(gpu_ctx, name):
     eps = 1e-3 * y_ref.mean()
     err = np.abs(y_gpu - y_ref) / np.maximum(y_ref, eps)
     assert err.mean() <= 1e-3, err.mean()
+
+
+def per_pixel_y_error(y, y_ref):
+    """|Y - Y_ref| / Y_ref per pixel; pixels darker than 1e-3 of the frame's mean
+    are measured against that floor (a relative bar means nothing at zero)."""
+    floor = 1e-3 * float(np.mean(y_ref))
+    return np.abs(y - y_ref) / np.maximum(y_ref, floor)
+
+
+@pytest.mark.parametrize("name", ["schwarzschild_disk", "kerr_disk_1080p"])
+def test_fp32_reference_per_pixel_radiance_matches_fp32_oracle(gpu_ctx, name):
+    """The plain path's parity bar with the reference integrator: linear
+    radiance within 1e-3 relative per pixel.  GPU FP32-reference trace + GPU
+    shading against the FP32 oracle trace + the oracle's shading, on the same
+    seeded rays; >= 99.9 % of pixels must meet the bar."""
+    gh, _, rh, _, (integ, sc, spec, cam, spp, hits) = trace_both(gpu_ctx, name, M.PREC_FP32_REFERENCE, O.F32)
+    y_gpu = gpu_ctx.shade(sc, spec, cam, SEED, 0, spp, hits)[..., 1].cpu().numpy().ravel().astype(np.float64)
+    W, _ = M.color_weights(spec)
+    npx = cam.x_resolution * cam.y_resolution
+    y_ref = O.shade(sc, spec, cam, SEED, 0, spp, np.arange(npx), rh, W)[:, 1]
+    err = per_pixel_y_error(y_gpu, y_ref)
+    frac = float(np.mean(err <= 1e-3))
+    print(f"{name}: FP32 reference vs FP32 oracle, pixels within 1e-3: {frac:.5f} (max {err.max():.3g}, "
+          f"rays of another class {np.mean(gh['cls'] != rh['cls']):.2e})", flush=True)
+    assert frac >= 0.999, frac
+    # the hits are identical, so what remains is the shading's own FP32 rounding
+    assert err.max() <= 1e-4, err.max()
+
+
+@pytest.mark.parametrize("name,min_frac", [("schwarzschild_disk", 0.98), ("kerr_disk_1080p", 0.8)])
+def test_mixed_per_pixel_radiance_vs_fp64_oracle(gpu_ctx, name, min_frac):
+    """The product (mixed-precision) path against the FP64 oracle per pixel:
+    the fraction of pixels within 1e-3 relative, with both hit sets shaded by
+    the oracle's shading.  The paths take different step sequences, and the
+    disk crossing is located on each step's chord, whose error is O(h^2): on
+    the Kerr view 86 % of pixels meet 1e-3 (mean 5e-4), so the bar here is the
+    mean plus a floor under the measured fraction."""
+    gh, _, rh, _, (integ, sc, spec, cam, spp, hits) = trace_both(gpu_ctx, name, M.PREC_MIXED, O.F64)
+    W, _ = M.color_weights(spec)
+    npx = cam.x_resolution * cam.y_resolution
+    y_gpu = O.shade(sc, spec, cam, SEED, 0, spp, np.arange(npx), gh, W)[:, 1]
+    y_ref = O.shade(sc, spec, cam, SEED, 0, spp, np.arange(npx), rh, W)[:, 1]
+    err = per_pixel_y_error(y_gpu, y_ref)
+    frac = float(np.mean(err <= 1e-3))
+    print(f"{name}: mixed vs FP64 oracle, pixels within 1e-3: {frac:.5f}, mean {err.mean():.3g}, "
+          f"p99 {np.quantile(err, 0.99):.3g}", flush=True)
+    assert err.mean() <= 1e-3 and frac >= min_frac, (frac, err.mean())
 
 
 @pytest.mark.parametrize("name", list(SIZES))
@@ -201,13 +256,10 @@ def test_magik_trace_geodesic_matches_host_restatement(metric):
         arr = np.frombuffer(steps, dtype=np.float32).reshape(n, 15)[: cont.n_written].copy()
         out.append(arr)
     g, r = out
-    # the initial record is exact arithmetic of the same float inputs
-    assert np.allclose(g[0], r[0], rtol=2e-6, atol=1e-7 * np.abs(r[0]).max())
-    # after that FP32 rounding (libm ulps in pow) feeds the step controller; the
-    # trajectories stay close for the first steps and end in the same place
-    k = min(10, len(g), len(r))
-    assert np.allclose(g[:k, :4], r[:k, :4], rtol=1e-3, atol=1e-3 * np.abs(r[:k, :4]).max())
-    assert abs(len(g) - len(r)) <= max(3, 0.1 * len(r))
+    # IEEE FP32 in the same operation order on both sides: every record, initial
+    # conditions included, is the host restatement's bit for bit
+    assert len(g) == len(r)
+    assert np.array_equal(g.view(np.uint32), r.view(np.uint32))
 
 
 def test_trace_final_states(gpu_ctx):
