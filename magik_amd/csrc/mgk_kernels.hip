@@ -385,9 +385,17 @@ template <int METRIC, class S> __device__ __forceinline__ void lane_constants(co
     }
 }
 
-// Hit record of a terminated path (g, a0, a1).  The azimuths are taken with
-// FP32 atan2 (they are stored in FP32): an FP64 atan2 here alone would cost the
-// integration loop ~45 VGPRs.
+// Azimuth of a hit record.  The FP32 reference path takes the correctly rounded
+// atan2 (the double result rounded once), as the CPU oracle does, so its
+// records match the oracle's bit for bit; the other paths take FP32 atan2 of
+// the rounded arguments (the record is FP32 either way): an FP64 atan2 in the
+// zone kernels alone would cost their integration loops ~45 VGPRs.
+template <class S> __device__ __forceinline__ float azimuth(S z, S x) {
+    if constexpr (sizeof(S) == 4) return float(atan2(double(z), double(x)));
+    else return atan2f(float(z), float(x));
+}
+
+// Hit record of a terminated path (g, a0, a1).
 template <int METRIC, class S>
 __device__ __forceinline__ void hit_record(const St<S>& y0, const St<S>& y1, uint32_t cls, S best, S r_hit, S contraction,
                                         const TraceParams& P, S a, int box_n, const S hq[3], uint32_t m_tri, uint32_t m_inst,
@@ -435,18 +443,18 @@ __device__ __forceinline__ void hit_record(const St<S>& y0, const St<S>& y1, uin
         rec.g = float(g);
         if (cls == MGK_HIT_DISK) {
             rec.a0 = float(r_hit);
-            rec.a1 = atan2f(float(hz), float(hx));
+            rec.a1 = azimuth<S>(hz, hx);
         } else {
             S nx = hx - S(P.sphere_c[0]), ny = hy - S(P.sphere_c[1]), nz = hz - S(P.sphere_c[2]);
             S l = sqrt_<false>(nx * nx + ny * ny + nz * nz);
             rec.a0 = float(ny / l);
-            rec.a1 = atan2f(float(nz / l), float(nx / l));
+            rec.a1 = azimuth<S>(nz / l, nx / l);
         }
     } else if (cls == MGK_HIT_ESCAPE) {
         S dx = y1.qx - y0.qx, dy = y1.qy - y0.qy, dz = y1.qz - y0.qz;
         S l = sqrt_<false>(dx * dx + dy * dy + dz * dz);
         rec.a0 = float(dy / l);
-        rec.a1 = atan2f(float(dz / l), float(dx / l));
+        rec.a1 = azimuth<S>(dz / l, dx / l);
         S g = contraction / y1.pt;
         if (isnan(g) || isinf(g)) g = S(1e-10f);
         rec.g = float(g);
@@ -765,7 +773,10 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
         const float dl = float(h);   // parameter length of this trial's step (Hermite chords)
         bool step_done;
         if constexpr (METRIC == MINKOWSKI) {
-            // explicit Euler with the stored h (metric_minkowski.cuh:49-53)
+#pragma clang fp contract(off)
+            // explicit Euler with the stored h (metric_minkowski.cuh:49-53), each
+            // product and sum rounded on its own as the reference's vector
+            // operators do
             y1 = y;
             y1.qx = y.qx + h * y.px; y1.qy = y.qy + h * y.py; y1.qz = y.qz + h * y.pz; y1.qt = y.qt + h * -y.pt;
             step_done = true;

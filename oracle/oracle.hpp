@@ -42,6 +42,10 @@ template <class R> inline R sqrt_(R v) { return std::sqrt(v); }
 // step sizes.
 template <class R> inline R pow_(R a, R b) { return std::pow(a, b); }
 template <> inline float pow_<float>(float a, float b) { return float(std::pow(double(a), double(b))); }
+// Azimuths of the hit records: for FP32 the correctly rounded atan2 (the double
+// result rounded once), the value the GPU's FP32 reference path also records.
+template <class R> inline R atan2_(R y, R x) { return std::atan2(y, x); }
+template <> inline float atan2_<float>(float y, float x) { return float(std::atan2(double(y), double(x))); }
 template <class R> inline R abs_(R v) { return std::fabs(v); }
 template <class R> inline R max_(R a, R b) { return std::fmax(a, b); }
 
@@ -1027,11 +1031,11 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
                 }
                 if (cls == HIT_DISK) {
                     hit.a0 = r_hit;
-                    hit.a1 = std::atan2(qh.z, qh.x);
+                    hit.a1 = atan2_(qh.z, qh.x);
                 } else {
                     V3<R> n = norm3(V3<R>{qh.x - sc.sphere_center.x, qh.y - sc.sphere_center.y, qh.z - sc.sphere_center.z});
                     hit.a0 = n.y;
-                    hit.a1 = std::atan2(n.z, n.x);
+                    hit.a1 = atan2_(n.z, n.x);
                 }
             } else if (cls == HIT_BOX_DIFFUSE || cls == HIT_BOX_EMITTER) {
                 // the hit point on the chord, the momentum of the step's end (kernel_intersect.cu:129-131)
@@ -1050,7 +1054,7 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
             } else if (cls == HIT_ESCAPE) {
                 V3<R> d = norm3(V3<R>{y1.q.x - y0.q.x, y1.q.y - y0.q.y, y1.q.z - y0.q.z});
                 hit.a0 = d.y;
-                hit.a1 = std::atan2(d.z, d.x);
+                hit.a1 = atan2_(d.z, d.x);
                 hit.g = redshift_factor(ic.contraction, V4<R>{R(0), R(0), R(0), R(1)}, y1.p);
             }
             return hit;

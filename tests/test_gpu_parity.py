@@ -1,8 +1,8 @@
 """HIP path vs the CPU oracle on identical seeded rays (through the C-ABI).
 
 Bars:
-  * ray generation: FP32 reference path within 4 ulp of the FP32 oracle,
-    FP64 path within 1e-12 relative of the FP64 oracle;
+  * ray generation: FP32 reference path bit for bit with the FP32 oracle,
+    FP64 path within 1e-13 relative of the FP64 oracle;
   * FP32 reference tracing vs the FP32 oracle: bit for bit (every hit record
     and step count; magik_trace_geodesic's records likewise), and linear
     radiance within 1e-3 relative per pixel on >= 99.9 % of pixels;
@@ -51,6 +51,9 @@ def test_generate_rays_matches_oracle(gpu_ctx, name, prec):
     g = rays.cpu().numpy().T.astype(np.float64)
     p, s = pix_samples(cam, spp)
     ref = O.generate_rays(O.F32 if prec == M.PREC_FP32_REFERENCE else O.F64, integ, cam, SEED, p, s)
+    if prec == M.PREC_FP32_REFERENCE:
+        # IEEE FP32 in the reference's operation order on both sides: bit for bit
+        assert np.array_equal(g.astype(np.float32).view(np.uint32), ref.astype(np.float32).view(np.uint32))
     # per ray: position error relative to |x|, momentum error relative to |p|
     tol = 2e-6 if prec == M.PREC_FP32_REFERENCE else 1e-13
     for sl in (slice(0, 3), slice(4, 8)):
