@@ -416,6 +416,23 @@ MGK_API e_magik_result_types mgk_trace_rays(mgk_context_handle ctx, const mgk_in
                                             uint64_t n_rays, const uint32_t* d_queue, mgk_hit_t* d_hits,
                                             uint32_t* d_steps, void* stream);
 
+/* mgk_trace_rays that allocates nothing: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * checks and results as mgk_trace_rays; its work counters, statistics and
+ * (mixed precision) zone hand-off queues and parked states live in a fixed
+ * ~20 MB device array of the library, one per device, resolved when the
+ * context is created.  A mixed-precision call takes at most 16384 rays
+ * (MGK_ERROR_CAPACITY_EXCEEDED beyond); the FP32-reference and FP64
+ * precisions take any count mgk_trace_rays takes.  Reflections are not
+ * followed, and mgk_last_trace_stats keeps describing the last other trace.
+ * Calls that overlap on one device share that array and race: serialise them
+ * (one stream, or events). */
+MGK_API e_magik_result_types mgk_trace_rays_noalloc(mgk_context_handle ctx, const mgk_integrator_t* integ, const mgk_scene_t* scene,
+                                                    const float camera_position[3], const void* d_rays, uint64_t ray_stride,
+                                                    uint64_t n_rays, const uint32_t* d_queue, mgk_hit_t* d_hits,
+                                                    uint32_t* d_steps, void* stream);
+
 /* mgk_trace_rays that also writes each ray's state at termination (the end
  * of its last step; for box matter, the chord point of the hit with the
  * step's end momentum): d_final[c * ray_stride + i] for c = 0..7 = x, y,

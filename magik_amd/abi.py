@@ -221,6 +221,8 @@ SIGNATURES = {
                                C.c_uint64, _P]),
     "mgk_trace_rays": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(C.c_float), _P, C.c_uint64,
                             C.c_uint64, _P, _P, _P, _P]),
+    "mgk_trace_rays_noalloc": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(C.c_float), _P, C.c_uint64,
+                                    C.c_uint64, _P, _P, _P, _P]),
     "mgk_trace_rays_final": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(C.c_float), _P, C.c_uint64,
                                   C.c_uint64, _P, _P, _P, _P]),
     "mgk_trace_paths": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(Camera), C.c_uint64, C.c_uint32,

@@ -73,11 +73,76 @@ struct DeviceBuffer {
     }
 };
 
+// Device memory of mgk_trace_rays_noalloc (its work counters, statistics and,
+// for the mixed precision, the zone hand-off queues and parked states), so that
+// it allocates nothing.  Mixed-precision calls take at most kNoAllocZoneRays
+// rays: a pass reserves hand-off slots for at most its rays plus kOutBlock per
+// wave of its launch, and a launch of n <= kNoAllocZoneRays rays has at most
+// ceil(n / 256) blocks of 4 waves.  About 20 MB per device, held until the
+// process ends.
+constexpr uint64_t kNoAllocZoneRays = 16384;
+constexpr uint64_t kNoAllocZoneCap = kNoAllocZoneRays + (kNoAllocZoneRays + 255) / 256 * 4 * kOutBlock;
+constexpr uint64_t kScrCounters = 0;                                                     // kCtrTotal u64
+constexpr uint64_t kScrState = (kCtrTotal * 8 + 255) / 256 * 256;                        // 2 x kParked x cap doubles
+constexpr uint64_t kScrSteps = kScrState + 2 * kNoAllocZoneCap * kParked * 8;            // 2 x cap u32
+constexpr uint64_t kScrQueues = kScrSteps + 2 * kNoAllocZoneCap * 4;                     // 2 x cap u32
+constexpr uint64_t kScratchBytes = kScrQueues + 2 * kNoAllocZoneCap * 4;
+
+// The buffers of one trace that the caller does not pass: either the
+// context's (grown on demand) or a fixed workspace (zone_cap > 0).
+struct Workspace {
+    unsigned long long* counters = nullptr;   // kCtr* layout
+    double* zone_state[2] = {nullptr, nullptr};
+    uint32_t* zone_steps[2] = {nullptr, nullptr};
+    uint32_t* zone_q[2] = {nullptr, nullptr};
+    uint64_t zone_cap = 0;
+};
+
+}  // namespace
+
+// In host code `scratch` names a host-side stand-in; its device address comes
+// from hipGetSymbolAddress (per device, with that device current) and only that
+// address is handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of this device's scratch array, resolved once per device; the
+// device must be current.
+hipError_t scratch_address(int device, void** out) {
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+Workspace scratch_workspace(void* base) {
+    unsigned char* b = static_cast<unsigned char*>(base);
+    Workspace w;
+    w.counters = reinterpret_cast<unsigned long long*>(b + kScrCounters);
+    for (int k = 0; k < 2; ++k) {
+        w.zone_state[k] = reinterpret_cast<double*>(b + kScrState + uint64_t(k) * kNoAllocZoneCap * kParked * 8);
+        w.zone_steps[k] = reinterpret_cast<uint32_t*>(b + kScrSteps + uint64_t(k) * kNoAllocZoneCap * 4);
+        w.zone_q[k] = reinterpret_cast<uint32_t*>(b + kScrQueues + uint64_t(k) * kNoAllocZoneCap * 4);
+    }
+    w.zone_cap = kNoAllocZoneCap;
+    return w;
+}
 }  // namespace
 
 struct mgk_context {
     int device = 0;
     int n_cu = 0;
+    void* scratch = nullptr;                     // this device's scratch array (device address)
     unsigned long long* d_counters = nullptr;   // work counters, statistics, queue counts (kCtr*)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -241,22 +306,42 @@ int trace_grid(mgk_context* c, const TraceParams& P) {
 
 // Mixed precision: alternate the far-zone (FP32) and near-zone (FP64) passes
 // until no ray is handed over.  Reads one queue count per pass (a 8-byte copy).
-e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t s) {
+e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t s, const Workspace& ws) {
     if (P.n >= uint64_t(kHole)) return MGK_ERROR_CAPACITY_EXCEEDED;   // ray ids travel as uint32 below kHole
     // A pass hands off each of its rays at most once; slots are reserved kOutBlock
     // per wave, so the unused tails of the waves' last blocks add at most
-    // (waves of the largest launch) * kOutBlock.
-    const uint64_t max_waves = uint64_t(c->n_cu) * 4u * 4u;
-    const uint64_t cap = P.n + max_waves * kOutBlock;
+    // (waves of its launch) * kOutBlock.  A later pass reads slots, holes
+    // included, so its slot count may exceed the rays and its launch the first
+    // pass's: with the context's buffers every launch may take the whole GPU
+    // (each wave then starts on one 64-slot chunk, as in the first pass); in a
+    // fixed workspace the later launches are capped at the first pass's, which
+    // bounds every pass by n rays plus (waves of the first launch) * kOutBlock.
+    TraceParams Q0 = P;
+    const bool fixed = ws.zone_cap > 0;
+    const int grid0 = fixed ? trace_grid(c, Q0) : c->n_cu * 4;
+    const uint64_t max_waves = uint64_t(grid0) * 4u;
+    uint64_t cap = P.n + max_waves * kOutBlock;
     if (cap >= (1ull << 32)) return MGK_ERROR_CAPACITY_EXCEEDED;   // the kernels hold slot indices in 32 bits
-    for (int k = 0; k < 2; ++k) {
-        HIP_TRY(c->zone_state[k].reserve(cap * kParked * sizeof(double)));
-        HIP_TRY(c->zone_steps[k].reserve(cap * sizeof(uint32_t)));
+    double* state[2];
+    uint32_t* stp[2];
+    uint32_t* q[2];
+    if (ws.zone_cap > 0) {
+        if (cap > ws.zone_cap) return MGK_ERROR_CAPACITY_EXCEEDED;
+        cap = ws.zone_cap;
+        for (int k = 0; k < 2; ++k) { state[k] = ws.zone_state[k]; stp[k] = ws.zone_steps[k]; q[k] = ws.zone_q[k]; }
+    } else {
+        for (int k = 0; k < 2; ++k) {
+            HIP_TRY(c->zone_state[k].reserve(cap * kParked * sizeof(double)));
+            HIP_TRY(c->zone_steps[k].reserve(cap * sizeof(uint32_t)));
+            state[k] = reinterpret_cast<double*>(c->zone_state[k].ptr);
+            stp[k] = reinterpret_cast<uint32_t*>(c->zone_steps[k].ptr);
+        }
+        HIP_TRY(c->zone_q0.reserve(cap * sizeof(uint32_t)));
+        HIP_TRY(c->zone_q1.reserve(cap * sizeof(uint32_t)));
+        q[0] = reinterpret_cast<uint32_t*>(c->zone_q0.ptr);
+        q[1] = reinterpret_cast<uint32_t*>(c->zone_q1.ptr);
     }
-    HIP_TRY(c->zone_q0.reserve(cap * sizeof(uint32_t)));
-    HIP_TRY(c->zone_q1.reserve(cap * sizeof(uint32_t)));
-    uint32_t* q[2] = {reinterpret_cast<uint32_t*>(c->zone_q0.ptr), reinterpret_cast<uint32_t*>(c->zone_q1.ptr)};
-    unsigned long long* cnt[2] = {c->d_counters + kCtrZone0, c->d_counters + kCtrZone1};
+    unsigned long long* cnt[2] = {ws.counters + kCtrZone0, ws.counters + kCtrZone1};
     const uint32_t* caller_queue = P.queue;
     uint64_t n_in = P.n;
     bool near = false;
@@ -273,17 +358,16 @@ e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t 
         Z.out_count = cnt[pass & 1];
         // pass p parks into buffer p % 2 and reads what pass p - 1 parked
         Z.cap = cap;
-        Z.state_out = reinterpret_cast<double*>(c->zone_state[pass & 1].ptr);
-        Z.steps_out = reinterpret_cast<uint32_t*>(c->zone_steps[pass & 1].ptr);
-        Z.state_in = reinterpret_cast<const double*>(c->zone_state[(pass + 1) & 1].ptr);
-        Z.steps_in = reinterpret_cast<const uint32_t*>(c->zone_steps[(pass + 1) & 1].ptr);
-        hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(64), 0, s, c->d_counters);
+        Z.state_out = state[pass & 1];
+        Z.steps_out = stp[pass & 1];
+        Z.state_in = state[(pass + 1) & 1];
+        Z.steps_in = stp[(pass + 1) & 1];
+        hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(64), 0, s, ws.counters);
         hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, cnt[pass & 1], 1);
         HIP_TRY(hipGetLastError());
         TraceParams Q = P;
         Q.n = n_in;
-        const int grid = trace_grid(c, Q);
-        if (uint64_t(grid) * 4u > max_waves) return MGK_ERROR_CAPACITY_EXCEEDED;   // the slot bound above
+        const int grid = std::min(trace_grid(c, Q), grid0);   // the slot bound above
         HIP_TRY(launch_trace_zone(Q, Z, near || Z.no_handoff, grid, s));
         ++pass;
         unsigned long long next = 0;
@@ -299,10 +383,10 @@ e_magik_result_types do_trace_zones(mgk_context* c, TraceParams& P, hipStream_t 
 
 // One trace of P.n rays (of P.queue when given): the zone passes, or one
 // persistent launch.
-e_magik_result_types trace_once(mgk_context* c, TraceParams& P, hipStream_t s) {
+e_magik_result_types trace_once(mgk_context* c, TraceParams& P, hipStream_t s, const Workspace& ws) {
     if (P.n == 0) return MAGIK_SUCCESS;
-    if (P.precision == MGK_PRECISION_MIXED && P.metric != MINKOWSKI) return do_trace_zones(c, P, s);
-    hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(64), 0, s, c->d_counters);
+    if (P.precision == MGK_PRECISION_MIXED && P.metric != MINKOWSKI) return do_trace_zones(c, P, s, ws);
+    hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(64), 0, s, ws.counters);
     HIP_TRY(hipGetLastError());
     HIP_TRY(launch_trace(P, trace_grid(c, P), s));
     return MAGIK_SUCCESS;
@@ -318,9 +402,17 @@ struct PathInfo {
 // that ended on diffuse matter and trace them again, up to max_reflections
 // times (kernel_bxdf.cu:69-95, kernel_enforce_ray_limits.cu:48-50).  Each
 // round reads one count back (an 8-byte copy).  Reflected rays get new initial
-// conditions in place in P.rays.
+// conditions in place in P.rays.  With a fixed workspace (fixed->zone_cap > 0)
+// nothing is allocated: no reflections, and the context's statistics and
+// timing events are left as they were.
 e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, const PathInfo* paths = nullptr,
-                              uint32_t max_reflections = 0) {
+                              uint32_t max_reflections = 0, const Workspace* fixed = nullptr) {
+    Workspace ws;
+    if (fixed) {
+        ws = *fixed;
+    } else {
+        ws.counters = c->d_counters;
+    }
     if (P.flags & MGK_SCENE_MESHES) {
         if (!c->mesh_valid) return MGK_ERROR_INVALID_ARGUMENT;   // no meshes set on this context
         P.mesh_nodes = reinterpret_cast<const float4*>(c->mesh_nodes.ptr);
@@ -329,11 +421,12 @@ e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, con
         P.mesh_root = c->mesh_root;
         for (int k = 0; k < 3; ++k) { P.mesh_lo[k] = c->mesh_lo[k]; P.mesh_hi[k] = c->mesh_hi[k]; }
     }
-    P.head = c->d_counters;
-    P.stats = c->d_counters + kCtrStats;
-    hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(64), 0, s, c->d_counters);
+    P.head = ws.counters;
+    P.stats = ws.counters + kCtrStats;
+    hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(64), 0, s, ws.counters);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, P.stats, 3);
     HIP_TRY(hipGetLastError());
+    if (fixed) return trace_once(c, P, s, ws);
     HIP_TRY(hipEventRecord(c->ev0, s));
     c->last_reflections = 0;
     const bool reflect = paths && max_reflections > 0 && P.n > 0 && has_diffuse_matter(c, P);
@@ -350,7 +443,7 @@ e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, con
             P.steps_out = reinterpret_cast<uint32_t*>(c->path_steps.ptr);
         }
     }
-    e_magik_result_types r = trace_once(c, P, s);
+    e_magik_result_types r = trace_once(c, P, s, ws);
     if (r != MAGIK_SUCCESS) return r;
     if (reflect) {
         const uint32_t* in = P.queue;
@@ -388,7 +481,7 @@ e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, con
             Q.queue = R.queue_out;
             Q.n = cnt;
             Q.steps_in = P.steps_out;
-            if ((r = trace_once(c, Q, s)) != MAGIK_SUCCESS) return r;
+            if ((r = trace_once(c, Q, s, ws)) != MAGIK_SUCCESS) return r;
             in = R.queue_out;
             n_in = cnt;
         }
@@ -792,6 +885,13 @@ MGK_API e_magik_result_types mgk_context_create(int device, mgk_context_handle* 
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(MGK_ERROR_HIP);
     c->n_cu = prop.multiProcessorCount;
     if (hipMalloc(&c->d_counters, kCtrTotal * sizeof(unsigned long long)) != hipSuccess) return fail(MAGIK_ERROR_DEVICE_ALLOCATION_FAILED);
+    {
+        const hipError_t e = scratch_address(device, &c->scratch);
+        if (e != hipSuccess) {
+            std::fprintf(stderr, "magik_amd: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return fail(MGK_ERROR_HIP);
+        }
+    }
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(MGK_ERROR_HIP);
     *out = c;
     return MAGIK_SUCCESS;
@@ -889,6 +989,33 @@ MGK_API e_magik_result_types mgk_trace_rays(mgk_context_handle c, const mgk_inte
     P.hits = d_hits;
     P.steps_out = d_steps;
     return do_trace(c, P, hipStream_t(stream));
+}
+
+MGK_API e_magik_result_types mgk_trace_rays_noalloc(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc,
+                                                    const float cam_pos[3], const void* d_rays, uint64_t stride, uint64_t n,
+                                                    const uint32_t* d_queue, mgk_hit_t* d_hits, uint32_t* d_steps, void* stream) {
+    if (!c || !sc || !cam_pos || !d_hits || (n > 0 && !d_rays)) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = check_integrator(in);
+    if (r != MAGIK_SUCCESS) return r;
+    if ((r = check_scene(sc)) != MAGIK_SUCCESS) return r;
+    if (!d_queue && n > stride) return MGK_ERROR_CAPACITY_EXCEEDED;
+    if (in->precision == MGK_PRECISION_MIXED && in->metric_type != MINKOWSKI && n > kNoAllocZoneRays) {
+        std::fprintf(stderr, "magik_amd: mgk_trace_rays_noalloc: %llu rays exceed the %llu a mixed-precision call may take\n",
+                     (unsigned long long)n, (unsigned long long)kNoAllocZoneRays);
+        return MGK_ERROR_CAPACITY_EXCEEDED;
+    }
+    if (!c->scratch) return MGK_ERROR_HIP;
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    TraceParams P;
+    fill_trace_params(in, sc, cam_pos, P);
+    P.rays = d_rays;
+    P.stride = stride;
+    P.n = n;
+    P.queue = d_queue;
+    P.hits = d_hits;
+    P.steps_out = d_steps;
+    const Workspace ws = scratch_workspace(c->scratch);
+    return do_trace(c, P, hipStream_t(stream), nullptr, 0, &ws);
 }
 
 MGK_API e_magik_result_types mgk_trace_rays_final(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc,
