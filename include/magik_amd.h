@@ -251,7 +251,12 @@ enum {
     MGK_SCENE_STARS = 4u,
     MGK_SCENE_BACKGROUND = 8u,
     MGK_SCENE_BOXES = 16u,
-    MGK_SCENE_MESHES = 32u       /* the context's triangle meshes (mgk_set_meshes) */
+    MGK_SCENE_MESHES = 32u,      /* the context's triangle meshes (mgk_set_meshes) */
+    /* With MGK_SCENE_DISK: locate the disk crossing on the geodesic (one RKF45
+     * trial from the step's start to the chord's crossing, then a Newton step
+     * along the tangent to the plane) instead of on the step's chord, whose
+     * crossing is off by O(h^2) of the step's curvature. */
+    MGK_SCENE_DISK_ON_GEODESIC = 64u
 };
 
 /* How a completed step's chord meets triangle meshes.  AFFINE: the straight

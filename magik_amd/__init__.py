@@ -16,7 +16,8 @@ from . import mesh
 from .abi import (CHORD_AFFINE, CHORD_HERMITE, HIT_BOX_DIFFUSE, HIT_BOX_EMITTER, HIT_DISK, HIT_ESCAPE, HIT_HORIZON,
                   HIT_MESH_DIFFUSE, HIT_MESH_EMITTER, HIT_NONE, HIT_SPHERE, HIT_STEP_LIMIT, KERR, MATTER_DIFFUSE,
                   MATTER_ILLUMINANT, MINKOWSKI, PREC_FP32_REFERENCE, PREC_FP64, PREC_MIXED, SCHWARZSCHILD,
-                  SCENE_BACKGROUND, SCENE_BOXES, SCENE_DISK, SCENE_MESHES, SCENE_SPHERE, SCENE_STARS)
+                  SCENE_BACKGROUND, SCENE_BOXES, SCENE_DISK, SCENE_DISK_ON_GEODESIC, SCENE_MESHES, SCENE_SPHERE,
+                  SCENE_STARS)
 
 DEFAULT_MASS_KG = 6.7329546075e26       # relativity_settings.h:21,24
 G, C_LIGHT = 6.67430e-11, 299792458.0
@@ -109,7 +110,9 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
                         FP64, tol 1e-8, cap 4096.
     schwarzschild_disk: M=1, 512x512x16, r=30M, theta=85 deg, 40 deg; thin disk
                         6M-20M with T = 1e4 K (r/6)^-3/4; 1e5 seeded stars;
-                        tol 1e-7, cap 1e4.
+                        tol 1e-7, cap 1e4.  The disk crossing is located on
+                        the geodesic (SCENE_DISK_ON_GEODESIC) in every disk
+                        workload.
     kerr_disk_1080p   : a=0.9, 1920x1080x32, r=50M, theta=80 deg, 25 deg; disk
                         r_ISCO-30M, same T law; 4 stratified hero lambdas;
                         same stars; tol 1e-7, cap 1e4.
@@ -139,7 +142,7 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
         W, H, spp = W or 512, H or 512, spp or 16
         integ = integrator(metric=SCHWARZSCHILD, precision=PREC_MIXED if precision is None else precision, spin=0.0)
         cam = camera_bl(30.0, 85.0, 0.0, 40.0, W, H)
-        sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_STARS, disk_r_in=6.0,
+        sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_DISK_ON_GEODESIC | SCENE_STARS, disk_r_in=6.0,
                    disk_r_out=20.0, disk_T_in=1e4, n_star=100000, star_seed=0x5EED, star_scale=1.0,
                    star_radius_rad=1e-3, star_T=6500.0)
         spec = spectral(n_hero=1)
@@ -150,7 +153,8 @@ def workload(name: str, W: int | None = None, H: int | None = None, spp: int | N
         a = 0.99 if big else 0.9
         integ = integrator(metric=KERR, precision=PREC_MIXED if precision is None else precision, spin=a)
         cam = camera_bl(50.0, 80.0, a, 25.0, W, H)
-        sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_STARS, disk_r_in=r_isco(a),
+        sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_DISK_ON_GEODESIC | SCENE_STARS,
+                   disk_r_in=r_isco(a),
                    disk_r_out=30.0, disk_T_in=1e4, n_star=100000, star_seed=0x5EED, star_scale=1.0,
                    star_radius_rad=1e-3, star_T=6500.0)
         spec = spectral(n_hero=4)
@@ -210,7 +214,8 @@ def batch_workload(seed: int = 4, n_cam: int = 64, resolutions=(128, 256, 512, 1
         cams[k].camera = camera_bl(r, th, a, fov_deg, res, res, camera_id=k)
         cams[k].spin = a
     integ = integrator(metric=KERR, precision=precision, spin=0.0)
-    sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_STARS, disk_r_in=0.0, disk_r_out=30.0,
+    sc = scene(celestial_radius_m=1000.0 * length_scale(), flags=SCENE_DISK | SCENE_DISK_ON_GEODESIC | SCENE_STARS, disk_r_in=0.0,
+               disk_r_out=30.0,
                disk_T_in=1e4, n_star=100000, star_seed=0x5EED, star_scale=1.0, star_radius_rad=1e-3, star_T=6500.0)
     return integ, sc, spectral(n_hero=4), cams, spp
 

@@ -36,6 +36,7 @@ HIT_NAMES = {0: "none", 1: "horizon", 2: "escape", 3: "disk", 4: "sphere", 5: "s
              7: "box_emitter", 8: "mesh_diffuse", 9: "mesh_emitter"}
 
 SCENE_DISK, SCENE_SPHERE, SCENE_STARS, SCENE_BACKGROUND, SCENE_BOXES, SCENE_MESHES = 1, 2, 4, 8, 16, 32
+SCENE_DISK_ON_GEODESIC = 64
 CHORD_AFFINE, CHORD_HERMITE = 0, 1
 MAX_MESH_TRIANGLES = MAX_INSTANCES = 16777215
 MATTER_DIFFUSE, MATTER_ILLUMINANT = 1, 2

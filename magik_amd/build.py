@@ -3,7 +3,8 @@
     python -m magik_amd.build          # or __graft_entry__.build()
 
 Generates csrc/cie1931_table.inc from data/cie1931_2deg_1nm.csv, then
-compiles csrc/*.hip into magik_amd/libmagik_amd.so.
+compiles csrc/*.hip (one object each, in parallel, under magik_amd/_build/) and
+links magik_amd/libmagik_amd.so.
 """
 from __future__ import annotations
 
@@ -47,10 +48,7 @@ def gen_table() -> str:
     return path
 
 
-def build(verbose: bool = False) -> str:
-    gen_table()
-    cmd = [HIPCC, *FLAGS, f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
-           *[os.path.join(CSRC, s) for s in SOURCES], "-o", OUT + ".tmp"]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -59,6 +57,22 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{err}")
     if err.strip() and verbose:
         print(err, file=sys.stderr)
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles every source to an object (in parallel), then links the library."""
+    from concurrent.futures import ThreadPoolExecutor
+    gen_table()
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    inc = [f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}"]
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    objs = [os.path.join(objdir, s.replace(".hip", ".o")) for s in SOURCES]
+    jobs = [[HIPCC, *compile_flags, *inc, "-c", os.path.join(CSRC, s), "-o", o] for s, o in zip(SOURCES, objs)]
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(lambda c: _run(c, verbose), jobs))
+    _run([HIPCC, *FLAGS, *objs, "-o", OUT + ".tmp"], verbose)
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

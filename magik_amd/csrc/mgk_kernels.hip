@@ -76,7 +76,8 @@ __device__ __forceinline__ double powr(double a, double b) { return pow(a, b); }
 // One RKF45 trial in the reference's arithmetic (internal_math.h:2167-2199),
 // contraction off so the FP32 instance rounds as the reference does.
 template <int METRIC, class S>
-__device__ __forceinline__ St<S> trial_reference(const St<S>& y0, S th, S a, S ms, S at, S rt, S h_max, S& h_new, bool& acc) {
+__device__ __forceinline__ St<S> trial_reference(const St<S>& y0, S th, S a, S ms, S at, S rt, S h_max, S& h_new, bool& acc,
+                                                 St<S>* k5_out = nullptr) {
 #pragma clang fp contract(off)
     St<S> k1 = rhs_ref<METRIC, S>(y0, a, ms);
     St<S> k2 = rhs_ref<METRIC, S>(add(y0, mul(S(1) / S(4) * th, k1)), a, ms);
@@ -92,6 +93,7 @@ __device__ __forceinline__ St<S> trial_reference(const St<S>& y0, S th, S a, S m
                 mul(S(1859) / S(4104) * th, k4)),
             mul(S(11) / S(40) * th, k5)),
         a, ms);
+    if (k5_out) *k5_out = k5;
     St<S> y1 = add(sub(add(add(add(y0, mul(S(16) / S(135) * th, k1)), mul(S(6656) / S(12825) * th, k3)),
                            mul(S(28561) / S(56430) * th, k4)),
                        mul(S(9) / S(50) * th, k5)),
@@ -149,7 +151,7 @@ constexpr double kET[6] = {ET1, ET2, ET3, ET4, ET5, 0};
 
 template <int METRIC, class SS, bool FAST>
 __device__ __forceinline__ void trial_split(const St<double>& y0, double h, float a, float ms, const TraceParams& P, St<double>& y1,
-                                            double& rms_out) {
+                                            double& rms_out, SS* tangent5 = nullptr) {
     const SS qx = SS(y0.qx), qy = SS(y0.qy), qz = SS(y0.qz);
     const SS px = SS(y0.px), py = SS(y0.py), pz = SS(y0.pz), pt = SS(y0.pt);
     const SS hs = SS(h), as = SS(a), mss = SS(ms);
@@ -171,6 +173,9 @@ __device__ __forceinline__ void trial_split(const St<double>& y0, double h, floa
         const SS spx = px + Sp[i][0], spy = py + Sp[i][1], spz = pz + Sp[i][2];
         const Rhs<SS> r = rhs<METRIC, SS, FAST>(sqx, sqy, sqz, spx, spy, spz, pt, as, mss);
         const SS kx = spx - r.cx, ky = spy - r.cy, kz = spz - r.cz;
+        if (i == 4 && tangent5) {   // dx/dlambda of the fifth stage, evaluated at the trial's end (c5 = 1)
+            tangent5[0] = kx; tangent5[1] = ky; tangent5[2] = kz;
+        }
 #pragma unroll
         for (int j = i + 1; j < 6; ++j) {
             const SS c = SS(kA[j][i]) * hs;
@@ -504,9 +509,35 @@ __device__ __forceinline__ bool box_chord(T ox, T oy, T oz, T dx, T dy, T dz, T 
     return true;
 }
 
-template <int METRIC, class S, bool FAST_TESTS = false, bool MESH = false>
+// The disk crossing on the geodesic (MGK_SCENE_DISK_ON_GEODESIC): from the
+// step's start y0, one RKF45 trial over the chord's crossing parameter t h --
+// the 5th-order solution, no step control -- then one Newton step to the plane
+// y = 0 along the tangent dx/dlambda of the trial's fifth stage, which
+// Fehlberg's tableau evaluates at the trial's end (c5 = 1).  The remaining
+// offset is second order in the chord estimate's error.  yc and the tangent
+// (dx, dy, dz) come from the caller's trial; false when the corrected point
+// lies off the disk (the path then goes on).
+template <class S>
+__device__ __forceinline__ bool disk_point(const St<S>& yc, S dx, S dy, S dz, S a, S r_in, S r_max, S& r, S& x, S& z) {
+#pragma clang fp contract(off)
+    const S dl = -yc.qy / dy;
+    x = yc.qx + dl * dx;
+    z = yc.qz + dl * dz;
+    const S rho2 = x * x + z * z - a * a;
+    if (!(rho2 > S(0))) return false;
+    r = sqrt_<false>(rho2);
+    return !(r < r_in || r > r_max);
+}
+
+// A refine functor that does nothing: scenes without the on-geodesic disk.
+struct NoDiskRefine {
+    template <class S> __device__ bool operator()(S, S&, S&, S&) const { return true; }
+};
+
+template <int METRIC, class S, bool FAST_TESTS = false, bool MESH = false, class DiskRefine = NoDiskRefine>
 __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, uint32_t n_step, S contraction, const TraceParams& P,
-                                             const Lane<S>& ln, float dl, mgk_hit_t& rec, S& t_hit, S hq[3]) {
+                                             const Lane<S>& ln, float dl, mgk_hit_t& rec, S& t_hit, S hq[3],
+                                             const DiskRefine& refine = DiskRefine{}) {
 #pragma clang fp contract(off)
     const S a = METRIC == KERR ? ln.a : S(0);
     S best = S(1e30f);
@@ -514,6 +545,8 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
     S r_hit = S(0);
     int box_n = 0;
     uint32_t m_tri = 0, m_inst = 0;
+    bool disk_geo = false;
+    S disk_x = S(0), disk_z = S(0);
     if (P.flags & MGK_SCENE_DISK) {
         S ya = y0.qy, yb = y1.qy;
         if ((ya > S(0) && yb <= S(0)) || (ya < S(0) && yb >= S(0))) {
@@ -522,7 +555,14 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
             S rho2 = x * x + z * z - a * a;
             if (rho2 > S(0)) {
                 S r = sqrt_<false>(rho2);
-                if (!(r < ln.disk_r_in || r > S(P.disk_r_out)) && t < best) { best = t; cls = MGK_HIT_DISK; r_hit = r; }
+                if (!(r < ln.disk_r_in || r > S(P.disk_r_out)) && t < best) {
+                    bool keep = true;
+                    if (METRIC != MINKOWSKI && (P.flags & MGK_SCENE_DISK_ON_GEODESIC)) {
+                        keep = refine(t, r, disk_x, disk_z);
+                        disk_geo = keep;
+                    }
+                    if (keep) { best = t; cls = MGK_HIT_DISK; r_hit = r; }
+                }
             }
         }
     }
@@ -619,6 +659,7 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
     if (cls == MGK_HIT_NONE && n_step > P.step_limit) cls = MGK_HIT_STEP_LIMIT;
     if (cls == MGK_HIT_NONE) return cls;
     hit_record<METRIC, S>(y0, y1, cls, best, r_hit, contraction, P, a, box_n, hq, m_tri, m_inst, rec);
+    if (cls == MGK_HIT_DISK && disk_geo) rec.a1 = azimuth<S>(disk_z, disk_x);
     t_hit = best;
     return cls;
 }
@@ -770,7 +811,8 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
 
         // ---- one trial ----
         St<S> y1;
-        const float dl = float(h);   // parameter length of this trial's step (Hermite chords)
+        const S h_step = h;          // parameter length of this trial's step
+        const float dl = float(h);   // ... in float (Hermite chords)
         bool step_done;
         if constexpr (METRIC == MINKOWSKI) {
 #pragma clang fp contract(off)
@@ -822,7 +864,22 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
         mgk_hit_t rec;
         S t_hit = S(0);
         S hq[3];
-        uint32_t cls = classify<METRIC, S, false, MESH>(y, y1, n_step, contraction, P, ln, dl, rec, t_hit, hq);
+        // the disk crossing on the geodesic: a trial of the kernel's own arithmetic
+        auto refine = [&](S t, S& r, S& x, S& z) -> bool {
+#pragma clang fp contract(off)
+            // (mixed precision runs the zone kernels; this kernel's mixed instance traces flat space only)
+            if constexpr (METRIC == MINKOWSKI || PREC == MGK_PRECISION_MIXED) {
+                return true;
+            } else {
+                S hn;
+                bool acc;
+                St<S> d;   // the fifth stage, evaluated at the trial's end (c5 = 1): the tangent there
+                const St<S> yc = trial_reference<METRIC, S>(y, t * h_step, ln.a, S(P.mass_sign), S(P.abs_tol), S(P.rel_tol),
+                                                            S(P.h_max), hn, acc, &d);
+                return disk_point<S>(yc, d.qx, d.qy, d.qz, ln.a, ln.disk_r_in, S(P.disk_r_out), r, x, z);
+            }
+        };
+        uint32_t cls = classify<METRIC, S, false, MESH>(y, y1, n_step, contraction, P, ln, dl, rec, t_hit, hq, refine);
         if (cls != MGK_HIT_NONE) {
             P.hits[ray] = rec;
             if (P.steps_out) P.steps_out[ray] = n_step;
@@ -854,9 +911,15 @@ __global__ void __launch_bounds__(256) k_trace(TraceParams P) {
 // alternates the two until both queues are empty.  A ray's arithmetic thus
 // depends on its own position only, never on the rays sharing its wave.
 // ---------------------------------------------------------------------------
-template <int METRIC, bool NEAR, bool MESH>
+// Zone-kernel instantiations by scene feature: kZoneMesh (triangle meshes),
+// kZoneDiskGeo (the disk crossing on the geodesic).  Each keeps its code out of
+// the others' register allocation.
+constexpr int kZoneMesh = 1, kZoneDiskGeo = 2;
+
+template <int METRIC, bool NEAR, int FEAT>
 __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P, ZoneParams Z) {
     using S = double;
+    constexpr bool MESH = (FEAT & kZoneMesh) != 0, DISKGEO = (FEAT & kZoneDiskGeo) != 0;
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
@@ -870,6 +933,10 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
     bool active = false, handoff = false;
     uint64_t ray = 0;
     uint32_t n_step = 0, n_sub = 0;
+    // the disk crossing on the geodesic (MGK_SCENE_DISK_ON_GEODESIC): the next
+    // trial is the sub-step to the chord's crossing
+    bool disk_sub = false;
+    double h_resume = 0.0;        // the controller's step, resumed after the sub-step
     uint64_t steps_acc = 0, trials_acc = 0, fp64_acc = 0;
     uint64_t chunk_next = 0, chunk_end = 0;
     uint32_t head_c = blockIdx.x % kHeads, head_tried = 0;   // wave-uniform: work counter in use, counters found empty
@@ -979,16 +1046,37 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
         if (!active || handoff) continue;
 
         // ---- one trial ----
+        using SS = typename std::conditional<NEAR, double, float>::type;
         St<S> y1;
         double rms;
-        const float dl = float(h);   // parameter length of this trial's step (Hermite chords)
-        if constexpr (NEAR) {
-            trial_split<METRIC, double, true>(y, h, float(ln.a), P.mass_sign, P, y1, rms);
-            ++fp64_acc;
-        } else {
-            trial_split<METRIC, float, true>(y, h, float(ln.a), P.mass_sign, P, y1, rms);
-        }
+        const double h_step = h;     // parameter length of this trial's step
+        const float dl = float(h);   // ... in float (Hermite chords)
+        SS tan5[3];                  // dx/dlambda of the fifth stage (at the trial's end)
+        trial_split<METRIC, SS, true>(y, h, float(ln.a), P.mass_sign, P, y1, rms, DISKGEO ? tan5 : nullptr);
+        if constexpr (NEAR) ++fp64_acc;
         ++trials_acc;
+        if (DISKGEO && disk_sub) {
+            // the sub-step to the chord's disk crossing, taken without step control,
+            // then one Newton step along its end tangent to the plane: on the disk the
+            // path ends there; off it, the path goes on from the sub-step's end
+            disk_sub = false;
+            double r, x, z;
+            if (disk_point<double>(y1, double(tan5[0]), double(tan5[1]), double(tan5[2]), double(ln.a), double(ln.disk_r_in),
+                                   double(P.disk_r_out), r, x, z)) {
+                mgk_hit_t rec;
+                S hq[3];
+                hit_record<METRIC, S>(y1, y1, MGK_HIT_DISK, S(0), r, contraction, P, ln.a, 0, hq, 0u, 0u, rec);
+                rec.a1 = azimuth<S>(z, x);
+                P.hits[ray] = rec;
+                if (P.steps_out) P.steps_out[ray] = n_step;
+                if (P.final_state) write_final(P, ray, y1);
+                active = false;
+            } else {
+                y = y1;
+                h = h_resume;
+            }
+            continue;
+        }
         const double rs = rms > 1e-30 ? rms : 1e-30;
         // Step factor: the reference's 0.9 err^-1/5 (internal_math.h:2190-2195).
         // After an accepted trial it is also bounded by Gustafsson's
@@ -1027,8 +1115,21 @@ __global__ void __launch_bounds__(256, NEAR ? 2 : 3) k_trace_zone(TraceParams P,
         mgk_hit_t rec;
         S t_hit = S(0);
         S hq[3];
-        const uint32_t cls = classify<METRIC, S, true, MESH>(y, y1, n_step, contraction, P, ln, dl, rec, t_hit, hq);
-        if (cls != MGK_HIT_NONE) {
+        // the disk crossing on the geodesic: classify only notes the chord's
+        // crossing; the sub-step to it runs through this loop's own trial
+        double disk_req = -1.0;
+        auto note = [&](S t, S&, S&, S&) -> bool {
+            disk_req = t;
+            return false;
+        };
+        uint32_t cls;
+        if constexpr (DISKGEO) cls = classify<METRIC, S, true, MESH>(y, y1, n_step, contraction, P, ln, dl, rec, t_hit, hq, note);
+        else cls = classify<METRIC, S, true, MESH>(y, y1, n_step, contraction, P, ln, dl, rec, t_hit, hq);
+        if (DISKGEO && disk_req >= 0.0 && (cls == MGK_HIT_NONE || t_hit > disk_req)) {
+            h_resume = h;
+            h = disk_req * h_step;
+            disk_sub = true;          // y stays: the sub-step starts at the step's start
+        } else if (cls != MGK_HIT_NONE) {
             P.hits[ray] = rec;
             if (P.steps_out) P.steps_out[ray] = n_step;
             if (P.final_state) write_terminal(P, ray, cls, y, y1, t_hit, hq);
@@ -1641,19 +1742,25 @@ hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s) {
     return (p.flags & MGK_SCENE_MESHES) ? launch_trace_t<true>(p, grid, s) : launch_trace_t<false>(p, grid, s);
 }
 
-template <bool MESH>
+template <int FEAT>
 hipError_t launch_trace_zone_t(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s) {
     return dispatch_metric<tag_t>(p.metric, [&](auto m) -> hipError_t {
         constexpr int M = decltype(m)::value;
-        if (near) hipLaunchKernelGGL((k_trace_zone<M, true, MESH>), dim3(grid), dim3(256), 0, s, p, z);
-        else hipLaunchKernelGGL((k_trace_zone<M, false, MESH>), dim3(grid), dim3(256), 0, s, p, z);
-        return hipGetLastError();
+        if constexpr (M == MINKOWSKI) {
+            return hipErrorInvalidValue;   // flat space has no zones: one persistent launch (launch_trace)
+        } else {
+            if (near) hipLaunchKernelGGL((k_trace_zone<M, true, FEAT>), dim3(grid), dim3(256), 0, s, p, z);
+            else hipLaunchKernelGGL((k_trace_zone<M, false, FEAT>), dim3(grid), dim3(256), 0, s, p, z);
+            return hipGetLastError();
+        }
     });
 }
 
 hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s) {
-    return (p.flags & MGK_SCENE_MESHES) ? launch_trace_zone_t<true>(p, z, near, grid, s)
-                                        : launch_trace_zone_t<false>(p, z, near, grid, s);
+    const bool mesh = (p.flags & MGK_SCENE_MESHES) != 0;
+    const bool geo = (p.flags & MGK_SCENE_DISK) && (p.flags & MGK_SCENE_DISK_ON_GEODESIC);
+    if (mesh) return geo ? launch_trace_zone_t<kZoneMesh | kZoneDiskGeo>(p, z, near, grid, s) : launch_trace_zone_t<kZoneMesh>(p, z, near, grid, s);
+    return geo ? launch_trace_zone_t<kZoneDiskGeo>(p, z, near, grid, s) : launch_trace_zone_t<0>(p, z, near, grid, s);
 }
 
 hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t* d_offsets, uint32_t n_cam, uint64_t n, IcCamera* d_ic,

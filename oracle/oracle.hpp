@@ -430,26 +430,33 @@ template <class R> inline R redshift_factor(R initial_contraction, V4<R> u, V4<R
 // h_taken is the last tried h; *h becomes the next h.
 // Stage sums are evaluated left to right exactly as written there.
 // ---------------------------------------------------------------------------
+// One trial: the 5th-order solution over th and its error estimate (stage sums
+// left to right as internal_math.h:2167-2189 writes them).
+template <class R, class F> inline State<R> rkf45_trial(F f, const State<R>& y0, R th, State<R>* err, State<R>* k5_out = nullptr) {
+    State<R> k1, k2, k3, k4, k5, k6;
+    k1 = f(y0);
+    k2 = f(y0 + (R(1) / R(4) * th) * k1);
+    k3 = f(y0 + (R(3) / R(32) * th) * k1 + (R(9) / R(32) * th) * k2);
+    k4 = f(y0 + (R(1932) / R(2197) * th) * k1 - (R(7200) / R(2197) * th) * k2 + (R(7296) / R(2197) * th) * k3);
+    k5 = f(y0 + (R(439) / R(216) * th) * k1 - (R(8) * th) * k2 + (R(3680) / R(513) * th) * k3 - (R(845) / R(4104) * th) * k4);
+    k6 = f(y0 - (R(8) / R(27) * th) * k1 + (R(2) * th) * k2 - (R(3544) / R(2565) * th) * k3 + (R(1859) / R(4104) * th) * k4 -
+           (R(11) / R(40) * th) * k5);
+    *err = (R(-1) / R(360) * th) * k1 + (R(128) / R(4275) * th) * k3 + (R(2197) / R(75240) * th) * k4 - (R(1) / R(50) * th) * k5 -
+           (R(2) / R(55) * th) * k6;
+    if (k5_out) *k5_out = k5;
+    return y0 + (R(16) / R(135) * th) * k1 + (R(6656) / R(12825) * th) * k3 + (R(28561) / R(56430) * th) * k4 -
+           (R(9) / R(50) * th) * k5 + (R(2) / R(55) * th) * k6;
+}
+
 template <class R, class F>
 inline State<R> rkf45_step(F f, const State<R>& y0, R* h, R* h_taken, R h_max, uint32_t n_max, R abs_tol, R rel_tol,
                            uint32_t* trials = nullptr) {
-    State<R> y1{}, k1, k2, k3, k4, k5, k6, err;
+    State<R> y1{}, err;
     R h_new = 0, th = *h, rms = 0, rms_safe = 0;
     uint32_t n = 0;
     for (uint32_t i = 0; i < n_max; i++) {
         ++n;
-        k1 = f(y0);
-        k2 = f(y0 + (R(1) / R(4) * th) * k1);
-        k3 = f(y0 + (R(3) / R(32) * th) * k1 + (R(9) / R(32) * th) * k2);
-        k4 = f(y0 + (R(1932) / R(2197) * th) * k1 - (R(7200) / R(2197) * th) * k2 + (R(7296) / R(2197) * th) * k3);
-        k5 = f(y0 + (R(439) / R(216) * th) * k1 - (R(8) * th) * k2 + (R(3680) / R(513) * th) * k3 -
-               (R(845) / R(4104) * th) * k4);
-        k6 = f(y0 - (R(8) / R(27) * th) * k1 + (R(2) * th) * k2 - (R(3544) / R(2565) * th) * k3 +
-               (R(1859) / R(4104) * th) * k4 - (R(11) / R(40) * th) * k5);
-        y1 = y0 + (R(16) / R(135) * th) * k1 + (R(6656) / R(12825) * th) * k3 + (R(28561) / R(56430) * th) * k4 -
-             (R(9) / R(50) * th) * k5 + (R(2) / R(55) * th) * k6;
-        err = (R(-1) / R(360) * th) * k1 + (R(128) / R(4275) * th) * k3 + (R(2197) / R(75240) * th) * k4 -
-              (R(1) / R(50) * th) * k5 - (R(2) / R(55) * th) * k6;
+        y1 = rkf45_trial(f, y0, th, &err);
         rms = rms_error(err, y0, y1, abs_tol, rel_tol);
         rms_safe = rms > R(1e-30f) ? rms : R(1e-30f);
         h_new = R(0.9f) * th * pow_(R(1) / rms_safe, R(0.2f));
@@ -570,6 +577,7 @@ template <class R> struct MeshSet {
 template <class R> struct Scene {
     R celestial_radius_m = R(1000);   // measured from the camera origin, SI metres
     bool disk = false;
+    bool disk_on_geodesic = false;    // locate the disk crossing on the geodesic, not on the step's chord
     R disk_r_in = R(6), disk_r_out = R(20), disk_T_in = R(1e4);
     bool sphere = false;
     V3<R> sphere_center{};            // dimensionless coordinates
@@ -725,6 +733,29 @@ template <class R> inline bool disk_hit(R* t_out, R* r_out, V4<R> q0, V4<R> q1, 
     if (r < r_in || r > r_max) return false;
     *t_out = t;
     *r_out = r;
+    return true;
+}
+
+// The disk crossing on the geodesic instead of on the step's chord (a scene
+// option; the chord's crossing is off by O(h^2) of the step's curvature).  One
+// RKF45 trial -- the 5th-order solution, no step control -- from the step's
+// start over the chord's crossing parameter, t h, then one Newton step to the
+// plane y = 0 along the tangent dx/dlambda of the trial's fifth stage, which
+// Fehlberg's tableau evaluates at the trial's end (c5 = 1).  The remaining
+// offset is second order in the first estimate's error.  False when the
+// corrected point lies off the disk (the path then goes on).
+template <class R, class F>
+inline bool disk_on_geodesic(F f, const State<R>& y0, R t, R h, R a, R r_in, R r_max, R* r_out, V3<R>* x_out) {
+    State<R> err, d;
+    const State<R> yc = rkf45_trial(f, y0, t * h, &err, &d);
+    const R dl = -yc.q.y / d.q.y;
+    const R x = yc.q.x + dl * d.q.x, z = yc.q.z + dl * d.q.z;
+    const R rho2 = x * x + z * z - a * a;
+    if (!(rho2 > R(0))) return false;
+    const R r = sqrt_(rho2);
+    if (r < r_in || r > r_max) return false;
+    *r_out = r;
+    *x_out = V3<R>{x, R(0), z};
     return true;
 }
 
@@ -964,8 +995,16 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
         R best = R(1e30f);
         uint32_t cls = HIT_NONE;
         R t, rr = 0, r_hit = 0;
+        V3<R> x_disk{};
+        bool disk_geo = false;
         if (sc.disk && disk_hit(&t, &rr, y0.q, y1.q, p.metric == kKerr ? p.spin : R(0), sc.disk_r_in, sc.disk_r_out)) {
-            if (t < best) { best = t; cls = HIT_DISK; r_hit = rr; }
+            bool keep = t < best;
+            if (keep && sc.disk_on_geodesic && p.metric != kMinkowski) {
+                auto f = [&](const State<R>& s) { return rhs(s, p); };
+                keep = disk_on_geodesic(f, y0, t, h_taken, p.metric == kKerr ? p.spin : R(0), sc.disk_r_in, sc.disk_r_out, &rr, &x_disk);
+                disk_geo = keep;
+            }
+            if (keep) { best = t; cls = HIT_DISK; r_hit = rr; }
         }
         if (sc.sphere && sphere_hit(&t, y0.q, y1.q, sc.sphere_center, sc.sphere_radius)) {
             if (t < best) { best = t; cls = HIT_SPHERE; }
@@ -1031,7 +1070,7 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
                 }
                 if (cls == HIT_DISK) {
                     hit.a0 = r_hit;
-                    hit.a1 = atan2_(qh.z, qh.x);
+                    hit.a1 = disk_geo ? atan2_(x_disk.z, x_disk.x) : atan2_(qh.z, qh.x);
                 } else {
                     V3<R> n = norm3(V3<R>{qh.x - sc.sphere_center.x, qh.y - sc.sphere_center.y, qh.z - sc.sphere_center.z});
                     hit.a0 = n.y;

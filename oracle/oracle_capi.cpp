@@ -90,6 +90,7 @@ template <class R> Scene<R> scene_from(const mgk_scene_t* s, uint32_t step_limit
     Scene<R> sc;
     sc.celestial_radius_m = R(s->celestial_radius_m);
     sc.disk = (s->flags & MGK_SCENE_DISK) != 0;
+    sc.disk_on_geodesic = (s->flags & MGK_SCENE_DISK_ON_GEODESIC) != 0;
     sc.disk_r_in = R(s->disk_r_in);
     sc.disk_r_out = R(s->disk_r_out);
     sc.disk_T_in = R(s->disk_T_in);
