@@ -61,7 +61,7 @@ def subset(n, seed, n_sub=1 << 16):
     return np.sort(rng.choice(n, size=min(n_sub, n), replace=False)).astype(np.int64)
 
 
-def radiance_error(ctx, M, integ, sc, spec, cam, spp, seed, test_hits=None, n_sub=1 << 16):
+def radiance_error(ctx, M, integ, sc, spec, cam, spp, seed, test_hits=None, n_sub=1 << 16, chord_scene=None):
     """Mean relative per-ray Y error against the FP64 path on a seeded subset of
     the frame's rays, with identical per-ray samples.  test_hits: the frame's
     hit records of the path under test ((n, 4) device tensor); when None the
@@ -81,7 +81,8 @@ def radiance_error(ctx, M, integ, sc, spec, cam, spp, seed, test_hits=None, n_su
     out, fin = {}, {}
     runs = [("fp64", it64, rays)] + ([("test", integ, rays_test)] if test_hits is None else [])
     for name, it, rr in runs:
-        hits, final = ctx.trace_final(it, sc, list(cam.position), rr, n=len(q), queue=qt)
+        scn = chord_scene if (name == "test" and chord_scene is not None) else sc
+        hits, final = ctx.trace_final(it, scn, list(cam.position), rr, n=len(q), queue=qt)
         out[name] = hits[qt.long()].contiguous()
         fin[name] = final[:, qt.long()].cpu().numpy()
         del hits, final
@@ -103,6 +104,7 @@ def radiance_error(ctx, M, integ, sc, spec, cam, spp, seed, test_hits=None, n_su
     eps = 1e-3 * max(y["fp64"].mean(), 1e-30)
     err = np.abs(y["test"] - y["fp64"]) / np.maximum(y["fp64"], eps)
     res = {"mean_rel_Y": float(err.mean()), "p99_rel_Y": float(np.quantile(err, 0.99)),
+           "frac_within_1e-3": float(np.mean(err <= 1e-3)),
            "class_agreement": float(np.mean(ht["cls"] == hr["cls"])), "rays": len(q)}
     # endpoint direction of the rays that escape on both paths
     esc = (ht["cls"] == M.HIT_ESCAPE) & (hr["cls"] == M.HIT_ESCAPE)
@@ -320,6 +322,26 @@ def main():
     elif not args.no_error:
         plain["radiance_err_vs_fp64"] = radiance_error(ctx, M, integ, sc, spec, cam, spp, args.seed)
         log(f"  radiance error {plain['radiance_err_vs_fp64']}")
+    if prec == M.PREC_MIXED and not boxes and (sc.flags & M.SCENE_DISK_ON_GEODESIC):
+        # the same frames with the disk crossing on the step's chord (one trial
+        # per disk path fewer; the chord's O(h^2) error instead)
+        sc_chord = M.scene(**{f: getattr(sc, f) for f, _ in sc._fields_})
+        sc_chord.flags &= ~M.SCENE_DISK_ON_GEODESIC
+        log("plain RKF45 frames, disk crossing on the chord")
+
+        def chord_frame(seed_off):
+            ctx.render_frame(integ, sc_chord, spec, cam, args.seed + seed_off, spp, out)
+            return ctx.stats()
+
+        wallc, perc = time_frames(chord_frame, args.steps, 1)
+        plain["chord_rays_per_s"] = n_paths * args.steps / wallc
+        plain["chord_trace_ms_per_frame"] = sum(st.ms for st in perc) / args.steps
+        log(f"  {plain['chord_rays_per_s']:.4e} rays/s")
+        if not args.no_error:
+            plain["chord_radiance_err_vs_fp64_geodesic"] = radiance_error(ctx, M, integ, sc, spec, cam, spp, args.seed,
+                                                                          chord_scene=sc_chord)
+            log(f"  radiance error against the FP64 frame with the crossing on the geodesic "
+                f"{plain['chord_radiance_err_vs_fp64_geodesic']['mean_rel_Y']:.3g}")
     if prec == M.PREC_MIXED and not boxes:
         # the plain path in the reference's FP32 arithmetic (the integrator the
         # reference ships), for the speed ratios
@@ -550,7 +572,11 @@ def bench_surrogate(M, ctx, args, integ, sc, spec, cam, spp, n_paths, out):
                                        "dir": float(hist[-1, 2]), "accuracy": float(hist[-1, 3])}},
            "configs": {}}
     hits = torch.zeros((n_paths, 4), dtype=torch.int32, device="cuda:0")
-    rules = {"default": M.fallback(),
+    # analytic: the network's class checked, and a disk path's radius refined,
+    # by the Mino-time integrals (the configuration meant for the error bar);
+    # default: the network's answers as they are; trace_disk: every disk path traced
+    rules = {"analytic": M.fallback(analytic=True),
+             "default": M.fallback(),
              "trace_disk": M.fallback(trace_disk=True)}
     for name, fb in rules.items():
         log(f"surrogate frames ({name})")

@@ -581,6 +581,12 @@ typedef struct mgk_fallback_t {
     float disk_edge_margin;               /* disk: predicted r within this fraction of r_in or r_out */
     uint32_t judge_stride;                /* every judge_stride-th path is also traced by RKF45 (0 = none) */
     uint32_t trace_disk;                  /* 1: every disk-class path is traced */
+    /* 1: every answered path is checked, and a disk path's radius refined, by
+     * the Mino-time integrals of its separated radial and polar motion
+     * (FP64, seeded by the network's radius); a path whose class they do not
+     * confirm, or cannot resolve (near the critical curve), is traced.  The
+     * disk_edge_margin rule is then not needed and not applied. */
+    uint32_t analytic;
 } mgk_fallback_t;
 
 typedef struct mgk_surrogate_stats_t {
@@ -599,6 +605,7 @@ typedef struct mgk_surrogate_stats_t {
     double judge_mean_dir, judge_max_dir; /* escape direction error (rad) over judge paths that escape in both */
     uint64_t n_trials;                    /* RKF45 trials of the fallback trace */
     double ms_raygen, ms_mlp, ms_trace, ms_shade, ms_total;   /* device time of the stages */
+    uint64_t n_fallback_analytic;         /* fallback.analytic: class not confirmed by, or unresolved in, the Mino-time integrals */
 } mgk_surrogate_stats_t;
 
 MGK_API e_magik_result_types mgk_surrogate_create(int device, uint64_t seed, const mgk_surrogate_ranges_t* ranges,

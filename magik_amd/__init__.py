@@ -611,10 +611,10 @@ def split_params(w, b):
 
 
 def fallback(min_confidence=0.9, critical_margin=0.05, direction_margin=2e-3, disk_edge_margin=0.02, judge_stride=1024,
-             trace_disk=False) -> abi.Fallback:
+             trace_disk=False, analytic=False) -> abi.Fallback:
     return abi.Fallback(min_confidence=min_confidence, critical_margin=critical_margin,
                         direction_margin=direction_margin, disk_edge_margin=disk_edge_margin,
-                        judge_stride=judge_stride, trace_disk=int(trace_disk))
+                        judge_stride=judge_stride, trace_disk=int(trace_disk), analytic=int(analytic))
 
 
 def render_frame_surrogate(ctx: Context, sur: Surrogate, integ, sc, spec, cam, seed: int, spp: int, fb=None, out=None,

@@ -153,7 +153,8 @@ class TrainConfig(C.Structure):  # mgk_train_config_t
 
 class Fallback(C.Structure):  # mgk_fallback_t
     _fields_ = [("min_confidence", C.c_float), ("critical_margin", C.c_float), ("direction_margin", C.c_float),
-                ("disk_edge_margin", C.c_float), ("judge_stride", C.c_uint32), ("trace_disk", C.c_uint32)]
+                ("disk_edge_margin", C.c_float), ("judge_stride", C.c_uint32), ("trace_disk", C.c_uint32),
+                ("analytic", C.c_uint32)]
 
 
 class SurrogateStats(C.Structure):  # mgk_surrogate_stats_t
@@ -162,7 +163,8 @@ class SurrogateStats(C.Structure):  # mgk_surrogate_stats_t
                                           "judge_class_agree", "judge_n_disk", "judge_n_escape")] + \
                [(n, C.c_double) for n in ("judge_mean_rel_r", "judge_mean_rel_g", "judge_mean_dir", "judge_max_dir")] + \
                [("n_trials", C.c_uint64)] + \
-               [(n, C.c_double) for n in ("ms_raygen", "ms_mlp", "ms_trace", "ms_shade", "ms_total")]
+               [(n, C.c_double) for n in ("ms_raygen", "ms_mlp", "ms_trace", "ms_shade", "ms_total")] + \
+               [("n_fallback_analytic", C.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -1,0 +1,329 @@
+// Disk crossing of a Kerr null geodesic from its constants of motion, by the
+// Mino-time integrals of the separated radial and polar motion (Carter 1968;
+// Mino 2003; root structure as in Gralla & Lupsasca, PRD 101, 044032, 2020).
+//
+// With E-normalised constants lambda = L / E and eta = Q / E^2 and the Mino
+// time d tau = |E| d lambda_affine / Sigma, a photon's Boyer-Lindquist r and
+// u = cos(theta) obey
+//     (dr/dtau)^2 = R(r) = r^4 + A r^2 + B r + C,
+//         A = a^2 - eta - lambda^2, B = 2 (eta + (lambda - a)^2), C = -a^2 eta,
+//     (du/dtau)^2 = U(u) = eta - (eta + lambda^2 - a^2) u^2 - a^2 u^4
+//                        = (u_+ - u^2) (a^2 u^2 + A_m),
+// so the n-th crossing of the equator happens at a Mino time G_n fixed by the
+// polar motion alone, and the radius there solves  int dr / sqrt(R) = G_n
+// along the radial branch (inbound, or outbound past the turning point r_4).
+// Kerr-Schild r and theta are Boyer-Lindquist's, and E = -p_t, L = z p_x -
+// x p_z (spin along +y) and Carter's Q are the same in both coordinates.
+//
+// Every integral is a 16-node Gauss-Legendre rule over a smooth integrand:
+// the radial ones in s = 1/r (P(s) = s^4 R(1/s) = 1 + A s^2 + B s^3 + C s^4),
+// with s = s_4 - w^2 taking out the square-root singularity at a turning
+// point, and the polar ones in u = sqrt(u_+) - w^2 likewise.  The crossing
+// radius is found by a bracketed Newton iteration seeded by the caller's
+// estimate.  Near-critical rays (the outer pair of R's roots within
+// kMinoMinGap of each other outside the horizon), rays that cross the equator
+// more than kMinoMaxCrossings times and radii that do not converge within
+// kMinoMaxNewton steps are left unresolved.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mgk {
+
+constexpr int kMinoMaxCrossings = 4;
+constexpr int kMinoMaxNewton = 12;   // Newton steps on a crossing's radius (bisection when one leaves its bracket)
+constexpr double kMinoMinGap = 0.02;  // relative separation of the outer root pair below which a ray is unresolved
+
+// Gauss-Legendre nodes and weights on [0, 1] (16 points; the Legendre
+// polynomial's roots mapped from [-1, 1], weights halved).
+struct Gl16 {
+    static constexpr double x[16] = {0.005299532504175031, 0.0277124884633837, 0.06718439880608412, 0.1222977958224985,
+                                     0.19106187779867811, 0.2709916111713863, 0.35919822461037054, 0.4524937450811813,
+                                     0.5475062549188188, 0.6408017753896295, 0.7290083888286136, 0.8089381222013219,
+                                     0.8777022041775016, 0.9328156011939159, 0.9722875115366163, 0.994700467495825};
+    static constexpr double w[16] = {0.013576229705877019, 0.031126761969323853, 0.047579255841246296, 0.062314485627767015,
+                                     0.07479799440828838, 0.08457825969750131, 0.0913017075224618, 0.09472530522753429,
+                                     0.09472530522753429, 0.0913017075224618, 0.08457825969750131, 0.07479799440828838,
+                                     0.062314485627767015, 0.047579255841246296, 0.031126761969323853, 0.013576229705877019};
+};
+
+enum MinoKind : int { kMinoUnresolved = 0, kMinoCaptured = 1, kMinoEscapes = 2, kMinoDisk = 3 };
+
+struct MinoResult {
+    int kind;        // MinoKind
+    int crossing;    // which crossing of the equator meets the disk (1, 2, ...)
+    int iters;       // Newton iterations of the crossing's radius
+    double r;        // the crossing's radius (kMinoDisk)
+};
+
+// Constants of motion and the Mino-time directions of a Kerr-Schild state
+// (q = x, y, z; p = p_x, p_y, p_z, p_t covariant; a the spin of the equations).
+struct MinoRay {
+    double lam, eta, a;
+    double r, u;     // Boyer-Lindquist r and cos(theta) of the state
+    int sr, su;      // signs of dr/dtau and du/dtau
+};
+
+__host__ __device__ inline MinoRay mino_ray(const double q[3], const double p[4], double a) {
+    MinoRay m;
+    const double x = q[0], y = q[1], z = q[2];
+    const double rho2 = x * x + y * y + z * z;
+    const double w = rho2 - a * a;
+    const double sig = sqrt(w * w + 4.0 * a * a * y * y);
+    const double r = sqrt(0.5 * (w + sig));
+    const double d = r * r + a * a;
+    // dx^i/dlambda = g^{i nu} p_nu = p_i - f (k.p - p_t) k_i with the Kerr-Schild f and k
+    const double f = 2.0 * r * r * r / (r * r * r * r + a * a * y * y);
+    const double kx = (r * x - a * z) / d, ky = y / r, kz = (r * z + a * x) / d;
+    const double kp = kx * p[0] + ky * p[1] + kz * p[2] - p[3];
+    const double vx = p[0] - f * kp * kx, vy = p[1] - f * kp * ky, vz = p[2] - f * kp * kz;
+    // r(x, y, z) of the quartic r^4 - (rho^2 - a^2) r^2 - a^2 y^2 = 0: grad r = (x r, y d / r, z r) / sig
+    const double rdot = (x * r * vx + y * d / r * vy + z * r * vz) / sig;
+    const double u = y / r;
+    const double udot = (vy * r - y * rdot) / (r * r);
+    const double E = -p[3];
+    const double L = z * p[0] - x * p[2];
+    // Carter's constant with p_theta = cot(theta) (x p_x + z p_z) - r sin(theta) p_y
+    const double s2 = fmax(1.0 - u * u, 1e-300), s = sqrt(s2);
+    const double pth = (u / s) * (x * p[0] + z * p[2]) - r * s * p[1];
+    const double Q = pth * pth + u * u * (L * L / s2 - a * a * E * E);
+    m.lam = L / E;
+    m.eta = Q / (E * E);
+    m.a = a;
+    m.r = r;
+    m.u = u;
+    m.sr = rdot < 0.0 ? -1 : 1;
+    m.su = udot < 0.0 ? -1 : 1;
+    return m;
+}
+
+namespace mino_detail {
+
+// Largest real root of the depressed cubic t^3 + p t + q = 0.
+__host__ __device__ inline double cubic_max_root(double p, double q) {
+    const double disc = 0.25 * q * q + p * p * p / 27.0;
+    if (disc > 0.0) {
+        const double sd = sqrt(disc);
+        return cbrt(-0.5 * q + sd) + cbrt(-0.5 * q - sd);
+    }
+    if (p >= 0.0) return cbrt(-q);   // p = 0 = disc: a triple root
+    const double rp = sqrt(-p / 3.0);
+    const double c = fmin(1.0, fmax(-1.0, (1.5 * q / p) / rp));
+    return 2.0 * rp * cos(acos(c) / 3.0);
+}
+
+// Radial potential in s = 1/r: P(s) = 1 + A s^2 + B s^3 + C s^4.
+struct Radial {
+    double A, B, C;
+    __host__ __device__ double P(double s) const { return 1.0 + s * s * (A + s * (B + s * C)); }
+    __host__ __device__ double R(double r) const { return r * r * r * r + A * r * r + B * r + C; }
+    __host__ __device__ double dR(double r) const { return 4.0 * r * r * r + 2.0 * A * r + B; }
+};
+
+// The outer root pair of R(r) (Ferrari: R = product of two quadratics through
+// the positive root m of 8 m^3 + 8 A m^2 + (2 A^2 - 8 C) m - B^2 = 0): r_3,4 =
+// z -/+ sqrt(-A/2 - z^2 - B/(4z)), z = sqrt(m/2).  Real pair: r3 <= r4, the
+// largest root polished by Newton steps.  Complex pair: z +/- i w.
+struct RootPair {
+    bool ok, real;
+    double r3, r4;   // real pair
+    double z, w;     // complex pair
+};
+__host__ __device__ inline RootPair radial_pair(const Radial& R) {
+    RootPair rp{false, false, 0.0, 0.0, 0.0, 0.0};
+    const double A = R.A, B = R.B, C = R.C;
+    // m^3 + A m^2 + (A^2/4 - C) m - B^2/8 = 0, m = t - A/3
+    const double p = -A * A / 12.0 - C;
+    const double q = -A * A * A / 108.0 + A * C / 3.0 - B * B / 8.0;
+    const double m = cubic_max_root(p, q) - A / 3.0;
+    if (!(m > 0.0)) return rp;
+    const double z = sqrt(0.5 * m);
+    const double rad = -0.5 * A - z * z - B / (4.0 * z);
+    rp.ok = true;
+    rp.z = z;
+    if (!(rad >= 0.0)) {
+        rp.w = sqrt(-rad);
+        return rp;
+    }
+    rp.real = true;
+    const double sr = sqrt(rad);
+    double r = z + sr;
+    for (int k = 0; k < 3; ++k) {
+        const double d = R.dR(r);
+        if (!(d != 0.0)) break;
+        r -= R.R(r) / d;
+    }
+    rp.r4 = r;
+    rp.r3 = z - sr;
+    return rp;
+}
+
+}  // namespace mino_detail
+
+// Disk crossing (or capture / escape) of the ray: the first crossing of the
+// equator within [r_in, r_out], with r_seed the estimate the Newton iteration
+// starts from when it lies on the crossing's branch (<= 0: none).  r_horizon
+// is the outer horizon radius 1 + sqrt(1 - a^2).  Only rays that start
+// inward (sr < 0) and outside the disk plane (u != 0) are resolved.
+__host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_horizon, double r_in, double r_out, double r_seed) {
+    using namespace mino_detail;
+    MinoResult res{kMinoUnresolved, 0, 0, 0.0};
+    const double a = ray.a, lam = ray.lam, eta = ray.eta;
+    if (ray.sr >= 0 || ray.u == 0.0) return res;
+    // ---- polar motion: the Mino times of the equator crossings
+    const double Ap = a * a;                       // U = (u_+ - u^2)(Ap u^2 + Am)
+    const double Bq = eta + lam * lam - a * a, Cq = eta;
+    const Radial rad{a * a - eta - lam * lam, 2.0 * (eta + (lam - a) * (lam - a)), -a * a * eta};
+    const double s_obs = 1.0 / ray.r, s_h = 1.0 / r_horizon;
+    const RootPair pr = radial_pair(rad);
+    if (!pr.ok) return res;
+    const double r4 = pr.r4;
+    const bool turning = pr.real && r4 > r_horizon && r4 < ray.r;
+    // a near-double root outside the horizon (the photon orbits of a ray close to
+    // the critical curve) makes the radial integrands nearly singular: left unresolved
+    if (pr.real && pr.r3 > r_horizon && pr.r4 - pr.r3 < kMinoMinGap * pr.r4) return res;
+    if (!pr.real && pr.z > r_horizon && pr.z < ray.r && pr.w < kMinoMinGap * pr.z) return res;
+    if (!(eta > 0.0)) {
+        // no equator crossing: a plunging ray is captured, the others are left
+        if (!turning) res.kind = kMinoCaptured;
+        return res;
+    }
+    const double D = sqrt(Bq * Bq + 4.0 * Ap * Cq);
+    const double up = Bq >= 0.0 ? 2.0 * Cq / (Bq + D) : (D - Bq) / (2.0 * Ap);
+    const double Am = Bq >= 0.0 ? 0.5 * (Bq + D) : 2.0 * Ap * Cq / (D - Bq);
+    const double su = sqrt(up);
+    const double uo = fabs(ray.u);
+    if (!(uo < su)) return res;
+    // tail(u) = int_u^{sqrt(u_+)} du / sqrt(U), with u = sqrt(u_+) - w^2
+    auto tail = [&](double u) {
+        const double W = sqrt(su - u);
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const double w = W * Gl16::x[k], uu = su - w * w;
+            acc += Gl16::w[k] / sqrt((su + uu) * (Ap * uu * uu + Am));
+        }
+        return 2.0 * W * acc;
+    };
+    const double G_half = tail(0.0);
+    const bool toward = (ray.su < 0) == (ray.u > 0.0);   // moving toward u = 0
+    const double t_obs = tail(uo);
+    const double G1 = toward ? G_half - t_obs : G_half + t_obs;
+
+    // ---- radial motion
+    const double s_in = 1.0 / r_in, s_out = 1.0 / r_out;
+    double s4 = 0.0;
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;   // Q(s) = P(s) / (s4 - s) = -(b3 s^3 + b2 s^2 + b1 s + b0)
+    if (turning) {
+        s4 = 1.0 / r4;
+        b3 = rad.C;
+        b2 = rad.B + s4 * b3;
+        b1 = rad.A + s4 * b2;
+        b0 = s4 * b1;
+    }
+    auto Qs = [&](double s) { return -(((b3 * s + b2) * s + b1) * s + b0); };
+    // K(W) = int_{s4 - W^2}^{s4} ds / sqrt(P) (turning rays)
+    auto K = [&](double W) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const double w = W * Gl16::x[k];
+            acc += Gl16::w[k] / sqrt(Qs(s4 - w * w));
+        }
+        return 2.0 * W * acc;
+    };
+    // J(s) = int_{s_obs}^{s} ds / sqrt(P) (plunging rays)
+    auto J = [&](double s) {
+        double acc = 0.0;
+        const double ds = s - s_obs;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) acc += Gl16::w[k] / sqrt(rad.P(s_obs + ds * Gl16::x[k]));
+        return ds * acc;
+    };
+
+    if (turning) {
+        const double W_obs = sqrt(s4 - s_obs);
+        const double K_obs = K(W_obs);
+        const double K_inf = K(sqrt(s4));
+        // windows of the disk on the two branches, as W = sqrt(s4 - s) ranges
+        // inbound: s in [max(s_out, s_obs), min(s_in, s4)] -> tau = K_obs - K(W), W decreasing in s
+        // outbound: s in [s_out, min(s_in, s4)] -> tau = K_obs + K(W)
+        const double s_lo_in = fmax(s_out, s_obs), s_hi = fmin(s_in, s4);
+        const bool win_in = s_lo_in < s_hi, win_out = s_out < s_hi;
+        const double Wa_in = win_in ? sqrt(s4 - s_lo_in) : 0.0;   // larger W (outer edge of the window)
+        const double Wb = sqrt(s4 - s_hi);                       // smaller W (inner edge)
+        const double Wa_out = win_out ? sqrt(s4 - s_out) : 0.0;
+        const double Ka_in = win_in ? K(Wa_in) : 0.0, Kb = K(Wb), Ka_out = win_out ? K(Wa_out) : 0.0;
+        for (int n = 1; n <= kMinoMaxCrossings; ++n) {
+            const double G = G1 + 2.0 * double(n - 1) * G_half;
+            if (G >= K_obs + K_inf) {
+                res.kind = kMinoEscapes;
+                return res;
+            }
+            double target, Wlo, Whi;
+            bool hit = false;
+            if (win_in && G >= K_obs - Ka_in && G <= K_obs - Kb) {
+                target = K_obs - G; Wlo = Wb; Whi = Wa_in; hit = true;           // K(W) = K_obs - G
+            } else if (win_out && G >= K_obs + Kb && G <= K_obs + Ka_out) {
+                target = G - K_obs; Wlo = Wb; Whi = Wa_out; hit = true;          // K(W) = G - K_obs
+            }
+            if (!hit) continue;
+            // bracketed Newton on K(W) = target, dK/dW = 2 / sqrt(Q(s4 - W^2))
+            double W = 0.5 * (Wlo + Whi);
+            if (r_seed > 0.0) {
+                const double ss = 1.0 / r_seed;
+                if (ss < s4 && ss > s4 - Whi * Whi && ss < s4 - Wlo * Wlo) W = sqrt(s4 - ss);
+            }
+            bool done = false;
+            for (int it = 0; it < kMinoMaxNewton && !done; ++it) {
+                const double f = K(W) - target;
+                if (f > 0.0) Whi = W; else Wlo = W;
+                double Wn = W - 0.5 * f * sqrt(Qs(s4 - W * W));
+                if (!(Wn >= Wlo && Wn <= Whi)) Wn = 0.5 * (Wlo + Whi);
+                done = fabs(Wn - W) <= 1e-8 * W;   // Newton: the next error is ~ (1e-8)^2
+                W = Wn;
+                res.iters = it + 1;
+            }
+            if (!done) return res;
+            res.kind = kMinoDisk;
+            res.crossing = n;
+            res.r = 1.0 / (s4 - W * W);
+            return res;
+        }
+        return res;
+    }
+    // plunging: every crossing is inbound, until the horizon
+    const double J_h = J(s_h);
+    const double s_lo = fmax(s_out, s_obs), s_hi = fmin(s_in, s_h);
+    const bool win = s_lo < s_hi;
+    const double Ja = win ? J(s_lo) : 0.0, Jb = win ? J(s_hi) : 0.0;
+    for (int n = 1; n <= kMinoMaxCrossings; ++n) {
+        const double G = G1 + 2.0 * double(n - 1) * G_half;
+        if (G >= J_h) {
+            res.kind = kMinoCaptured;
+            return res;
+        }
+        if (!(win && G >= Ja && G <= Jb)) continue;
+        double lo = s_lo, hi = s_hi;
+        double s = 0.5 * (lo + hi);
+        if (r_seed > 0.0 && 1.0 / r_seed > lo && 1.0 / r_seed < hi) s = 1.0 / r_seed;
+        bool done = false;
+        for (int it = 0; it < kMinoMaxNewton && !done; ++it) {
+            const double f = J(s) - G;
+            if (f > 0.0) hi = s; else lo = s;
+            double sn = s - f * sqrt(rad.P(s));
+            if (!(sn >= lo && sn <= hi)) sn = 0.5 * (lo + hi);
+            done = fabs(sn - s) <= 1e-8 * s;   // Newton: the next error is ~ (1e-8)^2
+            s = sn;
+            res.iters = it + 1;
+        }
+        if (!done) return res;
+        res.kind = kMinoDisk;
+        res.crossing = n;
+        res.r = 1.0 / s;
+        return res;
+    }
+    return res;
+}
+
+}  // namespace mgk

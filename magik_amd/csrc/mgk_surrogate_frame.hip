@@ -6,7 +6,9 @@
 //                    point's camera frame -> normalised features (n x 16)
 //   k_mlp_infer    : (mgk_mlp.hip) features -> outputs (n x 8)
 //   k_sur_decode   : outputs -> class / disk radius / escape direction, the
-//                    fallback rules, the hit record of answered paths, and
+//                    fallback rules, (fallback.analytic) the Mino-time check
+//                    of the class and refinement of the disk radius
+//                    (mgk_mino.hpp), the hit record of answered paths, and
 //                    the compacted queue of paths to trace (one atomic per
 //                    wave, ballot ranks within it)
 //   mgk_trace_rays : the queue, written back into the same hit slots
@@ -28,6 +30,7 @@
 #include "magik_amd.h"
 #include "mgk_device.hpp"
 #include "mgk_internal.hpp"
+#include "mgk_mino.hpp"
 #include "mgk_mlp.hpp"
 #include "mgk_surrogate.hpp"
 
@@ -49,9 +52,10 @@ constexpr uint32_t kNotAnswered = 0xFFFFFFFFu;
 // device counters of one frame (u64 slots; the double sums live in the
 // same buffer, bit-cast)
 enum {
-    C_QUEUE = 0, C_CONF, C_CRIT, C_DISK, C_STAR, C_DOMAIN, C_JUDGE, C_AGREE, C_JDISK, C_JESC,
+    C_QUEUE = 0, C_CONF, C_CRIT, C_DISK, C_STAR, C_DOMAIN, C_ANALYTIC, C_JUDGE, C_AGREE, C_JDISK, C_JESC,
     C_SUM_R, C_SUM_G, C_SUM_DIR, C_MAX_DIR, C_N
 };
+constexpr int C_REASON_LAST = C_ANALYTIC;   // the fallback reasons are C_CONF .. C_REASON_LAST
 
 struct FeatureParams {
     SurrogateRanges R;
@@ -94,10 +98,10 @@ struct DecodeParams {
     StarView stars;
     float star_chord2_in, star_chord2_out;
     uint32_t flags;
-    double a_eq, disk_r_in, disk_r_out;
+    double a_eq, disk_r_in, disk_r_out, r_horizon;
     float min_conf, crit_margin, edge_margin;
     int star_test;
-    uint32_t judge_stride, trace_disk;
+    uint32_t judge_stride, trace_disk, analytic;
 };
 
 // Any star whose edge lies within the band [chord_in, chord_out] of direction d.
@@ -154,14 +158,31 @@ __device__ __forceinline__ bool decode_one(const DecodeParams& P, uint64_t i, mg
     }
     if (fallback) return true;
     const double con = P.rays[9 * P.n + i], pt = P.rays[7 * P.n + i];
+    double r = exp(P.R.lnr_mid + double(y0.z) * P.R.lnr_half);
+    if (cls == 1 && P.trace_disk) {
+        reason = C_DISK;
+        return true;
+    }
+    if (P.analytic) {
+        // the Mino-time integrals must confirm the network's class; a disk
+        // path's radius becomes the one they give, seeded by the network's
+        const double q[3] = {P.rays[0 * P.n + i], P.rays[1 * P.n + i], P.rays[2 * P.n + i]};
+        const double p[4] = {P.rays[4 * P.n + i], P.rays[5 * P.n + i], P.rays[6 * P.n + i], pt};
+        const MinoRay m = mino_ray(q, p, P.a_eq);
+        const MinoResult mr = mino_disk(m, P.r_horizon, P.disk_r_in, P.disk_r_out, cls == 1 ? r : -1.0);
+        const int want = cls == 0 ? kMinoCaptured : (cls == 1 ? kMinoDisk : kMinoEscapes);
+        if (mr.kind != want) {
+            reason = C_ANALYTIC;
+            return true;
+        }
+        if (cls == 1) r = mr.r;
+    } else if (cls == 1 && (r < P.disk_r_in * (1.0 + P.edge_margin) || r > P.disk_r_out * (1.0 - P.edge_margin))) {
+        reason = C_DISK;
+        return true;
+    }
     if (cls == 0) {
         h.cls = MGK_HIT_HORIZON;
     } else if (cls == 1) {
-        const double r = exp(P.R.lnr_mid + double(y0.z) * P.R.lnr_half);
-        if (P.trace_disk || r < P.disk_r_in * (1.0 + P.edge_margin) || r > P.disk_r_out * (1.0 - P.edge_margin)) {
-            reason = C_DISK;
-            return true;
-        }
         double ut, w;
         kepler_ut_w<KERR, double>(r, P.a_eq, ut, w);
         const double qx = P.rays[0 * P.n + i], qz = P.rays[2 * P.n + i];
@@ -187,12 +208,12 @@ __device__ __forceinline__ bool decode_one(const DecodeParams& P, uint64_t i, mg
 __global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
     __shared__ uint32_t qbuf[kDecodeChunk];
     __shared__ uint32_t n_local;
-    __shared__ uint32_t reasons[C_DOMAIN + 1];
+    __shared__ uint32_t reasons[C_REASON_LAST + 1];
     __shared__ unsigned long long base_s;
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     if (threadIdx.x == 0) n_local = 0;
-    if (threadIdx.x <= C_DOMAIN) reasons[threadIdx.x] = 0;
+    if (threadIdx.x <= C_REASON_LAST) reasons[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t begin = uint64_t(blockIdx.x) * kDecodeChunk;
     for (int it = 0; it < kDecodeChunk / kDecodeBlock; ++it) {
@@ -211,7 +232,7 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
             }
             if (!fallback) P.hits[i] = h;
         }
-        for (int c = C_CONF; c <= C_DOMAIN; ++c) {
+        for (int c = C_CONF; c <= C_REASON_LAST; ++c) {
             const uint64_t m = __ballot(reason == c);
             if (m != 0 && lane == 0) atomicAdd(&reasons[c], uint32_t(__popcll(m)));
         }
@@ -226,7 +247,7 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
     __syncthreads();
     const uint32_t n_q = n_local;
     if (threadIdx.x == 0) base_s = n_q ? atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)n_q) : 0ull;
-    if (threadIdx.x >= C_CONF && threadIdx.x <= C_DOMAIN && reasons[threadIdx.x])
+    if (threadIdx.x >= C_CONF && threadIdx.x <= C_REASON_LAST && reasons[threadIdx.x])
         atomicAdd(&P.cnt[threadIdx.x], (unsigned long long)reasons[threadIdx.x]);
     __syncthreads();
     const unsigned long long base = base_s;
@@ -486,6 +507,7 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     }
     D.flags = scene->flags;
     D.a_eq = integ->time_reversed ? -a : a;
+    D.r_horizon = 1.0 + std::sqrt(std::max(0.0, 1.0 - a * a));
     D.disk_r_in = scene->disk_r_in;
     D.disk_r_out = scene->disk_r_out;
     D.min_conf = fb->min_confidence;
@@ -493,6 +515,7 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.edge_margin = fb->disk_edge_margin;
     D.judge_stride = fb->judge_stride;
     D.trace_disk = fb->trace_disk;
+    D.analytic = fb->analytic;
     hipLaunchKernelGGL(k_sur_decode, dim3(unsigned((n + kDecodeChunk - 1) / kDecodeChunk)), dim3(kDecodeBlock), 0, st, D);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(s->ev[2], st));
@@ -526,6 +549,7 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
         stats->n_fallback_disk = h[C_DISK];
         stats->n_fallback_star = h[C_STAR];
         stats->n_fallback_domain = h[C_DOMAIN];
+        stats->n_fallback_analytic = h[C_ANALYTIC];
         stats->n_judge = h[C_JUDGE];
         stats->judge_class_agree = h[C_AGREE];
         stats->judge_n_disk = h[C_JDISK];

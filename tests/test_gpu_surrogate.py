@@ -275,3 +275,79 @@ def test_surrogate_spin_099_frame(gpu_ctx, trained_099):
     torch.cuda.synchronize()
     assert st_all["n_fallback"] == n
     assert torch.equal(xyz_all, plain)
+
+
+def analytic_frame_vs_fp64_oracle(gpu_ctx, s, workload, W, H, spp, seed, direction_margin=2e-2):
+    """Surrogate frame with fallback.analytic against the FP64 oracle on the
+    same seeded rays: per-ray class and radius, escape directions, and the
+    per-pixel linear radiance (both hit sets through the oracle's shading)."""
+    import torch
+    from oracle import oracle as O
+    integ, sc, spec, cam, _ = M.workload(workload, W=W, H=H, spp=spp)
+    gpu_ctx.set_scene(sc)
+    n = W * H * spp
+    hits = torch.zeros((n, 4), dtype=torch.int32, device="cuda:0")
+    # the briefly trained test networks place escape directions to ~5e-3 rad:
+    # the star-edge margin is widened to match (the bench's network: ~1e-4)
+    fb = M.fallback(analytic=True, judge_stride=7, direction_margin=direction_margin)
+    xyz, st = M.render_frame_surrogate(gpu_ctx, s, integ, sc, spec, cam, seed=seed, spp=spp, fb=fb, hits=hits)
+    it64 = M.integrator()
+    C.memmove(C.byref(it64), C.byref(integ), C.sizeof(integ))
+    it64.precision = M.PREC_FP64
+    rays = gpu_ctx.generate_rays(it64, cam, seed, 0, spp)
+    torch.cuda.synchronize()
+    gh = M.hits_to_numpy(hits)
+    rh, _, _ = O.trace_rays(O.F64, it64, sc, list(cam.position), rays.cpu().numpy().T.copy())
+    same = gh["cls"] == rh["cls"]
+    disk = same & (rh["cls"] == M.HIT_DISK)
+    rel_r = np.abs(gh["a0"][disk].astype(np.float64) - rh["a0"][disk]) / rh["a0"][disk]
+    esc = same & (rh["cls"] == M.HIT_ESCAPE)
+
+    def unit(h):
+        c, ph = h["a0"].astype(np.float64), h["a1"].astype(np.float64)
+        sn = np.sqrt(np.maximum(0.0, 1.0 - c * c))
+        return np.stack([sn * np.cos(ph), c, sn * np.sin(ph)])
+    ang = np.arccos(np.clip(np.sum(unit(gh[esc]) * unit(rh[esc]), axis=0), -1.0, 1.0)) if esc.any() else np.zeros(1)
+    Wt, _ = M.color_weights(spec)
+    npx = W * H
+    y_s = O.shade(sc, spec, cam, seed, 0, spp, np.arange(npx), gh, Wt)[:, 1]
+    y_o = O.shade(sc, spec, cam, seed, 0, spp, np.arange(npx), rh, Wt)[:, 1]
+    err = np.abs(y_s - y_o) / np.maximum(y_o, 1e-3 * y_o.mean())
+    out = dict(class_agreement=float(same.mean()), rel_r_mean=float(rel_r.mean()), rel_r_q999=float(np.quantile(rel_r, 0.999)),
+               rel_r_max=float(rel_r.max()),
+               n_escape=int(esc.sum()), dir_mean=float(ang.mean()), dir_max=float(ang.max()), y_mean=float(err.mean()),
+               y_frac=float(np.mean(err <= 1e-3)), fallback=st["n_fallback"] / n, analytic=st["n_fallback_analytic"] / n,
+               judge_n_escape=st["judge_n_escape"], judge_class_agree=st["judge_class_agree"] / max(1, st["n_judge"]))
+    print(workload, out, flush=True)
+    return out
+
+
+def check_analytic_frame(gpu_ctx, s, workload, seed):
+    o = analytic_frame_vs_fp64_oracle(gpu_ctx, s, workload, 128, 128, 2, seed=seed)
+    assert o["class_agreement"] >= 0.999 and o["rel_r_mean"] < 1e-5 and o["rel_r_q999"] < 1e-4, o
+    assert o["n_escape"] > 0 and o["dir_max"] < 5e-2, o
+    assert o["y_mean"] <= 1e-3 and o["y_frac"] >= 0.995, o
+    assert o["fallback"] < 0.4 and o["judge_class_agree"] >= 0.99, o
+    # with the default star-edge margin the network answers escaping paths too:
+    # the judge measures their direction error against RKF45
+    o = analytic_frame_vs_fp64_oracle(gpu_ctx, s, workload, 128, 128, 2, seed=seed, direction_margin=2e-3)
+    assert o["judge_n_escape"] > 0 and o["judge_class_agree"] >= 0.99 and o["class_agreement"] >= 0.999, o
+
+
+def test_surrogate_analytic_frame_matches_fp64_oracle(gpu_ctx, trained):
+    """Kerr a = 0.9 (the headline view, square so that a part escapes): with
+    the Mino-time check and radius refinement the surrogate frame's linear
+    radiance is within 1e-3 of the FP64 oracle's per pixel (the network alone
+    misses that bar by ~30x: its radii are good to ~1 %).  The radii of
+    answered disk paths are exact to the quadrature (~1e-8); the few larger
+    differences (99.9th percentile) are paths traced by RKF45 next to the
+    critical curve, where the mixed and FP64 integrations part."""
+    s, _ = trained
+    check_analytic_frame(gpu_ctx, s, "kerr_disk_1080p", seed=8)
+
+
+def test_surrogate_analytic_spin_099_frame_matches_fp64_oracle(gpu_ctx, trained_099):
+    """The learned-surrogate configuration's spin (kerr_disk_4k, a = 0.99) at a
+    reduced, square size, against the FP64 oracle per ray and per pixel."""
+    s, _ = trained_099
+    check_analytic_frame(gpu_ctx, s, "kerr_disk_4k", seed=9)
