@@ -47,6 +47,18 @@ struct Gl16 {
                                      0.062314485627767015, 0.047579255841246296, 0.031126761969323853, 0.013576229705877019};
 };
 
+// 1/sqrt(x) of the quadrature nodes: on the device the v_rsq_f64 seed with one
+// Newton step (relative error e -> 1.5 e^2, ~1e-13), far cheaper than a
+// correctly rounded square root and division; the integrals need ~1e-12.
+__host__ __device__ inline double mino_rsqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    return y * fma(-0.5 * x * y, y, 1.5);
+#else
+    return 1.0 / sqrt(x);
+#endif
+}
+
 enum MinoKind : int { kMinoUnresolved = 0, kMinoCaptured = 1, kMinoEscapes = 2, kMinoDisk = 3 };
 
 struct MinoResult {
@@ -161,8 +173,9 @@ __host__ __device__ inline RootPair radial_pair(const Radial& R) {
 }  // namespace mino_detail
 
 // Disk crossing (or capture / escape) of the ray: the first crossing of the
-// equator within [r_in, r_out], with r_seed the estimate the Newton iteration
-// starts from when it lies on the crossing's branch (<= 0: none).  r_horizon
+// equator within [r_in, r_out].  r_seed > 0 asks for the crossing's radius and
+// is the estimate the Newton iteration starts from when it lies on the
+// crossing's branch; r_seed <= 0 asks for the class only (res.r stays 0).  r_horizon
 // is the outer horizon radius 1 + sqrt(1 - a^2).  Only rays that start
 // inward (sr < 0) and outside the disk plane (u != 0) are resolved.
 __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_horizon, double r_in, double r_out, double r_seed) {
@@ -198,10 +211,10 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
     auto tail = [&](double u) {
         const double W = sqrt(su - u);
         double acc = 0.0;
-#pragma unroll
+#pragma unroll 4
         for (int k = 0; k < 16; ++k) {
             const double w = W * Gl16::x[k], uu = su - w * w;
-            acc += Gl16::w[k] / sqrt((su + uu) * (Ap * uu * uu + Am));
+            acc += Gl16::w[k] * mino_rsqrt((su + uu) * (Ap * uu * uu + Am));
         }
         return 2.0 * W * acc;
     };
@@ -212,6 +225,7 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
 
     // ---- radial motion
     const double s_in = 1.0 / r_in, s_out = 1.0 / r_out;
+    const double s_seed = r_seed > 0.0 ? 1.0 / r_seed : -1.0;
     double s4 = 0.0;
     double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;   // Q(s) = P(s) / (s4 - s) = -(b3 s^3 + b2 s^2 + b1 s + b0)
     if (turning) {
@@ -225,10 +239,10 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
     // K(W) = int_{s4 - W^2}^{s4} ds / sqrt(P) (turning rays)
     auto K = [&](double W) {
         double acc = 0.0;
-#pragma unroll
+#pragma unroll 4
         for (int k = 0; k < 16; ++k) {
             const double w = W * Gl16::x[k];
-            acc += Gl16::w[k] / sqrt(Qs(s4 - w * w));
+            acc += Gl16::w[k] * mino_rsqrt(Qs(s4 - w * w));
         }
         return 2.0 * W * acc;
     };
@@ -236,58 +250,74 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
     auto J = [&](double s) {
         double acc = 0.0;
         const double ds = s - s_obs;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) acc += Gl16::w[k] / sqrt(rad.P(s_obs + ds * Gl16::x[k]));
+#pragma unroll 4
+        for (int k = 0; k < 16; ++k) acc += Gl16::w[k] * mino_rsqrt(rad.P(s_obs + ds * Gl16::x[k]));
         return ds * acc;
     };
-
+    // The crossings are taken in order.  With a seed (the caller wants the
+    // radius) each one's radius is solved for on its branch (bracketed Newton,
+    // seeded by the estimate when that lies on the branch) and a crossing
+    // within [r_in, r_out] is the disk hit; Newton stops once a step is below
+    // 1e-5 relative: the error left after it is of the order of its square
+    // (the integrals are smooth in W and s).  Without one only the class is
+    // wanted: a crossing meets the disk when its Mino time lies between those
+    // of the disk's edges on its branch, and no radius is solved for.
+    const bool want_r = s_seed > 0.0;
     if (turning) {
-        const double W_obs = sqrt(s4 - s_obs);
+        const double W_obs = sqrt(s4 - s_obs), W_inf = sqrt(s4);
         const double K_obs = K(W_obs);
-        const double K_inf = K(sqrt(s4));
-        // windows of the disk on the two branches, as W = sqrt(s4 - s) ranges
-        // inbound: s in [max(s_out, s_obs), min(s_in, s4)] -> tau = K_obs - K(W), W decreasing in s
-        // outbound: s in [s_out, min(s_in, s4)] -> tau = K_obs + K(W)
+        const double K_inf = K(W_inf);
+        // the disk's window on each branch, as W = sqrt(s4 - s) ranges over it
+        // (inbound: s in [max(s_out, s_obs), min(s_in, s4)]; outbound: s in
+        // [s_out, min(s_in, s4)]); K is increasing in W
+        double Ka_in = 0.0, Kb = 0.0, Ka_out = 0.0;
         const double s_lo_in = fmax(s_out, s_obs), s_hi = fmin(s_in, s4);
         const bool win_in = s_lo_in < s_hi, win_out = s_out < s_hi;
-        const double Wa_in = win_in ? sqrt(s4 - s_lo_in) : 0.0;   // larger W (outer edge of the window)
-        const double Wb = sqrt(s4 - s_hi);                       // smaller W (inner edge)
-        const double Wa_out = win_out ? sqrt(s4 - s_out) : 0.0;
-        const double Ka_in = win_in ? K(Wa_in) : 0.0, Kb = K(Wb), Ka_out = win_out ? K(Wa_out) : 0.0;
+        if (!want_r) {
+            Kb = K(sqrt(s4 - s_hi));
+            Ka_in = win_in ? K(sqrt(s4 - s_lo_in)) : 0.0;
+            Ka_out = win_out ? K(sqrt(s4 - s_out)) : 0.0;
+        }
         for (int n = 1; n <= kMinoMaxCrossings; ++n) {
             const double G = G1 + 2.0 * double(n - 1) * G_half;
             if (G >= K_obs + K_inf) {
                 res.kind = kMinoEscapes;
                 return res;
             }
-            double target, Wlo, Whi;
-            bool hit = false;
-            if (win_in && G >= K_obs - Ka_in && G <= K_obs - Kb) {
-                target = K_obs - G; Wlo = Wb; Whi = Wa_in; hit = true;           // K(W) = K_obs - G
-            } else if (win_out && G >= K_obs + Kb && G <= K_obs + Ka_out) {
-                target = G - K_obs; Wlo = Wb; Whi = Wa_out; hit = true;          // K(W) = G - K_obs
+            if (!want_r) {
+                if ((win_in && G >= K_obs - Ka_in && G <= K_obs - Kb) || (win_out && G >= K_obs + Kb && G <= K_obs + Ka_out)) {
+                    res.kind = kMinoDisk;
+                    res.crossing = n;
+                    return res;
+                }
+                continue;
             }
-            if (!hit) continue;
-            // bracketed Newton on K(W) = target, dK/dW = 2 / sqrt(Q(s4 - W^2))
+            // inbound (before the turning point): K(W) = K_obs - G, W in [0, W_obs];
+            // outbound: K(W) = G - K_obs, W in [0, W_inf]
+            const bool inbound = G <= K_obs;
+            const double target = inbound ? K_obs - G : G - K_obs;
+            double Wlo = 0.0, Whi = inbound ? W_obs : W_inf;
+            // a crossing beyond the disk's radial range on either side cannot be
+            // the hit: the branch's radii at its ends bound it
+            if (s4 < s_out) continue;                        // the whole branch lies outside r_out
             double W = 0.5 * (Wlo + Whi);
-            if (r_seed > 0.0) {
-                const double ss = 1.0 / r_seed;
-                if (ss < s4 && ss > s4 - Whi * Whi && ss < s4 - Wlo * Wlo) W = sqrt(s4 - ss);
-            }
+            if (s_seed > 0.0 && s_seed < s4 && s_seed > s4 - Whi * Whi) W = sqrt(s4 - s_seed);
             bool done = false;
             for (int it = 0; it < kMinoMaxNewton && !done; ++it) {
                 const double f = K(W) - target;
                 if (f > 0.0) Whi = W; else Wlo = W;
                 double Wn = W - 0.5 * f * sqrt(Qs(s4 - W * W));
                 if (!(Wn >= Wlo && Wn <= Whi)) Wn = 0.5 * (Wlo + Whi);
-                done = fabs(Wn - W) <= 1e-8 * W;   // Newton: the next error is ~ (1e-8)^2
+                done = fabs(Wn - W) <= 1e-5 * W + 1e-14;
                 W = Wn;
-                res.iters = it + 1;
+                res.iters += 1;
             }
             if (!done) return res;
+            const double s = s4 - W * W;
+            if (s < s_out || s > s_in) continue;             // this crossing misses the disk
             res.kind = kMinoDisk;
             res.crossing = n;
-            res.r = 1.0 / (s4 - W * W);
+            res.r = 1.0 / s;
             return res;
         }
         return res;
@@ -296,28 +326,36 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
     const double J_h = J(s_h);
     const double s_lo = fmax(s_out, s_obs), s_hi = fmin(s_in, s_h);
     const bool win = s_lo < s_hi;
-    const double Ja = win ? J(s_lo) : 0.0, Jb = win ? J(s_hi) : 0.0;
+    const double Ja = (!want_r && win) ? J(s_lo) : 0.0, Jb = (!want_r && win) ? J(s_hi) : 0.0;
     for (int n = 1; n <= kMinoMaxCrossings; ++n) {
         const double G = G1 + 2.0 * double(n - 1) * G_half;
         if (G >= J_h) {
             res.kind = kMinoCaptured;
             return res;
         }
-        if (!(win && G >= Ja && G <= Jb)) continue;
-        double lo = s_lo, hi = s_hi;
+        if (!want_r) {
+            if (win && G >= Ja && G <= Jb) {
+                res.kind = kMinoDisk;
+                res.crossing = n;
+                return res;
+            }
+            continue;
+        }
+        double lo = s_obs, hi = s_h;
         double s = 0.5 * (lo + hi);
-        if (r_seed > 0.0 && 1.0 / r_seed > lo && 1.0 / r_seed < hi) s = 1.0 / r_seed;
+        if (s_seed > lo && s_seed < hi) s = s_seed;
         bool done = false;
         for (int it = 0; it < kMinoMaxNewton && !done; ++it) {
             const double f = J(s) - G;
             if (f > 0.0) hi = s; else lo = s;
             double sn = s - f * sqrt(rad.P(s));
             if (!(sn >= lo && sn <= hi)) sn = 0.5 * (lo + hi);
-            done = fabs(sn - s) <= 1e-8 * s;   // Newton: the next error is ~ (1e-8)^2
+            done = fabs(sn - s) <= 1e-5 * s;
             s = sn;
-            res.iters = it + 1;
+            res.iters += 1;
         }
         if (!done) return res;
+        if (s < s_out || s > s_in) continue;                 // this crossing misses the disk
         res.kind = kMinoDisk;
         res.crossing = n;
         res.r = 1.0 / s;

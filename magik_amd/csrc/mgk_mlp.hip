@@ -7,13 +7,15 @@
 //                    fused epilogues (bias + SiLU -> z, h; SiLU' -> dz, g).
 //                    Used for the training forward, backward-data and
 //                    weight-gradient products.
-//   k_mlp_infer    : the whole network for 128 rays per workgroup, computed
-//                    transposed (Z^T = W^T H^T: weights are the MFMA A
-//                    operand, rays the columns) so that a layer's FP32
-//                    accumulators, after bias + SiLU, are already the next
-//                    layer's bf16 B operand in registers; the hidden layers'
-//                    weights stream through a double-buffered LDS stage
-//                    shared by the workgroup's four waves.
+//   k_mlp_infer    : the whole network, persistent (one workgroup per CU
+//                    looping over 256-ray blocks), computed transposed
+//                    (Z^T = W^T H^T: weights are the MFMA A operand, rays the
+//                    columns) so that a layer's FP32 accumulators, after
+//                    bias + SiLU, are already the next layer's bf16 B operand
+//                    in registers.  Each wave keeps two ray tiles half a
+//                    layer apart, so one tile's matrix work and the other's
+//                    SiLU epilogue issue side by side; a hidden layer's
+//                    weights stay in LDS for both tiles.
 //   k_loss         : class cross-entropy + masked regressions -> dL/dy.
 //   k_adam         : Adam on the FP32 master weights, refreshes the bf16 copy.
 #include "mgk_internal.hpp"
@@ -131,20 +133,23 @@ __device__ __forceinline__ int perm_k(int ks, int g, int j) {
     return 32 * (ks >> 1) + 16 * (ks & 1) + 8 * (j >> 2) + 4 * g + (j & 3);
 }
 
-// Repack a row-major bf16 [K][N] weight into MFMA A-fragment order of the
-// transposed product (A = W^T): fragment (ks, nt) holds, for lane l and
-// j = 0..7, W[k][32 nt + l % 32] with k = 16 ks + 8 (l / 32) + j (natural,
-// the input layer) or k = perm_k(ks, l / 32, j) (layers fed by a previous
-// layer's accumulators).
+// Repack a row-major FP32 [K][N] master weight into MFMA A-fragment order of
+// the transposed product (A = W^T), each element scaled by `scale` before its
+// one rounding to bf16 (the inference kernel's log2-domain factors: -log2(e)
+// for the input layer, -ln 2 for the output layer, 1 for the hidden layers).
+// Fragment (ks, nt) holds, for lane l and j = 0..7, W[k][32 nt + l % 32] with
+// k = 16 ks + 8 (l / 32) + j (natural, the input layer) or
+// k = perm_k(ks, l / 32, j) (layers fed by a previous layer's accumulators);
+// k-step major, so two k-steps of all output tiles are one contiguous 16 KB:
 // F[((ks * (N / 32) + nt) * 64 + lane) * 8 + j].
-__global__ void k_repack(const __bf16* W, int K, int N, __bf16* F, int permuted) {
+__global__ void k_repack_scaled(const float* W, int K, int N, __bf16* F, int permuted, float scale) {
     const int64_t total = int64_t(K) * N;
     for (int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < total; o += int64_t(gridDim.x) * blockDim.x) {
         const int j = int(o & 7), lane = int((o >> 3) & 63);
         const int64_t t = o >> 9;
         const int nt = int(t % (N / 32)), ks = int(t / (N / 32));
         const int k = permuted ? perm_k(ks, lane >> 5, j) : 16 * ks + 8 * (lane >> 5) + j;
-        F[o] = W[int64_t(k) * N + 32 * nt + (lane & 31)];
+        F[o] = __bf16(scale * W[int64_t(k) * N + 32 * nt + (lane & 31)]);
     }
 }
 
@@ -227,22 +232,26 @@ __global__ void k_adam(float* w, float* m, float* v, const float* g, int64_t n, 
 constexpr int kInferWaves = 4;
 constexpr int kStepFrags = kMlpHidden / 32;               // A fragments per hidden k-step
 
-__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+constexpr float kNegLog2e = -1.44269504088896341f;
+// SiLU in the log2 domain: for u = -log2(e) z, u / (1 + 2^u) = -log2(e) silu(z)
+__device__ __forceinline__ float silu_log2(float u) { return u * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u)); }
 
-// bias + SiLU of one layer's accumulators into the next layer's B operand:
-// H[c][2 nt + h] holds elements i = 8 h .. 8 h + 7 of accumulator tile nt.
-template <int kInferTiles>
+// (bias +) SiLU of one layer's accumulators into the next layer's B operand:
+// H[c][2 nt + h] holds elements i = 8 h .. 8 h + 7 of accumulator tile nt,
+// element i being row 32 nt + 8 (i / 4) + 4 g + i % 4.  The hidden layers'
+// accumulators start from their bias (kAddBias false).
+template <bool kAddBias, int kInferTiles>
 __device__ __forceinline__ void infer_epilogue(const f32x16 (&acc)[kInferTiles][kStepFrags], const float* bias, int g,
                                                bf16x8 (&H)[kInferTiles][2 * kStepFrags]) {
 #pragma unroll
     for (int nt = 0; nt < kStepFrags; ++nt) {
         float b[16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) b[i] = bias[32 * nt + 8 * (i >> 2) + 4 * g + (i & 3)];
+        for (int i = 0; i < 16; ++i) b[i] = kAddBias ? bias[32 * nt + 8 * (i >> 2) + 4 * g + (i & 3)] : 0.0f;
 #pragma unroll
         for (int c = 0; c < kInferTiles; ++c)
 #pragma unroll
-            for (int i = 0; i < 16; ++i) H[c][2 * nt + (i >> 3)][i & 7] = __bf16(silu_fast(acc[c][nt][i] + b[i]));
+            for (int i = 0; i < 16; ++i) H[c][2 * nt + (i >> 3)][i & 7] = __bf16(silu_log2(kAddBias ? acc[c][nt][i] + b[i] : acc[c][nt][i]));
     }
 }
 
@@ -257,7 +266,7 @@ __global__ void __launch_bounds__(256, WG_PER_CU) k_mlp_infer(InferArgs a) {
     __shared__ bf16x8 wbuf[2][kStageFrags];               // double buffered (kStage x 8 KB each)
     __shared__ float bias[4 * kMlpHidden + kMlpOutPad];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 5, r = lane & 31;
-    for (int k = tid; k < 4 * kMlpHidden; k += 256) bias[k] = a.b[k / kMlpHidden][k % kMlpHidden];
+    for (int k = tid; k < 4 * kMlpHidden; k += 256) bias[k] = kNegLog2e * a.b[k / kMlpHidden][k % kMlpHidden];
     if (tid < kMlpOutPad) bias[4 * kMlpHidden + tid] = a.b[kMlpLayers - 1][tid];
     {
         const bf16x8* F1 = reinterpret_cast<const bf16x8*>(a.Wf[1]);
@@ -274,8 +283,8 @@ __global__ void __launch_bounds__(256, WG_PER_CU) k_mlp_infer(InferArgs a) {
         for (int c = 0; c < kInferTiles; ++c) {
             const int64_t row = ray0 + 32 * c + r;
             float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
-            if (row < a.n) {
-                const float4* src = reinterpret_cast<const float4*>(a.X + row * kMlpInPad + 8 * g);
+            if (row < a.n && 8 * g < a.x_stride) {
+                const float4* src = reinterpret_cast<const float4*>(a.X + row * a.x_stride + 8 * g);
                 lo = src[0];
                 hi = src[1];
             }
@@ -296,16 +305,22 @@ __global__ void __launch_bounds__(256, WG_PER_CU) k_mlp_infer(InferArgs a) {
         }
     }
     __syncthreads();                                      // bias and the first weight stage
-    infer_epilogue(acc, bias, g, H);
+    infer_epilogue<true>(acc, bias, g, H);
     // hidden layers 2..4: 16 k-steps each; the stage of step s + 1 is loaded
     // while step s runs, one barrier per step
     for (int l = 1; l < kMlpLayers - 1; ++l) {
+        // the accumulators start from the layer's bias (rows of the C layout)
 #pragma unroll
-        for (int c = 0; c < kInferTiles; ++c)
+        for (int nt = 0; nt < kStepFrags; ++nt) {
+            f32x16 b0;
 #pragma unroll
-            for (int nt = 0; nt < kStepFrags; ++nt)
+            for (int q = 0; q < 4; ++q) {
+                const float4 b4 = *reinterpret_cast<const float4*>(bias + l * kMlpHidden + 32 * nt + 8 * q + 4 * g);
+                b0[4 * q] = b4.x; b0[4 * q + 1] = b4.y; b0[4 * q + 2] = b4.z; b0[4 * q + 3] = b4.w;
+            }
 #pragma unroll
-                for (int i = 0; i < 16; ++i) acc[c][nt][i] = 0.0f;
+            for (int c = 0; c < kInferTiles; ++c) acc[c][nt] = b0;
+        }
 #pragma unroll
         for (int st = 0; st < 2 * kStepFrags / kStage; ++st) {
             const bool last = st == 2 * kStepFrags / kStage - 1;
@@ -334,7 +349,7 @@ __global__ void __launch_bounds__(256, WG_PER_CU) k_mlp_infer(InferArgs a) {
             }
             __syncthreads();
         }
-        infer_epilogue(acc, bias + l * kMlpHidden, g, H);
+        infer_epilogue<false>(acc, bias, g, H);
     }
     // output layer: one 32-row tile (8 outputs used)
     {
@@ -386,10 +401,10 @@ hipError_t launch_to_bf16(const float* x, __bf16* y, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_bf16, dim3(unsigned(blocks)), dim3(256), 0, s, x, y, n);
     return hipGetLastError();
 }
-hipError_t launch_repack(const __bf16* W, int K, int N, __bf16* F, int permuted, hipStream_t s) {
+hipError_t launch_repack_scaled(const float* W, int K, int N, __bf16* F, int permuted, float scale, hipStream_t s) {
     int64_t blocks = (int64_t(K) * N + 255) / 256;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_repack, dim3(unsigned(blocks)), dim3(256), 0, s, W, K, N, F, permuted);
+    hipLaunchKernelGGL(k_repack_scaled, dim3(unsigned(blocks)), dim3(256), 0, s, W, K, N, F, permuted, scale);
     return hipGetLastError();
 }
 hipError_t launch_loss(const float* Y, const float* T, int n, float w_r, float w_d, float* dY, float* sums, hipStream_t s) {

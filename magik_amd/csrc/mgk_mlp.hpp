@@ -35,10 +35,13 @@ struct GemmArgs {
     int epi;
 };
 
+constexpr int kFrameXStride = 8; // a frame's feature rows: the 5 features padded to 8 floats
+
 struct InferArgs {
-    const float* X;     // n x 16 normalised features
+    const float* X;     // n rows of x_stride floats: normalised features, zero padded
+    int x_stride;       // kMlpInPad or kFrameXStride (the padding beyond it is implied)
     int64_t n;
-    const __bf16* Wf[kMlpLayers];   // bf16 weights in the A-fragment order of k_repack
+    const __bf16* Wf[kMlpLayers];   // scaled bf16 weights in the A-fragment order of k_repack_scaled
     const float* b[kMlpLayers];
     float* Y;           // n x 8 outputs
 };
@@ -47,7 +50,7 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
 hipError_t launch_colsum(const float* X, int M, int N, float* out, hipStream_t s);
 hipError_t launch_zero(float* p, int64_t n, hipStream_t s);
 hipError_t launch_to_bf16(const float* x, __bf16* y, int64_t n, hipStream_t s);
-hipError_t launch_repack(const __bf16* W, int K, int N, __bf16* F, int permuted, hipStream_t s);
+hipError_t launch_repack_scaled(const float* W, int K, int N, __bf16* F, int permuted, float scale, hipStream_t s);
 hipError_t launch_loss(const float* Y, const float* T, int n, float w_r, float w_d, float* dY, float* sums, hipStream_t s);
 hipError_t launch_adam(float* w, float* m, float* v, const float* g, int64_t n, float lr, int t, __bf16* wb, hipStream_t s);
 hipError_t launch_infer(const InferArgs& a, hipStream_t s);

@@ -170,7 +170,12 @@ e_magik_result_types train_backward(mgk_surrogate* s, int n, hipStream_t st) {
 }
 
 e_magik_result_types refresh_fragments(mgk_surrogate* s, hipStream_t st) {
-    for (int l = 0; l < kMlpLayers; ++l) HIP_TRY(launch_repack(s->Wb[l], kK[l], kN[l], s->Wf[l], l > 0 ? 1 : 0, st));
+    // the inference kernel's log2-domain factors: -log2(e) on the input
+    // layer, -ln 2 on the output layer, none on the hidden layers
+    for (int l = 0; l < kMlpLayers; ++l) {
+        const float scale = l == 0 ? -1.44269504088896341f : l == kMlpLayers - 1 ? -0.693147180559945309f : 1.0f;
+        HIP_TRY(launch_repack_scaled(s->W[l], kK[l], kN[l], s->Wf[l], l > 0 ? 1 : 0, scale, st));
+    }
     s->frag_dirty = false;
     return MAGIK_SUCCESS;
 }
@@ -294,16 +299,18 @@ MGK_API e_magik_result_types mgk_surrogate_set_weights(mgk_surrogate_handle s, c
     return MAGIK_SUCCESS;
 }
 
-MGK_API e_magik_result_types mgk_surrogate_forward(mgk_surrogate_handle s, const float* d_x, uint64_t n, float* d_y, void* stream) {
-    if (!s || (n && (!d_x || !d_y))) return MAGIK_ERROR_INVALID_POINTER;
+}  // extern "C"
+
+namespace mgk {
+e_magik_result_types surrogate_forward(mgk_surrogate* s, const float* d_x, int x_stride, uint64_t n, float* d_y, hipStream_t st) {
     HIP_TRY(hipSetDevice(s->device));
-    hipStream_t st = hipStream_t(stream);
     if (s->frag_dirty) {
         e_magik_result_types r = refresh_fragments(s, st);
         if (r != MAGIK_SUCCESS) return r;
     }
     InferArgs a;
     a.X = d_x;
+    a.x_stride = x_stride;
     a.n = int64_t(n);
     for (int l = 0; l < kMlpLayers; ++l) {
         a.Wf[l] = s->Wf[l];
@@ -313,6 +320,15 @@ MGK_API e_magik_result_types mgk_surrogate_forward(mgk_surrogate_handle s, const
     HIP_TRY(launch_infer(a, st));
     return MAGIK_SUCCESS;
 }
+}  // namespace mgk
+
+extern "C" {
+
+MGK_API e_magik_result_types mgk_surrogate_forward(mgk_surrogate_handle s, const float* d_x, uint64_t n, float* d_y, void* stream) {
+    if (!s || (n && (!d_x || !d_y))) return MAGIK_ERROR_INVALID_POINTER;
+    return surrogate_forward(s, d_x, kMlpInPad, n, d_y, hipStream_t(stream));
+}
+
 
 MGK_API e_magik_result_types mgk_surrogate_gradients(mgk_surrogate_handle s, const float* d_x, const float* d_dy, uint32_t n, float* d_y,
                                                      float* W_grad, float* b_grad) {

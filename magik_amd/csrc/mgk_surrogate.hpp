@@ -134,12 +134,18 @@ struct mgk_surrogate {
     float* sums = nullptr;
     mgk::SurrogateRanges ranges{};
     // render-path buffers
-    mgk::SBuf fr_rays, fr_x, fr_y, fr_hits, fr_queue, fr_pred, fr_counters;
+    mgk::SBuf fr_rays, fr_x, fr_y, fr_hits, fr_queue, fr_pred, fr_counters, fr_rqueue, fr_reason;
     // critical-curve table (screen radius per angle) and its key
     mgk::SBuf crit, crit_lo, crit_hi, crit_ok, crit_rays, crit_hits;
     double crit_key[6] = {-1, -1, -1, -1, -1, -1};
     hipEvent_t ev[5] = {};
 };
+
+namespace mgk {
+// The fused inference over n feature rows of x_stride floats (kMlpInPad, the
+// public layout, or kFrameXStride, a frame's own).
+e_magik_result_types surrogate_forward(mgk_surrogate* s, const float* d_x, int x_stride, uint64_t n, float* d_y, hipStream_t st);
+}  // namespace mgk
 
 // Internal (not exported): trace rays whose spin is stored per ray (component
 // 10), with the escape sphere about each ray's origin and the disk starting at

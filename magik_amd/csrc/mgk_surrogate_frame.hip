@@ -3,7 +3,8 @@
 // integrator; a held-out judge sample is traced as well and compared.
 //
 //   k_sur_features : pixel + sample -> screen point (alpha, beta) of the lens
-//                    point's camera frame -> normalised features (n x 16)
+//                    point's camera frame -> normalised features (n x 8:
+//                    the 5 features, zero padded)
 //   k_mlp_infer    : (mgk_mlp.hip) features -> outputs (n x 8)
 //   k_sur_decode   : outputs -> class / disk radius / escape direction, the
 //                    fallback rules, (fallback.analytic) the Mino-time check
@@ -52,7 +53,7 @@ constexpr uint32_t kNotAnswered = 0xFFFFFFFFu;
 // device counters of one frame (u64 slots; the double sums live in the
 // same buffer, bit-cast)
 enum {
-    C_QUEUE = 0, C_CONF, C_CRIT, C_DISK, C_STAR, C_DOMAIN, C_ANALYTIC, C_JUDGE, C_AGREE, C_JDISK, C_JESC,
+    C_QUEUE = 0, C_CONF, C_CRIT, C_DISK, C_STAR, C_DOMAIN, C_ANALYTIC, C_JUDGE, C_AGREE, C_JDISK, C_JESC, C_RQUEUE,
     C_SUM_R, C_SUM_G, C_SUM_DIR, C_MAX_DIR, C_N
 };
 constexpr int C_REASON_LAST = C_ANALYTIC;   // the fallback reasons are C_CONF .. C_REASON_LAST
@@ -80,8 +81,8 @@ __global__ void __launch_bounds__(256) k_sur_features(FeatureParams P) {
     const double al = P.r_lens * (-sx) / f_m, be = P.r_lens * (-sy) / f_m;
     float x[kMlpInPad];
     normalise_features(P.R, P.a, P.r_lens, P.th_lens_deg, al, be, x);
-    float4* dst = reinterpret_cast<float4*>(P.X + i * kMlpInPad);
-    for (int k = 0; k < kMlpInPad / 4; ++k) dst[k] = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
+    float4* dst = reinterpret_cast<float4*>(P.X + i * kFrameXStride);
+    for (int k = 0; k < kFrameXStride / 4; ++k) dst[k] = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
 }
 
 struct DecodeParams {
@@ -92,6 +93,8 @@ struct DecodeParams {
     const double* rays;          // 10 x n doubles (q, p, h0, u.p)
     mgk_hit_t* hits;
     uint32_t* queue;
+    uint32_t* rqueue;            // paths left to the resolve pass (k_sur_resolve)
+    uint8_t* reason;             // their fallback reasons
     mgk_hit_t* pred;             // judge predictions (one per judge slot)
     unsigned long long* cnt;
     const float* crit;           // kCritN screen radii of the critical curve (< 0: none)
@@ -124,7 +127,21 @@ __device__ bool near_star_edge(const DecodeParams& P, float dx, float dy, float 
 constexpr int kDecodeBlock = 256;
 constexpr int kDecodeChunk = 16 * kDecodeBlock;
 
-__device__ __forceinline__ bool decode_one(const DecodeParams& P, uint64_t i, mgk_hit_t& h, int& reason) {
+// A disk hit at radius r: the redshift of the emitter on its circular
+// (Keplerian) orbit seen along the path.
+__device__ __forceinline__ void disk_hit(const DecodeParams& P, uint64_t i, double r, double con, double pt, mgk_hit_t& h) {
+    double ut, w;
+    kepler_ut_w<KERR, double>(r, P.a_eq, ut, w);
+    const double qx = P.rays[0 * P.n + i], qz = P.rays[2 * P.n + i];
+    const double px = P.rays[4 * P.n + i], pz = P.rays[6 * P.n + i];
+    double g = con / (ut * pt + w * (qz * px - qx * pz));
+    if (isnan(g) || isinf(g)) g = 1e-10;
+    h = mgk_hit_t{MGK_HIT_DISK, float(g), float(r), 0.0f};
+}
+
+enum DecodeStatus : int { kAnswered = 0, kTrace = 1, kDefer = 2 };
+
+__device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, mgk_hit_t& h, int& reason) {
     bool fallback = false;
     reason = -1;
     h = mgk_hit_t{MGK_HIT_NONE, 0.0f, 0.0f, 0.0f};
@@ -137,8 +154,8 @@ __device__ __forceinline__ bool decode_one(const DecodeParams& P, uint64_t i, mg
     const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
     const int cls = (p0 >= p1 && p0 >= p2) ? 0 : (p1 >= p2 ? 1 : 2);
     const float conf = fmaxf(p0, fmaxf(p1, p2));
-    const float2 ab = *reinterpret_cast<const float2*>(P.X + i * kMlpInPad + 2);   // x[2], x[3]
-    const float be_n = P.X[i * kMlpInPad + 4];
+    const float2 ab = *reinterpret_cast<const float2*>(P.X + i * kFrameXStride + 2);   // x[2], x[3]
+    const float be_n = P.X[i * kFrameXStride + 4];
     const float al = ab.y * float(P.R.s_max), be = be_n * float(P.R.s_max);
     if (fabsf(al) > float(P.R.s_max) || fabsf(be) > float(P.R.s_max)) {
         fallback = true; reason = C_DOMAIN;
@@ -156,12 +173,18 @@ __device__ __forceinline__ bool decode_one(const DecodeParams& P, uint64_t i, mg
             fallback = true; reason = C_CRIT;
         }
     }
-    if (fallback) return true;
     const double con = P.rays[9 * P.n + i], pt = P.rays[7 * P.n + i];
     double r = exp(P.R.lnr_mid + double(y0.z) * P.R.lnr_half);
+    if (fallback) {
+        // the network is not trusted here (outside its domain, unsure, or near
+        // the critical curve): traced, or with the Mino-time integrals on,
+        // left to the resolve pass (k_sur_resolve), which runs them as full
+        // waves (scattered among the others they would hold up their waves)
+        return (!P.analytic || P.trace_disk) ? kTrace : kDefer;
+    }
     if (cls == 1 && P.trace_disk) {
         reason = C_DISK;
-        return true;
+        return kTrace;
     }
     if (P.analytic) {
         // the Mino-time integrals must confirm the network's class; a disk
@@ -173,85 +196,140 @@ __device__ __forceinline__ bool decode_one(const DecodeParams& P, uint64_t i, mg
         const int want = cls == 0 ? kMinoCaptured : (cls == 1 ? kMinoDisk : kMinoEscapes);
         if (mr.kind != want) {
             reason = C_ANALYTIC;
-            return true;
+            return kTrace;
         }
         if (cls == 1) r = mr.r;
     } else if (cls == 1 && (r < P.disk_r_in * (1.0 + P.edge_margin) || r > P.disk_r_out * (1.0 - P.edge_margin))) {
         reason = C_DISK;
-        return true;
+        return kTrace;
     }
     if (cls == 0) {
         h.cls = MGK_HIT_HORIZON;
     } else if (cls == 1) {
-        double ut, w;
-        kepler_ut_w<KERR, double>(r, P.a_eq, ut, w);
-        const double qx = P.rays[0 * P.n + i], qz = P.rays[2 * P.n + i];
-        const double px = P.rays[4 * P.n + i], pz = P.rays[6 * P.n + i];
-        double g = con / (ut * pt + w * (qz * px - qx * pz));
-        if (isnan(g) || isinf(g)) g = 1e-10;
-        h = mgk_hit_t{MGK_HIT_DISK, float(g), float(r), 0.0f};
+        disk_hit(P, i, r, con, pt, h);
     } else {
         const float c = fminf(1.0f, fmaxf(-1.0f, y0.w));
         const float ph = atan2f(y1.x, y1.y);
         const float sn = sqrtf(fmaxf(0.0f, 1.0f - c * c));
         if (P.star_test && near_star_edge(P, sn * cosf(ph), c, sn * sinf(ph), ph)) {
             reason = C_STAR;
-            return true;
+            return kTrace;
         }
         double g = con / pt;
         if (isnan(g) || isinf(g)) g = 1e-10;
         h = mgk_hit_t{MGK_HIT_ESCAPE, float(g), c, ph};
     }
-    return false;
+    return kAnswered;
+}
+
+// Appends the lanes' flagged path indices to a block-local list (one LDS
+// atomic per wave, ballot ranks within it).
+__device__ __forceinline__ void wave_append(bool flag, uint32_t i, uint32_t lane, uint64_t lt_mask, uint32_t* n_local, uint32_t* buf) {
+    const uint64_t m = __ballot(flag);
+    if (m == 0) return;
+    uint32_t off = 0;
+    if (lane == 0) off = atomicAdd(n_local, uint32_t(__popcll(m)));
+    off = __shfl(off, 0);
+    if (flag) buf[off + __popcll(m & lt_mask)] = i;
 }
 
 __global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
-    __shared__ uint32_t qbuf[kDecodeChunk];
-    __shared__ uint32_t n_local;
+    __shared__ uint32_t qbuf[kDecodeChunk], rbuf[kDecodeChunk];
+    __shared__ uint32_t n_local[2];
     __shared__ uint32_t reasons[C_REASON_LAST + 1];
-    __shared__ unsigned long long base_s;
+    __shared__ unsigned long long base_s[2];
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    if (threadIdx.x == 0) n_local = 0;
+    if (threadIdx.x < 2) n_local[threadIdx.x] = 0;
     if (threadIdx.x <= C_REASON_LAST) reasons[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t begin = uint64_t(blockIdx.x) * kDecodeChunk;
+#pragma unroll 1
     for (int it = 0; it < kDecodeChunk / kDecodeBlock; ++it) {
         const uint64_t i = begin + uint64_t(it) * kDecodeBlock + threadIdx.x;
         const bool live = i < P.n;
-        bool fallback = false;
+        int st = kAnswered;
         int reason = -1;
         mgk_hit_t h;
         if (live) {
-            fallback = decode_one(P, i, h, reason);
+            st = decode_one(P, i, h, reason);
             if (P.judge_stride && (i % P.judge_stride) == 0) {
+                // judge sample: the answer recorded, the path traced as well
+                // (a deferred one's answer is recorded by the resolve pass)
                 mgk_hit_t pr = h;
-                if (fallback) pr.cls = kNotAnswered;
+                if (st != kAnswered) pr.cls = kNotAnswered;
                 P.pred[i / P.judge_stride] = pr;
-                fallback = true;
+                if (st == kAnswered) st = kTrace;
             }
-            if (!fallback) P.hits[i] = h;
+            if (st == kAnswered) P.hits[i] = h;
+            if (st == kDefer) P.reason[i] = uint8_t(reason);
         }
+        // a deferred path's reason counts only if the resolve pass traces it
         for (int c = C_CONF; c <= C_REASON_LAST; ++c) {
-            const uint64_t m = __ballot(reason == c);
+            const uint64_t m = __ballot(st == kTrace && reason == c);
             if (m != 0 && lane == 0) atomicAdd(&reasons[c], uint32_t(__popcll(m)));
         }
-        const uint64_t fb = __ballot(live && fallback);
-        if (fb != 0) {
-            uint32_t off = 0;
-            if (lane == 0) off = atomicAdd(&n_local, uint32_t(__popcll(fb)));
-            off = __shfl(off, 0);
-            if (live && fallback) qbuf[off + __popcll(fb & lt_mask)] = uint32_t(i);
-        }
+        wave_append(live && st == kTrace, uint32_t(i), lane, lt_mask, &n_local[0], qbuf);
+        wave_append(live && st == kDefer, uint32_t(i), lane, lt_mask, &n_local[1], rbuf);
     }
     __syncthreads();
-    const uint32_t n_q = n_local;
-    if (threadIdx.x == 0) base_s = n_q ? atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)n_q) : 0ull;
+    const uint32_t n_q = n_local[0], n_r = n_local[1];
+    if (threadIdx.x == 0) base_s[0] = n_q ? atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)n_q) : 0ull;
+    if (threadIdx.x == 1) base_s[1] = n_r ? atomicAdd(&P.cnt[C_RQUEUE], (unsigned long long)n_r) : 0ull;
     if (threadIdx.x >= C_CONF && threadIdx.x <= C_REASON_LAST && reasons[threadIdx.x])
         atomicAdd(&P.cnt[threadIdx.x], (unsigned long long)reasons[threadIdx.x]);
     __syncthreads();
-    const unsigned long long base = base_s;
-    for (uint32_t k = threadIdx.x; k < n_q; k += kDecodeBlock) P.queue[base + k] = qbuf[k];
+    for (uint32_t k = threadIdx.x; k < n_q; k += kDecodeBlock) P.queue[base_s[0] + k] = qbuf[k];
+    for (uint32_t k = threadIdx.x; k < n_r; k += kDecodeBlock) P.rqueue[base_s[1] + k] = rbuf[k];
+}
+
+// The resolve pass over the paths the decode deferred (the network not
+// trusted; the Mino-time integrals on): one the integrals find captured or
+// on the disk is answered (its radius solved for, seeded by the network's;
+// a judge path's answer recorded and the path traced as well); an escape (its
+// direction would be the network's) or an unresolved path is traced, under
+// the reason the decode gave it.  A wave-uniform grid-stride loop over the
+// queue, whose length only the device knows.
+__global__ void __launch_bounds__(kDecodeBlock) k_sur_resolve(DecodeParams P) {
+    const uint64_t nr = P.cnt[C_RQUEUE];
+    const uint32_t lane = __lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t wave = (uint64_t(blockIdx.x) * kDecodeBlock + threadIdx.x) >> 6;
+    const uint64_t n_waves = (uint64_t(gridDim.x) * kDecodeBlock) >> 6;
+#pragma unroll 1
+    for (uint64_t base = wave * 64; base < nr; base += n_waves * 64) {
+        const uint64_t k = base + lane;
+        bool trace = false;
+        int reason = -1;
+        uint32_t i = 0;
+        if (k < nr) {
+            i = P.rqueue[k];
+            const double pt = P.rays[7 * P.n + i], con = P.rays[9 * P.n + i];
+            const double r = exp(P.R.lnr_mid + double(P.Y[uint64_t(i) * kMlpOut + 2]) * P.R.lnr_half);
+            const double q[3] = {P.rays[0 * P.n + i], P.rays[1 * P.n + i], P.rays[2 * P.n + i]};
+            const double p[4] = {P.rays[4 * P.n + i], P.rays[5 * P.n + i], P.rays[6 * P.n + i], pt};
+            const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.r_horizon, P.disk_r_in, P.disk_r_out, r);
+            mgk_hit_t h{MGK_HIT_HORIZON, 0.0f, 0.0f, 0.0f};
+            const bool ok = mr.kind == kMinoCaptured || mr.kind == kMinoDisk;
+            if (mr.kind == kMinoDisk) disk_hit(P, i, mr.r, con, pt, h);
+            const bool judge = P.judge_stride && (i % P.judge_stride) == 0;
+            if (ok && judge) P.pred[i / P.judge_stride] = h;
+            if (ok && !judge) P.hits[i] = h;
+            trace = !ok || judge;
+            if (!ok) reason = P.reason[i];
+        }
+        for (int c = C_CONF; c <= C_REASON_LAST; ++c) {
+            const uint64_t m = __ballot(reason == c);
+            if (m != 0 && lane == 0) atomicAdd(&P.cnt[c], (unsigned long long)__popcll(m));
+        }
+        const uint64_t m = __ballot(trace);
+        if (m != 0) {
+            unsigned long long off = 0;
+            if (lane == 0) off = atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)__popcll(m));
+            off = __shfl(off, 0);
+            if (trace) P.queue[off + __popcll(m & lt_mask)] = i;
+        }
+    }
 }
 
 __device__ inline void atomic_max_double(unsigned long long* p, double v) {
@@ -450,11 +528,15 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     const uint64_t n = uint64_t(cam->x_resolution) * cam->y_resolution * spp;
     const uint64_t n_slots = fb->judge_stride ? (n + fb->judge_stride - 1) / fb->judge_stride : 0;
     HIP_TRY(s->fr_rays.reserve(n * 10 * sizeof(double)));
-    HIP_TRY(s->fr_x.reserve(n * kMlpInPad * sizeof(float)));
+    HIP_TRY(s->fr_x.reserve(n * kFrameXStride * sizeof(float)));
     HIP_TRY(s->fr_y.reserve(n * kMlpOut * sizeof(float)));
     if (!d_hits) HIP_TRY(s->fr_hits.reserve(n * sizeof(mgk_hit_t)));
     mgk_hit_t* hits = d_hits ? d_hits : s->fr_hits.as<mgk_hit_t>();
     HIP_TRY(s->fr_queue.reserve(n * sizeof(uint32_t)));
+    if (fb->analytic && !fb->trace_disk) {
+        HIP_TRY(s->fr_rqueue.reserve(n * sizeof(uint32_t)));
+        HIP_TRY(s->fr_reason.reserve(n));
+    }
     HIP_TRY(s->fr_pred.reserve(std::max<uint64_t>(1, n_slots) * sizeof(mgk_hit_t)));
     HIP_TRY(s->fr_counters.reserve(C_N * sizeof(unsigned long long)));
     if (n > 0xFFFFFFFFull) return MGK_ERROR_CAPACITY_EXCEEDED;
@@ -483,7 +565,7 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     const unsigned blocks = unsigned((n + 255) / 256);
     hipLaunchKernelGGL(k_sur_features, dim3(blocks), dim3(256), 0, st, F);
     HIP_TRY(hipGetLastError());
-    if ((r = mgk_surrogate_forward(s, s->fr_x.as<float>(), n, s->fr_y.as<float>(), st)) != MAGIK_SUCCESS) return r;
+    if ((r = surrogate_forward(s, s->fr_x.as<float>(), kFrameXStride, n, s->fr_y.as<float>(), st)) != MAGIK_SUCCESS) return r;
 
     DecodeParams D;
     std::memset(&D, 0, sizeof(D));
@@ -494,6 +576,8 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.rays = s->fr_rays.as<const double>();
     D.hits = hits;
     D.queue = s->fr_queue.as<uint32_t>();
+    D.rqueue = s->fr_rqueue.as<uint32_t>();
+    D.reason = s->fr_reason.as<uint8_t>();
     D.pred = s->fr_pred.as<mgk_hit_t>();
     D.cnt = cnt;
     D.crit = s->crit.as<const float>();
@@ -518,6 +602,12 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.analytic = fb->analytic;
     hipLaunchKernelGGL(k_sur_decode, dim3(unsigned((n + kDecodeChunk - 1) / kDecodeChunk)), dim3(kDecodeBlock), 0, st, D);
     HIP_TRY(hipGetLastError());
+    if (D.analytic && !D.trace_disk) {
+        int n_cu = 0;
+        HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
+        hipLaunchKernelGGL(k_sur_resolve, dim3(unsigned(8 * n_cu)), dim3(kDecodeBlock), 0, st, D);
+        HIP_TRY(hipGetLastError());
+    }
     HIP_TRY(hipEventRecord(s->ev[2], st));
 
     unsigned long long nq = 0;

@@ -62,12 +62,15 @@ def test_fused_inference_matches_oracle(sur, n):
     sur.set_weights(W, b)
     x = features(np.random.default_rng(n), n)
     y = sur.forward(torch.from_numpy(x).cuda()).cpu().numpy()
-    ref = MO.forward(W, b, x)[:, :8]
+    ref = MO.forward_infer(W, b, x)[:, :8]
     # FP32 accumulation order differs from the oracle's float64: an activation
     # can round to the neighbouring bf16 value, so compare at bf16 resolution
     err = np.abs(y - ref)
     assert err.max() <= 2e-2 * (1 + np.abs(ref).max()), err.max()
     assert np.median(err) <= 1e-3 * (1 + np.abs(ref).max())
+    # and the same network as training's forward pass, to the bf16 error of
+    # the two rounding orders
+    assert np.abs(y - MO.forward(W, b, x)[:, :8]).max() <= 5e-2 * (1 + np.abs(ref).max())
 
 
 def test_gradients_match_oracle(sur):

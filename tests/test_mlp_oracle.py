@@ -114,3 +114,41 @@ def test_loss_cross_entropy_closed_form():
     sums, d = O.loss(y, t, 1.0, 1.0)
     assert sums[0] == pytest.approx(3 * np.log(3.0))
     assert sums[1] == 0 and sums[2] == 0
+
+
+def test_log2_domain_inference_is_the_same_network():
+    # unquantised, the log2-domain restatement of inference equals the plain
+    # float64 network: the factors -log2(e) and -ln 2 cancel exactly
+    rng = np.random.default_rng(11)
+    W, b = small_net(rng)
+    x = rng.uniform(-1, 1, size=(9, O.IN_PAD))
+    np.testing.assert_allclose(O.forward_infer(W, b, x, quant=False), O.exact_forward(W, b, x), rtol=1e-12, atol=1e-12)
+
+
+def test_log2_domain_inference_quantised_close_to_exact():
+    # with its bf16 rounding points it stays within the bf16 error of the
+    # exact network, like forward(); and a dropped factor (the output layer's
+    # -ln 2 left out) is far outside that
+    rng = np.random.default_rng(12)
+    W, b = small_net(rng)
+    x = rng.uniform(-1, 1, size=(256, O.IN_PAD))
+    x[:, 5:] = 0
+    y0 = O.exact_forward(W, b, x)
+    yi = O.forward_infer(W, b, x)
+    yf = O.forward(W, b, x)
+    tol = 5e-3 * (1 + np.abs(y0).max())
+    assert np.abs(yi - y0).max() <= tol
+    assert np.abs(yi - yf).max() <= tol
+    W_bad = [w.copy() for w in W]
+    W_bad[-1] = W_bad[-1] / np.float32(O.NEG_LN2)
+    assert np.abs(O.forward_infer(W_bad, b, x) - y0).max() > 10 * tol
+
+
+def test_log2_silu_closed_form():
+    # u / (1 + 2^u) at u = -log2(e) z is -log2(e) silu(z): z = 0 -> 0,
+    # z = 1 -> -log2(e) / (1 + e^-1), large |z| -> -log2(e) max(z, 0)
+    z = np.array([0.0, 1.0, -40.0, 40.0])
+    u = -z / np.log(2.0)
+    h = u / (1.0 + np.exp2(u))
+    want = -np.array([0.0, 1.0 / (1.0 + np.exp(-1.0)), 0.0, 40.0]) / np.log(2.0)
+    np.testing.assert_allclose(h, want, rtol=1e-12, atol=1e-15)
