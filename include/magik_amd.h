@@ -335,6 +335,15 @@ typedef struct mgk_scene_t {
     mgk_box_t box[MGK_MAX_BOX];
 } mgk_scene_t;
 
+/* Wavelength sampling of a path's hero wavelengths. */
+typedef enum mgk_lambda_sampling_t {
+    MGK_LAMBDA_UNIFORM = 0,      /* stratified over [hero_min, hero_max], weight span / n_hero */
+    MGK_LAMBDA_CMF = 1           /* stratified primary samples through the 256-bin inverse CDF of x+y+z with a
+                                    5% floor over [hero_min, hero_max] (host_color_utilities.cuh:188-233,
+                                    device_sensor_utilities.cuh:25-33), weight 1 / (n_hero pdf)
+                                    (kernel_generate_thin_lens_ray.cu:73-74) */
+} mgk_lambda_sampling_t;
+
 /* Spectral accumulation (color_settings.h:11-14, optical_system_settings.h:22-27). */
 typedef struct mgk_spectral_t {
     uint32_t n_hero;             /* hero wavelengths per path (1..4), stratified */
@@ -344,7 +353,7 @@ typedef struct mgk_spectral_t {
     float sigma_nm;              /* Gaussian redistribution width */
     float iso, fps, shutter_angle;
     float white_point_K;         /* scene white balance (color_settings.h:27) */
-    uint32_t reserved;
+    uint32_t lambda_sampling;    /* mgk_lambda_sampling_t */
 } mgk_spectral_t;
 
 /* One terminated path (16 bytes).  DISK: a0 = r_hit, a1 = azimuth;
@@ -510,6 +519,9 @@ MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle ctx, const mgk_
  * :29-68; color_pipeline.cuh:21-48): bin CMF weights (n_bin x 3, XYZ) and the
  * white-balanced XYZ -> linear Rec.709 matrix (row-major). */
 MGK_API e_magik_result_types mgk_color_weights(const mgk_spectral_t* spec, float* weights_xyz, float* xyz_to_scene);
+/* The 256-entry inverse CDF (nm) that MGK_LAMBDA_CMF samples over [hero_min, hero_max]
+ * (host_color_utilities.cuh:188-233 with the 5% floor of color_pipeline.cuh:36); host only. */
+MGK_API e_magik_result_types mgk_cmf_inverse_cdf(const mgk_spectral_t* spec, float* inverse_cdf_nm);
 
 /* Display settings: the colour part of the reference's stage
  * (color_settings.h:9-30, with its defaults in the comments). */

@@ -13,7 +13,7 @@ import math
 
 from . import abi
 from . import mesh
-from .abi import (CHORD_AFFINE, CHORD_HERMITE, HIT_BOX_DIFFUSE, HIT_BOX_EMITTER, HIT_DISK, HIT_ESCAPE, HIT_HORIZON,
+from .abi import (LAMBDA_CMF, LAMBDA_UNIFORM, CHORD_AFFINE, CHORD_HERMITE, HIT_BOX_DIFFUSE, HIT_BOX_EMITTER, HIT_DISK, HIT_ESCAPE, HIT_HORIZON,
                   HIT_MESH_DIFFUSE, HIT_MESH_EMITTER, HIT_NONE, HIT_SPHERE, HIT_STEP_LIMIT, KERR, MATTER_DIFFUSE,
                   MATTER_ILLUMINANT, MINKOWSKI, PREC_FP32_REFERENCE, PREC_FP64, PREC_MIXED, SCHWARZSCHILD,
                   SCENE_BACKGROUND, SCENE_BOXES, SCENE_DISK, SCENE_DISK_ON_GEODESIC, SCENE_MESHES, SCENE_SPHERE,
@@ -46,11 +46,20 @@ def integrator(metric=KERR, precision=PREC_MIXED, spin=0.9, mass_kg=DEFAULT_MASS
 
 
 def spectral(n_hero=4, hero_min=380.0, hero_max=780.0, n_bin=24, bin_min=360.0, bin_max=830.0, sigma=10.0,
-             iso=500.0, fps=24.0, shutter=180.0, white_K=5600.0) -> abi.Spectral:
-    """Defaults of color_settings.h:11-14 and optical_system_settings.h:22-27."""
+             iso=500.0, fps=24.0, shutter=180.0, white_K=5600.0, lambda_sampling=abi.LAMBDA_UNIFORM) -> abi.Spectral:
+    """Defaults of color_settings.h:11-14 and optical_system_settings.h:22-27; lambda_sampling
+    LAMBDA_CMF draws the hero wavelengths through the reference's CMF inverse CDF."""
     return abi.Spectral(n_hero=n_hero, hero_min_nm=hero_min, hero_max_nm=hero_max, n_bin=n_bin, bin_min_nm=bin_min,
                         bin_max_nm=bin_max, sigma_nm=sigma, iso=iso, fps=fps, shutter_angle=shutter,
-                        white_point_K=white_K)
+                        white_point_K=white_K, lambda_sampling=lambda_sampling)
+
+
+def cmf_inverse_cdf(spec):
+    """The 256 wavelengths (nm) of the inverse CDF that LAMBDA_CMF samples over the hero range."""
+    import numpy as np
+    out = np.zeros(256, dtype=np.float32)
+    abi.check(abi.load().mgk_cmf_inverse_cdf(C.byref(spec), out.ctypes.data_as(C.POINTER(C.c_float))), "mgk_cmf_inverse_cdf")
+    return out
 
 
 def camera_bl(r_obs: float, theta_deg: float, spin: float, fov_deg: float, W: int, H: int,

@@ -113,11 +113,15 @@ class MeshHierarchyInfo(C.Structure):  # mgk_mesh_hierarchy_info_t
                 ("world_min", C.c_float * 3), ("world_max", C.c_float * 3)]
 
 
+LAMBDA_UNIFORM = 0   # mgk_lambda_sampling_t
+LAMBDA_CMF = 1
+
+
 class Spectral(C.Structure):  # mgk_spectral_t
     _fields_ = [("n_hero", C.c_uint32), ("hero_min_nm", C.c_float), ("hero_max_nm", C.c_float),
                 ("n_bin", C.c_uint32), ("bin_min_nm", C.c_float), ("bin_max_nm", C.c_float), ("sigma_nm", C.c_float),
                 ("iso", C.c_float), ("fps", C.c_float), ("shutter_angle", C.c_float), ("white_point_K", C.c_float),
-                ("reserved", C.c_uint32)]
+                ("lambda_sampling", C.c_uint32)]
 
 
 class Display(C.Structure):  # mgk_display_t
@@ -240,6 +244,7 @@ SIGNATURES = {
     "mgk_render_frame": (_R, [_P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(Spectral), C.POINTER(Camera),
                               C.c_uint64, C.c_uint32, _P, _P]),
     "mgk_color_weights": (_R, [C.POINTER(Spectral), C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "mgk_cmf_inverse_cdf": (_R, [C.POINTER(Spectral), C.POINTER(C.c_float)]),
     "mgk_display_transform": (_R, [_P, C.POINTER(Display), _P, C.c_uint64, _P, _P]),
     "mgk_camera_boyer_lindquist": (_R, [C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, C.c_uint32,
                                         C.c_uint32, C.POINTER(Camera)]),

@@ -548,6 +548,35 @@ void bin_weights(const mgk_spectral_t* sp, float* W) {
     for (uint32_t i = 0; i < 3 * sp->n_bin; ++i) W[i] = W[i] * (k * inv_bw);
 }
 
+// The inverse CDF of x+y+z plus a floor of 5% of its mean over [lo, hi], as
+// kCmfCdfN wavelengths at u = k / (kCmfCdfN - 1): the CMF resampled at 471
+// points across [lo, hi], a trapezoid CDF accumulated in float, normalised,
+// and inverted by one linear walk (host_color_utilities.cuh:188-233; the
+// floor of color_pipeline.cuh:36).
+void cmf_inverse_cdf(float lo, float hi, float* inv) {
+    const uint32_t n_e = 471, n = n_e - 1;
+    std::vector<float> s(n_e), cdf(n + 1, 0.0f);
+    const float step = (hi - lo) / float(n);
+    for (uint32_t i = 0; i < n_e; ++i) {
+        const f3 c = cmf_at(lo + float(i) * step);
+        s[i] = c.x + c.y + c.z;
+    }
+    float mean = 0.0f;
+    for (uint32_t i = 0; i < n_e; ++i) mean += s[i];
+    mean /= float(n_e);
+    const float floor_ = 0.05f * mean;
+    for (uint32_t i = 0; i < n; ++i) cdf[i + 1] = cdf[i] + (0.5f * (s[i] + s[i + 1]) + floor_) * step;
+    const float total = cdf[n];
+    for (uint32_t j = 0; j <= n; ++j) cdf[j] /= total;
+    uint32_t b = 0;
+    for (int k = 0; k < kCmfCdfN; ++k) {
+        const float u = float(k) / float(kCmfCdfN - 1);
+        while (b < n - 1 && cdf[b + 1] < u) ++b;
+        const float t = (u - cdf[b]) / std::fmax(cdf[b + 1] - cdf[b], 1e-12f);
+        inv[k] = lo + (float(b) + t) * step;
+    }
+}
+
 f3 xy_to_XYZ(float x, float y) { return {x / y, 1.0f, (1.0f - x - y) / y}; }
 
 // Blackbody white point shifted by tint along the locus normal (host_color_utilities.cuh:29-57).
@@ -772,7 +801,8 @@ e_magik_result_types ensure_star_band(mgk_context* c, const mgk_scene_t* sc, flo
 e_magik_result_types fill_shade_params(mgk_context* c, const mgk_scene_t* sc, const mgk_spectral_t* sp, const mgk_camera_t* cam,
                                        uint64_t seed, uint32_t sample_begin, uint32_t spp, const mgk_hit_t* hits, float* xyz,
                                        ShadeParams& P) {
-    if (sp->n_bin == 0 || sp->n_bin > uint32_t(kMaxBins) || sp->n_hero == 0 || sp->n_hero > 4 || !(sp->sigma_nm > 0.0f))
+    if (sp->n_bin == 0 || sp->n_bin > uint32_t(kMaxBins) || sp->n_hero == 0 || sp->n_hero > 4 || !(sp->sigma_nm > 0.0f) ||
+        sp->lambda_sampling > MGK_LAMBDA_CMF || (sp->lambda_sampling == MGK_LAMBDA_CMF && !(sp->hero_max_nm > sp->hero_min_nm)))
         return MGK_ERROR_INVALID_ARGUMENT;
     e_magik_result_types r = ensure_stars(c, sc);
     if (r != MAGIK_SUCCESS) return r;
@@ -820,6 +850,9 @@ e_magik_result_types fill_shade_params(mgk_context* c, const mgk_scene_t* sc, co
     float npx = float(cam->x_resolution * cam->y_resolution);
     float thr = P.hero_span / float(sp->n_hero);
     P.irr_scale = thr * (area / npx) * sp->iso * 2.0f * kPi / ((360.0f * sp->fps) / sp->shutter_angle);
+    P.irr_base = (area / npx) * sp->iso * 2.0f * kPi / ((360.0f * sp->fps) / sp->shutter_angle);
+    P.lambda_cmf = sp->lambda_sampling == MGK_LAMBDA_CMF ? 1u : 0u;
+    if (P.lambda_cmf) cmf_inverse_cdf(sp->hero_min_nm, sp->hero_max_nm, P.inv_cdf);
     P.led_scale = sc->led_scale;
     bin_weights(sp, P.weights);
     HIP_TRY(c->shade_resp.reserve(size_t(kRespN) * sizeof(float4)));
@@ -1244,6 +1277,13 @@ MGK_API e_magik_result_types mgk_color_weights(const mgk_spectral_t* sp, float* 
         m3 m = xyz_to_scene(sp->white_point_K > 0.0f ? sp->white_point_K : 5600.0f);
         std::memcpy(xyz_to_scene_out, m.m, sizeof(m.m));
     }
+    return MAGIK_SUCCESS;
+}
+
+MGK_API e_magik_result_types mgk_cmf_inverse_cdf(const mgk_spectral_t* sp, float* inverse_cdf_nm) {
+    if (!sp || !inverse_cdf_nm) return MAGIK_ERROR_INVALID_POINTER;
+    if (!(sp->hero_max_nm > sp->hero_min_nm)) return MGK_ERROR_INVALID_ARGUMENT;
+    cmf_inverse_cdf(sp->hero_min_nm, sp->hero_max_nm, inverse_cdf_nm);
     return MAGIK_SUCCESS;
 }
 

@@ -141,6 +141,8 @@ struct ZoneParams {
     uint32_t* steps_out;
 };
 
+constexpr int kCmfCdfN = 256;   // entries of the CMF inverse CDF (host_color_utilities.cuh:10)
+
 struct ShadeParams {
     uint32_t W, H, spp, sample_begin, camera_id;
     uint64_t seed;
@@ -160,8 +162,11 @@ struct ShadeParams {
     float hero_min, hero_span;
     uint32_t n_bin;
     float bin_min, bin_width, sigma;
-    float irr_scale;             // radiance_to_irradiance factor per unit L * throughput
+    float irr_scale;             // radiance_to_irradiance factor per unit L, uniform sampling's throughput included
+    float irr_base;              // the same factor per unit L * throughput (inverse-CDF sampling)
     float led_scale;             // illuminant boxes: led_scale x white LED spectrum
+    uint32_t lambda_cmf;         // hero wavelengths through the CMF inverse CDF (else uniform)
+    float inv_cdf[kCmfCdfN];     // its 256 wavelengths at u = k / 255
     float weights[3 * kMaxBins];
     const float4* resp;          // spectral response R(lambda), kRespN samples over [hero_min, hero_min + hero_span]
 };

@@ -1387,7 +1387,19 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(ShadeParams P) {
         for (uint32_t k = 0; k < P.n_hero; ++k) {
             float f = u[2] + float(k) / float(P.n_hero);
             f -= floorf(f);
-            const float lam = P.hero_min + P.hero_span * f;
+            float lam, thr = 0.0f;
+            if (P.lambda_cmf) {
+                // the primary sample through the inverse CDF; throughput
+                // 1 / (n_hero pdf), pdf = 1 / (255 (c[k + 1] - c[k]))
+                // (device_sensor_utilities.cuh:25-33)
+                const float x = f * float(kCmfCdfN - 1);
+                const uint32_t kk = uint32_t(fminf(x, float(kCmfCdfN - 2)));
+                const float fr = x - float(kk), c0 = P.inv_cdf[kk], c1 = P.inv_cdf[kk + 1];
+                lam = c0 * (1.0f - fr) + c1 * fr;
+                thr = float(kCmfCdfN - 1) * (c1 - c0) / float(P.n_hero);
+            } else {
+                lam = P.hero_min + P.hero_span * f;
+            }
             const float le = lam * g;
             float L;
             if (h.cls == MGK_HIT_ESCAPE) {
@@ -1402,7 +1414,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(ShadeParams P) {
             L *= g5;
             if (isnan(L) || isinf(L) || L < 0.0f) L = 0.0f;
             if (!(L > 0.0f)) continue;
-            const float E = L * P.irr_scale;
+            const float E = P.lambda_cmf ? L * (P.irr_base * thr) : L * P.irr_scale;
             const float t = fmaxf(0.0f, (lam - P.hero_min) * resp_scale);
             const int i = min(int(t), kRespN - 2);
             const float w = t - float(i);

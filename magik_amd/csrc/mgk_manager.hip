@@ -149,6 +149,7 @@ struct Stage {
         sp.fps = fps;
         sp.shutter_angle = shutter;
         sp.white_point_K = display.white_point_K;
+        sp.lambda_sampling = MGK_LAMBDA_CMF;   // the stage's sampler (kernel_generate_thin_lens_ray.cu:73-74)
         return sp;
     }
     mgk_camera_t camera() const {

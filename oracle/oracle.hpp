@@ -1162,6 +1162,36 @@ inline void bin_weights(float* W /*n_bin x 3*/, uint32_t n_bin, float bmin, floa
     for (uint32_t i = 0; i < n_bin * 3; i++) W[i] = W[i] * (kk * inv_bw);
 }
 
+// The wavelength sampler's table (host_color_utilities.cuh:188-233, with the
+// 5% floor of color_pipeline.cuh:36): s = x + y + z resampled at n_tab points
+// across [lo, hi]; cdf[i + 1] = cdf[i] + ((s_i + s_{i+1}) / 2 + 0.05 mean(s)) h
+// in float, normalised; inverted at u = k / 255 by one walk up the CDF with
+// linear interpolation inside the bin.
+inline void cmf_inverse_cdf(const float* tab, uint32_t n_tab, float tmin, float tmax, float lo, float hi, float* inv /*256*/) {
+    const uint32_t n = n_tab - 1;
+    std::vector<float> s(n_tab), cdf(n + 1, 0.0f);
+    const float h = (hi - lo) / float(n);
+    for (uint32_t i = 0; i < n_tab; ++i) {
+        float c[3];
+        sample_cmf(tab, n_tab, tmin, tmax, lo + float(i) * h, c);
+        s[i] = c[0] + c[1] + c[2];
+    }
+    float mean = 0.0f;
+    for (uint32_t i = 0; i < n_tab; ++i) mean += s[i];
+    mean /= float(n_tab);
+    const float fl = 0.05f * mean;
+    for (uint32_t i = 0; i < n; ++i) cdf[i + 1] = cdf[i] + (0.5f * (s[i] + s[i + 1]) + fl) * h;
+    const float total = cdf[n];
+    for (uint32_t j = 0; j <= n; ++j) cdf[j] /= total;
+    uint32_t b = 0;
+    for (uint32_t k = 0; k < 256; ++k) {
+        const float u = float(k) / 255.0f;
+        while (b < n - 1 && cdf[b + 1] < u) ++b;
+        const float t = (u - cdf[b]) / std::fmax(cdf[b + 1] - cdf[b], 1e-12f);
+        inv[k] = lo + (float(b) + t) * h;
+    }
+}
+
 }  // namespace oracle
 
 namespace oracle {

@@ -37,10 +37,11 @@ _SIG = {
     "orc_trace_rays": (C.c_int, [C.c_int, C.POINTER(abi.Integrator), C.POINTER(abi.Scene), _F, _D, C.c_uint64,
                                  C.c_void_p, _U, _U, C.c_int]),
     "orc_bin_weights": (None, [C.c_uint32, C.c_float, C.c_float, _F, C.c_uint32, C.c_float, C.c_float, _F]),
+    "orc_cmf_inverse_cdf": (None, [C.c_float, C.c_float, _F, C.c_uint32, C.c_float, C.c_float, _F]),
     "orc_planck": (C.c_double, [C.c_int, C.c_double, C.c_double]),
     "orc_gaussian_cdf": (C.c_double, [C.c_int, C.c_double, C.c_double, C.c_double]),
     "orc_shade": (C.c_int, [C.POINTER(abi.Scene), C.POINTER(abi.Spectral), C.POINTER(abi.Camera), C.c_uint64,
-                            C.c_uint32, C.c_uint32, C.c_uint32, _U, C.c_void_p, _F, _D]),
+                            C.c_uint32, C.c_uint32, C.c_uint32, _U, C.c_void_p, _F, _F, _D]),
     "orc_host_trace_geodesic": (C.c_int, [C.POINTER(abi.Container), abi.InitialConditions, abi.MetricParameters,
                                           abi.IntegratorParameters]),
     "orc_star_catalogue": (None, [C.c_uint64, C.c_uint32, C.c_float, _F]),
@@ -286,8 +287,20 @@ def shade(scene, spec, cam, seed: int, sample_begin: int, spp: int, pixel_ids, h
     assert hits.dtype == HIT_DTYPE and hits.size == spp * len(pixel_ids)
     w = np.ascontiguousarray(weights, dtype=np.float32)
     out = np.zeros((len(pixel_ids), 3))
+    inv = None
+    if spec.lambda_sampling == abi.LAMBDA_CMF:
+        inv = cmf_inverse_cdf(spec.hero_min_nm, spec.hero_max_nm, cie_table())
     abi.check(lib().orc_shade(C.byref(scene), C.byref(spec), C.byref(cam), seed, sample_begin, spp, len(pixel_ids),
-                              uptr(pixel_ids), hits.ctypes.data_as(C.c_void_p), fptr(w), dptr(out)), "orc_shade")
+                              uptr(pixel_ids), hits.ctypes.data_as(C.c_void_p), fptr(w),
+                              fptr(inv) if inv is not None else None, dptr(out)), "orc_shade")
+    return out
+
+
+def cmf_inverse_cdf(lo: float, hi: float, table: np.ndarray) -> np.ndarray:
+    """The 256-entry inverse CDF of x+y+z (5% floor) over [lo, hi] nm, from a CIE table (n x 3, 360..830 nm)."""
+    tab = np.ascontiguousarray(table, dtype=np.float32)
+    out = np.zeros(256, dtype=np.float32)
+    lib().orc_cmf_inverse_cdf(lo, hi, fptr(tab), tab.shape[0], 360.0, 830.0, fptr(out))
     return out
 
 

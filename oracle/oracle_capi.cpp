@@ -449,6 +449,11 @@ double orc_white_led(int prec, double lam_nm) {
     return white_led<double>(lam_nm);
 }
 
+// The wavelength sampler's inverse CDF over [lo, hi] (256 entries).
+void orc_cmf_inverse_cdf(float lo, float hi, const float* tab, uint32_t n_tab, float tmin, float tmax, float* out) {
+    cmf_inverse_cdf(tab, n_tab, tmin, tmax, lo, hi, out);
+}
+
 // Bin CMF weights (host_color_utilities.cuh:145-178) from a CIE table (n x 3).
 void orc_bin_weights(uint32_t n_bin, float bmin, float bmax, const float* tab, uint32_t n_tab, float tmin, float tmax, float* W) {
     bin_weights(W, n_bin, bmin, bmax, tab, n_tab, tmin, tmax);
@@ -472,9 +477,16 @@ double orc_gaussian_cdf(int prec, double mean, double x, double sigma) {
 // Gaussian-CDF differences (kernel_redistribute_radiance.cu:64-88); the bins
 // are accumulated over samples and folded with the weights / spp
 // (device_color_utilities.cuh:40-54).  Computed in double.
+// With spec->lambda_sampling = MGK_LAMBDA_CMF the hero wavelengths are the
+// stratified primary samples mapped through inv_cdf (256 entries; the draw of
+// device_sensor_utilities.cuh:25-33) at throughput 1 / (n_hero pdf)
+// (kernel_generate_thin_lens_ray.cu:73-74).
 int orc_shade(const mgk_scene_t* scene, const mgk_spectral_t* spec, const mgk_camera_t* cam, uint64_t seed, uint32_t sample_begin,
-              uint32_t spp, uint32_t n_pix, const uint32_t* pixel_ids, const mgk_hit_t* hits, const float* weights, double* xyz) {
+              uint32_t spp, uint32_t n_pix, const uint32_t* pixel_ids, const mgk_hit_t* hits, const float* weights, const float* inv_cdf,
+              double* xyz) {
     if (!scene || !spec || !cam || !pixel_ids || !hits || !weights || !xyz) return MAGIK_ERROR_INVALID_POINTER;
+    const bool cmf = spec->lambda_sampling == MGK_LAMBDA_CMF;
+    if (cmf && !inv_cdf) return MAGIK_ERROR_INVALID_POINTER;
     StarIndex idx;
     bool have_stars = (scene->flags & MGK_SCENE_STARS) && scene->n_star > 0;
     if (have_stars) idx.build(star_catalogue(scene->star_seed, scene->n_star, scene->star_scale), scene->star_radius_rad);
@@ -494,9 +506,16 @@ int orc_shade(const mgk_scene_t* scene, const mgk_spectral_t* spec, const mgk_ca
                 double f = double(u[2]) + double(k) / double(spec->n_hero);
                 f -= std::floor(f);
                 double lam = double(spec->hero_min_nm) + span * f;
+                double thr = span / double(spec->n_hero);
+                if (cmf) {
+                    const double x = f * 255.0;
+                    const uint32_t kk = uint32_t(std::fmin(x, 254.0));
+                    const double fr = x - double(kk), c0 = double(inv_cdf[kk]), c1 = double(inv_cdf[kk + 1]);
+                    lam = c0 * (1.0 - fr) + c1 * fr;
+                    thr = 255.0 * (c1 - c0) / double(spec->n_hero);
+                }
                 double L = emission(scene, have_stars ? &idx : nullptr, h, lam * g) * (g * g * g * g * g);
                 if (std::isnan(L) || std::isinf(L) || L < 0) L = 0;
-                double thr = span / double(spec->n_hero);
                 if (!(L > 0)) continue;
                 double E = radiance_to_irradiance<double>(L, thr, sw, sh, spec->iso, spec->fps, spec->shutter_angle,
                                                           cam->x_resolution, cam->y_resolution);

@@ -17,12 +17,13 @@ pytestmark = pytest.mark.gpu
 
 
 def stage_structs(W, H, metric=M.KERR, reflections=10, white_K=5600.0):
-    """The manager's stage defaults as the library's structs."""
+    """The manager's stage defaults as the library's structs (its wavelengths
+    drawn through the CMF inverse CDF, as the reference's stage draws them)."""
     integ, sc, _, cam, _ = M.workload("box_led", W=W, H=H, spp=1, precision=M.PREC_MIXED)
     integ.metric_type = metric
     sc.max_reflections = reflections
     spec = M.spectral(n_hero=4, hero_min=360.0, hero_max=830.0, n_bin=24, bin_min=360.0, bin_max=830.0, sigma=10.0,
-                      white_K=white_K)
+                      white_K=white_K, lambda_sampling=M.LAMBDA_CMF)
     return integ, sc, spec, cam
 
 

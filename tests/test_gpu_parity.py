@@ -185,6 +185,30 @@ def test_shade_matches_oracle(gpu_ctx, name):
     assert np.quantile(rel, 0.999) <= 1e-4, np.quantile(rel, 0.999)
 
 
+@pytest.mark.parametrize("n_hero", [1, 4])
+def test_shade_cmf_sampler_matches_oracle(gpu_ctx, n_hero):
+    # the reference's wavelength sampler: hero wavelengths through the CMF
+    # inverse CDF at throughput 1 / (n_hero pdf), against the oracle's shading
+    integ, sc, spec, cam, spp = setup("kerr_disk_1080p", M.PREC_FP64)
+    spec.n_hero = n_hero
+    spec.lambda_sampling = M.LAMBDA_CMF
+    gpu_ctx.set_scene(sc)
+    rays = gpu_ctx.generate_rays(integ, cam, SEED, 0, spp)
+    hits, _ = gpu_ctx.trace(integ, sc, list(cam.position), rays)
+    xyz = gpu_ctx.shade(sc, spec, cam, SEED, 0, spp, hits).cpu().numpy().reshape(-1, 3).astype(np.float64)
+    W, _ = M.color_weights(spec)
+    npx = cam.x_resolution * cam.y_resolution
+    ref = O.shade(sc, spec, cam, SEED, 0, spp, np.arange(npx), M.hits_to_numpy(hits), W)
+    scale = np.maximum(np.abs(ref), 1e-6 * np.abs(ref).max())
+    rel = np.abs(xyz - ref) / scale
+    assert np.quantile(rel, 0.999) <= 1e-4, np.quantile(rel, 0.999)
+    # an unbiased estimate of the same image: the mean over the frame agrees
+    # with the uniform sampler's to the Monte Carlo noise of the two
+    spec.lambda_sampling = M.LAMBDA_UNIFORM
+    uni = gpu_ctx.shade(sc, spec, cam, SEED, 0, spp, hits).cpu().numpy().reshape(-1, 3).astype(np.float64)
+    assert abs(xyz[:, 1].mean() / uni[:, 1].mean() - 1.0) <= 2e-2
+
+
 def reproducible(a, b, tol=1e-5, frac=2e-3):
     """Two renders of the mixed path agree to FP32 rounding: the FP64
     renormalisation runs on a wave-uniform schedule, so a ray's renormalisation
