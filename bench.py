@@ -155,8 +155,9 @@ def frame_error(ctx, M, integ, sc, spec, cam, spp, seed, frame):
             "note": "aggregate comparison with the FP64 frame of the same seed (whole paths are chaotic)"}
 
 
-def cpu_baseline(M, integ, sc, cam, spp, seed, budget_s=12.0):
-    """Oracle FP32 restatement of the reference integrator on the host cores."""
+def cpu_baseline(M, integ, sc, cam, spp, seed, budget_s=12.0, single_budget_s=5.0):
+    """Oracle FP32 restatement of the reference integrator on the host cores,
+    and the same on one core (a smaller sample) as `single_core`."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
     it = M.integrator()
@@ -169,7 +170,7 @@ def cpu_baseline(M, integ, sc, cam, spp, seed, budget_s=12.0):
     if sc.flags & M.SCENE_MESHES:
         O.set_meshes(*M.workload_meshes("box_mesh"))
 
-    def run(n):
+    def run(n, threads=threads):
         idx = np.sort(rng.choice(npx * spp, size=n, replace=False))
         rays = O.generate_rays(O.F32, it, cam, seed, (idx % npx).astype(np.uint32), (idx // npx).astype(np.uint32))
         t = time.perf_counter()
@@ -192,6 +193,12 @@ def cpu_baseline(M, integ, sc, cam, spp, seed, budget_s=12.0):
            "steps_per_s": steps / dt}
     if not boxes:
         res["trials_per_s"] = trials / dt
+    dt1, _, _ = run(1 << 10, threads=1)
+    n1 = int(min(1 << 22, max(1 << 11, (1 << 10) * single_budget_s / max(dt1, 1e-3))))
+    dt1, steps1, _ = run(n1, threads=1)
+    res["single_core"] = {"value": n1 / dt1, "unit": "rays/s", "cores": 1,
+                          "sample": f"{n1} seeded {what} of the same frame on one thread, {dt1:.1f} s",
+                          "steps_per_s": steps1 / dt1}
     return res
 
 
@@ -425,7 +432,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(M, integ, sc, cam, spp, args.seed)
-        log(f"cpu baseline {res['cpu_baseline']['value']:.3e} rays/s")
+        log(f"cpu baseline {res['cpu_baseline']['value']:.3e} rays/s on {res['cpu_baseline']['cores']} threads, "
+            f"{res['cpu_baseline']['single_core']['value']:.3e} rays/s on one")
     ctx.close()
     print(json.dumps(res), flush=True)
 

@@ -54,7 +54,14 @@ typedef struct magik_renderer_statistics_t {
     double latest_render_loop_iteration_time_seconds;
     double total_wavefront_sample_time_seconds;
     double latest_wavefront_sample_time_seconds;
-    size_t latest_wavefront_history_length;          /* 0: persistent kernels keep no per-iteration queue history */
+    /* Queue counts per wavefront iteration of the latest sample (iteration k
+     * takes every live path to step k + 1): advance_A = intersect = paths
+     * with more than k steps, advance_B = those going on to iteration k + 1,
+     * bxdf = paths meeting matter at step k + 1 (ending or reflecting),
+     * redistribute = splat = paths ending there.  Paths stopped by the step
+     * limit leave advance_A only.  The six arrays are one host allocation
+     * made by a fresh fetch and freed by magik_destroy_renderer_statistics. */
+    size_t latest_wavefront_history_length;
     uint32_t* h_latest_advance_A_queue_count_history;
     uint32_t* h_latest_advance_B_queue_count_history;
     uint32_t* h_latest_intersect_queue_count_history;

@@ -378,6 +378,7 @@ class Manager:
         abi.check(self.lib.magik_render_manager_create(C.byref(h), d), "magik_render_manager_create")
         self.h = h
         self._last_stats = abi.RendererStatistics()
+        self._last_histories = {q: None for q in abi.QUEUE_HISTORIES}
         fb = C.c_void_p()
         abi.check(self.lib.magik_configure_framebuffer_collection(C.byref(fb), abi.FB_CONFIG_HOST),
                   "magik_configure_framebuffer_collection")
@@ -417,12 +418,30 @@ class Manager:
     def stats(self):
         """(statistics, whether they are new).  The library fills the struct only
         when the worker has published statistics since the last fetch; the last
-        statistics fetched are returned otherwise."""
+        statistics fetched are returned otherwise.  The queue-count histories of
+        a fresh fetch are copied into queue_histories() and their host memory is
+        released at once, so the returned struct's history pointers are NULL."""
+        import numpy as np
         st, new = abi.RendererStatistics(), C.c_bool(False)
         abi.check(self.lib.magik_fetch_renderer_statistics(self.h, C.byref(new), C.byref(st)), "magik_fetch_renderer_statistics")
         if new.value:
-            self._last_stats = st
+            n = st.latest_wavefront_history_length
+            hist = {}
+            for q in abi.QUEUE_HISTORIES:
+                ptr = getattr(st, f"h_latest_{q}_queue_count_history")
+                hist[q] = (np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_uint32)), shape=(n,)).copy() if n and ptr
+                           else np.zeros(0, np.uint32))
+            owned = abi.RendererStatistics.from_buffer_copy(st)
+            abi.check(self.lib.magik_destroy_renderer_statistics(C.byref(owned)), "magik_destroy_renderer_statistics")
+            for q in abi.QUEUE_HISTORIES:
+                setattr(st, f"h_latest_{q}_queue_count_history", None)
+            self._last_stats, self._last_histories = st, hist
         return self._last_stats, new.value
+
+    def queue_histories(self) -> dict:
+        """The queue counts per wavefront iteration of the latest sample, as of
+        the last fresh stats() fetch: {queue name: uint32 array}."""
+        return dict(self._last_histories)
 
     def fetch(self) -> tuple[int, bool]:
         new = C.c_bool(False)

@@ -180,13 +180,15 @@ class ManagerDescriptor(C.Structure):  # magik_manager_descriptor_t (include/mag
                 ("cqs_n_reserved_chunk", C.c_uint32), ("cqs_drop_overflow", C.c_bool)]
 
 
+QUEUE_HISTORIES = ("advance_A", "advance_B", "intersect", "bxdf", "redistribute", "splat")
+
+
 class RendererStatistics(C.Structure):  # magik_renderer_statistics_t
     _fields_ = [("x_sample", C.c_size_t), ("total_render_time_seconds", C.c_double),
                 ("latest_render_loop_iteration_time_seconds", C.c_double),
                 ("total_wavefront_sample_time_seconds", C.c_double), ("latest_wavefront_sample_time_seconds", C.c_double),
                 ("latest_wavefront_history_length", C.c_size_t)] + \
-               [(f"h_latest_{q}_queue_count_history", C.c_void_p)
-                for q in ("advance_A", "advance_B", "intersect", "bxdf", "redistribute", "splat")] + \
+               [(f"h_latest_{q}_queue_count_history", C.c_void_p) for q in QUEUE_HISTORIES] + \
                [("total_framebuffer_swapchain_size", C.c_size_t), ("total_wavefront_buffer_size", C.c_size_t)]
 
 

@@ -169,6 +169,7 @@ struct mgk_context {
     // paths continued after reflections: terminal states, step counts, reflected-ray queues
     DeviceBuffer path_state, path_steps, refl_q[2];
     uint64_t last_reflections = 0;
+    DeviceBuffer step_hist;      // end / limit / reflect step histograms of a render that asks for them
     // display parameters derived from the last display settings (the blackbody
     // white point alone is ~1e3 Planck evaluations): reused while the settings repeat
     mgk_display_t disp_key{};
@@ -398,15 +399,21 @@ struct PathInfo {
     uint32_t camera_id, npix, sample_begin;
 };
 
+// Bins of a render's step histograms: one per step up to the step limit
+// (4 MiB per histogram at most; later steps share the last bin).
+constexpr uint64_t kMaxHistBins = uint64_t(1) << 20;
+
 // Trace, then (with `paths` and a scene with diffuse boxes) reflect the paths
 // that ended on diffuse matter and trace them again, up to max_reflections
 // times (kernel_bxdf.cu:69-95, kernel_enforce_ray_limits.cu:48-50).  Each
 // round reads one count back (an 8-byte copy).  Reflected rays get new initial
 // conditions in place in P.rays.  With a fixed workspace (fixed->zone_cap > 0)
-// nothing is allocated: no reflections, and the context's statistics and
-// timing events are left as they were.
+// nothing is allocated: no reflections, no histograms, and the context's
+// statistics and timing events are left as they were.  With hist, the step
+// histograms of the paths (after the timed region; one synchronous copy).
 e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, const PathInfo* paths = nullptr,
-                              uint32_t max_reflections = 0, const Workspace* fixed = nullptr) {
+                              uint32_t max_reflections = 0, const Workspace* fixed = nullptr,
+                              StepHistogram* hist = nullptr) {
     Workspace ws;
     if (fixed) {
         ws = *fixed;
@@ -427,6 +434,18 @@ e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, con
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, P.stats, 3);
     HIP_TRY(hipGetLastError());
     if (fixed) return trace_once(c, P, s, ws);
+    const bool want_hist = hist && P.n > 0;
+    const uint32_t n_bin = uint32_t(std::min<uint64_t>(uint64_t(P.step_limit) + 2u, kMaxHistBins));
+    uint32_t* d_hist = nullptr;   // end, limit, reflect: n_bin bins each
+    if (want_hist) {
+        if (!P.steps_out) {
+            HIP_TRY(c->path_steps.reserve(P.stride * sizeof(uint32_t)));
+            P.steps_out = reinterpret_cast<uint32_t*>(c->path_steps.ptr);
+        }
+        HIP_TRY(c->step_hist.reserve(size_t(3) * n_bin * sizeof(uint32_t)));
+        d_hist = reinterpret_cast<uint32_t*>(c->step_hist.ptr);
+        HIP_TRY(hipMemsetAsync(d_hist, 0, size_t(3) * n_bin * sizeof(uint32_t), s));
+    }
     HIP_TRY(hipEventRecord(c->ev0, s));
     c->last_reflections = 0;
     const bool reflect = paths && max_reflections > 0 && P.n > 0 && has_diffuse_matter(c, P);
@@ -477,6 +496,9 @@ e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, con
             HIP_TRY(hipStreamSynchronize(s));
             if (cnt == 0) break;
             c->last_reflections += cnt;
+            if (want_hist)   // steps_out still holds each reflected path's step at the reflection
+                HIP_TRY(launch_step_hist(P.steps_out, nullptr, R.queue_out, cnt, n_bin, d_hist + 2 * n_bin, d_hist + 2 * n_bin,
+                                         c->n_cu, s));
             TraceParams Q = P;
             Q.queue = R.queue_out;
             Q.n = cnt;
@@ -489,6 +511,19 @@ e_magik_result_types do_trace(mgk_context* c, TraceParams& P, hipStream_t s, con
     HIP_TRY(hipEventRecord(c->ev1, s));
     c->timed = true;
     c->last_n = P.n;
+    if (want_hist) {
+        HIP_TRY(launch_step_hist(P.steps_out, P.hits, P.queue, P.n, n_bin, d_hist, d_hist + n_bin, c->n_cu, s));
+        std::vector<uint32_t> h(size_t(3) * n_bin);
+        HIP_TRY(hipMemcpyAsync(h.data(), d_hist, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        hist->end.assign(h.begin(), h.begin() + n_bin);
+        hist->limit.assign(h.begin() + n_bin, h.begin() + 2 * size_t(n_bin));
+        hist->reflect.assign(h.begin() + 2 * size_t(n_bin), h.end());
+    } else if (hist) {
+        hist->end.clear();
+        hist->limit.clear();
+        hist->reflect.clear();
+    }
     return MAGIK_SUCCESS;
 }
 
@@ -1219,7 +1254,7 @@ MGK_API e_magik_result_types mgk_render_frame(mgk_context_handle c, const mgk_in
 // (whole paths) + shade, XYZ / spp per pixel.
 e_magik_result_types mgk::render_samples(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc,
                                          const mgk_spectral_t* sp, const mgk_camera_t* cam, uint64_t seed, uint32_t sample_begin,
-                                         uint32_t spp, float* d_xyz, hipStream_t s) {
+                                         uint32_t spp, float* d_xyz, hipStream_t s, StepHistogram* hist) {
     if (!c || !sc || !sp || !cam || !d_xyz) return MAGIK_ERROR_INVALID_POINTER;
     e_magik_result_types r = check_integrator(in);
     if (r != MAGIK_SUCCESS) return r;
@@ -1240,7 +1275,7 @@ e_magik_result_types mgk::render_samples(mgk_context_handle c, const mgk_integra
     P.n = n;
     P.hits = reinterpret_cast<mgk_hit_t*>(c->hits.ptr);
     const PathInfo paths{seed, cam->camera_id, cam->x_resolution * cam->y_resolution, sample_begin};
-    if ((r = do_trace(c, P, s, &paths, sc->max_reflections)) != MAGIK_SUCCESS) return r;
+    if ((r = do_trace(c, P, s, &paths, sc->max_reflections, nullptr, hist)) != MAGIK_SUCCESS) return r;
     ShadeParams S;
     if ((r = fill_shade_params(c, sc, sp, cam, seed, sample_begin, spp, reinterpret_cast<mgk_hit_t*>(c->hits.ptr), d_xyz, S)) !=
         MAGIK_SUCCESS)

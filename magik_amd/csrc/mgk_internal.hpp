@@ -207,12 +207,26 @@ hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s);
 hipError_t launch_trace_zone(const TraceParams& p, const ZoneParams& z, bool near, int grid, hipStream_t s);
 hipError_t launch_shade(const ShadeParams& p, hipStream_t s);
 hipError_t launch_reflect(const ReflectParams& p, int precision, hipStream_t s);
+// Adds each path (all n, or queue[0..n)) to bin clamp(steps, 1, n_bin - 1) of
+// limit_out (hit class MGK_HIT_STEP_LIMIT) or end_out (any other, or hits NULL).
+hipError_t launch_step_hist(const uint32_t* steps, const mgk_hit_t* hits, const uint32_t* queue, uint64_t n, uint32_t n_bin,
+                            uint32_t* end_out, uint32_t* limit_out, int n_cu, hipStream_t s);
+
+// Per-path step counts of one render, binned by the step at which each event
+// happened: end[k] paths whose geodesic ended at step k (a hit, escape or
+// horizon), limit[k] paths stopped there by the step limit, reflect[k]
+// reflections taken there.  Bin 0 stays empty; steps past the last bin count
+// in it.  All three have the same length.
+struct StepHistogram {
+    std::vector<uint32_t> end, limit, reflect;
+};
 
 // Samples [sample_begin, sample_begin + spp) of a frame into d_xyz (XYZ / spp
-// per pixel): the body of mgk_render_frame.
+// per pixel): the body of mgk_render_frame.  With hist, also the render's step
+// histograms (one more kernel and a synchronous copy).
 e_magik_result_types render_samples(mgk_context_handle c, const mgk_integrator_t* in, const mgk_scene_t* sc, const mgk_spectral_t* sp,
                                     const mgk_camera_t* cam, uint64_t seed, uint32_t sample_begin, uint32_t spp, float* d_xyz,
-                                    hipStream_t s);
+                                    hipStream_t s, StepHistogram* hist = nullptr);
 
 // mat3 rotate(a, b, g) of internal_math.h:1784-1810: m_x(g) m_y(b) m_z(a), row-major.
 inline void rotate_matrix(float a, float b, float g, float out[9]) {
@@ -232,6 +246,7 @@ hipError_t launch_geodesic_steps(const GeodesicStepParams& p, hipStream_t s);
 // bottom levels first, then the top level.
 struct MeshBuild {
     std::vector<float4> nodes, tris, normals, inst;
+    std::vector<uint32_t> mesh_roots;   // each mesh's bottom-level root, in mesh order
     uint32_t root = kBvhEnd;
     float lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     bool any_diffuse = false;

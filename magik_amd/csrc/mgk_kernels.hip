@@ -1802,4 +1802,49 @@ hipError_t launch_geodesic_steps(const GeodesicStepParams& p, hipStream_t s) {
     });
 }
 
+// Step histogram: path id (i, or queue[i]) lands in bin clamp(steps[id], 1,
+// n_bin - 1) of limit_out when its hit class is MGK_HIT_STEP_LIMIT, else of
+// end_out (hits NULL: always end_out).  The first kHistLds bins are summed in
+// LDS per block, one global atomic per non-empty bin and block.
+constexpr uint32_t kHistLds = 2048;
+constexpr int kHistBlock = 256;
+
+__global__ __launch_bounds__(kHistBlock) void k_step_hist(const uint32_t* __restrict__ steps, const mgk_hit_t* __restrict__ hits,
+                                                           const uint32_t* __restrict__ queue, uint64_t n, uint32_t n_bin,
+                                                           uint32_t* end_out, uint32_t* limit_out) {
+    __shared__ uint32_t h_end[kHistLds], h_lim[kHistLds];
+    const uint32_t nl = n_bin < kHistLds ? n_bin : kHistLds;
+    for (uint32_t b = threadIdx.x; b < nl; b += kHistBlock) {
+        h_end[b] = 0u;
+        h_lim[b] = 0u;
+    }
+    __syncthreads();
+    const uint64_t stride = uint64_t(gridDim.x) * kHistBlock;
+    for (uint64_t i = uint64_t(blockIdx.x) * kHistBlock + threadIdx.x; i < n; i += stride) {
+        const uint32_t id = queue ? queue[i] : uint32_t(i);
+        const uint32_t s = steps[id];
+        const uint32_t b = s < 1u ? 1u : (s > n_bin - 1u ? n_bin - 1u : s);
+        const bool lim = hits && hits[id].cls == uint32_t(MGK_HIT_STEP_LIMIT);
+        if (b < nl)
+            atomicAdd(lim ? &h_lim[b] : &h_end[b], 1u);
+        else
+            atomicAdd(lim ? &limit_out[b] : &end_out[b], 1u);
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nl; b += kHistBlock) {
+        if (h_end[b]) atomicAdd(&end_out[b], h_end[b]);
+        if (h_lim[b]) atomicAdd(&limit_out[b], h_lim[b]);
+    }
+}
+
+hipError_t launch_step_hist(const uint32_t* steps, const mgk_hit_t* hits, const uint32_t* queue, uint64_t n, uint32_t n_bin,
+                            uint32_t* end_out, uint32_t* limit_out, int n_cu, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (n_bin < 2) return hipErrorInvalidValue;
+    const uint64_t want = (n + kHistBlock - 1) / kHistBlock, cap = uint64_t(n_cu > 0 ? n_cu : 1) * 4u;
+    const unsigned grid = unsigned(want < cap ? want : cap);
+    hipLaunchKernelGGL(k_step_hist, dim3(grid), dim3(kHistBlock), 0, s, steps, hits, queue, n, n_bin, end_out, limit_out);
+    return hipGetLastError();
+}
+
 }  // namespace mgk

@@ -12,6 +12,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -190,6 +191,41 @@ __global__ void __launch_bounds__(256) k_accumulate(const float* __restrict__ x,
 
 double seconds(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
+// The statistics' queue histories, in this order.
+enum QueueHistory { kHistAdvanceA, kHistAdvanceB, kHistIntersect, kHistBxdf, kHistRedistribute, kHistSplat };
+
+// The reference's wavefront loop stores each queue's count once per iteration
+// (ray.h:68-76, kernel_advance_geodesic.cu:97, kernel_intersect.cu:157,
+// kernel_bxdf.cu:132, kernel_splat_to_renderbuffer.cu:76); iteration k takes
+// every live path one step, to step k + 1.  The persistent kernels run each
+// path through all its steps at once, so the same counts follow from the step
+// at which each path ended or reflected:
+//   advance_A[k] = intersect[k] = paths with more than k steps
+//   bxdf[k]      = paths that met matter at step k + 1 (ended on it or reflected)
+//   advance_B[k] = paths that go on to iteration k + 1 (advance_A[k + 1])
+//   redistribute[k] = splat[k] = paths that ended at step k + 1
+// A path stopped by the step limit leaves advance_A with no bxdf, redistribute
+// or splat entry (kernel_enforce_ray_limits.cu:48-50 drops it).  The length
+// is the largest step of any path.
+void queue_histories(const mgk::StepHistogram& h, std::vector<uint32_t> (&q)[6]) {
+    size_t len = 0;
+    uint64_t n_paths = 0;
+    for (size_t k = 0; k < h.end.size(); ++k) {
+        n_paths += uint64_t(h.end[k]) + h.limit[k];
+        if (h.end[k] || h.limit[k] || h.reflect[k]) len = k;
+    }
+    for (auto& v : q) v.assign(len, 0u);
+    uint64_t alive = n_paths;   // paths with more than k steps
+    for (size_t k = 0; k < len; ++k) {
+        alive -= uint64_t(h.end[k]) + h.limit[k];
+        const uint64_t next = alive - h.end[k + 1] - h.limit[k + 1];
+        q[kHistAdvanceA][k] = q[kHistIntersect][k] = uint32_t(alive);
+        q[kHistAdvanceB][k] = uint32_t(next);
+        q[kHistBxdf][k] = h.end[k + 1] + h.reflect[k + 1];
+        q[kHistRedistribute][k] = q[kHistSplat][k] = h.end[k + 1];
+    }
+}
+
 }  // namespace
 
 struct magik_render_manager {
@@ -209,16 +245,19 @@ struct magik_render_manager {
     Stage stage;
     bool clear = true, paused = false;
     std::map<std::string, DeviceAov> render;   // the render framebuffer
-    // swapchain: the worker owns back, the DCC owns front, ready is exchanged
+    // swapchain: the worker owns back, the DCC owns front, and the ready slot
+    // is one word, coll index | kReadyNew, exchanged whole so that the index
+    // and its new bit never part (the reference's ready_slot_t, m_swapchain.h)
+    static constexpr uint32_t kReadyNew = 4u;
     Collection coll[3];
-    Collection* back = &coll[2];
-    std::atomic<Collection*> ready{&coll[1]};
-    Collection* front = &coll[0];
-    std::atomic<bool> ready_updated{false};
-    // statistics
+    uint32_t back = 2, front = 0;
+    std::atomic<uint32_t> ready{1u};
+    // statistics; history[q] holds queue q's count per iteration of the latest
+    // sample (the order of kQueueHistories), copied out by each fresh fetch
     std::mutex stats_mutex;
     magik_renderer_statistics_t stats{};
     bool stats_new = false;
+    std::vector<uint32_t> history[6];
 
     void fail(e_magik_result_types r) {
         int expected = MAGIK_SUCCESS;
@@ -416,6 +455,7 @@ void run(magik_render_manager* m) {
     mgk_context_handle ctx = nullptr;
     float *d_xyz = nullptr, *d_acc = nullptr;
     uint64_t px_cap = 0, n_sample = 0;
+    mgk::StepHistogram step_hist;   // of the latest sample, for the queue histories
     double total_sample_s = 0.0;
     const auto t_start = clk::now();
     if (hipSetDevice(m->device) != hipSuccess) {
@@ -459,7 +499,7 @@ void run(magik_render_manager* m) {
         if (m->display == MAGIK_DISPLAY_SWAPCHAIN) {
             // render -> back (framebuffer.cpp:202-241), then back <-> ready (:243-250)
             if (!dirty) {
-                Collection& b = *m->back;
+                Collection& b = m->coll[m->back];
                 for (auto it = b.aov.begin(); it != b.aov.end();) {
                     if (m->render.find(it->first) == m->render.end()) {
                         free_aov(it->second);
@@ -481,8 +521,7 @@ void run(magik_render_manager* m) {
                 }
                 W_TRY(hipStreamSynchronize(m->stream));
             }
-            m->back = m->ready.exchange(m->back, std::memory_order_acq_rel);
-            m->ready_updated.store(true, std::memory_order_release);
+            m->back = m->ready.exchange(m->back | magik_render_manager::kReadyNew, std::memory_order_acq_rel) & 3u;
         }
         if (m->clear) {
             n_sample = 0;
@@ -506,7 +545,7 @@ void run(magik_render_manager* m) {
             const mgk_scene_t sc = st.scene();
             const mgk_spectral_t sp = st.spectral();
             const mgk_camera_t cam = st.camera();
-            r = mgk::render_samples(ctx, &in, &sc, &sp, &cam, 0, uint32_t(n_sample), 1, d_xyz, m->stream);
+            r = mgk::render_samples(ctx, &in, &sc, &sp, &cam, 0, uint32_t(n_sample), 1, d_xyz, m->stream, &step_hist);
             if (r != MAGIK_SUCCESS) {
                 m->fail(r);
                 break;
@@ -525,8 +564,11 @@ void run(magik_render_manager* m) {
             ++n_sample;
             const double dt = seconds(t_s, clk::now());
             total_sample_s += dt;
+            std::vector<uint32_t> q[6];
+            queue_histories(step_hist, q);
             std::lock_guard<std::mutex> lk(m->stats_mutex);
             m->stats.latest_wavefront_sample_time_seconds = dt;
+            for (int i = 0; i < 6; ++i) m->history[i].swap(q[i]);
         } else {
             std::this_thread::sleep_for(std::chrono::milliseconds(m->paused ? 2 : 10));
         }
@@ -640,7 +682,27 @@ MGK_API e_magik_result_types magik_fetch_renderer_statistics(magik_render_manage
     std::lock_guard<std::mutex> lk(m->stats_mutex);
     const bool fresh = m->stats_new;
     if (fresh) {
+        // the histories go out in one host allocation owned by the caller
+        // until magik_destroy_renderer_statistics
+        const size_t len = m->history[0].size();
+        uint32_t* block = nullptr;
+        if (len > 0) {
+            block = static_cast<uint32_t*>(std::malloc(6 * len * sizeof(uint32_t)));
+            if (!block) {
+                if (is_new_fetch) *is_new_fetch = false;
+                return MAGIK_ERROR_HOST_MEMORY_ALLOCATION_FAILED;
+            }
+            for (int i = 0; i < 6; ++i) std::memcpy(block + i * len, m->history[i].data(), len * sizeof(uint32_t));
+        }
         *out = m->stats;
+        out->latest_wavefront_history_length = len;
+        uint32_t** dst[6] = {&out->h_latest_advance_A_queue_count_history, &out->h_latest_advance_B_queue_count_history,
+                             &out->h_latest_intersect_queue_count_history, &out->h_latest_bxdf_queue_count_history,
+                             &out->h_latest_redistribute_queue_count_history, &out->h_latest_splat_queue_count_history};
+        static_assert(kHistAdvanceA == 0 && kHistAdvanceB == 1 && kHistIntersect == 2 && kHistBxdf == 3 &&
+                          kHistRedistribute == 4 && kHistSplat == 5,
+                      "history order");
+        for (int i = 0; i < 6; ++i) *dst[i] = block ? block + i * len : nullptr;
         m->stats_new = false;
     }
     if (is_new_fetch) *is_new_fetch = fresh;
@@ -649,7 +711,8 @@ MGK_API e_magik_result_types magik_fetch_renderer_statistics(magik_render_manage
 
 MGK_API e_magik_result_types magik_destroy_renderer_statistics(magik_renderer_statistics_t* s) {
     if (!s) return MAGIK_ERROR_INVALID_POINTER;
-    std::memset(s, 0, sizeof(*s));   // no queue histories are handed out
+    std::free(s->h_latest_advance_A_queue_count_history);   // the start of the fetch's one allocation
+    std::memset(s, 0, sizeof(*s));
     return MAGIK_SUCCESS;
 }
 
@@ -673,17 +736,22 @@ MGK_API e_magik_result_types magik_framebuffer_collection_fetch(magik_render_man
     if (!m || !fb) return MAGIK_ERROR_INVALID_POINTER;
     if (!m->running.load(std::memory_order_acquire)) return MAGIK_SUCCESS;
     if (m->display == MAGIK_DISPLAY_HEADLESS) return MAGIK_ERROR_AOV_SWAPCHAIN_NOT_INITALIZED;
-    if (!m->ready_updated.exchange(false, std::memory_order_acquire)) return MAGIK_SUCCESS;
-    m->front = m->ready.exchange(m->front, std::memory_order_acq_rel);
+    // take the ready slot only while it holds a frame not fetched yet
+    uint32_t slot = m->ready.load(std::memory_order_acquire);
+    do {
+        if (!(slot & magik_render_manager::kReadyNew)) return MAGIK_SUCCESS;
+    } while (!m->ready.compare_exchange_weak(slot, m->front, std::memory_order_acq_rel, std::memory_order_acquire));
+    m->front = slot & 3u;
+    const Collection& front = m->coll[m->front];
     int old = 0;
     (void)hipGetDevice(&old);
     if (hipSetDevice(m->device) != hipSuccess) return MAGIK_INVALID_CUDA_DEVICE;
     e_magik_result_types r = MAGIK_SUCCESS;
     for (auto it = fb->aov.begin(); it != fb->aov.end();) {
-        if (m->front->aov.find(it->first) == m->front->aov.end()) it = fb->aov.erase(it);
+        if (front.aov.find(it->first) == front.aov.end()) it = fb->aov.erase(it);
         else ++it;
     }
-    for (auto& [name, d] : m->front->aov) {
+    for (auto& [name, d] : front.aov) {
         auto& h = fb->aov[name];
         const size_t n = size_t(d.W) * d.H * d.channels;
         h.W = d.W;

@@ -223,6 +223,7 @@ e_magik_result_types build_meshes(const mgk_mesh_t* meshes, uint32_t n_mesh, con
         mesh_box[m] = local[0].box;
         root[m] = append(local, out.nodes);
     }
+    out.mesh_roots = root;
     // instances: records and world bounds
     std::vector<Prim> prims(n_inst);
     Box3 world;
@@ -293,10 +294,18 @@ bool walk(const mgk::MeshBuild& b, uint32_t root, std::vector<uint32_t>& seen, b
         const uint32_t skip = u(lo.w), info = u(hi.w);
         if (skip != kBvhEnd && skip <= n) return false;
         if (info == 0) {
-            if (uint64_t(n) + 1 >= n_nodes) return false;
-            const float4 clo = b.nodes[2 * (uint64_t(n) + 1)], chi = b.nodes[2 * (uint64_t(n) + 1) + 1];
-            if (clo.x < lo.x || clo.y < lo.y || clo.z < lo.z || chi.x > hi.x || chi.y > hi.y || chi.z > hi.z) return false;
-            ends.push_back(skip == kBvhEnd ? uint32_t(n_nodes) : skip);
+            // two children inside the parent's box: n + 1, and the node that the
+            // first child's skip link reaches, before the parent's own skip target
+            const uint64_t end = skip == kBvhEnd ? n_nodes : skip;
+            auto inside = [&](uint64_t c) {
+                const float4 clo = b.nodes[2 * c], chi = b.nodes[2 * c + 1];
+                return !(clo.x < lo.x || clo.y < lo.y || clo.z < lo.z || chi.x > hi.x || chi.y > hi.y || chi.z > hi.z);
+            };
+            const uint64_t c1 = uint64_t(n) + 1;
+            if (c1 >= end || !inside(c1)) return false;
+            const uint32_t c2 = u(b.nodes[2 * c1].w);
+            if (c2 == kBvhEnd || c2 <= c1 || c2 >= end || !inside(c2)) return false;
+            ends.push_back(uint32_t(end));
             ++n;
             continue;
         }
@@ -324,17 +333,18 @@ MGK_API e_magik_result_types mgk_mesh_hierarchy_info(const mgk_mesh_t* meshes, u
     if (r != MAGIK_SUCCESS || n_mesh == 0) return r;
     auto u = [](float f) { uint32_t x; std::memcpy(&x, &f, 4); return x; };
     const uint64_t n_slots = b.tris.size() / 3;
-    // bottom levels: each mesh's root from its first instance; every slot in exactly one leaf
-    std::vector<uint32_t> seen(n_slots, 0), roots;
-    for (uint32_t k = 0; k < n_instance; ++k) {
-        const uint32_t root = u(b.inst[uint64_t(kInstF4) * k + 3].w);
-        if (std::find(roots.begin(), roots.end(), root) == roots.end()) roots.push_back(root);
-    }
+    // bottom levels: every mesh's own root, whether an instance uses it or not;
+    // every slot in exactly one leaf; each instance names one of those roots
+    std::vector<uint32_t> seen(n_slots, 0);
     uint32_t depth = 0, max_leaf = 0, top_depth = 0, top_leaf = 0;
-    for (uint32_t root : roots)
+    for (uint32_t root : b.mesh_roots)
         if (!walk(b, root, seen, false, n_instance, depth, max_leaf)) return MGK_ERROR_INVALID_ARGUMENT;
     for (uint32_t c : seen)
         if (c != 1u) return MGK_ERROR_INVALID_ARGUMENT;
+    for (uint32_t k = 0; k < n_instance; ++k) {
+        const uint32_t root = u(b.inst[uint64_t(kInstF4) * k + 3].w);
+        if (std::find(b.mesh_roots.begin(), b.mesh_roots.end(), root) == b.mesh_roots.end()) return MGK_ERROR_INVALID_ARGUMENT;
+    }
     std::vector<uint32_t> seen_top(n_instance, 0);
     if (!walk(b, b.root, seen_top, true, n_instance, top_depth, top_leaf)) return MGK_ERROR_INVALID_ARGUMENT;
     for (uint32_t c : seen_top)

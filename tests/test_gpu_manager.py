@@ -111,11 +111,57 @@ def test_progressive_frame_equals_the_n_sample_frame(gpu_ctx):
         disp = M.display(view_transform=M.VIEW_NONE, display_colorspace=M.DISPLAY_SRGB, white_point_K=6500.0)
         compare(frame2, *direct(gpu_ctx, 40, 32, n2, disp, metric=M.SCHWARZSCHILD, reflections=3, white_K=6500.0))
         st, _ = m.stats()
-        assert st.total_wavefront_sample_time_seconds > 0 and st.latest_wavefront_history_length == 0
+        assert st.total_wavefront_sample_time_seconds > 0 and st.latest_wavefront_history_length > 0
         print(f"manager: {n} + {n2} samples, latest sample {st.latest_wavefront_sample_time_seconds * 1e3:.2f} ms, "
               f"loop iteration {st.latest_render_loop_iteration_time_seconds * 1e3:.2f} ms")
     finally:
         m.close()
+
+
+def test_queue_histories_follow_the_latest_samples_paths(gpu_ctx):
+    """The statistics' per-iteration queue counts equal the counts that the
+    same sample's paths give, traced directly with their step counts
+    (mgk_trace_paths): advance_A[k] = paths with more than k steps,
+    splat[k] = paths ending at step k + 1 other than by the step limit,
+    advance_B[k] = advance_A[k + 1], and bxdf - splat = the reflections."""
+    import torch
+    W, H = 40, 32
+    m = M.Manager(0)
+    try:
+        for cmd in (M.Manager.command("camera_resolution", W, H), M.Manager.aov_command("add_aov", "test", 3)):
+            assert m.push(cmd) == abi.MAGIK_SUCCESS
+        wait(m.dispatch, what="dispatch")
+        n, _ = run_until_paused(m, 2)
+        wait(lambda: m.stats()[1], what="statistics after the pause")
+        st, _ = m.stats()
+        q = m.queue_histories()
+    finally:
+        m.close()
+    assert st.x_sample == n and st.h_latest_advance_A_queue_count_history is None
+    length = st.latest_wavefront_history_length
+    assert all(v.shape == (length,) for v in q.values())
+
+    integ, sc, _, cam = stage_structs(W, H)
+    gpu_ctx.set_scene(sc)
+    rays = gpu_ctx.generate_rays(integ, cam, 0, n - 1, 1)
+    hits, steps = gpu_ctx.trace_paths(integ, sc, cam, 0, rays, 1, sample_begin=n - 1)
+    torch.cuda.synchronize()
+    n_refl = gpu_ctx.stats().n_reflections
+    s = np.maximum(steps.cpu().numpy().astype(np.int64), 1)
+    lim = hits[:, 0].cpu().numpy() == M.HIT_STEP_LIMIT
+    assert length == s.max() and n_refl > 0
+    k = np.arange(length)
+    alive = (s[None, :] > k[:, None]).sum(axis=1)
+    ends = np.bincount(s[~lim], minlength=length + 1)[1:]
+    np.testing.assert_array_equal(q["advance_A"], alive)
+    np.testing.assert_array_equal(q["intersect"], alive)
+    np.testing.assert_array_equal(q["advance_B"], np.append(alive[1:], 0))
+    np.testing.assert_array_equal(q["splat"], ends)
+    np.testing.assert_array_equal(q["redistribute"], ends)
+    extra = q["bxdf"].astype(np.int64) - ends
+    assert extra.min() >= 0 and extra.sum() == n_refl
+    assert q["advance_A"][0] == W * H
+    print(f"queue histories: {length} iterations, {lim.sum()} paths at the step limit, {n_refl} reflections")
 
 
 def test_headless_manager_has_no_swapchain(gpu_ctx):

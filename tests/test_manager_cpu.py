@@ -86,6 +86,16 @@ def test_collections():
     assert lib.magik_configure_framebuffer_collection(None, abi.FB_CONFIG_HOST) == abi.MAGIK_ERROR_INVALID_POINTER
     st = abi.RendererStatistics(x_sample=5)
     assert lib.magik_destroy_renderer_statistics(C.byref(st)) == abi.MAGIK_SUCCESS and st.x_sample == 0
+    # a fetch hands out the six histories as one host block that destroy frees
+    libc = C.CDLL(None)
+    libc.malloc.restype = C.c_void_p
+    libc.malloc.argtypes = [C.c_size_t]
+    block = libc.malloc(6 * 3 * 4)
+    st = abi.RendererStatistics(latest_wavefront_history_length=3)
+    for i, q in enumerate(abi.QUEUE_HISTORIES):
+        setattr(st, f"h_latest_{q}_queue_count_history", block + 12 * i)
+    assert lib.magik_destroy_renderer_statistics(C.byref(st)) == abi.MAGIK_SUCCESS
+    assert st.latest_wavefront_history_length == 0 and st.h_latest_splat_queue_count_history is None
     d = abi.ManagerDescriptor(display_type=9)
     h = C.c_void_p()
     assert lib.magik_render_manager_create(C.byref(h), d) == abi.MAGIK_UNKNOWN_ENUM_TYPE and not h.value

@@ -243,6 +243,10 @@ def test_hierarchy_build_is_sound_on_the_host():
         assert r == abi.MAGIK_SUCCESS, (r, tris)
         assert info.n_triangles == tris and 1 <= info.max_leaf <= 8 and info.max_depth < 64
         assert info.world_max[0] > info.world_min[0]
+    # a mesh that no instance uses is built and checked like the others
+    r, info = mesh.hierarchy_info([mesh.icosphere(2), mesh.box_mesh((0, 0, 0), (1, 1, 1)), mesh.icosphere(1)],
+                                  [(1, M.MATTER_DIFFUSE, mesh.affine())])
+    assert r == abi.MAGIK_SUCCESS and info.n_triangles == 320 + 12 + 80
     v1, f1 = mesh.box_mesh((0, 0, 0), (1, 1, 1))
     good = (0, M.MATTER_DIFFUSE, mesh.affine())
     for meshes, inst in (([(v1, np.where(f1 == 7, 8, f1))], [good]),                         # vertex index out of range
