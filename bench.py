@@ -44,8 +44,8 @@ ERROR_BAR = 1e-3
 # stage sums, error norm, controller), counted by oracle/opcount.cpp:
 # 453 add/sub + 921 mul + 40 div + 13 sqrt + 1 pow.
 FLOP_PER_TRIAL = {2: 1428, 1: 1272, 0: 0}
-# MLP forward, 5 -> 4 x 256 SiLU -> 6 outputs: 2 (5*256 + 3*256*256 + 256*6)
-FLOP_PER_INFERENCE = 398848
+# MLP forward, 5 -> 4 x 128 ReLU -> 6 outputs: 2 (5*128 + 3*128*128 + 128*6)
+FLOP_PER_INFERENCE = 101120
 # HBM bytes moved by the integration of one Kerr 1080p x 32 frame (all zone
 # passes): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors being what
 # the same counters read for a copy kernel of known size (bench.py --copy-probe).
@@ -629,7 +629,7 @@ def bench_surrogate(M, ctx, args, integ, sc, spec, cam, spp, n_paths, out):
     mlp_roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_BF16_TFLOPS, "traffic": None, "kernel": "k_mlp_infer",
                 "ms_per_frame": ms,
-                "note": f"{FLOP_PER_INFERENCE} FLOP per path (5 -> 4x256 SiLU -> 6) x {n_paths} paths / "
+                "note": f"{FLOP_PER_INFERENCE} FLOP per path (5 -> 4x128 ReLU -> 6) x {n_paths} paths / "
                         "HIP-event time of the kernel on the launch stream"}
     log(f"  k_mlp_infer {ms:.2f} ms per frame, {achieved:.1f} TFLOP/s")
     del xf, yf

@@ -562,7 +562,7 @@ MGK_API e_magik_result_types mgk_camera_boyer_lindquist(float r_obs, float theta
 /* ---------------------------------------------------------------------------
  * 3. Learned surrogate (no counterpart in the reference: the MLP of the
  *    tracer's Kerr path).  5 features (spin a, observer r and theta in
- *    Boyer-Lindquist, screen alpha, beta) -> 4 x 256 SiLU -> outputs
+ *    Boyer-Lindquist, screen alpha, beta) -> 4 x 128 ReLU -> outputs
  *    (horizon / disk logits, normalised log r of the disk hit, escape
  *    direction cos(theta), sin(phi), cos(phi)); bf16 weights and activations,
  *    FP32 accumulation on the matrix cores.
@@ -623,7 +623,7 @@ typedef struct mgk_surrogate_stats_t {
 MGK_API e_magik_result_types mgk_surrogate_create(int device, uint64_t seed, const mgk_surrogate_ranges_t* ranges,
                                                   mgk_surrogate_handle* out);
 MGK_API e_magik_result_types mgk_surrogate_destroy(mgk_surrogate_handle s);
-/* Parameter counts of the padded layout (weights 16x256, 3 x 256x256, 256x32). */
+/* Parameter counts of the padded layout (weights 16x128, 3 x 128x128, 128x32). */
 MGK_API e_magik_result_types mgk_surrogate_n_params(uint64_t* n_weights, uint64_t* n_biases);
 MGK_API e_magik_result_types mgk_surrogate_get_weights(mgk_surrogate_handle s, float* weights, float* biases);
 MGK_API e_magik_result_types mgk_surrogate_set_weights(mgk_surrogate_handle s, const float* weights, const float* biases);

@@ -524,7 +524,7 @@ def display_transform(ctx, disp, xyz, out=None):
 # ---------------------------------------------------------------------------
 # Surrogate (learned path)
 # ---------------------------------------------------------------------------
-LAYER_SHAPES = [(16, 256), (256, 256), (256, 256), (256, 256), (256, 32)]
+LAYER_SHAPES = [(16, 128), (128, 128), (128, 128), (128, 128), (128, 32)]
 
 
 def surrogate_ranges(spin=(0.0, 0.99), r=(20.0, 100.0), theta=(10.0, 89.0), screen=15.0, disk_r_max=30.0):

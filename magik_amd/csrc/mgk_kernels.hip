@@ -52,10 +52,16 @@ __device__ __forceinline__ S rms_norm(const St<S>& e, const St<S>& y0, const St<
     return sqrt_<false>(total);
 }
 
-// The same norm for the split (mixed / near-zone) trials: the FP64 error
-// components and scales are formed first, their ratios only feed the
-// accept/reject test and the step-size factor, so FP32 ratios with a hardware
-// reciprocal are ample.
+// The same norm for the split (mixed / near-zone) trials.  The FP64 error
+// components and scales are formed first and their ratios taken in FP32 with
+// a hardware reciprocal.  In units u = 2^-24 each ratio is within 2.5 u of
+// the exact one (two roundings to float, the 1 ulp reciprocal, the product),
+// each square within 5.5 u, the sum of the eight within 9 u and the root
+// within 5.5 u (3.3e-7) of the FP64 norm of the same components.  Within
+// kNormBand of 1 the norm is taken again in FP64, so every accept/reject
+// decision is the FP64 norm's; the step-size factor keeps the FP32 value.
+constexpr float kNormBand = 0x1p-14f;   // 6.1e-5, ~185x the FP32 form's error bound
+
 __device__ __forceinline__ double rms_norm_split(const St<double>& e, const St<double>& y0, const St<double>& y1, double at, double rt) {
     auto c = [&](double ev, double a, double b) {
         const float sc = float(at + rt * fmax(fabs(a), fabs(b)));
@@ -64,7 +70,17 @@ __device__ __forceinline__ double rms_norm_split(const St<double>& e, const St<d
     const float s0 = c(e.qt, y0.qt, y1.qt), s1 = c(e.qx, y0.qx, y1.qx), s2 = c(e.qy, y0.qy, y1.qy), s3 = c(e.qz, y0.qz, y1.qz);
     const float s4 = c(e.pt, y0.pt, y1.pt), s5 = c(e.px, y0.px, y1.px), s6 = c(e.py, y0.py, y1.py), s7 = c(e.pz, y0.pz, y1.pz);
     const float total = (s0 * s0 + s1 * s1 + s2 * s2 + s3 * s3 + s4 * s4 + s5 * s5 + s6 * s6 + s7 * s7) * 0.125f;
-    return double(__builtin_amdgcn_sqrtf(total));
+    const float n32 = __builtin_amdgcn_sqrtf(total);
+    if (fabsf(n32 - 1.0f) < kNormBand) {
+        auto d = [&](double ev, double a, double b) {
+            const double q = ev / (at + rt * fmax(fabs(a), fabs(b)));
+            return q * q;
+        };
+        const double t = d(e.qt, y0.qt, y1.qt) + d(e.qx, y0.qx, y1.qx) + d(e.qy, y0.qy, y1.qy) + d(e.qz, y0.qz, y1.qz) +
+                         d(e.pt, y0.pt, y1.pt) + d(e.px, y0.px, y1.px) + d(e.py, y0.py, y1.py) + d(e.pz, y0.pz, y1.pz);
+        return sqrt(t * 0.125);
+    }
+    return double(n32);
 }
 
 // x^0.2 of the step controller: the FP32 instance is the correctly rounded

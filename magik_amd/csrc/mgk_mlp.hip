@@ -1,21 +1,19 @@
-// Surrogate MLP on gfx950 bf16 matrix cores: 16 (5 used) -> 4 x 256 SiLU -> 32
+// Surrogate MLP on gfx950 bf16 matrix cores: 16 (5 used) -> 4 x 128 ReLU -> 32
 // (8 used), bf16 weights and activations, FP32 accumulation.
 //
 //   k_gemm         : C = A B with arbitrary strides (so A^T / B^T are free),
 //                    v_mfma_f32_32x32x16_bf16 tiles, 64x64 output per
 //                    256-thread workgroup, optional split-K with FP32 atomics,
-//                    fused epilogues (bias + SiLU -> z, h; SiLU' -> dz, g).
+//                    fused epilogues (bias + ReLU -> z, h; ReLU' -> dz, g).
 //                    Used for the training forward, backward-data and
 //                    weight-gradient products.
-//   k_mlp_infer    : the whole network, persistent (one workgroup per CU
-//                    looping over 256-ray blocks), computed transposed
+//   k_mlp_infer    : the whole network in one kernel, computed transposed
 //                    (Z^T = W^T H^T: weights are the MFMA A operand, rays the
 //                    columns) so that a layer's FP32 accumulators, after
-//                    bias + SiLU, are already the next layer's bf16 B operand
-//                    in registers.  Each wave keeps two ray tiles half a
-//                    layer apart, so one tile's matrix work and the other's
-//                    SiLU epilogue issue side by side; a hidden layer's
-//                    weights stay in LDS for both tiles.
+//                    bias + ReLU, are already the next layer's bf16 B operand
+//                    in registers.  The hidden weights are staged through LDS
+//                    a few k-steps at a time and serve every ray tile of the
+//                    workgroup.
 //   k_loss         : class cross-entropy + masked regressions -> dL/dy.
 //   k_adam         : Adam on the FP32 master weights, refreshes the bf16 copy.
 #include "mgk_internal.hpp"
@@ -34,11 +32,8 @@ __device__ __forceinline__ int frag_rc(int l) { return l & 31; }
 __device__ __forceinline__ int frag_k(int l, int j) { return ((l >> 5) << 3) + j; }
 __device__ __forceinline__ int acc_row(int l, int i) { return ((i >> 2) << 3) + ((l >> 5) << 2) + (i & 3); }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
-__device__ __forceinline__ float dsilu(float z) {
-    float s = 1.0f / (1.0f + __expf(-z));
-    return s * (1.0f + z * (1.0f - s));
-}
+__device__ __forceinline__ float relu(float z) { return fmaxf(z, 0.0f); }
+__device__ __forceinline__ float drelu(float z) { return z > 0.0f ? 1.0f : 0.0f; }
 
 template <class T> __device__ __forceinline__ float ld(const void* p, int64_t i) {
     return float(reinterpret_cast<const T*>(p)[i]);
@@ -87,14 +82,14 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
             case EPI_ATOMIC:
                 atomicAdd(&g.C[ic], v);
                 break;
-            case EPI_SILU: {
+            case EPI_RELU: {
                 if (g.bias) v += g.bias[n];
                 g.C[ic] = v;
-                g.H[int64_t(m) * g.ldh + n] = __bf16(silu(v));
+                g.H[int64_t(m) * g.ldh + n] = __bf16(relu(v));
                 break;
             }
-            case EPI_DSILU: {
-                float dz = v * dsilu(g.Z[int64_t(m) * g.ldz + n]);
+            case EPI_DRELU: {
+                float dz = v * drelu(g.Z[int64_t(m) * g.ldz + n]);
                 g.C[ic] = dz;
                 g.H[int64_t(m) * g.ldh + n] = __bf16(dz);
                 break;
@@ -134,22 +129,19 @@ __device__ __forceinline__ int perm_k(int ks, int g, int j) {
 }
 
 // Repack a row-major FP32 [K][N] master weight into MFMA A-fragment order of
-// the transposed product (A = W^T), each element scaled by `scale` before its
-// one rounding to bf16 (the inference kernel's log2-domain factors: -log2(e)
-// for the input layer, -ln 2 for the output layer, 1 for the hidden layers).
-// Fragment (ks, nt) holds, for lane l and j = 0..7, W[k][32 nt + l % 32] with
+// the transposed product (A = W^T), rounded once to bf16.  Fragment (ks, nt) holds, for lane l and j = 0..7, W[k][32 nt + l % 32] with
 // k = 16 ks + 8 (l / 32) + j (natural, the input layer) or
 // k = perm_k(ks, l / 32, j) (layers fed by a previous layer's accumulators);
-// k-step major, so two k-steps of all output tiles are one contiguous 16 KB:
+// k-step major, so a few k-steps of all output tiles are one contiguous block:
 // F[((ks * (N / 32) + nt) * 64 + lane) * 8 + j].
-__global__ void k_repack_scaled(const float* W, int K, int N, __bf16* F, int permuted, float scale) {
+__global__ void k_repack(const float* W, int K, int N, __bf16* F, int permuted) {
     const int64_t total = int64_t(K) * N;
     for (int64_t o = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; o < total; o += int64_t(gridDim.x) * blockDim.x) {
         const int j = int(o & 7), lane = int((o >> 3) & 63);
         const int64_t t = o >> 9;
         const int nt = int(t % (N / 32)), ks = int(t / (N / 32));
         const int k = permuted ? perm_k(ks, lane >> 5, j) : 16 * ks + 8 * (lane >> 5) + j;
-        F[o] = __bf16(scale * W[int64_t(k) * N + 32 * nt + (lane & 31)]);
+        F[o] = __bf16(W[int64_t(k) * N + 32 * nt + (lane & 31)]);
     }
 }
 
@@ -218,25 +210,21 @@ __global__ void k_adam(float* w, float* m, float* v, const float* g, int64_t n, 
 // ---------------------------------------------------------------------------
 // Fused inference: 4 waves x MGK_INFER_TILES ray tiles x 32 rays per workgroup.
 // ---------------------------------------------------------------------------
-// One 32-ray tile per wave and two workgroups per CU (one workgroup's SiLU
-// epilogue overlaps the other's matrix work), hidden weights staged two
-// k-steps at a time: measured fastest of 1-2 tiles x 1-2 workgroups x 1-4
-// k-steps per stage (32.1 ms for a 1080p x 32 frame, 825 TFLOP/s).
+// Two 32-ray tiles per wave (each staged weight fragment read from LDS serves
+// two MFMAs), two workgroups per CU, hidden weights staged four k-steps at a
+// time: measured fastest of 1-2 tiles x 1-2 workgroups x 2-4 k-steps per
+// stage (7.7 ms for the 66.4M feature rows of a 1080p x 32 frame).
 #ifndef MGK_INFER_TILES
-#define MGK_INFER_TILES 1
+#define MGK_INFER_TILES 2
 #define MGK_INFER_OCC 2
 #endif
 #ifndef MGK_INFER_STAGE
-#define MGK_INFER_STAGE 2
+#define MGK_INFER_STAGE 4
 #endif
 constexpr int kInferWaves = 4;
 constexpr int kStepFrags = kMlpHidden / 32;               // A fragments per hidden k-step
 
-constexpr float kNegLog2e = -1.44269504088896341f;
-// SiLU in the log2 domain: for u = -log2(e) z, u / (1 + 2^u) = -log2(e) silu(z)
-__device__ __forceinline__ float silu_log2(float u) { return u * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u)); }
-
-// (bias +) SiLU of one layer's accumulators into the next layer's B operand:
+// (bias +) ReLU of one layer's accumulators into the next layer's B operand:
 // H[c][2 nt + h] holds elements i = 8 h .. 8 h + 7 of accumulator tile nt,
 // element i being row 32 nt + 8 (i / 4) + 4 g + i % 4.  The hidden layers'
 // accumulators start from their bias (kAddBias false).
@@ -251,27 +239,29 @@ __device__ __forceinline__ void infer_epilogue(const f32x16 (&acc)[kInferTiles][
 #pragma unroll
         for (int c = 0; c < kInferTiles; ++c)
 #pragma unroll
-            for (int i = 0; i < 16; ++i) H[c][2 * nt + (i >> 3)][i & 7] = __bf16(silu_log2(kAddBias ? acc[c][nt][i] + b[i] : acc[c][nt][i]));
+            for (int i = 0; i < 16; ++i) H[c][2 * nt + (i >> 3)][i & 7] = __bf16(fmaxf(kAddBias ? acc[c][nt][i] + b[i] : acc[c][nt][i], 0.0f));
     }
 }
 
 // TILES 32-ray tiles per wave (each staged weight fragment serves TILES
-// MFMAs); WG_PER_CU workgroups per CU (with two, one workgroup's SiLU
-// epilogue overlaps the other's matrix work).
+// MFMAs); WG_PER_CU workgroups per CU (one workgroup's barrier waits and
+// epilogue overlap the other's matrix work).
 template <int kInferTiles, int WG_PER_CU>
 __global__ void __launch_bounds__(256, WG_PER_CU) k_mlp_infer(InferArgs a) {
     constexpr int kInferRays = kInferWaves * kInferTiles * 32;
     constexpr int kStage = MGK_INFER_STAGE;               // hidden k-steps per LDS stage
     constexpr int kStageFrags = kStage * kStepFrags * 64;  // bf16x8 per stage
-    __shared__ bf16x8 wbuf[2][kStageFrags];               // double buffered (kStage x 8 KB each)
+    constexpr int kStageLoads = kStageFrags / 256;         // of them per thread
+    static_assert(kStageFrags % 256 == 0 && (2 * kStepFrags) % kStage == 0, "stage shape");
+    __shared__ bf16x8 wbuf[2][kStageFrags];               // double buffered (kStage x 4 KB each)
     __shared__ float bias[4 * kMlpHidden + kMlpOutPad];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 5, r = lane & 31;
-    for (int k = tid; k < 4 * kMlpHidden; k += 256) bias[k] = kNegLog2e * a.b[k / kMlpHidden][k % kMlpHidden];
+    for (int k = tid; k < 4 * kMlpHidden; k += 256) bias[k] = a.b[k / kMlpHidden][k % kMlpHidden];
     if (tid < kMlpOutPad) bias[4 * kMlpHidden + tid] = a.b[kMlpLayers - 1][tid];
     {
         const bf16x8* F1 = reinterpret_cast<const bf16x8*>(a.Wf[1]);
 #pragma unroll
-        for (int k = 0; k < 2 * kStage; ++k) wbuf[0][tid + 256 * k] = F1[tid + 256 * k];
+        for (int k = 0; k < kStageLoads; ++k) wbuf[0][tid + 256 * k] = F1[tid + 256 * k];
     }
     const int64_t ray0 = int64_t(blockIdx.x) * kInferRays + w * (kInferTiles * 32);
     bf16x8 H[kInferTiles][2 * kStepFrags];
@@ -325,12 +315,12 @@ __global__ void __launch_bounds__(256, WG_PER_CU) k_mlp_infer(InferArgs a) {
         for (int st = 0; st < 2 * kStepFrags / kStage; ++st) {
             const bool last = st == 2 * kStepFrags / kStage - 1;
             const bool has_next = !last || l + 1 < kMlpLayers - 1;
-            bf16x8 nx[2 * kStage];
+            bf16x8 nx[kStageLoads];
             if (has_next) {
                 const bf16x8* Fn = last ? reinterpret_cast<const bf16x8*>(a.Wf[l + 1])
                                         : reinterpret_cast<const bf16x8*>(a.Wf[l]) + (st + 1) * kStageFrags;
 #pragma unroll
-                for (int k = 0; k < 2 * kStage; ++k) nx[k] = Fn[tid + 256 * k];
+                for (int k = 0; k < kStageLoads; ++k) nx[k] = Fn[tid + 256 * k];
             }
 #pragma unroll
             for (int kk = 0; kk < kStage; ++kk) {
@@ -345,7 +335,7 @@ __global__ void __launch_bounds__(256, WG_PER_CU) k_mlp_infer(InferArgs a) {
             }
             if (has_next) {
 #pragma unroll
-                for (int k = 0; k < 2 * kStage; ++k) wbuf[(st + 1) & 1][tid + 256 * k] = nx[k];
+                for (int k = 0; k < kStageLoads; ++k) wbuf[(st + 1) & 1][tid + 256 * k] = nx[k];
             }
             __syncthreads();
         }
@@ -401,10 +391,10 @@ hipError_t launch_to_bf16(const float* x, __bf16* y, int64_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_to_bf16, dim3(unsigned(blocks)), dim3(256), 0, s, x, y, n);
     return hipGetLastError();
 }
-hipError_t launch_repack_scaled(const float* W, int K, int N, __bf16* F, int permuted, float scale, hipStream_t s) {
+hipError_t launch_repack(const float* W, int K, int N, __bf16* F, int permuted, hipStream_t s) {
     int64_t blocks = (int64_t(K) * N + 255) / 256;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_repack_scaled, dim3(unsigned(blocks)), dim3(256), 0, s, W, K, N, F, permuted, scale);
+    hipLaunchKernelGGL(k_repack, dim3(unsigned(blocks)), dim3(256), 0, s, W, K, N, F, permuted);
     return hipGetLastError();
 }
 hipError_t launch_loss(const float* Y, const float* T, int n, float w_r, float w_d, float* dY, float* sums, hipStream_t s) {

@@ -8,13 +8,13 @@
 namespace mgk {
 
 constexpr int kMlpInPad = 16;    // 5 features, zero padded to one MFMA k-step
-constexpr int kMlpHidden = 256;
+constexpr int kMlpHidden = 128;  // ReLU units per hidden layer
 constexpr int kMlpLayers = 5;    // 4 hidden layers + output
 constexpr int kMlpOut = 8;       // used outputs
 constexpr int kMlpOutPad = 32;   // one MFMA n-tile
 constexpr int kTargetStride = 8; // cls, r_n, cos(theta), sin(phi), cos(phi), spare x3
 
-enum { EPI_STORE = 0, EPI_ATOMIC = 1, EPI_SILU = 2, EPI_DSILU = 3 };
+enum { EPI_STORE = 0, EPI_ATOMIC = 1, EPI_RELU = 2, EPI_DRELU = 3 };
 
 struct GemmArgs {
     const void* A;
@@ -30,7 +30,7 @@ struct GemmArgs {
     int64_t ldc;
     __bf16* H;          // bf16 result (h or g)
     int64_t ldh;
-    const float* Z;     // saved pre-activation for EPI_DSILU
+    const float* Z;     // saved pre-activation for EPI_DRELU
     int64_t ldz;
     int epi;
 };
@@ -41,7 +41,7 @@ struct InferArgs {
     const float* X;     // n rows of x_stride floats: normalised features, zero padded
     int x_stride;       // kMlpInPad or kFrameXStride (the padding beyond it is implied)
     int64_t n;
-    const __bf16* Wf[kMlpLayers];   // scaled bf16 weights in the A-fragment order of k_repack_scaled
+    const __bf16* Wf[kMlpLayers];   // bf16 weights in the A-fragment order of k_repack
     const float* b[kMlpLayers];
     float* Y;           // n x 8 outputs
 };
@@ -50,7 +50,7 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
 hipError_t launch_colsum(const float* X, int M, int N, float* out, hipStream_t s);
 hipError_t launch_zero(float* p, int64_t n, hipStream_t s);
 hipError_t launch_to_bf16(const float* x, __bf16* y, int64_t n, hipStream_t s);
-hipError_t launch_repack_scaled(const float* W, int K, int N, __bf16* F, int permuted, float scale, hipStream_t s);
+hipError_t launch_repack(const float* W, int K, int N, __bf16* F, int permuted, hipStream_t s);
 hipError_t launch_loss(const float* Y, const float* T, int n, float w_r, float w_d, float* dY, float* sums, hipStream_t s);
 hipError_t launch_adam(float* w, float* m, float* v, const float* g, int64_t n, float lr, int t, __bf16* wb, hipStream_t s);
 hipError_t launch_infer(const InferArgs& a, hipStream_t s);

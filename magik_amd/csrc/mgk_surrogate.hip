@@ -118,7 +118,7 @@ e_magik_result_types train_forward(mgk_surrogate* s, const float* X, int n, hipS
         g.M = n; g.N = kN[l]; g.K = kK[l];
         g.bias = s->b[l];
         if (l < kMlpLayers - 1) {
-            g.epi = EPI_SILU;
+            g.epi = EPI_RELU;
             g.C = s->Z[l]; g.ldc = kN[l];
             g.H = s->H[l + 1]; g.ldh = kN[l];
         } else {
@@ -153,12 +153,12 @@ e_magik_result_types train_backward(mgk_surrogate* s, int n, hipStream_t st) {
         HIP_TRY(launch_gemm(g, st));
         HIP_TRY(launch_colsum(dz, n, kN[l], s->gb[l], st));
         if (l == 0) break;
-        // dz_{l-1} = (g_l W_l^T) * silu'(z_{l-1});  g_{l-1} = bf16(dz_{l-1})
+        // dz_{l-1} = (g_l W_l^T) * relu'(z_{l-1});  g_{l-1} = bf16(dz_{l-1})
         GemmArgs d = gemm_args();
         d.A = s->G[cur]; d.a_bf16 = 1; d.sam = kN[l]; d.sak = 1;
         d.B = s->Wb[l]; d.b_bf16 = 1; d.sbk = 1; d.sbn = kN[l];
         d.M = n; d.N = kK[l]; d.K = kN[l];
-        d.epi = EPI_DSILU;
+        d.epi = EPI_DRELU;
         d.Z = s->Z[l - 1]; d.ldz = kK[l];
         d.C = s->DZ; d.ldc = kK[l];
         d.H = s->G[1 - cur]; d.ldh = kK[l];
@@ -170,12 +170,7 @@ e_magik_result_types train_backward(mgk_surrogate* s, int n, hipStream_t st) {
 }
 
 e_magik_result_types refresh_fragments(mgk_surrogate* s, hipStream_t st) {
-    // the inference kernel's log2-domain factors: -log2(e) on the input
-    // layer, -ln 2 on the output layer, none on the hidden layers
-    for (int l = 0; l < kMlpLayers; ++l) {
-        const float scale = l == 0 ? -1.44269504088896341f : l == kMlpLayers - 1 ? -0.693147180559945309f : 1.0f;
-        HIP_TRY(launch_repack_scaled(s->W[l], kK[l], kN[l], s->Wf[l], l > 0 ? 1 : 0, scale, st));
-    }
+    for (int l = 0; l < kMlpLayers; ++l) HIP_TRY(launch_repack(s->W[l], kK[l], kN[l], s->Wf[l], l > 0 ? 1 : 0, st));
     s->frag_dirty = false;
     return MAGIK_SUCCESS;
 }

@@ -62,15 +62,51 @@ def test_fused_inference_matches_oracle(sur, n):
     sur.set_weights(W, b)
     x = features(np.random.default_rng(n), n)
     y = sur.forward(torch.from_numpy(x).cuda()).cpu().numpy()
-    ref = MO.forward_infer(W, b, x)[:, :8]
+    ref = MO.forward(W, b, x)[:, :8]
     # FP32 accumulation order differs from the oracle's float64: an activation
     # can round to the neighbouring bf16 value, so compare at bf16 resolution
     err = np.abs(y - ref)
     assert err.max() <= 2e-2 * (1 + np.abs(ref).max()), err.max()
     assert np.median(err) <= 1e-3 * (1 + np.abs(ref).max())
-    # and the same network as training's forward pass, to the bf16 error of
-    # the two rounding orders
-    assert np.abs(y - MO.forward(W, b, x)[:, :8]).max() <= 5e-2 * (1 + np.abs(ref).max())
+
+
+def test_hand_wired_network_is_exact(sur):
+    """Weights wired by hand (the oracle's hand-wired network: y1 =
+    3 relu(2 relu(x0 - 0.5) - 1) - 0.25, y0 = -relu(2 relu(x0 - 0.5) - 1)):
+    every value is exact in bf16 and FP32, so the fused inference kernel and
+    the training forward must return it bit for bit, whatever their
+    summation order; a swapped layer, a misplaced bias or a wrong fragment
+    permutation changes it."""
+    import torch
+    shapes = M.LAYER_SHAPES
+    W = [np.zeros(s, np.float32) for s in shapes]
+    b = [np.zeros(s[1], np.float32) for s in shapes]
+    W[0][0, 0], b[0][0] = 1.0, -0.5
+    W[1][0, 0] = 1.0
+    W[2][0, 0], b[2][0] = 2.0, -1.0
+    W[3][0, 0] = 1.0
+    W[4][0, 1], b[4][1] = 3.0, -0.25
+    W[4][0, 0] = -1.0
+    # the same through other units and k-slots: unit 77 of every hidden layer
+    # carries relu(x3 + 0.25) into output 5
+    W[0][3, 77], b[0][77] = 1.0, 0.25
+    for l in (1, 2, 3):
+        W[l][77, 77] = 1.0
+    W[4][77, 5] = 0.5
+    sur.set_weights(W, b)
+    n = 700
+    x = np.zeros((n, 16), np.float32)
+    x[:, 0] = np.tile([0.25, 0.75, 1.5, 3.0, 2.0, 0.5, 1.0], n // 7)
+    x[:, 3] = np.tile([-1.0, -0.25, 0.0, 0.5, 1.75], n // 5)
+    h = np.maximum(2 * np.maximum(x[:, 0] - 0.5, 0) - 1, 0)
+    want = np.zeros((n, 8), np.float32)
+    want[:, 0], want[:, 1] = -h, 3 * h - 0.25
+    want[:, 5] = 0.5 * np.maximum(x[:, 3] + 0.25, 0)
+    np.testing.assert_array_equal(MO.forward(W, b, x)[:, :8], want)
+    y = sur.forward(torch.from_numpy(x).cuda()).cpu().numpy()
+    np.testing.assert_array_equal(y, want)
+    yt, _ = sur.gradients(torch.from_numpy(x).cuda(), torch.zeros((n, 32), dtype=torch.float32, device="cuda"))
+    np.testing.assert_array_equal(yt.cpu().numpy()[:, :8], want)
 
 
 def test_gradients_match_oracle(sur):

@@ -2,7 +2,7 @@
 
 The backward pass and the loss gradient are checked against central finite
 differences of the unquantised float64 network, so a transposed operand, a
-dropped SiLU' factor or a wrong class column fails; bf16 rounding is checked
+dropped ReLU' factor or a wrong class column fails; bf16 rounding is checked
 against exact bit patterns.
 """
 import numpy as np
@@ -66,6 +66,9 @@ def test_backward_matches_finite_differences():
 
 
 def test_quantised_backward_close_to_exact():
+    """bf16 rounding moves a few pre-activations across zero, where ReLU'
+    jumps; taken with the quantised forward's ReLU pattern, the exact
+    float64 backward stays within the bf16 error of the quantised one."""
     rng = np.random.default_rng(3)
     W, b = small_net(rng)
     x = rng.uniform(-1, 1, size=(64, O.IN_PAD))
@@ -74,9 +77,11 @@ def test_quantised_backward_close_to_exact():
     y, hs, zs = O.forward(W, b, x, keep=True)
     dW, db = O.backward(W, hs, zs, dy)
     y0, hs0, zs0 = O.forward(W, b, x, keep=True, quant=False)
-    dW0, db0 = O.backward(W, hs0, zs0, dy, quant=False)
+    flips = np.mean([np.mean((z > 0) != (z0 > 0)) for z, z0 in zip(zs, zs0)])
+    assert flips < 2e-2
+    dW0, db0 = O.backward(W, hs0, zs, dy, quant=False)
     for l in range(len(W)):
-        assert np.linalg.norm(dW[l] - dW0[l]) <= 2e-2 * np.linalg.norm(dW0[l])
+        assert np.linalg.norm(dW[l] - dW0[l]) <= 2e-2 * np.linalg.norm(dW0[l]), l
     assert np.abs(y - y0).max() <= 5e-2 * (1 + np.abs(y0).max())
 
 
@@ -116,39 +121,32 @@ def test_loss_cross_entropy_closed_form():
     assert sums[1] == 0 and sums[2] == 0
 
 
-def test_log2_domain_inference_is_the_same_network():
-    # unquantised, the log2-domain restatement of inference equals the plain
-    # float64 network: the factors -log2(e) and -ln 2 cancel exactly
-    rng = np.random.default_rng(11)
-    W, b = small_net(rng)
-    x = rng.uniform(-1, 1, size=(9, O.IN_PAD))
-    np.testing.assert_allclose(O.forward_infer(W, b, x, quant=False), O.exact_forward(W, b, x), rtol=1e-12, atol=1e-12)
+def test_relu_closed_form():
+    z = np.array([-3.0, -1e-30, 0.0, 1e-30, 2.5])
+    np.testing.assert_array_equal(O.relu(z), [0.0, 0.0, 0.0, 1e-30, 2.5])
+    np.testing.assert_array_equal(O.drelu(z), [0.0, 0.0, 0.0, 1.0, 1.0])
 
 
-def test_log2_domain_inference_quantised_close_to_exact():
-    # with its bf16 rounding points it stays within the bf16 error of the
-    # exact network, like forward(); and a dropped factor (the output layer's
-    # -ln 2 left out) is far outside that
-    rng = np.random.default_rng(12)
-    W, b = small_net(rng)
-    x = rng.uniform(-1, 1, size=(256, O.IN_PAD))
-    x[:, 5:] = 0
-    y0 = O.exact_forward(W, b, x)
-    yi = O.forward_infer(W, b, x)
-    yf = O.forward(W, b, x)
-    tol = 5e-3 * (1 + np.abs(y0).max())
-    assert np.abs(yi - y0).max() <= tol
-    assert np.abs(yi - yf).max() <= tol
-    W_bad = [w.copy() for w in W]
-    W_bad[-1] = W_bad[-1] / np.float32(O.NEG_LN2)
-    assert np.abs(O.forward_infer(W_bad, b, x) - y0).max() > 10 * tol
-
-
-def test_log2_silu_closed_form():
-    # u / (1 + 2^u) at u = -log2(e) z is -log2(e) silu(z): z = 0 -> 0,
-    # z = 1 -> -log2(e) / (1 + e^-1), large |z| -> -log2(e) max(z, 0)
-    z = np.array([0.0, 1.0, -40.0, 40.0])
-    u = -z / np.log(2.0)
-    h = u / (1.0 + np.exp2(u))
-    want = -np.array([0.0, 1.0 / (1.0 + np.exp(-1.0)), 0.0, 40.0]) / np.log(2.0)
-    np.testing.assert_allclose(h, want, rtol=1e-12, atol=1e-15)
+def test_hand_wired_network():
+    """A network wired by hand: hidden unit 0 of layer 1 is relu(x0 - 0.5),
+    units 0 of layers 2..4 pass it on (weight 1, or 2 into layer 3 with bias
+    -1), the output y1 = 3 h - 0.25 and y0 = -h.  Every other weight is zero,
+    so y1 = 3 relu(2 relu(x0 - 0.5) - 1) - 0.25, exactly representable in bf16
+    for these inputs."""
+    shapes = O.layer_shapes()
+    W = [np.zeros(s, np.float32) for s in shapes]
+    b = [np.zeros(s[1], np.float32) for s in shapes]
+    W[0][0, 0], b[0][0] = 1.0, -0.5
+    W[1][0, 0] = 1.0
+    W[2][0, 0], b[2][0] = 2.0, -1.0
+    W[3][0, 0] = 1.0
+    W[4][0, 1], b[4][1] = 3.0, -0.25
+    W[4][0, 0] = -1.0
+    x = np.zeros((4, O.IN_PAD), np.float32)
+    x[:, 0] = [0.25, 0.75, 1.5, 3.0]
+    h = np.maximum(2 * np.maximum(x[:, 0] - 0.5, 0) - 1, 0)
+    for quant in (True, False):
+        y = O.forward(W, b, x, quant=quant)
+        np.testing.assert_array_equal(y[:, 1], 3 * h - 0.25)
+        np.testing.assert_array_equal(y[:, 0], -h)
+        assert np.all(y[:, 2:] == 0)
