@@ -655,6 +655,18 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
                                                         const mgk_scene_t* scene, const mgk_spectral_t* spec, const mgk_camera_t* cam,
                                                         uint64_t seed, uint32_t spp, const mgk_fallback_t* fb, float* d_xyz,
                                                         mgk_hit_t* d_hits, mgk_surrogate_stats_t* stats, void* stream);
+/* The analytic check of fallback.analytic on the host: the disk crossing,
+ * capture or escape of n Kerr rays from the Mino-time integrals of their
+ * separated radial and polar motion, in FP64.  rays: HOST memory in the SoA
+ * layout of mgk_generate_rays (FP64, component c of ray i at
+ * rays[c * ray_stride + i]); integ gives the spin and time_reversed, scene the
+ * disk's r_in .. r_out.  kind[i]: 0 unresolved (near the critical curve, or
+ * not starting inward), 1 captured, 2 escapes, 3 disk.  With r_seed (an
+ * estimate of the crossing radius, > 0) r_hit[i] is the disk radius, solved
+ * for from that seed; without it only the class is found (r_hit[i] = 0).  No
+ * counterpart in the reference. */
+MGK_API e_magik_result_types mgk_mino_disk(const mgk_integrator_t* integ, const mgk_scene_t* scene, const double* rays,
+                                           uint64_t ray_stride, uint64_t n, const double* r_seed, int32_t* kind, double* r_hit);
 
 #ifdef __cplusplus
 }

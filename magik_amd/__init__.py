@@ -525,6 +525,23 @@ def display_transform(ctx, disp, xyz, out=None):
 # Surrogate (learned path)
 # ---------------------------------------------------------------------------
 LAYER_SHAPES = [(16, 128), (128, 128), (128, 128), (128, 128), (128, 32)]
+MINO_UNRESOLVED, MINO_CAPTURED, MINO_ESCAPES, MINO_DISK = range(4)
+
+
+def mino_disk(integ, sc, rays, r_seed=None):
+    """The analytic disk check of fallback.analytic, on the host (mgk_mino_disk):
+    rays (n, 10) float64 initial conditions (host) -> (kind (n,) int32, r (n,))."""
+    import numpy as np
+    rays_soa = np.ascontiguousarray(np.asarray(rays, dtype=np.float64).T)
+    n = rays_soa.shape[1]
+    kind = np.zeros(n, np.int32)
+    r = np.zeros(n, np.float64)
+    seed = None if r_seed is None else np.ascontiguousarray(r_seed, dtype=np.float64)
+    lib = abi.load()
+    abi.check(lib.mgk_mino_disk(C.byref(integ), C.byref(sc), rays_soa.ctypes.data, n, n,
+                                seed.ctypes.data if seed is not None else None, kind.ctypes.data, r.ctypes.data),
+              "mgk_mino_disk")
+    return kind, r
 
 
 def surrogate_ranges(spin=(0.0, 0.99), r=(20.0, 100.0), theta=(10.0, 89.0), screen=15.0, disk_r_max=30.0):

@@ -265,6 +265,7 @@ SIGNATURES = {
     "mgk_render_frame_surrogate": (_R, [_P, _P, C.POINTER(Integrator), C.POINTER(Scene), C.POINTER(Spectral),
                                         C.POINTER(Camera), C.c_uint64, C.c_uint32, C.POINTER(Fallback), _P, _P,
                                         C.POINTER(SurrogateStats), _P]),
+    "mgk_mino_disk": (_R, [C.POINTER(Integrator), C.POINTER(Scene), _P, C.c_uint64, C.c_uint64, _P, _P, _P]),
     "magik_render_manager_create": (_R, [C.POINTER(_P), ManagerDescriptor]),
     "magik_render_manager_destroy": (_R, [_P]),
     "magik_render_manager_fetch_error": (_R, [_P]),

@@ -266,7 +266,7 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
     if (turning) {
         const double W_obs = sqrt(s4 - s_obs), W_inf = sqrt(s4);
         const double K_obs = K(W_obs);
-        const double K_inf = K(W_inf);
+        double K_inf = -1.0;   // taken when a crossing lies past the turning point
         // the disk's window on each branch, as W = sqrt(s4 - s) ranges over it
         // (inbound: s in [max(s_out, s_obs), min(s_in, s4)]; outbound: s in
         // [s_out, min(s_in, s4)]); K is increasing in W
@@ -280,9 +280,12 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
         }
         for (int n = 1; n <= kMinoMaxCrossings; ++n) {
             const double G = G1 + 2.0 * double(n - 1) * G_half;
-            if (G >= K_obs + K_inf) {
-                res.kind = kMinoEscapes;
-                return res;
+            if (G > K_obs) {   // past the turning point: escaped unless before K_obs + K_inf
+                if (K_inf < 0.0) K_inf = K(W_inf);
+                if (G >= K_obs + K_inf) {
+                    res.kind = kMinoEscapes;
+                    return res;
+                }
             }
             if (!want_r) {
                 if ((win_in && G >= K_obs - Ka_in && G <= K_obs - Kb) || (win_out && G >= K_obs + Kb && G <= K_obs + Ka_out)) {

@@ -659,4 +659,23 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     return MAGIK_SUCCESS;
 }
 
+MGK_API e_magik_result_types mgk_mino_disk(const mgk_integrator_t* integ, const mgk_scene_t* scene, const double* rays,
+                                           uint64_t ray_stride, uint64_t n, const double* r_seed, int32_t* kind, double* r_hit) {
+    if (!integ || !scene || (n > 0 && (!rays || !kind || !r_hit))) return MAGIK_ERROR_INVALID_POINTER;
+    if (integ->metric_type != KERR || n > ray_stride) return MGK_ERROR_INVALID_ARGUMENT;
+    if (!(scene->disk_r_out > scene->disk_r_in) || !(scene->disk_r_in > 0.0f)) return MGK_ERROR_INVALID_ARGUMENT;
+    const double a = integ->spin;
+    if (!(std::fabs(a) < 1.0)) return MGK_ERROR_INVALID_ARGUMENT;
+    const double a_eq = integ->time_reversed ? -a : a;   // as the frame's decode takes it
+    const double r_h = 1.0 + std::sqrt(1.0 - a * a);
+    for (uint64_t i = 0; i < n; ++i) {
+        const double q[3] = {rays[0 * ray_stride + i], rays[1 * ray_stride + i], rays[2 * ray_stride + i]};
+        const double p[4] = {rays[4 * ray_stride + i], rays[5 * ray_stride + i], rays[6 * ray_stride + i], rays[7 * ray_stride + i]};
+        const MinoResult mr = mino_disk(mino_ray(q, p, a_eq), r_h, scene->disk_r_in, scene->disk_r_out, r_seed ? r_seed[i] : -1.0);
+        kind[i] = mr.kind;
+        r_hit[i] = mr.r;
+    }
+    return MAGIK_SUCCESS;
+}
+
 }  // extern "C"
