@@ -237,6 +237,10 @@ def main():
                     help="only time the inference kernel over one 1080p x 32 frame of feature rows (counter runs)")
     ap.add_argument("--manager", action="store_true",
                     help="only time the render manager (command queue + worker) against direct one-sample renders")
+    ap.add_argument("--surrogate-only", action="store_true",
+                    help="only train the surrogate and time its frames (profiler runs); prints the surrogate results")
+    ap.add_argument("--rules", default="analytic,default,trace_disk",
+                    help="comma-separated fallback rule sets of the surrogate frames")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a copy kernel over BYTES three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -295,6 +299,12 @@ def main():
             sc.chord = M.CHORD_HERMITE if args.chord == "hermite" else M.CHORD_AFFINE
     out = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
     log(f"bench: {args.workload} {W}x{H}x{spp} = {n_paths} paths, precision {args.precision}")
+    if args.surrogate_only:
+        sur_res, mlp_roof = bench_surrogate(M, ctx, args, integ, sc, spec, cam, spp, n_paths, out)
+        ctx.close()
+        print(json.dumps({"workload": args.workload, "paths_per_frame": n_paths, "surrogate": sur_res,
+                          "roofline_mlp": mlp_roof}), flush=True)
+        return
 
     # ---- plain RKF45 path ----
     acc = {"ms": 0.0, "steps": 0, "trials": 0, "fp64": 0}
@@ -586,6 +596,10 @@ def bench_surrogate(M, ctx, args, integ, sc, spec, cam, spp, n_paths, out):
     rules = {"analytic": M.fallback(analytic=True),
              "default": M.fallback(),
              "trace_disk": M.fallback(trace_disk=True)}
+    wanted = [r for r in args.rules.split(",") if r]
+    if any(r not in rules for r in wanted):
+        raise SystemExit(f"--rules: unknown rule set in {args.rules!r}; known: {', '.join(rules)}")
+    rules = {k: rules[k] for k in wanted}
     for name, fb in rules.items():
         log(f"surrogate frames ({name})")
         stats = []

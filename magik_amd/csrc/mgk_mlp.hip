@@ -18,11 +18,9 @@
 //   k_adam         : Adam on the FP32 master weights, refreshes the bf16 copy.
 #include "mgk_internal.hpp"
 #include "mgk_mlp.hpp"
+#include "mgk_mlp_infer.hpp"
 
 namespace mgk {
-
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
 
 // Fragment layout of v_mfma_f32_32x32x16_bf16 (wave64):
 //   A (32 x 16): lane l holds row l % 32, k = 8 (l / 32) + j, j = 0..7
@@ -207,166 +205,6 @@ __global__ void k_adam(float* w, float* m, float* v, const float* g, int64_t n, 
     }
 }
 
-// ---------------------------------------------------------------------------
-// Fused inference: 4 waves x MGK_INFER_TILES ray tiles x 32 rays per workgroup.
-// ---------------------------------------------------------------------------
-// Two 32-ray tiles per wave (each staged weight fragment read from LDS serves
-// two MFMAs), two workgroups per CU, hidden weights staged four k-steps at a
-// time: measured fastest of 1-2 tiles x 1-2 workgroups x 2-4 k-steps per
-// stage (7.7 ms for the 66.4M feature rows of a 1080p x 32 frame).
-#ifndef MGK_INFER_TILES
-#define MGK_INFER_TILES 2
-#define MGK_INFER_OCC 2
-#endif
-#ifndef MGK_INFER_STAGE
-#define MGK_INFER_STAGE 4
-#endif
-constexpr int kInferWaves = 4;
-constexpr int kStepFrags = kMlpHidden / 32;               // A fragments per hidden k-step
-
-// (bias +) ReLU of one layer's accumulators into the next layer's B operand:
-// H[c][2 nt + h] holds elements i = 8 h .. 8 h + 7 of accumulator tile nt,
-// element i being row 32 nt + 8 (i / 4) + 4 g + i % 4.  The hidden layers'
-// accumulators start from their bias (kAddBias false).
-template <bool kAddBias, int kInferTiles>
-__device__ __forceinline__ void infer_epilogue(const f32x16 (&acc)[kInferTiles][kStepFrags], const float* bias, int g,
-                                               bf16x8 (&H)[kInferTiles][2 * kStepFrags]) {
-#pragma unroll
-    for (int nt = 0; nt < kStepFrags; ++nt) {
-        float b[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) b[i] = kAddBias ? bias[32 * nt + 8 * (i >> 2) + 4 * g + (i & 3)] : 0.0f;
-#pragma unroll
-        for (int c = 0; c < kInferTiles; ++c)
-#pragma unroll
-            for (int i = 0; i < 16; ++i) H[c][2 * nt + (i >> 3)][i & 7] = __bf16(fmaxf(kAddBias ? acc[c][nt][i] + b[i] : acc[c][nt][i], 0.0f));
-    }
-}
-
-// TILES 32-ray tiles per wave (each staged weight fragment serves TILES
-// MFMAs); WG_PER_CU workgroups per CU (one workgroup's barrier waits and
-// epilogue overlap the other's matrix work).
-template <int kInferTiles, int WG_PER_CU>
-__global__ void __launch_bounds__(256, WG_PER_CU) k_mlp_infer(InferArgs a) {
-    constexpr int kInferRays = kInferWaves * kInferTiles * 32;
-    constexpr int kStage = MGK_INFER_STAGE;               // hidden k-steps per LDS stage
-    constexpr int kStageFrags = kStage * kStepFrags * 64;  // bf16x8 per stage
-    constexpr int kStageLoads = kStageFrags / 256;         // of them per thread
-    static_assert(kStageFrags % 256 == 0 && (2 * kStepFrags) % kStage == 0, "stage shape");
-    __shared__ bf16x8 wbuf[2][kStageFrags];               // double buffered (kStage x 4 KB each)
-    __shared__ float bias[4 * kMlpHidden + kMlpOutPad];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 5, r = lane & 31;
-    for (int k = tid; k < 4 * kMlpHidden; k += 256) bias[k] = a.b[k / kMlpHidden][k % kMlpHidden];
-    if (tid < kMlpOutPad) bias[4 * kMlpHidden + tid] = a.b[kMlpLayers - 1][tid];
-    {
-        const bf16x8* F1 = reinterpret_cast<const bf16x8*>(a.Wf[1]);
-#pragma unroll
-        for (int k = 0; k < kStageLoads; ++k) wbuf[0][tid + 256 * k] = F1[tid + 256 * k];
-    }
-    const int64_t ray0 = int64_t(blockIdx.x) * kInferRays + w * (kInferTiles * 32);
-    bf16x8 H[kInferTiles][2 * kStepFrags];
-    f32x16 acc[kInferTiles][kStepFrags];
-    // layer 1 (K = 16): B = X^T, lane l holds ray l % 32, features 8 (l / 32) .. + 7
-    {
-        bf16x8 xb[kInferTiles];
-#pragma unroll
-        for (int c = 0; c < kInferTiles; ++c) {
-            const int64_t row = ray0 + 32 * c + r;
-            float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
-            if (row < a.n && 8 * g < a.x_stride) {
-                const float4* src = reinterpret_cast<const float4*>(a.X + row * a.x_stride + 8 * g);
-                lo = src[0];
-                hi = src[1];
-            }
-            xb[c][0] = __bf16(lo.x); xb[c][1] = __bf16(lo.y); xb[c][2] = __bf16(lo.z); xb[c][3] = __bf16(lo.w);
-            xb[c][4] = __bf16(hi.x); xb[c][5] = __bf16(hi.y); xb[c][6] = __bf16(hi.z); xb[c][7] = __bf16(hi.w);
-        }
-        const bf16x8* F0 = reinterpret_cast<const bf16x8*>(a.Wf[0]);
-#pragma unroll
-        for (int nt = 0; nt < kStepFrags; ++nt) {
-            const bf16x8 A = F0[nt * 64 + lane];
-#pragma unroll
-            for (int c = 0; c < kInferTiles; ++c) {
-                f32x16 z;
-#pragma unroll
-                for (int i = 0; i < 16; ++i) z[i] = 0.0f;
-                acc[c][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, xb[c], z, 0, 0, 0);
-            }
-        }
-    }
-    __syncthreads();                                      // bias and the first weight stage
-    infer_epilogue<true>(acc, bias, g, H);
-    // hidden layers 2..4: 16 k-steps each; the stage of step s + 1 is loaded
-    // while step s runs, one barrier per step
-    for (int l = 1; l < kMlpLayers - 1; ++l) {
-        // the accumulators start from the layer's bias (rows of the C layout)
-#pragma unroll
-        for (int nt = 0; nt < kStepFrags; ++nt) {
-            f32x16 b0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float4 b4 = *reinterpret_cast<const float4*>(bias + l * kMlpHidden + 32 * nt + 8 * q + 4 * g);
-                b0[4 * q] = b4.x; b0[4 * q + 1] = b4.y; b0[4 * q + 2] = b4.z; b0[4 * q + 3] = b4.w;
-            }
-#pragma unroll
-            for (int c = 0; c < kInferTiles; ++c) acc[c][nt] = b0;
-        }
-#pragma unroll
-        for (int st = 0; st < 2 * kStepFrags / kStage; ++st) {
-            const bool last = st == 2 * kStepFrags / kStage - 1;
-            const bool has_next = !last || l + 1 < kMlpLayers - 1;
-            bf16x8 nx[kStageLoads];
-            if (has_next) {
-                const bf16x8* Fn = last ? reinterpret_cast<const bf16x8*>(a.Wf[l + 1])
-                                        : reinterpret_cast<const bf16x8*>(a.Wf[l]) + (st + 1) * kStageFrags;
-#pragma unroll
-                for (int k = 0; k < kStageLoads; ++k) nx[k] = Fn[tid + 256 * k];
-            }
-#pragma unroll
-            for (int kk = 0; kk < kStage; ++kk) {
-                const int ks = st * kStage + kk;
-#pragma unroll
-                for (int nt = 0; nt < kStepFrags; ++nt) {
-                    const bf16x8 A = wbuf[st & 1][(kk * kStepFrags + nt) * 64 + lane];
-#pragma unroll
-                    for (int c = 0; c < kInferTiles; ++c)
-                        acc[c][nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, H[c][ks], acc[c][nt], 0, 0, 0);
-                }
-            }
-            if (has_next) {
-#pragma unroll
-                for (int k = 0; k < kStageLoads; ++k) wbuf[(st + 1) & 1][tid + 256 * k] = nx[k];
-            }
-            __syncthreads();
-        }
-        infer_epilogue<false>(acc, bias, g, H);
-    }
-    // output layer: one 32-row tile (8 outputs used)
-    {
-        const bf16x8* F4 = reinterpret_cast<const bf16x8*>(a.Wf[kMlpLayers - 1]);
-        f32x16 o[kInferTiles];
-#pragma unroll
-        for (int c = 0; c < kInferTiles; ++c)
-#pragma unroll
-            for (int i = 0; i < 16; ++i) o[c][i] = 0.0f;
-#pragma unroll
-        for (int ks = 0; ks < 2 * kStepFrags; ++ks) {
-            const bf16x8 A = F4[ks * 64 + lane];
-#pragma unroll
-            for (int c = 0; c < kInferTiles; ++c) o[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, H[c][ks], o[c], 0, 0, 0);
-        }
-        // lane group g holds outputs 4 g .. 4 g + 3 (accumulator rows i = 0..3) of ray l % 32
-        const float* bo = bias + 4 * kMlpHidden + 4 * g;
-#pragma unroll
-        for (int c = 0; c < kInferTiles; ++c) {
-            const int64_t row = ray0 + 32 * c + r;
-            if (row < a.n)
-                *reinterpret_cast<float4*>(a.Y + row * kMlpOut + 4 * g) =
-                    make_float4(o[c][0] + bo[0], o[c][1] + bo[1], o[c][2] + bo[2], o[c][3] + bo[3]);
-        }
-    }
-}
-
 hipError_t launch_gemm(const GemmArgs& g, hipStream_t s) {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
     dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, g.k_split > 1 ? g.k_split : 1);
@@ -409,11 +247,24 @@ hipError_t launch_adam(float* w, float* m, float* v, const float* g, int64_t n, 
     hipLaunchKernelGGL(k_adam, dim3(unsigned(blocks)), dim3(256), 0, s, w, m, v, g, n, lr, b1, b2, eps, c1, c2, wb);
     return hipGetLastError();
 }
+// Workgroups of the inference kernel: one per CU, fewer for short inputs.
+unsigned infer_grid(int64_t n) {
+    static int n_cu = 0;
+    if (n_cu == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            n_cu <= 0)
+            n_cu = 256;
+    }
+    const int64_t chunks = (n + kInferChunk - 1) / kInferChunk;
+    const int64_t per_wg = kInferBlock / 64;
+    return unsigned(std::max<int64_t>(1, std::min<int64_t>(n_cu, (chunks + per_wg - 1) / per_wg)));
+}
+
 hipError_t launch_infer(const InferArgs& a, hipStream_t s) {
     if (a.n <= 0) return hipSuccess;
-    constexpr int kTiles = MGK_INFER_TILES, kOcc = MGK_INFER_OCC;
-    constexpr int kRays = kInferWaves * kTiles * 32;
-    hipLaunchKernelGGL((k_mlp_infer<kTiles, kOcc>), dim3(unsigned((a.n + kRays - 1) / kRays)), dim3(256), 0, s, a);
+    InferFromRows src{a.X, a.x_stride};
+    hipLaunchKernelGGL(k_mlp_infer_res<InferFromRows>, dim3(infer_grid(a.n)), dim3(kInferBlock), 0, s, a, src);
     return hipGetLastError();
 }
 

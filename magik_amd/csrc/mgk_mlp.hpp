@@ -54,5 +54,6 @@ hipError_t launch_repack(const float* W, int K, int N, __bf16* F, int permuted, 
 hipError_t launch_loss(const float* Y, const float* T, int n, float w_r, float w_d, float* dY, float* sums, hipStream_t s);
 hipError_t launch_adam(float* w, float* m, float* v, const float* g, int64_t n, float lr, int t, __bf16* wb, hipStream_t s);
 hipError_t launch_infer(const InferArgs& a, hipStream_t s);
+unsigned infer_grid(int64_t n);   // workgroups of the inference kernel for n rows
 
 }  // namespace mgk

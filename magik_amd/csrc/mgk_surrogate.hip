@@ -297,20 +297,26 @@ MGK_API e_magik_result_types mgk_surrogate_set_weights(mgk_surrogate_handle s, c
 }  // extern "C"
 
 namespace mgk {
-e_magik_result_types surrogate_forward(mgk_surrogate* s, const float* d_x, int x_stride, uint64_t n, float* d_y, hipStream_t st) {
+e_magik_result_types surrogate_infer_args(mgk_surrogate* s, InferArgs& a, hipStream_t st) {
     HIP_TRY(hipSetDevice(s->device));
     if (s->frag_dirty) {
         e_magik_result_types r = refresh_fragments(s, st);
         if (r != MAGIK_SUCCESS) return r;
     }
-    InferArgs a;
-    a.X = d_x;
-    a.x_stride = x_stride;
-    a.n = int64_t(n);
     for (int l = 0; l < kMlpLayers; ++l) {
         a.Wf[l] = s->Wf[l];
         a.b[l] = s->b[l];
     }
+    return MAGIK_SUCCESS;
+}
+
+e_magik_result_types surrogate_forward(mgk_surrogate* s, const float* d_x, int x_stride, uint64_t n, float* d_y, hipStream_t st) {
+    InferArgs a;
+    e_magik_result_types r = surrogate_infer_args(s, a, st);
+    if (r != MAGIK_SUCCESS) return r;
+    a.X = d_x;
+    a.x_stride = x_stride;
+    a.n = int64_t(n);
     a.Y = d_y;
     HIP_TRY(launch_infer(a, st));
     return MAGIK_SUCCESS;

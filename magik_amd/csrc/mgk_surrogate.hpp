@@ -134,7 +134,7 @@ struct mgk_surrogate {
     float* sums = nullptr;
     mgk::SurrogateRanges ranges{};
     // render-path buffers
-    mgk::SBuf fr_rays, fr_x, fr_y, fr_hits, fr_queue, fr_pred, fr_counters, fr_rqueue, fr_reason;
+    mgk::SBuf fr_rays, fr_y, fr_hits, fr_queue, fr_pred, fr_counters, fr_rqueue, fr_reason;
     // critical-curve table (screen radius per angle) and its key
     mgk::SBuf crit, crit_lo, crit_hi, crit_ok, crit_rays, crit_hits;
     double crit_key[6] = {-1, -1, -1, -1, -1, -1};
@@ -145,6 +145,9 @@ namespace mgk {
 // The fused inference over n feature rows of x_stride floats (kMlpInPad, the
 // public layout, or kFrameXStride, a frame's own).
 e_magik_result_types surrogate_forward(mgk_surrogate* s, const float* d_x, int x_stride, uint64_t n, float* d_y, hipStream_t st);
+// Refreshes the inference fragments after a weight change (a no-op otherwise)
+// and fills the weight and bias pointers of an inference launch.
+e_magik_result_types surrogate_infer_args(mgk_surrogate* s, InferArgs& a, hipStream_t st);
 }  // namespace mgk
 
 // Internal (not exported): trace rays whose spin is stored per ray (component

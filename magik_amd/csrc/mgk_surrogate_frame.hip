@@ -2,10 +2,10 @@
 // frame is answered by the MLP unless a fallback rule sends it to the RKF45
 // integrator; a held-out judge sample is traced as well and compared.
 //
-//   k_sur_features : pixel + sample -> screen point (alpha, beta) of the lens
-//                    point's camera frame -> normalised features (n x 8:
-//                    the 5 features, zero padded)
-//   k_mlp_infer    : (mgk_mlp.hip) features -> outputs (n x 8)
+//   k_mlp_infer_res: (mgk_mlp_infer.hpp) pixel + sample -> screen point
+//                    (alpha, beta) of the lens point's camera frame ->
+//                    normalised features, computed in the kernel -> network
+//                    outputs (n x 8; outputs 6 and 7 carry alpha and beta)
 //   k_sur_decode   : outputs -> class / disk radius / escape direction, the
 //                    fallback rules, (fallback.analytic) the Mino-time check
 //                    of the class and refinement of the disk radius
@@ -33,6 +33,7 @@
 #include "mgk_internal.hpp"
 #include "mgk_mino.hpp"
 #include "mgk_mlp.hpp"
+#include "mgk_mlp_infer.hpp"
 #include "mgk_surrogate.hpp"
 
 using namespace mgk;
@@ -64,32 +65,52 @@ struct FeatureParams {
     uint64_t seed, n;
     float sensor_width_mm, focal_mm;
     double a, r_lens, th_lens_deg;
-    float* X;
 };
 
-__global__ void __launch_bounds__(256) k_sur_features(FeatureParams P) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= P.n) return;
-    const uint32_t npix = P.W * P.H;
-    const uint32_t pixel = uint32_t(i % npix), sample = uint32_t(i / npix);
-    float u[4];
-    path_uniforms(P.seed, P.camera_id, pixel, sample, 0, u);
-    double sx, sy;
-    sensor_point<double>(P.W, P.H, P.sensor_width_mm, pixel % P.W, pixel / P.W, u[0], u[1], sx, sy);
-    // camera-local direction (-sx, -sy, -f): alpha = r d_x / (-d_z)
-    const double f_m = double(P.focal_mm) / 1000.0;
-    const double al = P.r_lens * (-sx) / f_m, be = P.r_lens * (-sy) / f_m;
-    float x[kMlpInPad];
-    normalise_features(P.R, P.a, P.r_lens, P.th_lens_deg, al, be, x);
-    float4* dst = reinterpret_cast<float4*>(P.X + i * kFrameXStride);
-    for (int k = 0; k < kFrameXStride / 4; ++k) dst[k] = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
-}
+// The input rows of the inference kernel, computed in it: lane l of a wave
+// computes the screen point of path ray0 + l from its pixel and Philox
+// samples (the path's own ray-generation samples), and the lanes of each
+// 32-ray tile gather theirs by cross-lane reads.  Spin, r and theta are the
+// camera's (the same normalised values for every path).  The normalised
+// alpha and beta also leave the kernel as outputs 6 and 7 of the row.
+struct InferFromCamera {
+    FeatureParams P;
+    float x0, x1, x2;
+    static constexpr bool kExtra = true;
+    __device__ __forceinline__ void load(int64_t ray0, int64_t n, int lane, bf16x8 (&xb)[kInferTiles],
+                                         float (&extra)[kInferTiles][2]) const {
+        const uint64_t i = uint64_t(ray0) + uint64_t(lane);
+        float al_n = 0.0f, be_n = 0.0f;
+        if (int64_t(i) < n) {
+            const uint32_t npix = P.W * P.H;
+            const uint32_t pixel = uint32_t(i % npix), sample = uint32_t(i / npix);
+            float u[4];
+            path_uniforms(P.seed, P.camera_id, pixel, sample, 0, u);
+            double sx, sy;
+            sensor_point<double>(P.W, P.H, P.sensor_width_mm, pixel % P.W, pixel / P.W, u[0], u[1], sx, sy);
+            // camera-local direction (-sx, -sy, -f): alpha = r d_x / (-d_z)
+            const double f_m = double(P.focal_mm) / 1000.0;
+            al_n = float(P.r_lens * (-sx) / f_m / P.R.s_max);
+            be_n = float(P.r_lens * (-sy) / f_m / P.R.s_max);
+        }
+        const int g = lane >> 5, r = lane & 31;
+#pragma unroll
+        for (int c = 0; c < kInferTiles; ++c) {
+            const float a3 = __shfl(al_n, 32 * c + r), a4 = __shfl(be_n, 32 * c + r);
+            const bool lo = g == 0;
+            xb[c][0] = __bf16(lo ? x0 : 0.0f); xb[c][1] = __bf16(lo ? x1 : 0.0f); xb[c][2] = __bf16(lo ? x2 : 0.0f);
+            xb[c][3] = __bf16(lo ? a3 : 0.0f); xb[c][4] = __bf16(lo ? a4 : 0.0f);
+            xb[c][5] = xb[c][6] = xb[c][7] = __bf16(0.0f);
+            extra[c][0] = a3;
+            extra[c][1] = a4;
+        }
+    }
+};
 
 struct DecodeParams {
     SurrogateRanges R;
     uint64_t n;
-    const float* X;
-    const float* Y;
+    const float* Y;              // n x 8 network outputs (6 and 7: the normalised alpha, beta)
     const double* rays;          // 10 x n doubles (q, p, h0, u.p)
     mgk_hit_t* hits;
     uint32_t* queue;
@@ -154,9 +175,7 @@ __device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, mgk
     const float p0 = e0 * inv, p1 = e1 * inv, p2 = e2 * inv;
     const int cls = (p0 >= p1 && p0 >= p2) ? 0 : (p1 >= p2 ? 1 : 2);
     const float conf = fmaxf(p0, fmaxf(p1, p2));
-    const float2 ab = *reinterpret_cast<const float2*>(P.X + i * kFrameXStride + 2);   // x[2], x[3]
-    const float be_n = P.X[i * kFrameXStride + 4];
-    const float al = ab.y * float(P.R.s_max), be = be_n * float(P.R.s_max);
+    const float al = y1.z * float(P.R.s_max), be = y1.w * float(P.R.s_max);   // outputs 6, 7: alpha, beta normalised
     if (fabsf(al) > float(P.R.s_max) || fabsf(be) > float(P.R.s_max)) {
         fallback = true; reason = C_DOMAIN;
     } else if (conf < P.min_conf) {
@@ -528,7 +547,6 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     const uint64_t n = uint64_t(cam->x_resolution) * cam->y_resolution * spp;
     const uint64_t n_slots = fb->judge_stride ? (n + fb->judge_stride - 1) / fb->judge_stride : 0;
     HIP_TRY(s->fr_rays.reserve(n * 10 * sizeof(double)));
-    HIP_TRY(s->fr_x.reserve(n * kFrameXStride * sizeof(float)));
     HIP_TRY(s->fr_y.reserve(n * kMlpOut * sizeof(float)));
     if (!d_hits) HIP_TRY(s->fr_hits.reserve(n * sizeof(mgk_hit_t)));
     mgk_hit_t* hits = d_hits ? d_hits : s->fr_hits.as<mgk_hit_t>();
@@ -555,23 +573,31 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     if ((r = mgk_generate_rays(ctx, integ, cam, seed, 0, spp, s->fr_rays.ptr, n, st)) != MAGIK_SUCCESS) return r;
     HIP_TRY(hipEventRecord(s->ev[1], st));
 
-    FeatureParams F;
+    InferFromCamera src;
+    FeatureParams& F = src.P;
     F.R = R;
     F.W = cam->x_resolution; F.H = cam->y_resolution; F.camera_id = cam->camera_id; F.spp = spp;
     F.seed = seed; F.n = n;
     F.sensor_width_mm = cam->sensor_width_mm; F.focal_mm = cam->focal_length_mm;
     F.a = a; F.r_lens = r_lens; F.th_lens_deg = th_deg;
-    F.X = s->fr_x.as<float>();
-    const unsigned blocks = unsigned((n + 255) / 256);
-    hipLaunchKernelGGL(k_sur_features, dim3(blocks), dim3(256), 0, st, F);
+    {
+        float x[kMlpInPad];
+        normalise_features(R, a, r_lens, th_deg, 0.0, 0.0, x);
+        src.x0 = x[0]; src.x1 = x[1]; src.x2 = x[2];
+    }
+    InferArgs ia;
+    if ((r = surrogate_infer_args(s, ia, st)) != MAGIK_SUCCESS) return r;
+    ia.X = nullptr;
+    ia.x_stride = 0;
+    ia.n = int64_t(n);
+    ia.Y = s->fr_y.as<float>();
+    hipLaunchKernelGGL(k_mlp_infer_res<InferFromCamera>, dim3(infer_grid(ia.n)), dim3(kInferBlock), 0, st, ia, src);
     HIP_TRY(hipGetLastError());
-    if ((r = surrogate_forward(s, s->fr_x.as<float>(), kFrameXStride, n, s->fr_y.as<float>(), st)) != MAGIK_SUCCESS) return r;
 
     DecodeParams D;
     std::memset(&D, 0, sizeof(D));
     D.R = R;
     D.n = n;
-    D.X = s->fr_x.as<const float>();
     D.Y = s->fr_y.as<const float>();
     D.rays = s->fr_rays.as<const double>();
     D.hits = hits;
