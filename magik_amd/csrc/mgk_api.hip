@@ -1284,6 +1284,21 @@ e_magik_result_types mgk::render_samples(mgk_context_handle c, const mgk_integra
     return MAGIK_SUCCESS;
 }
 
+e_magik_result_types mgk_camera_constants(mgk_context_handle c, const mgk_integrator_t* in, const mgk_camera_t* cam, uint64_t seed,
+                                          uint32_t spp, RayGenParams* params, const IcCamera** d_ic, void* stream) {
+    if (!c || !cam || !params || !d_ic) return MAGIK_ERROR_INVALID_POINTER;
+    e_magik_result_types r = check_integrator(in);
+    if (r != MAGIK_SUCCESS) return r;
+    if (in->precision == MGK_PRECISION_FP32_REFERENCE) return MGK_ERROR_INVALID_ARGUMENT;
+    if ((r = set_device(c)) != MAGIK_SUCCESS) return r;
+    const uint64_t n = uint64_t(cam->x_resolution) * cam->y_resolution * spp;
+    if ((r = fill_raygen(in, cam, seed, 0, spp, nullptr, n, *params)) != MAGIK_SUCCESS) return r;
+    HIP_TRY(c->ic_camera.reserve(sizeof(IcCamera)));
+    HIP_TRY(launch_ic_camera(*params, reinterpret_cast<IcCamera*>(c->ic_camera.ptr), hipStream_t(stream)));
+    *d_ic = reinterpret_cast<const IcCamera*>(c->ic_camera.ptr);
+    return MAGIK_SUCCESS;
+}
+
 e_magik_result_types mgk_star_band_view(mgk_context_handle c, const mgk_scene_t* sc, float margin, StarView* out) {
     if (!c || !sc || !out) return MAGIK_ERROR_INVALID_POINTER;
     std::memset(out, 0, sizeof(*out));

@@ -201,6 +201,7 @@ hipError_t launch_display(const disp::Params& p, const float* xyz, uint64_t n, f
 
 // d_ic: device space for the per-camera constants (one IcCamera per camera)
 hipError_t launch_generate_rays(const RayGenParams& p, IcCamera* d_ic, hipStream_t s);
+hipError_t launch_ic_camera(const RayGenParams& p, IcCamera* d_ic, hipStream_t s);
 hipError_t launch_generate_rays_batch(const RayGenParams* d_cams, const uint64_t* d_offsets, uint32_t n_cam, uint64_t n, IcCamera* d_ic,
                                       hipStream_t s);
 hipError_t launch_trace(const TraceParams& p, int grid, hipStream_t s);
@@ -260,3 +261,11 @@ e_magik_result_types build_meshes(const mgk_mesh_t* meshes, uint32_t n_mesh, con
 // widened to star_radius_rad + margin (the surrogate's star-edge test); the
 // grid stays valid until the next call on the context.
 e_magik_result_types mgk_star_band_view(mgk_context_handle ctx, const mgk_scene_t* scene, float margin, mgk::StarView* out);
+
+// Internal (not exported): the FP64 per-camera constants of mgk_generate_rays
+// (FP64 and mixed precision) computed into the context, without generating the
+// rays: *params and *d_ic then give any path's initial conditions through
+// ray_ic_fp64 (mgk_raygen.hpp), bit for bit those mgk_generate_rays writes.
+// d_ic stays valid until the next ray generation on the context.
+e_magik_result_types mgk_camera_constants(mgk_context_handle ctx, const mgk_integrator_t* in, const mgk_camera_t* cam, uint64_t seed,
+                                          uint32_t spp, mgk::RayGenParams* params, const mgk::IcCamera** d_ic, void* stream);

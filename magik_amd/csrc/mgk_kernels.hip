@@ -22,6 +22,7 @@
 #include "mgk_device.hpp"
 #include "mgk_internal.hpp"
 #include "mgk_mesh.hpp"
+#include "mgk_raygen.hpp"
 
 namespace mgk {
 
@@ -545,6 +546,10 @@ __device__ __forceinline__ bool disk_point(const St<S>& yc, S dx, S dy, S dz, S 
     return !(r < r_in || r > r_max);
 }
 
+// Factors on the disk's radii that widen the chord's crossing test when the
+// crossing is located on the geodesic (MGK_SCENE_DISK_ON_GEODESIC).
+constexpr float kDiskChordInner = 0.7f, kDiskChordOuter = 1.02f;
+
 // A refine functor that does nothing: scenes without the on-geodesic disk.
 struct NoDiskRefine {
     template <class S> __device__ bool operator()(S, S&, S&, S&) const { return true; }
@@ -571,9 +576,17 @@ __device__ __forceinline__ uint32_t classify(const St<S>& y0, const St<S>& y1, u
             S rho2 = x * x + z * z - a * a;
             if (rho2 > S(0)) {
                 S r = sqrt_<false>(rho2);
-                if (!(r < ln.disk_r_in || r > S(P.disk_r_out)) && t < best) {
+                // with the crossing located on the geodesic, the chord's radius
+                // only selects the steps to refine: it may lie off the disk by
+                // the chord's O(h^2) error while the geodesic's lies on it
+                // (steps near the inner edge at high spin are ~20 % of r), so
+                // the test is widened and the geodesic point decides
+                const bool geo = METRIC != MINKOWSKI && (P.flags & MGK_SCENE_DISK_ON_GEODESIC);
+                const S lo = geo ? ln.disk_r_in * S(kDiskChordInner) : ln.disk_r_in;
+                const S hi = geo ? S(P.disk_r_out) * S(kDiskChordOuter) : S(P.disk_r_out);
+                if (!(r < lo || r > hi) && t < best) {
                     bool keep = true;
-                    if (METRIC != MINKOWSKI && (P.flags & MGK_SCENE_DISK_ON_GEODESIC)) {
+                    if (geo) {
                         keep = refine(t, r, disk_x, disk_z);
                         disk_geo = keep;
                     }
@@ -1269,35 +1282,15 @@ __global__ void __launch_bounds__(64) k_ic_cameras(const RayGenParams* cams, uin
     if (c < n_cam) ic_camera_any(cams[c], out[c]);
 }
 
-// The per-ray part: sensor point and direction (device_sensor_utilities.cuh:35-80,
-// thin_lens.cuh:60-76), then p = M l and u_obs . p = w . l.
+// The per-ray part (ray_ic_fp64, mgk_raygen.hpp), stored SoA.
 __device__ __forceinline__ void gen_one_fp64(const RayGenParams& P, const IcCamera& C, uint64_t i, uint64_t o) {
-#pragma clang fp contract(off)
-    using S = double;
-    const uint32_t npix = P.W * P.H;
-    const uint32_t pixel = uint32_t(i % npix);
-    const uint32_t sample = P.sample_begin + uint32_t(i / npix);
-    const uint32_t px = pixel % P.W, py = pixel / P.W;
-    float u[4];
-    path_uniforms(P.seed, P.camera_id, pixel, sample, 0, u);
-    S sx, sy;
-    sensor_point<S>(P.W, P.H, P.sensor_width_mm, px, py, u[0], u[1], sx, sy);
-    const S f_m = S(P.focal_mm) / S(1000);
-    S dx = S(0) - sx, dy = S(0) - sy, dz = -f_m - S(0);
-    const S dl = sqrt_<false>(dx * dx + dy * dy + dz * dz);
-    dx = dx / dl; dy = dy / dl; dz = dz / dl;
-    const S L = S(P.length_scale);
-    const S lx = dx / L, ly = dy / L, lz = dz / L;
-    const S lt = sqrt_<false>((lx * lx + ly * ly + lz * lz) - S(0));
-    S* out = reinterpret_cast<S*>(P.rays);
+    double y[10];
+    ray_ic_fp64(P, C, i, y);
+    double* out = reinterpret_cast<double*>(P.rays);
     const uint64_t st = P.stride;
-    out[0 * st + o] = C.q[0]; out[1 * st + o] = C.q[1]; out[2 * st + o] = C.q[2]; out[3 * st + o] = C.q[3];
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu)
-        out[(4 + mu) * st + o] = ((C.M[mu][0] * lx + C.M[mu][1] * ly) + C.M[mu][2] * lz) + C.M[mu][3] * lt;
-    out[8 * st + o] = P.metric == MINKOWSKI ? S(1000) : S(1);
-    out[9 * st + o] = ((C.w[0] * lx + C.w[1] * ly) + C.w[2] * lz) + C.w[3] * lt;
-    if (P.per_ray_spin) out[10 * st + o] = S(P.spin);   // the spin of the equations, per ray (batched cameras)
+    for (int k = 0; k < 10; ++k) out[k * st + o] = y[k];
+    if (P.per_ray_spin) out[10 * st + o] = double(P.spin);   // the spin of the equations, per ray (batched cameras)
 }
 
 __global__ void __launch_bounds__(256) k_generate_rays_fp64(RayGenParams P, const IcCamera* __restrict__ ic) {
@@ -1730,6 +1723,11 @@ template <template <int> class K, class Fn> static hipError_t dispatch_metric(in
     }
 }
 template <int> struct tag_t {};
+
+hipError_t launch_ic_camera(const RayGenParams& p, IcCamera* d_ic, hipStream_t s) {
+    hipLaunchKernelGGL(k_ic_camera, dim3(1), dim3(64), 0, s, p, d_ic);
+    return hipGetLastError();
+}
 
 hipError_t launch_generate_rays(const RayGenParams& p, IcCamera* d_ic, hipStream_t s) {
     const uint64_t n = uint64_t(p.W) * p.H * p.spp;

@@ -32,6 +32,7 @@
 #include "mgk_device.hpp"
 #include "mgk_internal.hpp"
 #include "mgk_mino.hpp"
+#include "mgk_raygen.hpp"
 #include "mgk_mlp.hpp"
 #include "mgk_mlp_infer.hpp"
 #include "mgk_surrogate.hpp"
@@ -111,7 +112,9 @@ struct DecodeParams {
     SurrogateRanges R;
     uint64_t n;
     const float* Y;              // n x 8 network outputs (6 and 7: the normalised alpha, beta)
-    const double* rays;          // 10 x n doubles (q, p, h0, u.p)
+    RayGenParams G;              // the camera's ray generation, and its FP64 constants:
+    const IcCamera* ic;          // a path's initial conditions are formed in registers
+    double* rays;                // 10 x n doubles (q, p, h0, u.p), written for the paths to trace or resolve
     mgk_hit_t* hits;
     uint32_t* queue;
     uint32_t* rqueue;            // paths left to the resolve pass (k_sur_resolve)
@@ -150,19 +153,19 @@ constexpr int kDecodeChunk = 16 * kDecodeBlock;
 
 // A disk hit at radius r: the redshift of the emitter on its circular
 // (Keplerian) orbit seen along the path.
-__device__ __forceinline__ void disk_hit(const DecodeParams& P, uint64_t i, double r, double con, double pt, mgk_hit_t& h) {
+// From the path's conserved p_t and p_phi = z p_x - x p_z and its u.p at the camera.
+__device__ __forceinline__ void disk_hit(const DecodeParams& P, double pt, double pphi, double con, double r, mgk_hit_t& h) {
     double ut, w;
     kepler_ut_w<KERR, double>(r, P.a_eq, ut, w);
-    const double qx = P.rays[0 * P.n + i], qz = P.rays[2 * P.n + i];
-    const double px = P.rays[4 * P.n + i], pz = P.rays[6 * P.n + i];
-    double g = con / (ut * pt + w * (qz * px - qx * pz));
+    double g = con / (ut * pt + w * pphi);
     if (isnan(g) || isinf(g)) g = 1e-10;
     h = mgk_hit_t{MGK_HIT_DISK, float(g), float(r), 0.0f};
 }
 
 enum DecodeStatus : int { kAnswered = 0, kTrace = 1, kDefer = 2 };
 
-__device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, mgk_hit_t& h, int& reason) {
+// y: the path's initial conditions (x, y, z, t, p_x, p_y, p_z, p_t, h0, u.p).
+__device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, const double (&y)[10], mgk_hit_t& h, int& reason) {
     bool fallback = false;
     reason = -1;
     h = mgk_hit_t{MGK_HIT_NONE, 0.0f, 0.0f, 0.0f};
@@ -192,7 +195,7 @@ __device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, mgk
             fallback = true; reason = C_CRIT;
         }
     }
-    const double con = P.rays[9 * P.n + i], pt = P.rays[7 * P.n + i];
+    const double con = y[9], pt = y[7], pphi = y[2] * y[4] - y[0] * y[6];
     double r = exp(P.R.lnr_mid + double(y0.z) * P.R.lnr_half);
     if (fallback) {
         // the network is not trusted here (outside its domain, unsure, or near
@@ -208,8 +211,8 @@ __device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, mgk
     if (P.analytic) {
         // the Mino-time integrals must confirm the network's class; a disk
         // path's radius becomes the one they give, seeded by the network's
-        const double q[3] = {P.rays[0 * P.n + i], P.rays[1 * P.n + i], P.rays[2 * P.n + i]};
-        const double p[4] = {P.rays[4 * P.n + i], P.rays[5 * P.n + i], P.rays[6 * P.n + i], pt};
+        const double q[3] = {y[0], y[1], y[2]};
+        const double p[4] = {y[4], y[5], y[6], pt};
         const MinoRay m = mino_ray(q, p, P.a_eq);
         const MinoResult mr = mino_disk(m, P.r_horizon, P.disk_r_in, P.disk_r_out, cls == 1 ? r : -1.0);
         const int want = cls == 0 ? kMinoCaptured : (cls == 1 ? kMinoDisk : kMinoEscapes);
@@ -225,7 +228,7 @@ __device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, mgk
     if (cls == 0) {
         h.cls = MGK_HIT_HORIZON;
     } else if (cls == 1) {
-        disk_hit(P, i, r, con, pt, h);
+        disk_hit(P, pt, pphi, con, r, h);
     } else {
         const float c = fminf(1.0f, fmaxf(-1.0f, y0.w));
         const float ph = atan2f(y1.x, y1.y);
@@ -252,15 +255,19 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t i, uint32_t lane
     if (flag) buf[off + __popcll(m & lt_mask)] = i;
 }
 
-__global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
+__global__ void __launch_bounds__(kDecodeBlock, 2) k_sur_decode(DecodeParams P) {
     __shared__ uint32_t qbuf[kDecodeChunk], rbuf[kDecodeChunk];
     __shared__ uint32_t n_local[2];
     __shared__ uint32_t reasons[C_REASON_LAST + 1];
     __shared__ unsigned long long base_s[2];
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    __shared__ IcCamera C;   // the camera's FP64 constants, read from LDS by every path
     if (threadIdx.x < 2) n_local[threadIdx.x] = 0;
     if (threadIdx.x <= C_REASON_LAST) reasons[threadIdx.x] = 0;
+    static_assert(sizeof(IcCamera) % sizeof(double) == 0 && sizeof(IcCamera) / sizeof(double) <= kDecodeBlock, "IcCamera copy");
+    if (threadIdx.x < sizeof(IcCamera) / sizeof(double))
+        reinterpret_cast<double*>(&C)[threadIdx.x] = reinterpret_cast<const double*>(P.ic)[threadIdx.x];
     __syncthreads();
     const uint64_t begin = uint64_t(blockIdx.x) * kDecodeChunk;
 #pragma unroll 1
@@ -271,7 +278,11 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
         int reason = -1;
         mgk_hit_t h;
         if (live) {
-            st = decode_one(P, i, h, reason);
+            {
+                double y[10];
+                ray_ic_fp64(P.G, C, i, y);
+                st = decode_one(P, i, y, h, reason);
+            }
             if (P.judge_stride && (i % P.judge_stride) == 0) {
                 // judge sample: the answer recorded, the path traced as well
                 // (a deferred one's answer is recorded by the resolve pass)
@@ -282,6 +293,14 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
             }
             if (st == kAnswered) P.hits[i] = h;
             if (st == kDefer) P.reason[i] = uint8_t(reason);
+            if (st != kAnswered) {
+                // traced or resolved: its initial conditions go to HBM (formed
+                // again, so that they need not stay live through the integrals)
+                double y[10];
+                ray_ic_fp64(P.G, C, i, y);
+#pragma unroll
+                for (int k = 0; k < 10; ++k) P.rays[k * P.n + i] = y[k];
+            }
         }
         // a deferred path's reason counts only if the resolve pass traces it
         for (int c = C_CONF; c <= C_REASON_LAST; ++c) {
@@ -323,14 +342,16 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_resolve(DecodeParams P) {
         uint32_t i = 0;
         if (k < nr) {
             i = P.rqueue[k];
-            const double pt = P.rays[7 * P.n + i], con = P.rays[9 * P.n + i];
+            double y[10];
+#pragma unroll
+            for (int c = 0; c < 10; ++c) y[c] = P.rays[c * P.n + i];
             const double r = exp(P.R.lnr_mid + double(P.Y[uint64_t(i) * kMlpOut + 2]) * P.R.lnr_half);
-            const double q[3] = {P.rays[0 * P.n + i], P.rays[1 * P.n + i], P.rays[2 * P.n + i]};
-            const double p[4] = {P.rays[4 * P.n + i], P.rays[5 * P.n + i], P.rays[6 * P.n + i], pt};
+            const double q[3] = {y[0], y[1], y[2]};
+            const double p[4] = {y[4], y[5], y[6], y[7]};
             const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.r_horizon, P.disk_r_in, P.disk_r_out, r);
             mgk_hit_t h{MGK_HIT_HORIZON, 0.0f, 0.0f, 0.0f};
             const bool ok = mr.kind == kMinoCaptured || mr.kind == kMinoDisk;
-            if (mr.kind == kMinoDisk) disk_hit(P, i, mr.r, con, pt, h);
+            if (mr.kind == kMinoDisk) disk_hit(P, y[7], y[2] * y[4] - y[0] * y[6], y[9], mr.r, h);
             const bool judge = P.judge_stride && (i % P.judge_stride) == 0;
             if (ok && judge) P.pred[i / P.judge_stride] = h;
             if (ok && !judge) P.hits[i] = h;
@@ -570,7 +591,10 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     HIP_TRY(hipEventRecord(s->ev[0], st));
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, st, cnt, int(C_N));
     HIP_TRY(hipGetLastError());
-    if ((r = mgk_generate_rays(ctx, integ, cam, seed, 0, spp, s->fr_rays.ptr, n, st)) != MAGIK_SUCCESS) return r;
+    RayGenParams G;
+    const IcCamera* d_ic = nullptr;
+    if ((r = mgk_camera_constants(ctx, integ, cam, seed, spp, &G, &d_ic, st)) != MAGIK_SUCCESS) return r;
+    G.rays = s->fr_rays.ptr;
     HIP_TRY(hipEventRecord(s->ev[1], st));
 
     InferFromCamera src;
@@ -599,7 +623,9 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.R = R;
     D.n = n;
     D.Y = s->fr_y.as<const float>();
-    D.rays = s->fr_rays.as<const double>();
+    D.G = G;
+    D.ic = d_ic;
+    D.rays = s->fr_rays.as<double>();
     D.hits = hits;
     D.queue = s->fr_queue.as<uint32_t>();
     D.rqueue = s->fr_rqueue.as<uint32_t>();

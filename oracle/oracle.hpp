@@ -997,9 +997,14 @@ inline Hit<R> trace_ray(const RayIC<R>& ic, const MetricParams<R>& p, const Scen
         R t, rr = 0, r_hit = 0;
         V3<R> x_disk{};
         bool disk_geo = false;
-        if (sc.disk && disk_hit(&t, &rr, y0.q, y1.q, p.metric == kKerr ? p.spin : R(0), sc.disk_r_in, sc.disk_r_out)) {
+        // with the crossing on the geodesic the chord's radius only selects the
+        // steps to refine, over radii widened by the chord's error (factors
+        // 0.7 and 1.02 on the disk's radii); the geodesic point decides
+        const bool geo = sc.disk_on_geodesic && p.metric != kMinkowski;
+        const R chord_in = geo ? sc.disk_r_in * R(0.7f) : sc.disk_r_in, chord_out = geo ? sc.disk_r_out * R(1.02f) : sc.disk_r_out;
+        if (sc.disk && disk_hit(&t, &rr, y0.q, y1.q, p.metric == kKerr ? p.spin : R(0), chord_in, chord_out)) {
             bool keep = t < best;
-            if (keep && sc.disk_on_geodesic && p.metric != kMinkowski) {
+            if (keep && geo) {
                 auto f = [&](const State<R>& s) { return rhs(s, p); };
                 keep = disk_on_geodesic(f, y0, t, h_taken, p.metric == kKerr ? p.spin : R(0), sc.disk_r_in, sc.disk_r_out, &rr, &x_disk);
                 disk_geo = keep;
