@@ -40,6 +40,19 @@ constexpr int kInferBlock = 512;                             // 8 waves: two per
 constexpr int kInferTiles = 2;                               // 32-ray tiles per wave
 constexpr int kInferChunk = 32 * kInferTiles;                // rays per wave and iteration
 
+// k-slot (g, j) of k-step ks of a B operand formed from a layer's
+// accumulators: with the C layout of v_mfma_f32_32x32x16_bf16 (lane l holds
+// column l % 32, rows 8 (i / 4) + 4 (l / 32) + i % 4 of element i), k-step ks
+// takes elements i = 8 (ks % 2) + j of row tile ks / 2, which are the rows
+// perm_k(ks, l / 32, j).  The A fragments of the next product are packed in
+// this order, so accumulators become B operands without moving between lanes.
+__host__ __device__ constexpr int perm_k(int ks, int g, int j) {
+    return 32 * (ks >> 1) + 16 * (ks & 1) + 8 * (j >> 2) + 4 * g + (j & 3);
+}
+
+// Row of accumulator element i in lane group g (the C layout above).
+__host__ __device__ constexpr int acc_row(int g, int i) { return 8 * (i >> 2) + 4 * g + (i & 3); }
+
 // Offset (in fragments) of layer l's A fragments in the LDS image.
 __host__ __device__ constexpr int frag_base(int l) { return l == 0 ? 0 : (l < kMlpLayers - 1 ? kFragL0 + (l - 1) * kFragHidden : kFragL0 + 3 * kFragHidden); }
 

@@ -29,8 +29,8 @@ using namespace mgk;
 
 namespace {
 
-const int kK[kMlpLayers] = {kMlpInPad, kMlpHidden, kMlpHidden, kMlpHidden, kMlpHidden};
-const int kN[kMlpLayers] = {kMlpHidden, kMlpHidden, kMlpHidden, kMlpHidden, kMlpOutPad};
+const int* const kK = kLayerK;
+const int* const kN = kLayerN;
 
 template <class T> hipError_t alloc(T** p, size_t n) { return hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)); }
 
@@ -78,99 +78,73 @@ __global__ void __launch_bounds__(256) k_targets(const mgk_hit_t* hits, uint64_t
 namespace {
 
 void release_scratch(mgk_surrogate* s) {
-    for (int l = 0; l < kMlpLayers; ++l) if (s->H[l]) (void)hipFree(s->H[l]), s->H[l] = nullptr;
-    for (int l = 0; l < kMlpLayers - 1; ++l) if (s->Z[l]) (void)hipFree(s->Z[l]), s->Z[l] = nullptr;
-    for (float** p : {&s->Y, &s->dY, &s->DZ}) if (*p) (void)hipFree(*p), *p = nullptr;
-    for (int k = 0; k < 2; ++k) if (s->G[k]) (void)hipFree(s->G[k]), s->G[k] = nullptr;
-    s->cap = 0;
+    for (int l = 0; l < kMlpLayers; ++l) {
+        if (s->HT[l]) (void)hipFree(s->HT[l]), s->HT[l] = nullptr;
+        if (s->GT[l]) (void)hipFree(s->GT[l]), s->GT[l] = nullptr;
+    }
+    if (s->part) (void)hipFree(s->part), s->part = nullptr;
+    for (int l = 0; l < kMlpLayers; ++l)
+        if (s->gWp[l]) (void)hipFree(s->gWp[l]), s->gWp[l] = nullptr;
+    s->cap_pad = 0;
 }
 
 e_magik_result_types ensure_scratch(mgk_surrogate* s, int n) {
-    if (n <= s->cap) return MAGIK_SUCCESS;
+    const int np = train_n_pad(n);
+    if (np <= s->cap_pad) return MAGIK_SUCCESS;
     release_scratch(s);
-    size_t N = size_t(n);
-    HIP_TRY(alloc(&s->H[0], N * kMlpInPad));
-    for (int l = 1; l < kMlpLayers; ++l) HIP_TRY(alloc(&s->H[l], N * kMlpHidden));
-    for (int l = 0; l < kMlpLayers - 1; ++l) HIP_TRY(alloc(&s->Z[l], N * kMlpHidden));
-    HIP_TRY(alloc(&s->Y, N * kMlpOutPad));
-    HIP_TRY(alloc(&s->dY, N * kMlpOutPad));
-    HIP_TRY(alloc(&s->DZ, N * kMlpHidden));
-    HIP_TRY(alloc(&s->G[0], N * kMlpHidden));
-    HIP_TRY(alloc(&s->G[1], N * kMlpHidden));
-    s->cap = n;
-    return MAGIK_SUCCESS;
-}
-
-GemmArgs gemm_args() {
-    GemmArgs g;
-    std::memset(&g, 0, sizeof(g));
-    g.k_split = 1;
-    return g;
-}
-
-// Training forward: h_0 = bf16(x); z_l, h_l per layer; y.
-e_magik_result_types train_forward(mgk_surrogate* s, const float* X, int n, hipStream_t st) {
-    HIP_TRY(launch_to_bf16(X, s->H[0], int64_t(n) * kMlpInPad, st));
     for (int l = 0; l < kMlpLayers; ++l) {
-        GemmArgs g = gemm_args();
-        g.A = s->H[l]; g.a_bf16 = 1; g.sam = kK[l]; g.sak = 1;
-        g.B = s->Wb[l]; g.b_bf16 = 1; g.sbk = kN[l]; g.sbn = 1;
-        g.M = n; g.N = kN[l]; g.K = kK[l];
-        g.bias = s->b[l];
-        if (l < kMlpLayers - 1) {
-            g.epi = EPI_RELU;
-            g.C = s->Z[l]; g.ldc = kN[l];
-            g.H = s->H[l + 1]; g.ldh = kN[l];
-        } else {
-            g.epi = EPI_STORE;
-            g.C = s->Y; g.ldc = kMlpOutPad;
-        }
-        HIP_TRY(launch_gemm(g, st));
+        HIP_TRY(alloc(&s->HT[l], size_t(kK[l]) * np));
+        HIP_TRY(alloc(&s->GT[l], size_t(kN[l]) * np));
     }
+    HIP_TRY(alloc(&s->part, size_t(np / 32) * kPartStride));
+    for (int l = 0; l < kMlpLayers; ++l) HIP_TRY(alloc(&s->gWp[l], size_t(kDwSplits) * kK[l] * kN[l]));
+    s->cap_pad = np;
     return MAGIK_SUCCESS;
 }
 
-// Backward from dY (n x 32) into gW, gb.
-e_magik_result_types train_backward(mgk_surrogate* s, int n, hipStream_t st) {
+AdamArgs adam_args(mgk_surrogate* s) {
+    AdamArgs a;
+    std::memset(&a, 0, sizeof(a));
     for (int l = 0; l < kMlpLayers; ++l) {
-        HIP_TRY(launch_zero(s->gW[l], int64_t(kK[l]) * kN[l], st));
-        HIP_TRY(launch_zero(s->gb[l], kN[l], st));
+        a.W[l] = s->W[l]; a.b[l] = s->b[l];
+        a.mW[l] = s->mW[l]; a.vW[l] = s->vW[l]; a.mb[l] = s->mb[l]; a.vb[l] = s->vb[l];
+        a.gWp[l] = s->gWp[l]; a.gW[l] = s->gW[l]; a.gb[l] = s->gb[l];
+        a.Wf[l] = s->Wf[l]; a.Wg[l] = s->Wg[l];
     }
-    // g_5 = bf16(dY)
-    HIP_TRY(launch_to_bf16(s->dY, s->G[0], int64_t(n) * kMlpOutPad, st));
-    const float* dz = s->dY;   // FP32 gradient of the current layer's pre-activation
-    int cur = 0;
-    const int ksplit = n >= 4096 ? 16 : 1;
-    for (int l = kMlpLayers - 1; l >= 0; --l) {
-        // dW_l = h_l^T g_l   (M = K_l inputs, N = N_l outputs, K = batch)
-        GemmArgs g = gemm_args();
-        g.A = s->H[l]; g.a_bf16 = 1; g.sam = 1; g.sak = kK[l];
-        g.B = s->G[cur]; g.b_bf16 = 1; g.sbk = kN[l]; g.sbn = 1;
-        g.M = kK[l]; g.N = kN[l]; g.K = n;
-        g.k_split = ksplit;
-        g.epi = EPI_ATOMIC;
-        g.C = s->gW[l]; g.ldc = kN[l];
-        HIP_TRY(launch_gemm(g, st));
-        HIP_TRY(launch_colsum(dz, n, kN[l], s->gb[l], st));
-        if (l == 0) break;
-        // dz_{l-1} = (g_l W_l^T) * relu'(z_{l-1});  g_{l-1} = bf16(dz_{l-1})
-        GemmArgs d = gemm_args();
-        d.A = s->G[cur]; d.a_bf16 = 1; d.sam = kN[l]; d.sak = 1;
-        d.B = s->Wb[l]; d.b_bf16 = 1; d.sbk = 1; d.sbn = kN[l];
-        d.M = n; d.N = kK[l]; d.K = kN[l];
-        d.epi = EPI_DRELU;
-        d.Z = s->Z[l - 1]; d.ldz = kK[l];
-        d.C = s->DZ; d.ldc = kK[l];
-        d.H = s->G[1 - cur]; d.ldh = kK[l];
-        HIP_TRY(launch_gemm(d, st));
-        dz = s->DZ;
-        cur = 1 - cur;
+    return a;
+}
+
+// Forward, loss (targets T) or a given dL/dy (dY), backward and the weight
+// and bias gradients of one batch, into gW and gb; the loss sums into sums.
+e_magik_result_types train_gradients(mgk_surrogate* s, const float* X, const float* T, const float* dY, float* Y, int n, float w_r,
+                                     float w_d, hipStream_t st) {
+    TrainArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.X = X; a.T = T; a.dY = dY; a.Y = Y;
+    a.n = n;
+    a.n_pad = train_n_pad(n);
+    a.w_r = w_r; a.w_d = w_d;
+    DwArgs d;
+    std::memset(&d, 0, sizeof(d));
+    d.n_pad = a.n_pad;
+    for (int l = 0; l < kMlpLayers; ++l) {
+        a.Wf[l] = s->Wf[l]; a.Wg[l] = s->Wg[l]; a.b[l] = s->b[l];
+        a.HT[l] = s->HT[l]; a.GT[l] = s->GT[l];
+        d.HT[l] = s->HT[l]; d.GT[l] = s->GT[l]; d.gWp[l] = s->gWp[l]; d.gb[l] = s->gb[l];
     }
+    a.part = s->part;
+    d.part = s->part;
+    d.sums = s->sums;
+    HIP_TRY(launch_train_fb(a, st));
+    HIP_TRY(launch_train_dw(d, st));
     return MAGIK_SUCCESS;
 }
 
+// The bf16 forward and backward fragments of the current master weights.
 e_magik_result_types refresh_fragments(mgk_surrogate* s, hipStream_t st) {
-    for (int l = 0; l < kMlpLayers; ++l) HIP_TRY(launch_repack(s->W[l], kK[l], kN[l], s->Wf[l], l > 0 ? 1 : 0, st));
+    AdamArgs a = adam_args(s);
+    a.update = 0;
+    HIP_TRY(launch_train_adam(a, 0, st));
     s->frag_dirty = false;
     return MAGIK_SUCCESS;
 }
@@ -200,7 +174,7 @@ MGK_API e_magik_result_types mgk_surrogate_create(int device, uint64_t seed, con
     };
     for (int l = 0; l < kMlpLayers; ++l) {
         size_t nw = size_t(kK[l]) * kN[l];
-        if (alloc(&s->W[l], nw) || alloc(&s->mW[l], nw) || alloc(&s->vW[l], nw) || alloc(&s->gW[l], nw) || alloc(&s->Wb[l], nw) ||
+        if (alloc(&s->W[l], nw) || alloc(&s->mW[l], nw) || alloc(&s->vW[l], nw) || alloc(&s->gW[l], nw) || alloc(&s->Wg[l], nw) ||
             alloc(&s->Wf[l], nw) || alloc(&s->b[l], size_t(kN[l])) || alloc(&s->mb[l], size_t(kN[l])) ||
             alloc(&s->vb[l], size_t(kN[l])) || alloc(&s->gb[l], size_t(kN[l])))
             return fail(MAGIK_ERROR_DEVICE_ALLOCATION_FAILED);
@@ -228,7 +202,7 @@ MGK_API e_magik_result_types mgk_surrogate_create(int device, uint64_t seed, con
             hipMemcpy(s->b[l], z.data(), size_t(N) * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(s->mb[l], z.data(), size_t(N) * 4, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(s->vb[l], z.data(), size_t(N) * 4, hipMemcpyHostToDevice) != hipSuccess ||
-            launch_to_bf16(s->W[l], s->Wb[l], int64_t(K) * N, nullptr) != hipSuccess)
+            hipMemcpy(s->gb[l], z.data(), size_t(N) * 4, hipMemcpyHostToDevice) != hipSuccess)
             return fail(MGK_ERROR_HIP);
     }
     if (refresh_fragments(s, nullptr) != MAGIK_SUCCESS || hipDeviceSynchronize() != hipSuccess) return fail(MGK_ERROR_HIP);
@@ -243,7 +217,7 @@ MGK_API e_magik_result_types mgk_surrogate_destroy(mgk_surrogate_handle s) {
     release_scratch(s);
     for (int l = 0; l < kMlpLayers; ++l)
         for (void* p : {(void*)s->W[l], (void*)s->b[l], (void*)s->mW[l], (void*)s->vW[l], (void*)s->mb[l], (void*)s->vb[l],
-                        (void*)s->gW[l], (void*)s->gb[l], (void*)s->Wb[l], (void*)s->Wf[l]})
+                        (void*)s->gW[l], (void*)s->gb[l], (void*)s->Wg[l], (void*)s->Wf[l]})
             if (p) (void)hipFree(p);
     if (s->sums) (void)hipFree(s->sums);
     for (hipEvent_t e : s->ev)
@@ -284,7 +258,6 @@ MGK_API e_magik_result_types mgk_surrogate_set_weights(mgk_surrogate_handle s, c
     for (int l = 0; l < kMlpLayers; ++l) {
         HIP_TRY(hipMemcpy(s->W[l], W + ow, size_t(kK[l]) * kN[l] * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(s->b[l], b + ob, size_t(kN[l]) * 4, hipMemcpyHostToDevice));
-        HIP_TRY(launch_to_bf16(s->W[l], s->Wb[l], int64_t(kK[l]) * kN[l], nullptr));
         ow += size_t(kK[l]) * kN[l];
         ob += kN[l];
     }
@@ -338,10 +311,11 @@ MGK_API e_magik_result_types mgk_surrogate_gradients(mgk_surrogate_handle s, con
     HIP_TRY(hipSetDevice(s->device));
     e_magik_result_types r = ensure_scratch(s, int(n));
     if (r != MAGIK_SUCCESS) return r;
-    if ((r = train_forward(s, d_x, int(n), nullptr)) != MAGIK_SUCCESS) return r;
-    if (d_y) HIP_TRY(hipMemcpy(d_y, s->Y, size_t(n) * kMlpOutPad * 4, hipMemcpyDeviceToDevice));
-    HIP_TRY(hipMemcpy(s->dY, d_dy, size_t(n) * kMlpOutPad * 4, hipMemcpyDeviceToDevice));
-    if ((r = train_backward(s, int(n), nullptr)) != MAGIK_SUCCESS) return r;
+    if (s->frag_dirty && (r = refresh_fragments(s, nullptr)) != MAGIK_SUCCESS) return r;
+    if ((r = train_gradients(s, d_x, nullptr, d_dy, d_y, int(n), 0.0f, 0.0f, nullptr)) != MAGIK_SUCCESS) return r;
+    AdamArgs a = adam_args(s);   // sums the weight gradients' partials (the weights stay)
+    a.update = 0;
+    HIP_TRY(launch_train_adam(a, 0, nullptr));
     HIP_TRY(hipDeviceSynchronize());
     size_t ow = 0, ob = 0;
     for (int l = 0; l < kMlpLayers; ++l) {
@@ -361,16 +335,13 @@ MGK_API e_magik_result_types mgk_surrogate_train_step(mgk_surrogate_handle s, co
     e_magik_result_types r = ensure_scratch(s, int(n));
     if (r != MAGIK_SUCCESS) return r;
     hipStream_t st = nullptr;
-    if ((r = train_forward(s, d_x, int(n), st)) != MAGIK_SUCCESS) return r;
-    HIP_TRY(launch_zero(s->sums, 4, st));
-    HIP_TRY(launch_loss(s->Y, d_t, int(n), w_r, w_d, s->dY, s->sums, st));
-    if ((r = train_backward(s, int(n), st)) != MAGIK_SUCCESS) return r;
+    if (s->frag_dirty && (r = refresh_fragments(s, st)) != MAGIK_SUCCESS) return r;
+    if ((r = train_gradients(s, d_x, d_t, nullptr, nullptr, int(n), w_r, w_d, st)) != MAGIK_SUCCESS) return r;
     s->t += 1;
-    for (int l = 0; l < kMlpLayers; ++l) {
-        HIP_TRY(launch_adam(s->W[l], s->mW[l], s->vW[l], s->gW[l], int64_t(kK[l]) * kN[l], lr, s->t, s->Wb[l], st));
-        HIP_TRY(launch_adam(s->b[l], s->mb[l], s->vb[l], s->gb[l], kN[l], lr, s->t, nullptr, st));
-    }
-    s->frag_dirty = true;
+    AdamArgs a = adam_args(s);
+    a.lr = lr;
+    a.update = 1;
+    HIP_TRY(launch_train_adam(a, s->t, st));   // also writes the fragments of the new weights
     if (losses) {
         float h[4];
         HIP_TRY(hipMemcpy(h, s->sums, sizeof(h), hipMemcpyDeviceToHost));

@@ -119,18 +119,17 @@ struct mgk_surrogate {
     float* vb[mgk::kMlpLayers] = {};
     float* gW[mgk::kMlpLayers] = {};
     float* gb[mgk::kMlpLayers] = {};
-    __bf16* Wb[mgk::kMlpLayers] = {};
-    __bf16* Wf[mgk::kMlpLayers] = {};
+    __bf16* Wf[mgk::kMlpLayers] = {};   // forward A fragments (inference and training)
+    __bf16* Wg[mgk::kMlpLayers] = {};   // backward A fragments (training; layers 1..4)
     bool frag_dirty = true;
     int t = 0;   // Adam step
-    // training scratch
-    int cap = 0;
-    __bf16* H[mgk::kMlpLayers] = {};   // h_0 .. h_4 (bf16)
-    float* Z[mgk::kMlpLayers - 1] = {};
-    float* Y = nullptr;
-    float* dY = nullptr;
-    float* DZ = nullptr;
-    __bf16* G[2] = {};
+    // training scratch: the layer inputs h_l and output gradients g_l of a
+    // batch, feature-major with rows of cap_pad rays
+    int cap_pad = 0;
+    __bf16* HT[mgk::kMlpLayers] = {};
+    __bf16* GT[mgk::kMlpLayers] = {};
+    float* part = nullptr;   // per 32-ray tile: partial bias-gradient and loss sums
+    float* gWp[mgk::kMlpLayers] = {};   // partial weight gradients of the batch splits
     float* sums = nullptr;
     mgk::SurrogateRanges ranges{};
     // render-path buffers
