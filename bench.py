@@ -49,7 +49,7 @@ FLOP_PER_INFERENCE = 101120
 # HBM bytes moved by the integration of one Kerr 1080p x 32 frame (all zone
 # passes): rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors being what
 # the same counters read for a copy kernel of known size (bench.py --copy-probe).
-TRACE_TRAFFIC_BYTES = {"kerr_disk_1080p": (2821880 + 1263842) * 2.0 * 1024 + (3551106 + 545575) * 1024.0}
+TRACE_TRAFFIC_BYTES = {"kerr_disk_1080p": (2975543 + 1247368) * 2.0 * 1024 + (4245581 + 542684) * 1024.0}
 
 
 def log(msg):
