@@ -7,11 +7,15 @@
 //                    normalised features, computed in the kernel -> network
 //                    outputs (n x 8; outputs 6 and 7 carry alpha and beta)
 //   k_sur_decode   : outputs -> class / disk radius / escape direction, the
-//                    fallback rules, (fallback.analytic) the Mino-time check
-//                    of the class and refinement of the disk radius
-//                    (mgk_mino.hpp), the hit record of answered paths, and
+//                    fallback rules, the hit record of answered paths, and
 //                    the compacted queue of paths to trace (one atomic per
 //                    wave, ballot ranks within it)
+//   k_sur_mino     : (fallback.analytic) the Mino-time check of the class of
+//                    every confident path and the refinement of a disk
+//                    path's radius (mgk_mino.hpp), in a kernel of its own so
+//                    that its registers allow four waves per SIMD
+//   k_sur_resolve  : (fallback.analytic) the same integrals for the unsure
+//                    paths the decode deferred
 //   mgk_trace_rays : the queue, written back into the same hit slots
 //   k_sur_judge    : surrogate vs RKF45 on the judge sample
 //   mgk_shade      : the unchanged spectral shading of every hit
@@ -119,6 +123,8 @@ struct DecodeParams {
     uint32_t* queue;
     uint32_t* rqueue;            // paths left to the resolve pass (k_sur_resolve)
     uint8_t* reason;             // their fallback reasons
+    float* cand;                 // (fallback.analytic) per path, the class for k_sur_mino: the network's
+                                 // disk radius (> 0), -1 captured, -2 escape, 0 none
     mgk_hit_t* pred;             // judge predictions (one per judge slot)
     unsigned long long* cnt;
     const float* crit;           // kCritN screen radii of the critical curve (< 0: none)
@@ -162,10 +168,17 @@ __device__ __forceinline__ void disk_hit(const DecodeParams& P, double pt, doubl
     h = mgk_hit_t{MGK_HIT_DISK, float(g), float(r), 0.0f};
 }
 
-enum DecodeStatus : int { kAnswered = 0, kTrace = 1, kDefer = 2 };
+enum DecodeStatus : int { kAnswered = 0, kTrace = 1, kDefer = 2, kMino = 3 };
+constexpr float kCandCaptured = -1.0f, kCandEscape = -2.0f;
 
-// y: the path's initial conditions (x, y, z, t, p_x, p_y, p_z, p_t, h0, u.p).
-__device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, const double (&y)[10], mgk_hit_t& h, int& reason) {
+// The answer of path i from its network outputs: kAnswered (in h), kTrace or
+// kDefer (the fallback reason in reason), or kMino: confident, the class to be
+// confirmed by k_sur_mino, in cand; h then holds a captured or escaping path's
+// answer.  C: the camera's FP64 constants, from which the path's initial
+// conditions y = (x, y, z, t, p_x, p_y, p_z, p_t, h0, u.p) are formed where an
+// answer needs them.
+__device__ __forceinline__ int decode_one(const DecodeParams& P, const IcCamera& C, uint64_t i, mgk_hit_t& h, int& reason,
+                                          float& cand) {
     bool fallback = false;
     reason = -1;
     h = mgk_hit_t{MGK_HIT_NONE, 0.0f, 0.0f, 0.0f};
@@ -195,8 +208,7 @@ __device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, con
             fallback = true; reason = C_CRIT;
         }
     }
-    const double con = y[9], pt = y[7], pphi = y[2] * y[4] - y[0] * y[6];
-    double r = exp(P.R.lnr_mid + double(y0.z) * P.R.lnr_half);
+    const double r = exp(P.R.lnr_mid + double(y0.z) * P.R.lnr_half);
     if (fallback) {
         // the network is not trusted here (outside its domain, unsure, or near
         // the critical curve): traced, or with the Mino-time integrals on,
@@ -208,27 +220,23 @@ __device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, con
         reason = C_DISK;
         return kTrace;
     }
-    if (P.analytic) {
-        // the Mino-time integrals must confirm the network's class; a disk
-        // path's radius becomes the one they give, seeded by the network's
-        const double q[3] = {y[0], y[1], y[2]};
-        const double p[4] = {y[4], y[5], y[6], pt};
-        const MinoRay m = mino_ray(q, p, P.a_eq);
-        const MinoResult mr = mino_disk(m, P.r_horizon, P.disk_r_in, P.disk_r_out, cls == 1 ? r : -1.0);
-        const int want = cls == 0 ? kMinoCaptured : (cls == 1 ? kMinoDisk : kMinoEscapes);
-        if (mr.kind != want) {
-            reason = C_ANALYTIC;
-            return kTrace;
-        }
-        if (cls == 1) r = mr.r;
-    } else if (cls == 1 && (r < P.disk_r_in * (1.0 + P.edge_margin) || r > P.disk_r_out * (1.0 - P.edge_margin))) {
+    // with the Mino-time integrals on, k_sur_mino must confirm the network's
+    // class, and a disk path's radius becomes the one they give, seeded by
+    // the network's
+    if (P.analytic && cls == 1) {
+        cand = float(r);
+        return kMino;
+    }
+    if (!P.analytic && cls == 1 && (r < P.disk_r_in * (1.0 + P.edge_margin) || r > P.disk_r_out * (1.0 - P.edge_margin))) {
         reason = C_DISK;
         return kTrace;
     }
     if (cls == 0) {
         h.cls = MGK_HIT_HORIZON;
     } else if (cls == 1) {
-        disk_hit(P, pt, pphi, con, r, h);
+        double y[10];
+        ray_ic_fp64(P.G, C, i, y);
+        disk_hit(P, y[7], y[2] * y[4] - y[0] * y[6], y[9], r, h);
     } else {
         const float c = fminf(1.0f, fmaxf(-1.0f, y0.w));
         const float ph = atan2f(y1.x, y1.y);
@@ -239,9 +247,15 @@ __device__ __forceinline__ int decode_one(const DecodeParams& P, uint64_t i, con
             reason = C_STAR;
             return kTrace;
         }
-        double g = con / pt;
+        double y[10];
+        ray_ic_fp64(P.G, C, i, y);
+        double g = y[9] / y[7];
         if (isnan(g) || isinf(g)) g = 1e-10;
         h = mgk_hit_t{MGK_HIT_ESCAPE, float(g), c, ph};
+    }
+    if (P.analytic) {
+        cand = cls == 0 ? kCandCaptured : kCandEscape;
+        return kMino;
     }
     return kAnswered;
 }
@@ -257,22 +271,18 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t i, uint32_t lane
     if (flag) buf[off + __popcll(m & lt_mask)] = i;
 }
 
-// Two waves per SIMD (at most 256 registers, a few spilled): with the Mino-time
-// integrals inline the kernel takes 282 registers, and one wave per SIMD made
-// the analytic frame 3 ms slower.
-__global__ void __launch_bounds__(kDecodeBlock, 2) k_sur_decode(DecodeParams P) {
+__global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
     __shared__ uint32_t qbuf[kDecodeChunk], rbuf[kDecodeChunk];
     __shared__ uint32_t n_local[2];
     __shared__ uint32_t reasons[C_REASON_LAST + 1];
     __shared__ unsigned long long base_s[2];
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    __shared__ IcCamera C;   // the camera's FP64 constants, read from LDS by every path
+    // the camera's FP64 constants: uniform and read before any store, so
+    // scalar loads into SGPRs (a copy in LDS took 40 more vector registers)
+    const IcCamera C = *P.ic;
     if (threadIdx.x < 2) n_local[threadIdx.x] = 0;
     if (threadIdx.x <= C_REASON_LAST) reasons[threadIdx.x] = 0;
-    static_assert(sizeof(IcCamera) % sizeof(double) == 0 && sizeof(IcCamera) / sizeof(double) <= kDecodeBlock, "IcCamera copy");
-    if (threadIdx.x < sizeof(IcCamera) / sizeof(double))
-        reinterpret_cast<double*>(&C)[threadIdx.x] = reinterpret_cast<const double*>(P.ic)[threadIdx.x];
     __syncthreads();
     const uint64_t begin = uint64_t(blockIdx.x) * kDecodeChunk;
 #pragma unroll 1
@@ -283,24 +293,28 @@ __global__ void __launch_bounds__(kDecodeBlock, 2) k_sur_decode(DecodeParams P) 
         int reason = -1;
         mgk_hit_t h;
         if (live) {
-            {
-                double y[10];
-                ray_ic_fp64(P.G, C, i, y);
-                st = decode_one(P, i, y, h, reason);
-            }
-            if (P.judge_stride && (i % P.judge_stride) == 0) {
+            float cand = 0.0f;
+            st = decode_one(P, C, i, h, reason, cand);
+            const bool judge = P.judge_stride && (i % P.judge_stride) == 0;
+            if (judge && st == kMino) {
+                // judge sample left to k_sur_mino, which records a disk path's
+                // answer and queues the path; a captured or escaping path's
+                // answer is recorded here
+                if (cand < 0.0f) P.pred[i / P.judge_stride] = h;
+            } else if (judge) {
                 // judge sample: the answer recorded, the path traced as well
                 // (a deferred one's answer is recorded by the resolve pass)
                 mgk_hit_t pr = h;
                 if (st != kAnswered) pr.cls = kNotAnswered;
                 P.pred[i / P.judge_stride] = pr;
                 if (st == kAnswered) st = kTrace;
+            } else if (st == kAnswered || (st == kMino && cand < 0.0f)) {
+                P.hits[i] = h;   // a disk path's is written by k_sur_mino
             }
-            if (st == kAnswered) P.hits[i] = h;
+            if (P.cand) P.cand[i] = cand;
             if (st == kDefer) P.reason[i] = uint8_t(reason);
-            if (st != kAnswered) {
-                // traced or resolved: its initial conditions go to HBM (formed
-                // again, so that they need not stay live through the integrals)
+            if (st == kTrace || st == kDefer) {
+                // traced or resolved: its initial conditions go to HBM
                 double y[10];
                 ray_ic_fp64(P.G, C, i, y);
 #pragma unroll
@@ -324,6 +338,59 @@ __global__ void __launch_bounds__(kDecodeBlock, 2) k_sur_decode(DecodeParams P) 
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < n_q; k += kDecodeBlock) P.queue[base_s[0] + k] = qbuf[k];
     for (uint32_t k = threadIdx.x; k < n_r; k += kDecodeBlock) P.rqueue[base_s[1] + k] = rbuf[k];
+}
+
+// The Mino-time check of the confident paths (fallback.analytic; inside the
+// decode the integrals' registers held it to two waves per SIMD, with spills):
+// a path whose class the integrals confirm keeps the decode's answer, a disk
+// path's with the radius they give; the others are traced (reason
+// C_ANALYTIC).  A judge path is traced as well, its answer recorded in pred.
+// One path per thread: a loop over several kept loop-invariant constants in
+// vector registers across it (256 of them, one wave per SIMD); this way the
+// kernel takes 115, four waves per SIMD.
+__global__ void __launch_bounds__(kDecodeBlock) k_sur_mino(DecodeParams P) {
+    const IcCamera C = *P.ic;   // uniform, read before any store: scalar loads into SGPRs
+    const uint32_t lane = __lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t i = uint64_t(blockIdx.x) * kDecodeBlock + threadIdx.x;
+    const float c = i < P.n ? P.cand[i] : 0.0f;
+    bool fail = false, trace = false;
+    if (c != 0.0f) {
+        double y[10];
+        ray_ic_fp64(P.G, C, i, y);
+        const double q[3] = {y[0], y[1], y[2]};
+        const double p[4] = {y[4], y[5], y[6], y[7]};
+        const double pt = y[7], pphi = y[2] * y[4] - y[0] * y[6], con = y[9];
+        const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.r_horizon, P.disk_r_in, P.disk_r_out, c > 0.0f ? double(c) : -1.0);
+        const int want = c > 0.0f ? kMinoDisk : (c == kCandCaptured ? kMinoCaptured : kMinoEscapes);
+        fail = mr.kind != want;
+        const bool judge = P.judge_stride && (i % P.judge_stride) == 0;
+        if (!fail && c > 0.0f) {
+            mgk_hit_t h;
+            disk_hit(P, pt, pphi, con, mr.r, h);
+            if (judge) P.pred[i / P.judge_stride] = h;
+            else P.hits[i] = h;
+        }
+        if (fail && judge) P.pred[i / P.judge_stride] = mgk_hit_t{kNotAnswered, 0.0f, 0.0f, 0.0f};
+        trace = fail || judge;
+        if (trace) {
+            // its initial conditions go to HBM (formed again, so that they
+            // need not stay live through the integrals)
+            double z[10];
+            ray_ic_fp64(P.G, C, i, z);
+#pragma unroll
+            for (int k = 0; k < 10; ++k) P.rays[k * P.n + i] = z[k];
+        }
+    }
+    const uint64_t mf = __ballot(fail);
+    if (mf != 0 && lane == 0) atomicAdd(&P.cnt[C_ANALYTIC], (unsigned long long)__popcll(mf));
+    const uint64_t m = __ballot(trace);
+    if (m != 0) {
+        unsigned long long off = 0;
+        if (lane == 0) off = atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)__popcll(m));
+        off = __shfl(off, 0);
+        if (trace) P.queue[off + __popcll(m & lt_mask)] = uint32_t(i);
+    }
 }
 
 // The resolve pass over the paths the decode deferred (the network not
@@ -581,6 +648,7 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
         HIP_TRY(s->fr_rqueue.reserve(n * sizeof(uint32_t)));
         HIP_TRY(s->fr_reason.reserve(n));
     }
+    if (fb->analytic) HIP_TRY(s->fr_cand.reserve(n * sizeof(float)));
     HIP_TRY(s->fr_pred.reserve(std::max<uint64_t>(1, n_slots) * sizeof(mgk_hit_t)));
     HIP_TRY(s->fr_counters.reserve(C_N * sizeof(unsigned long long)));
     if (n > 0xFFFFFFFFull) return MGK_ERROR_CAPACITY_EXCEEDED;
@@ -635,6 +703,7 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.queue = s->fr_queue.as<uint32_t>();
     D.rqueue = s->fr_rqueue.as<uint32_t>();
     D.reason = s->fr_reason.as<uint8_t>();
+    D.cand = fb->analytic ? s->fr_cand.as<float>() : nullptr;
     D.pred = s->fr_pred.as<mgk_hit_t>();
     D.cnt = cnt;
     D.crit = s->crit.as<const float>();
@@ -659,6 +728,10 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.analytic = fb->analytic;
     hipLaunchKernelGGL(k_sur_decode, dim3(unsigned((n + kDecodeChunk - 1) / kDecodeChunk)), dim3(kDecodeBlock), 0, st, D);
     HIP_TRY(hipGetLastError());
+    if (D.analytic) {
+        hipLaunchKernelGGL(k_sur_mino, dim3(unsigned((n + kDecodeBlock - 1) / kDecodeBlock)), dim3(kDecodeBlock), 0, st, D);
+        HIP_TRY(hipGetLastError());
+    }
     if (D.analytic && !D.trace_disk) {
         int n_cu = 0;
         HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));

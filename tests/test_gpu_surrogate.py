@@ -378,7 +378,7 @@ def test_surrogate_analytic_frame_matches_fp64_oracle(gpu_ctx, trained):
     the Mino-time check and radius refinement the surrogate frame's linear
     radiance is within 1e-3 of the FP64 oracle's per pixel (the network alone
     misses that bar by ~30x: its radii are good to ~1 %).  The radii of
-    answered disk paths are exact to the quadrature (~1e-8); the few larger
+    answered disk paths are exact to the quadrature (~5e-8); the few larger
     differences (99.9th percentile) are paths traced by RKF45 next to the
     critical curve, where the mixed and FP64 integrations part."""
     s, _ = trained
