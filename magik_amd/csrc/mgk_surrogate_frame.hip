@@ -112,12 +112,26 @@ struct InferFromCamera {
     }
 };
 
+// Per-camera constants of the Mino-time check's own initial conditions (FP64,
+// from the host): the sensor mapping of sensor_point and the direction
+// normalisation of ray_ic_fp64 with the camera's divisions done once.
+struct MinoIc {
+    double inv_w, inv_h;   // 1 / W, 1 / H
+    double aspect;         // H / W on the vertical offset (wide) or W / H on the horizontal one
+    double side;           // jitter square side (mm)
+    double sw_m;           // sensor width / 1000
+    double f_m;            // focal length (m)
+    double inv_l;          // 1 / L
+    uint32_t wide, npix, W, sample_begin;
+};
+
 struct DecodeParams {
     SurrogateRanges R;
     uint64_t n;
     const float* Y;              // n x 8 network outputs (6 and 7: the normalised alpha, beta)
     RayGenParams G;              // the camera's ray generation, and its FP64 constants:
     const IcCamera* ic;          // a path's initial conditions are formed in registers
+    MinoIc mi;
     double* rays;                // 10 x n doubles (q, p, h0, u.p), written for the paths to trace or resolve
     mgk_hit_t* hits;
     uint32_t* queue;
@@ -340,6 +354,33 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
     for (uint32_t k = threadIdx.x; k < n_r; k += kDecodeBlock) P.rqueue[base_s[1] + k] = rbuf[k];
 }
 
+// A path's covariant momentum p and u_obs . p for the Mino-time check: the
+// steps of ray_ic_fp64 (sensor point, normalised direction, l = d / (|d| L),
+// p = M l) with the camera's divisions hoisted into mi, reciprocals for the
+// path's own, contraction on and 32-bit index arithmetic (a frame holds fewer
+// than 2^32 paths).  Within a few ulp of ray_ic_fp64's, which the traced
+// paths keep: the integrals need nothing closer.
+__device__ __forceinline__ void mino_ic(const DecodeParams& P, const IcCamera& C, uint32_t i, double (&p)[4], double& con) {
+    const MinoIc& F = P.mi;
+    const uint32_t pixel = i % F.npix, sample = F.sample_begin + i / F.npix;
+    const uint32_t px = pixel % F.W, py = pixel / F.W;
+    float u[4];
+    path_uniforms(P.G.seed, P.G.camera_id, pixel, sample, 0, u);
+    double a = double(px) * F.inv_w - 0.5, b = double(py) * F.inv_h - 0.5;
+    if (F.wide) b *= F.aspect;
+    else a *= F.aspect;
+    const double sx = a * F.sw_m + (double(u[0]) - 0.5) * F.side;
+    const double sy = b * F.sw_m + (double(u[1]) - 0.5) * F.side;
+    const double d2 = sx * sx + sy * sy + F.f_m * F.f_m;
+    double y = __builtin_amdgcn_rsq(d2);
+    y = y * fma(-0.5 * d2 * y, y, 1.5);   // 1 / |d| to ~1e-15
+    const double k = -y * F.inv_l;
+    const double l[4] = {sx * k, sy * k, F.f_m * k, F.inv_l};
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) p[mu] = C.M[mu][0] * l[0] + C.M[mu][1] * l[1] + C.M[mu][2] * l[2] + C.M[mu][3] * l[3];
+    con = C.w[0] * l[0] + C.w[1] * l[1] + C.w[2] * l[2] + C.w[3] * l[3];
+}
+
 // The Mino-time check of the confident paths (fallback.analytic; inside the
 // decode the integrals' registers held it to two waves per SIMD, with spills):
 // a path whose class the integrals confirm keeps the decode's answer, a disk
@@ -352,15 +393,14 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_mino(DecodeParams P) {
     const IcCamera C = *P.ic;   // uniform, read before any store: scalar loads into SGPRs
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint64_t i = uint64_t(blockIdx.x) * kDecodeBlock + threadIdx.x;
+    const uint32_t i = blockIdx.x * kDecodeBlock + threadIdx.x;
     const float c = i < P.n ? P.cand[i] : 0.0f;
     bool fail = false, trace = false;
     if (c != 0.0f) {
-        double y[10];
-        ray_ic_fp64(P.G, C, i, y);
-        const double q[3] = {y[0], y[1], y[2]};
-        const double p[4] = {y[4], y[5], y[6], y[7]};
-        const double pt = y[7], pphi = y[2] * y[4] - y[0] * y[6], con = y[9];
+        double p[4], con;
+        mino_ic(P, C, i, p, con);
+        const double q[3] = {C.q[0], C.q[1], C.q[2]};
+        const double pt = p[3], pphi = q[2] * p[0] - q[0] * p[2];
         const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.r_horizon, P.disk_r_in, P.disk_r_out, c > 0.0f ? double(c) : -1.0);
         const int want = c > 0.0f ? kMinoDisk : (c == kCandCaptured ? kMinoCaptured : kMinoEscapes);
         fail = mr.kind != want;
@@ -389,7 +429,7 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_mino(DecodeParams P) {
         unsigned long long off = 0;
         if (lane == 0) off = atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)__popcll(m));
         off = __shfl(off, 0);
-        if (trace) P.queue[off + __popcll(m & lt_mask)] = uint32_t(i);
+        if (trace) P.queue[off + __popcll(m & lt_mask)] = i;
     }
 }
 
@@ -698,6 +738,26 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.Y = s->fr_y.as<const float>();
     D.G = G;
     D.ic = d_ic;
+    {
+        // the per-camera parts of sensor_point and ray_ic_fp64, in FP64
+        MinoIc& F = D.mi;
+        const uint32_t W = cam->x_resolution, H = cam->y_resolution;
+        const double sw = double(cam->sensor_width_mm);
+        const double inv_s = sw / std::sqrt(double(float(W * W + H * H)));
+        const double area = (double(float(W)) * inv_s) * (double(float(H)) * inv_s) / 1e6;
+        F.side = std::sqrt(area / double(float(W * H)));
+        F.inv_w = 1.0 / double(W);
+        F.inv_h = 1.0 / double(H);
+        const double ratio = double(float(H)) / double(float(W));
+        F.wide = ratio < 1.0 ? 1u : 0u;
+        F.aspect = ratio < 1.0 ? ratio : 1.0 / ratio;
+        F.sw_m = sw / 1000.0;
+        F.f_m = double(cam->focal_length_mm) / 1000.0;
+        F.inv_l = 1.0 / double(G.length_scale);
+        F.npix = W * H;
+        F.W = W;
+        F.sample_begin = G.sample_begin;
+    }
     D.rays = s->fr_rays.as<double>();
     D.hits = hits;
     D.queue = s->fr_queue.as<uint32_t>();

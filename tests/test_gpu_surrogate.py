@@ -393,6 +393,19 @@ def test_surrogate_analytic_frame_matches_fp64_oracle(gpu_ctx, trained):
     check_analytic_frame(gpu_ctx, s, "kerr_disk_1080p", seed=8)
 
 
+@pytest.mark.parametrize("W,H", [(160, 90), (90, 160)])
+def test_surrogate_analytic_wide_and_tall_frames(gpu_ctx, trained, W, H):
+    """The Mino-time check forms each path's initial conditions itself, with
+    the camera's sensor mapping folded into per-camera constants (both aspect
+    branches of sensor_point): the radii it gives match the FP64 oracle's, on
+    the oracle's own initial conditions, on a wide and on a tall frame, and it
+    confirms the network's class as often as on the square frame."""
+    s, _ = trained
+    o = analytic_frame_vs_fp64_oracle(gpu_ctx, s, "kerr_disk_1080p", W, H, 2, seed=10)
+    assert o["class_agreement"] >= 0.999 and o["rel_r_mean"] < 1e-5 and o["rel_r_q999"] < 1e-4, o
+    assert o["analytic"] < 0.02 and o["y_mean"] <= 1e-3, o
+
+
 def test_surrogate_analytic_spin_099_frame_matches_fp64_oracle(gpu_ctx, trained_099):
     """The learned-surrogate configuration's spin (kerr_disk_4k, a = 0.99) at a
     reduced, square size, against the FP64 oracle per ray and per pixel."""
