@@ -28,6 +28,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 namespace mgk {
 
 constexpr int kMinoMaxCrossings = 4;
@@ -116,7 +118,20 @@ __host__ __device__ inline MinoRay mino_ray(const double q[3], const double p[4]
 
 namespace mino_detail {
 
-// Largest real root of the depressed cubic t^3 + p t + q = 0.
+// cos(acos(c) / 3) in float: on the device acosf and the hardware cosine (the
+// angle lies in [0, pi/3]), ~1e-7.
+__host__ __device__ inline float cos_third_acos(float c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __cosf(acosf(c) * (1.0f / 3.0f));
+#else
+    return std::cos(std::acos(c) * (1.0f / 3.0f));
+#endif
+}
+
+// Largest real root of the depressed cubic t^3 + p t + q = 0.  Three real
+// roots: the trigonometric form in float, then one Newton step on the cubic in
+// FP64 (the error e -> ~e^2; linear only where the two largest roots merge),
+// ~60 instructions in place of ~250 for acos and cos in FP64.
 __host__ __device__ inline double cubic_max_root(double p, double q) {
     const double disc = 0.25 * q * q + p * p * p * (1.0 / 27.0);
     if (disc > 0.0) {
@@ -126,7 +141,10 @@ __host__ __device__ inline double cubic_max_root(double p, double q) {
     if (p >= 0.0) return cbrt(-q);   // p = 0 = disc: a triple root
     const double rp = sqrt(p * (-1.0 / 3.0));
     const double c = fmin(1.0, fmax(-1.0, (1.5 * q) / (p * rp)));
-    return 2.0 * rp * cos(acos(c) * (1.0 / 3.0));
+    double t = 2.0 * rp * double(cos_third_acos(float(c)));
+    const double d = 3.0 * t * t + p;
+    if (d != 0.0) t -= ((t * t + p) * t + q) / d;
+    return t;
 }
 
 // Radial potential in s = 1/r: P(s) = 1 + A s^2 + B s^3 + C s^4.
@@ -276,26 +294,32 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
         double K_inf = -1.0;   // taken when a crossing lies past the turning point
         // the disk's window on each branch, as W = sqrt(s4 - s) ranges over it
         // (inbound: s in [max(s_out, s_obs), min(s_in, s4)]; outbound: s in
-        // [s_out, min(s_in, s4)]); K is increasing in W
-        double Ka_in = 0.0, Kb = 0.0, Ka_out = 0.0;
+        // [s_out, min(s_in, s4)]); K is increasing in W.  Each end is
+        // integrated when a crossing first needs it (K >= 0; -1: not yet).
+        double Ka_in = -1.0, Kb = -1.0, Ka_out = -1.0;
         const double s_lo_in = fmax(s_out, s_obs), s_hi = fmin(s_in, s4);
         const bool win_in = s_lo_in < s_hi, win_out = s_out < s_hi;
-        if (!want_r) {
-            Kb = K(sqrt(s4 - s_hi));
-            Ka_in = win_in ? K(sqrt(s4 - s_lo_in)) : 0.0;
-            Ka_out = win_out ? K(sqrt(s4 - s_out)) : 0.0;
-        }
+        auto kb = [&]() { return Kb >= 0.0 ? Kb : (Kb = K(sqrt(s4 - s_hi))); };
+        auto ka_in = [&]() { return Ka_in >= 0.0 ? Ka_in : (Ka_in = K(sqrt(s4 - s_lo_in))); };
+        auto ka_out = [&]() { return Ka_out >= 0.0 ? Ka_out : (Ka_out = K(sqrt(s4 - s_out))); };
         for (int n = 1; n <= kMinoMaxCrossings; ++n) {
             const double G = G1 + 2.0 * double(n - 1) * G_half;
-            if (G > K_obs) {   // past the turning point: escaped unless before K_obs + K_inf
+            // past the turning point the ray has escaped unless G < K_obs +
+            // K_inf.  With a seed the test waits until the radius solve on the
+            // outbound branch finds no crossing: one it finds lies before the
+            // escape, and most disk paths then never need K_inf.
+            auto escaped = [&]() {
                 if (K_inf < 0.0) K_inf = K(W_inf);
-                if (G >= K_obs + K_inf) {
-                    res.kind = kMinoEscapes;
-                    return res;
-                }
+                return G >= K_obs + K_inf;
+            };
+            if (!want_r && G > K_obs && escaped()) {
+                res.kind = kMinoEscapes;
+                return res;
             }
             if (!want_r) {
-                if ((win_in && G >= K_obs - Ka_in && G <= K_obs - Kb) || (win_out && G >= K_obs + Kb && G <= K_obs + Ka_out)) {
+                const bool hit = G <= K_obs ? win_in && G <= K_obs - kb() && G >= K_obs - ka_in()
+                                            : win_out && G >= K_obs + kb() && G <= K_obs + ka_out();
+                if (hit) {
                     res.kind = kMinoDisk;
                     res.crossing = n;
                     return res;
@@ -309,18 +333,31 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
             double Wlo = 0.0, Whi = inbound ? W_obs : W_inf;
             // a crossing beyond the disk's radial range on either side cannot be
             // the hit: the branch's radii at its ends bound it
-            if (s4 < s_out) continue;                        // the whole branch lies outside r_out
+            if (s4 < s_out) {                                // the whole branch lies outside r_out
+                if (!inbound && escaped()) {
+                    res.kind = kMinoEscapes;
+                    return res;
+                }
+                continue;
+            }
             double W = 0.5 * (Wlo + Whi);
             if (s_seed > 0.0 && s_seed < s4 && s_seed > s4 - Whi * Whi) W = sqrt(s4 - s_seed);
-            bool done = false;
+            bool done = false, newton = false;
             for (int it = 0; it < kMinoMaxNewton && !done; ++it) {
                 const double f = K(W) - target;
                 if (f > 0.0) Whi = W; else Wlo = W;
                 double Wn = W - 0.5 * f * sqrt(Qs(s4 - W * W));
-                if (!(Wn >= Wlo && Wn <= Whi)) Wn = 0.5 * (Wlo + Whi);
+                newton = Wn >= Wlo && Wn <= Whi;
+                if (!newton) Wn = 0.5 * (Wlo + Whi);
                 done = fabs(Wn - W) <= 1e-4 * W + 1e-14;
                 W = Wn;
                 res.iters += 1;
+            }
+            // no crossing found on the outbound branch (bisection steps toward
+            // W_inf, where K stays below the target): the escape test decides
+            if (!inbound && !(done && newton) && escaped()) {
+                res.kind = kMinoEscapes;
+                return res;
             }
             if (!done) return res;
             const double s = s4 - W * W;
@@ -336,7 +373,8 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
     const double J_h = J(s_h);
     const double s_lo = fmax(s_out, s_obs), s_hi = fmin(s_in, s_h);
     const bool win = s_lo < s_hi;
-    const double Ja = (!want_r && win) ? J(s_lo) : 0.0, Jb = (!want_r && win) ? J(s_hi) : 0.0;
+    double Ja = 0.0, Jb = 0.0;   // the disk's window, integrated when a crossing before the horizon needs it
+    bool have_window = false;
     for (int n = 1; n <= kMinoMaxCrossings; ++n) {
         const double G = G1 + 2.0 * double(n - 1) * G_half;
         if (G >= J_h) {
@@ -344,6 +382,11 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
             return res;
         }
         if (!want_r) {
+            if (win && !have_window) {
+                Ja = J(s_lo);
+                Jb = J(s_hi);
+                have_window = true;
+            }
             if (win && G >= Ja && G <= Jb) {
                 res.kind = kMinoDisk;
                 res.crossing = n;
