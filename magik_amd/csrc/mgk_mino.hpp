@@ -78,6 +78,7 @@ struct MinoResult {
 struct MinoRay {
     double lam, eta, a;
     double r, u;     // Boyer-Lindquist r and cos(theta) of the state
+    double s;        // 1 / r
     int sr, su;      // signs of dr/dtau and du/dtau
 };
 
@@ -110,6 +111,7 @@ __host__ __device__ inline MinoRay mino_ray(const double q[3], const double p[4]
     m.eta = Q * (inv_E * inv_E);
     m.a = a;
     m.r = r;
+    m.s = inv_r;
     m.u = u;
     m.sr = rdot < 0.0 ? -1 : 1;
     m.su = udot < 0.0 ? -1 : 1;
@@ -196,14 +198,25 @@ __host__ __device__ inline RootPair radial_pair(const Radial& R) {
 
 }  // namespace mino_detail
 
+// The disk and horizon radii of mino_disk with their reciprocals (per scene;
+// a kernel sets them up once instead of dividing per path).
+struct MinoDisk {
+    double r_horizon, r_in, r_out;
+    double s_h, s_in, s_out;   // 1 / r_horizon, 1 / r_in, 1 / r_out
+};
+__host__ __device__ inline MinoDisk mino_disk_radii(double r_horizon, double r_in, double r_out) {
+    return MinoDisk{r_horizon, r_in, r_out, 1.0 / r_horizon, 1.0 / r_in, 1.0 / r_out};
+}
+
 // Disk crossing (or capture / escape) of the ray: the first crossing of the
 // equator within [r_in, r_out].  r_seed > 0 asks for the crossing's radius and
 // is the estimate the Newton iteration starts from when it lies on the
-// crossing's branch; r_seed <= 0 asks for the class only (res.r stays 0).  r_horizon
-// is the outer horizon radius 1 + sqrt(1 - a^2).  Only rays that start
+// crossing's branch; r_seed <= 0 asks for the class only (res.r stays 0).
+// dk.r_horizon is the outer horizon radius 1 + sqrt(1 - a^2).  Only rays that start
 // inward (sr < 0) and outside the disk plane (u != 0) are resolved.
-__host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_horizon, double r_in, double r_out, double r_seed) {
+__host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, const MinoDisk& dk, double r_seed) {
     using namespace mino_detail;
+    const double r_horizon = dk.r_horizon;
     MinoResult res{kMinoUnresolved, 0, 0, 0.0};
     const double a = ray.a, lam = ray.lam, eta = ray.eta;
     if (ray.sr >= 0 || ray.u == 0.0) return res;
@@ -211,7 +224,7 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
     const double Ap = a * a;                       // U = (u_+ - u^2)(Ap u^2 + Am)
     const double Bq = eta + lam * lam - a * a, Cq = eta;
     const Radial rad{a * a - eta - lam * lam, 2.0 * (eta + (lam - a) * (lam - a)), -a * a * eta};
-    const double s_obs = 1.0 / ray.r, s_h = 1.0 / r_horizon;
+    const double s_obs = ray.s, s_h = dk.s_h;
     const RootPair pr = radial_pair(rad);
     if (!pr.ok) return res;
     const double r4 = pr.r4;
@@ -248,7 +261,7 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
     const double G1 = toward ? G_half - t_obs : G_half + t_obs;
 
     // ---- radial motion
-    const double s_in = 1.0 / r_in, s_out = 1.0 / r_out;
+    const double s_in = dk.s_in, s_out = dk.s_out;
     const double s_seed = r_seed > 0.0 ? 1.0 / r_seed : -1.0;
     double s4 = 0.0;
     double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;   // Q(s) = P(s) / (s4 - s) = -(b3 s^3 + b2 s^2 + b1 s + b0)
@@ -415,6 +428,10 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_hor
         return res;
     }
     return res;
+}
+
+__host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, double r_horizon, double r_in, double r_out, double r_seed) {
+    return mino_disk(ray, mino_disk_radii(r_horizon, r_in, r_out), r_seed);
 }
 
 }  // namespace mgk

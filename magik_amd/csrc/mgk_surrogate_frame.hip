@@ -146,6 +146,7 @@ struct DecodeParams {
     float star_chord2_in, star_chord2_out;
     uint32_t flags;
     double a_eq, disk_r_in, disk_r_out, r_horizon;
+    MinoDisk dk;                 // the same radii with their reciprocals
     float min_conf, crit_margin, edge_margin;
     int star_test;
     uint32_t judge_stride, trace_disk, analytic;
@@ -401,7 +402,7 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_mino(DecodeParams P) {
         mino_ic(P, C, i, p, con);
         const double q[3] = {C.q[0], C.q[1], C.q[2]};
         const double pt = p[3], pphi = q[2] * p[0] - q[0] * p[2];
-        const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.r_horizon, P.disk_r_in, P.disk_r_out, c > 0.0f ? double(c) : -1.0);
+        const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.dk, c > 0.0f ? double(c) : -1.0);
         const int want = c > 0.0f ? kMinoDisk : (c == kCandCaptured ? kMinoCaptured : kMinoEscapes);
         fail = mr.kind != want;
         const bool judge = P.judge_stride && (i % P.judge_stride) == 0;
@@ -460,7 +461,7 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_resolve(DecodeParams P) {
             const double r = exp(P.R.lnr_mid + double(P.Y[uint64_t(i) * kMlpOut + 2]) * P.R.lnr_half);
             const double q[3] = {y[0], y[1], y[2]};
             const double p[4] = {y[4], y[5], y[6], y[7]};
-            const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.r_horizon, P.disk_r_in, P.disk_r_out, r);
+            const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.dk, r);
             mgk_hit_t h{MGK_HIT_HORIZON, 0.0f, 0.0f, 0.0f};
             const bool ok = mr.kind == kMinoCaptured || mr.kind == kMinoDisk;
             if (mr.kind == kMinoDisk) disk_hit(P, y[7], y[2] * y[4] - y[0] * y[6], y[9], mr.r, h);
@@ -780,6 +781,7 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.r_horizon = 1.0 + std::sqrt(std::max(0.0, 1.0 - a * a));
     D.disk_r_in = scene->disk_r_in;
     D.disk_r_out = scene->disk_r_out;
+    D.dk = mino_disk_radii(D.r_horizon, D.disk_r_in, D.disk_r_out);
     D.min_conf = fb->min_confidence;
     D.crit_margin = fb->critical_margin;
     D.edge_margin = fb->disk_edge_margin;
