@@ -70,6 +70,10 @@ struct FeatureParams {
     uint64_t seed, n;
     float sensor_width_mm, focal_mm;
     double a, r_lens, th_lens_deg;
+    // sensor_point's per-camera parts in float, and the screen scale: the
+    // normalised alpha = -sx k_screen, beta = -sy k_screen
+    float inv_w, inv_h, aspect, side, sw_m, k_screen;
+    uint32_t wide;
 };
 
 // The input rows of the inference kernel, computed in it: lane l of a wave
@@ -87,16 +91,20 @@ struct InferFromCamera {
         const uint64_t i = uint64_t(ray0) + uint64_t(lane);
         float al_n = 0.0f, be_n = 0.0f;
         if (int64_t(i) < n) {
-            const uint32_t npix = P.W * P.H;
-            const uint32_t pixel = uint32_t(i % npix), sample = uint32_t(i / npix);
+            // sensor_point in float with the camera's divisions done on the host
+            // (the features enter the network in bf16); 32-bit indices (a frame
+            // holds fewer than 2^32 paths)
+            const uint32_t npix = P.W * P.H, i32 = uint32_t(i);
+            const uint32_t pixel = i32 % npix, sample = i32 / npix;
             float u[4];
             path_uniforms(P.seed, P.camera_id, pixel, sample, 0, u);
-            double sx, sy;
-            sensor_point<double>(P.W, P.H, P.sensor_width_mm, pixel % P.W, pixel / P.W, u[0], u[1], sx, sy);
+            float a = float(pixel % P.W) * P.inv_w - 0.5f, b = float(pixel / P.W) * P.inv_h - 0.5f;
+            if (P.wide) b *= P.aspect;
+            else a *= P.aspect;
+            const float sx = a * P.sw_m + (u[0] - 0.5f) * P.side, sy = b * P.sw_m + (u[1] - 0.5f) * P.side;
             // camera-local direction (-sx, -sy, -f): alpha = r d_x / (-d_z)
-            const double f_m = double(P.focal_mm) / 1000.0;
-            al_n = float(P.r_lens * (-sx) / f_m / P.R.s_max);
-            be_n = float(P.r_lens * (-sy) / f_m / P.R.s_max);
+            al_n = -sx * P.k_screen;
+            be_n = -sy * P.k_screen;
         }
         const int g = lane >> 5, r = lane & 31;
 #pragma unroll
@@ -718,6 +726,20 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     F.seed = seed; F.n = n;
     F.sensor_width_mm = cam->sensor_width_mm; F.focal_mm = cam->focal_length_mm;
     F.a = a; F.r_lens = r_lens; F.th_lens_deg = th_deg;
+    {
+        const uint32_t W = F.W, H = F.H;
+        const double sw = double(cam->sensor_width_mm);
+        const double inv_s = sw / std::sqrt(double(float(W * W + H * H)));
+        const double area = (double(float(W)) * inv_s) * (double(float(H)) * inv_s) / 1e6;
+        const double ratio = double(float(H)) / double(float(W));
+        F.side = float(std::sqrt(area / double(float(W * H))));
+        F.inv_w = float(1.0 / double(W));
+        F.inv_h = float(1.0 / double(H));
+        F.wide = ratio < 1.0 ? 1u : 0u;
+        F.aspect = float(ratio < 1.0 ? ratio : 1.0 / ratio);
+        F.sw_m = float(sw / 1000.0);
+        F.k_screen = float(r_lens / (double(cam->focal_length_mm) / 1000.0) / R.s_max);
+    }
     {
         float x[kMlpInPad];
         normalise_features(R, a, r_lens, th_deg, 0.0, 0.0, x);

@@ -237,12 +237,12 @@ def frame_setup(W=192, H=108):
 
 
 def test_surrogate_all_fallback_equals_plain_render(gpu_ctx, trained):
-    """With every path sent to RKF45 the frame is the plain render, bit for bit
-    but for the rare path that a wave of the mixed-precision trace picks up in
-    mid-run (a refill of idle lanes): its FP64 renormalisation, scheduled on
-    the wave's iterations, then falls on other steps and moves its last bits,
-    more for a path near the critical curve.  Each trace has its own
-    scheduling, so the two frames may differ there."""
+    """With every path sent to RKF45 the frame is the plain render, to the
+    mixed path's run-to-run agreement (FP32 rounding): its FP64
+    renormalisation runs on a wave-uniform schedule, so a ray's renormalised
+    steps depend on the rays that share its wave, which each trace's dynamic
+    work distribution decides (the same bound as two plain renders,
+    test_gpu_parity.py)."""
     import torch
     s, _ = trained
     integ, sc, spec, cam = frame_setup()
@@ -252,10 +252,9 @@ def test_surrogate_all_fallback_equals_plain_render(gpu_ctx, trained):
     xyz, st = M.render_frame_surrogate(gpu_ctx, s, integ, sc, spec, cam, seed=3, spp=2, fb=fb)
     torch.cuda.synchronize()
     assert st["n_fallback"] == st["n_paths"] == 192 * 108 * 2
-    same = (xyz == plain).all(dim=-1).float().mean().item()
-    assert same >= 0.995, same
-    total = (xyz.double().sum() / plain.double().sum() - 1.0).abs().item()
-    assert total < 1e-3, total
+    a, b = xyz.double().cpu(), plain.double().cpu()
+    rel = (a - b).abs() / b.abs().clamp_min(1e-6 * float(b.abs().max()))
+    assert torch.isfinite(a).all() and float((rel > 1e-5).double().mean()) <= 2e-3, float((rel > 1e-5).double().mean())
 
 
 def test_surrogate_judge_and_gain(gpu_ctx, trained):
