@@ -133,7 +133,7 @@ struct mgk_surrogate {
     float* sums = nullptr;
     mgk::SurrogateRanges ranges{};
     // render-path buffers
-    mgk::SBuf fr_rays, fr_y, fr_hits, fr_queue, fr_pred, fr_counters, fr_rqueue, fr_reason, fr_cand;
+    mgk::SBuf fr_rays, fr_y, fr_hits, fr_queue, fr_pred, fr_counters, fr_reason, fr_cand;
     // critical-curve table (screen radius per angle) and its key
     mgk::SBuf crit, crit_lo, crit_hi, crit_ok, crit_rays, crit_hits;
     double crit_key[6] = {-1, -1, -1, -1, -1, -1};

@@ -12,10 +12,9 @@
 //                    wave, ballot ranks within it)
 //   k_sur_mino     : (fallback.analytic) the Mino-time check of the class of
 //                    every confident path and the refinement of a disk
-//                    path's radius (mgk_mino.hpp), in a kernel of its own so
-//                    that its registers allow four waves per SIMD
-//   k_sur_resolve  : (fallback.analytic) the same integrals for the unsure
-//                    paths the decode deferred
+//                    path's radius (mgk_mino.hpp), and the same integrals for
+//                    the unsure paths the decode deferred, in a kernel of its
+//                    own so that its registers allow four waves per SIMD
 //   mgk_trace_rays : the queue, written back into the same hit slots
 //   k_sur_judge    : surrogate vs RKF45 on the judge sample
 //   mgk_shade      : the unchanged spectral shading of every hit
@@ -59,7 +58,7 @@ constexpr uint32_t kNotAnswered = 0xFFFFFFFFu;
 // device counters of one frame (u64 slots; the double sums live in the
 // same buffer, bit-cast)
 enum {
-    C_QUEUE = 0, C_CONF, C_CRIT, C_DISK, C_STAR, C_DOMAIN, C_ANALYTIC, C_JUDGE, C_AGREE, C_JDISK, C_JESC, C_RQUEUE,
+    C_QUEUE = 0, C_CONF, C_CRIT, C_DISK, C_STAR, C_DOMAIN, C_ANALYTIC, C_JUDGE, C_AGREE, C_JDISK, C_JESC,
     C_SUM_R, C_SUM_G, C_SUM_DIR, C_MAX_DIR, C_N
 };
 constexpr int C_REASON_LAST = C_ANALYTIC;   // the fallback reasons are C_CONF .. C_REASON_LAST
@@ -140,13 +139,13 @@ struct DecodeParams {
     RayGenParams G;              // the camera's ray generation, and its FP64 constants:
     const IcCamera* ic;          // a path's initial conditions are formed in registers
     MinoIc mi;
-    double* rays;                // 10 x n doubles (q, p, h0, u.p), written for the paths to trace or resolve
+    double* rays;                // 10 x n doubles (q, p, h0, u.p), written for the paths to trace
     mgk_hit_t* hits;
     uint32_t* queue;
-    uint32_t* rqueue;            // paths left to the resolve pass (k_sur_resolve)
-    uint8_t* reason;             // their fallback reasons
+    uint8_t* reason;             // the fallback reasons of the deferred paths
     float* cand;                 // (fallback.analytic) per path, the class for k_sur_mino: the network's
-                                 // disk radius (> 0), -1 captured, -2 escape, 0 none
+                                 // disk radius (> 0), -1 captured, -2 escape, -10 - the network's radius
+                                 // for a deferred path, 0 none
     mgk_hit_t* pred;             // judge predictions (one per judge slot)
     unsigned long long* cnt;
     const float* crit;           // kCritN screen radii of the critical curve (< 0: none)
@@ -192,7 +191,7 @@ __device__ __forceinline__ void disk_hit(const DecodeParams& P, double pt, doubl
 }
 
 enum DecodeStatus : int { kAnswered = 0, kTrace = 1, kDefer = 2, kMino = 3 };
-constexpr float kCandCaptured = -1.0f, kCandEscape = -2.0f;
+constexpr float kCandCaptured = -1.0f, kCandEscape = -2.0f, kCandDeferred = -10.0f;
 
 // The answer of path i from its network outputs: kAnswered (in h), kTrace or
 // kDefer (the fallback reason in reason), or kMino: confident, the class to be
@@ -235,9 +234,10 @@ __device__ __forceinline__ int decode_one(const DecodeParams& P, const IcCamera&
     if (fallback) {
         // the network is not trusted here (outside its domain, unsure, or near
         // the critical curve): traced, or with the Mino-time integrals on,
-        // left to the resolve pass (k_sur_resolve), which runs them as full
-        // waves (scattered among the others they would hold up their waves)
-        return (!P.analytic || P.trace_disk) ? kTrace : kDefer;
+        // deferred to k_sur_mino, the network's radius as the seed
+        if (!P.analytic || P.trace_disk) return kTrace;
+        cand = kCandDeferred - ((r > 0.0 && r < 1e30) ? float(r) : 1.0f);
+        return kDefer;
     }
     if (cls == 1 && P.trace_disk) {
         reason = C_DISK;
@@ -295,16 +295,16 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t i, uint32_t lane
 }
 
 __global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
-    __shared__ uint32_t qbuf[kDecodeChunk], rbuf[kDecodeChunk];
-    __shared__ uint32_t n_local[2];
+    __shared__ uint32_t qbuf[kDecodeChunk];
+    __shared__ uint32_t n_local[1];
     __shared__ uint32_t reasons[C_REASON_LAST + 1];
-    __shared__ unsigned long long base_s[2];
+    __shared__ unsigned long long base_s[1];
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     // the camera's FP64 constants: uniform and read before any store, so
     // scalar loads into SGPRs (a copy in LDS took 40 more vector registers)
     const IcCamera C = *P.ic;
-    if (threadIdx.x < 2) n_local[threadIdx.x] = 0;
+    if (threadIdx.x < 1) n_local[threadIdx.x] = 0;
     if (threadIdx.x <= C_REASON_LAST) reasons[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t begin = uint64_t(blockIdx.x) * kDecodeChunk;
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
                 if (cand < 0.0f) P.pred[i / P.judge_stride] = h;
             } else if (judge) {
                 // judge sample: the answer recorded, the path traced as well
-                // (a deferred one's answer is recorded by the resolve pass)
+                // (a deferred one's answer is recorded by k_sur_mino)
                 mgk_hit_t pr = h;
                 if (st != kAnswered) pr.cls = kNotAnswered;
                 P.pred[i / P.judge_stride] = pr;
@@ -336,31 +336,28 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_decode(DecodeParams P) {
             }
             if (P.cand) P.cand[i] = cand;
             if (st == kDefer) P.reason[i] = uint8_t(reason);
-            if (st == kTrace || st == kDefer) {
-                // traced or resolved: its initial conditions go to HBM
+            if (st == kTrace) {
+                // traced: its initial conditions go to HBM
                 double y[10];
                 ray_ic_fp64(P.G, C, i, y);
 #pragma unroll
                 for (int k = 0; k < 10; ++k) P.rays[k * P.n + i] = y[k];
             }
         }
-        // a deferred path's reason counts only if the resolve pass traces it
+        // a deferred path's reason counts only if k_sur_mino traces it
         for (int c = C_CONF; c <= C_REASON_LAST; ++c) {
             const uint64_t m = __ballot(st == kTrace && reason == c);
             if (m != 0 && lane == 0) atomicAdd(&reasons[c], uint32_t(__popcll(m)));
         }
         wave_append(live && st == kTrace, uint32_t(i), lane, lt_mask, &n_local[0], qbuf);
-        wave_append(live && st == kDefer, uint32_t(i), lane, lt_mask, &n_local[1], rbuf);
     }
     __syncthreads();
-    const uint32_t n_q = n_local[0], n_r = n_local[1];
+    const uint32_t n_q = n_local[0];
     if (threadIdx.x == 0) base_s[0] = n_q ? atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)n_q) : 0ull;
-    if (threadIdx.x == 1) base_s[1] = n_r ? atomicAdd(&P.cnt[C_RQUEUE], (unsigned long long)n_r) : 0ull;
     if (threadIdx.x >= C_CONF && threadIdx.x <= C_REASON_LAST && reasons[threadIdx.x])
         atomicAdd(&P.cnt[threadIdx.x], (unsigned long long)reasons[threadIdx.x]);
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < n_q; k += kDecodeBlock) P.queue[base_s[0] + k] = qbuf[k];
-    for (uint32_t k = threadIdx.x; k < n_r; k += kDecodeBlock) P.rqueue[base_s[1] + k] = rbuf[k];
 }
 
 // A path's covariant momentum p and u_obs . p for the Mino-time check: the
@@ -390,14 +387,17 @@ __device__ __forceinline__ void mino_ic(const DecodeParams& P, const IcCamera& C
     con = C.w[0] * l[0] + C.w[1] * l[1] + C.w[2] * l[2] + C.w[3] * l[3];
 }
 
-// The Mino-time check of the confident paths (fallback.analytic; inside the
-// decode the integrals' registers held it to two waves per SIMD, with spills):
-// a path whose class the integrals confirm keeps the decode's answer, a disk
-// path's with the radius they give; the others are traced (reason
-// C_ANALYTIC).  A judge path is traced as well, its answer recorded in pred.
-// One path per thread: a loop over several kept loop-invariant constants in
-// vector registers across it (256 of them, one wave per SIMD); this way the
-// kernel takes 115, four waves per SIMD.
+// The Mino-time check of the confident paths (fallback.analytic): a path
+// whose class the integrals confirm keeps the decode's answer, a disk path's
+// with the radius they give; the others are traced (reason C_ANALYTIC).  The
+// paths the decode deferred (the network not trusted) take the same integrals
+// seeded by the network's radius: one they find captured or on the disk is
+// answered; an escape (its direction would be the network's) or an unresolved
+// path is traced under the reason the decode gave it.  A judge path is traced
+// as well, its answer recorded in pred or marked unanswered.  Inside the decode the integrals' registers held it to
+// two waves per SIMD, with spills, and a loop over several paths per thread
+// kept loop-invariant constants in vector registers (246): one path per
+// thread, the kernel takes about 100 (four waves per SIMD).
 __global__ void __launch_bounds__(kDecodeBlock) k_sur_mino(DecodeParams P) {
     const IcCamera C = *P.ic;   // uniform, read before any store: scalar loads into SGPRs
     const uint32_t lane = __lane_id();
@@ -405,23 +405,32 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_mino(DecodeParams P) {
     const uint32_t i = blockIdx.x * kDecodeBlock + threadIdx.x;
     const float c = i < P.n ? P.cand[i] : 0.0f;
     bool fail = false, trace = false;
+    int reason = -1;
     if (c != 0.0f) {
         double p[4], con;
         mino_ic(P, C, i, p, con);
         const double q[3] = {C.q[0], C.q[1], C.q[2]};
         const double pt = p[3], pphi = q[2] * p[0] - q[0] * p[2];
-        const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.dk, c > 0.0f ? double(c) : -1.0);
-        const int want = c > 0.0f ? kMinoDisk : (c == kCandCaptured ? kMinoCaptured : kMinoEscapes);
-        fail = mr.kind != want;
+        const bool deferred = c <= kCandDeferred;
+        const double seed = deferred ? double(kCandDeferred - c) : (c > 0.0f ? double(c) : -1.0);
+        const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.dk, seed);
         const bool judge = P.judge_stride && (i % P.judge_stride) == 0;
-        if (!fail && c > 0.0f) {
-            mgk_hit_t h;
-            disk_hit(P, pt, pphi, con, mr.r, h);
+        bool ok;
+        if (deferred) {
+            ok = mr.kind == kMinoCaptured || mr.kind == kMinoDisk;
+            if (!ok) reason = P.reason[i];
+        } else {
+            ok = mr.kind == (c > 0.0f ? kMinoDisk : (c == kCandCaptured ? kMinoCaptured : kMinoEscapes));
+            fail = !ok;
+        }
+        if (ok && (c > 0.0f || deferred)) {
+            mgk_hit_t h{MGK_HIT_HORIZON, 0.0f, 0.0f, 0.0f};
+            if (mr.kind == kMinoDisk) disk_hit(P, pt, pphi, con, mr.r, h);
             if (judge) P.pred[i / P.judge_stride] = h;
             else P.hits[i] = h;
         }
-        if (fail && judge) P.pred[i / P.judge_stride] = mgk_hit_t{kNotAnswered, 0.0f, 0.0f, 0.0f};
-        trace = fail || judge;
+        if (!ok && judge) P.pred[i / P.judge_stride] = mgk_hit_t{kNotAnswered, 0.0f, 0.0f, 0.0f};
+        trace = !ok || judge;
         if (trace) {
             // its initial conditions go to HBM (formed again, so that they
             // need not stay live through the integrals)
@@ -433,63 +442,18 @@ __global__ void __launch_bounds__(kDecodeBlock) k_sur_mino(DecodeParams P) {
     }
     const uint64_t mf = __ballot(fail);
     if (mf != 0 && lane == 0) atomicAdd(&P.cnt[C_ANALYTIC], (unsigned long long)__popcll(mf));
+    if (__ballot(reason >= 0) != 0) {
+        for (int r = C_CONF; r <= C_REASON_LAST; ++r) {
+            const uint64_t mr = __ballot(reason == r);
+            if (mr != 0 && lane == 0) atomicAdd(&P.cnt[r], (unsigned long long)__popcll(mr));
+        }
+    }
     const uint64_t m = __ballot(trace);
     if (m != 0) {
         unsigned long long off = 0;
         if (lane == 0) off = atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)__popcll(m));
         off = __shfl(off, 0);
         if (trace) P.queue[off + __popcll(m & lt_mask)] = i;
-    }
-}
-
-// The resolve pass over the paths the decode deferred (the network not
-// trusted; the Mino-time integrals on): one the integrals find captured or
-// on the disk is answered (its radius solved for, seeded by the network's;
-// a judge path's answer recorded and the path traced as well); an escape (its
-// direction would be the network's) or an unresolved path is traced, under
-// the reason the decode gave it.  A wave-uniform grid-stride loop over the
-// queue, whose length only the device knows.
-__global__ void __launch_bounds__(kDecodeBlock) k_sur_resolve(DecodeParams P) {
-    const uint64_t nr = P.cnt[C_RQUEUE];
-    const uint32_t lane = __lane_id();
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint64_t wave = (uint64_t(blockIdx.x) * kDecodeBlock + threadIdx.x) >> 6;
-    const uint64_t n_waves = (uint64_t(gridDim.x) * kDecodeBlock) >> 6;
-#pragma unroll 1
-    for (uint64_t base = wave * 64; base < nr; base += n_waves * 64) {
-        const uint64_t k = base + lane;
-        bool trace = false;
-        int reason = -1;
-        uint32_t i = 0;
-        if (k < nr) {
-            i = P.rqueue[k];
-            double y[10];
-#pragma unroll
-            for (int c = 0; c < 10; ++c) y[c] = P.rays[c * P.n + i];
-            const double r = exp(P.R.lnr_mid + double(P.Y[uint64_t(i) * kMlpOut + 2]) * P.R.lnr_half);
-            const double q[3] = {y[0], y[1], y[2]};
-            const double p[4] = {y[4], y[5], y[6], y[7]};
-            const MinoResult mr = mino_disk(mino_ray(q, p, P.a_eq), P.dk, r);
-            mgk_hit_t h{MGK_HIT_HORIZON, 0.0f, 0.0f, 0.0f};
-            const bool ok = mr.kind == kMinoCaptured || mr.kind == kMinoDisk;
-            if (mr.kind == kMinoDisk) disk_hit(P, y[7], y[2] * y[4] - y[0] * y[6], y[9], mr.r, h);
-            const bool judge = P.judge_stride && (i % P.judge_stride) == 0;
-            if (ok && judge) P.pred[i / P.judge_stride] = h;
-            if (ok && !judge) P.hits[i] = h;
-            trace = !ok || judge;
-            if (!ok) reason = P.reason[i];
-        }
-        for (int c = C_CONF; c <= C_REASON_LAST; ++c) {
-            const uint64_t m = __ballot(reason == c);
-            if (m != 0 && lane == 0) atomicAdd(&P.cnt[c], (unsigned long long)__popcll(m));
-        }
-        const uint64_t m = __ballot(trace);
-        if (m != 0) {
-            unsigned long long off = 0;
-            if (lane == 0) off = atomicAdd(&P.cnt[C_QUEUE], (unsigned long long)__popcll(m));
-            off = __shfl(off, 0);
-            if (trace) P.queue[off + __popcll(m & lt_mask)] = i;
-        }
     }
 }
 
@@ -693,10 +657,7 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     if (!d_hits) HIP_TRY(s->fr_hits.reserve(n * sizeof(mgk_hit_t)));
     mgk_hit_t* hits = d_hits ? d_hits : s->fr_hits.as<mgk_hit_t>();
     HIP_TRY(s->fr_queue.reserve(n * sizeof(uint32_t)));
-    if (fb->analytic && !fb->trace_disk) {
-        HIP_TRY(s->fr_rqueue.reserve(n * sizeof(uint32_t)));
-        HIP_TRY(s->fr_reason.reserve(n));
-    }
+    if (fb->analytic && !fb->trace_disk) HIP_TRY(s->fr_reason.reserve(n));
     if (fb->analytic) HIP_TRY(s->fr_cand.reserve(n * sizeof(float)));
     HIP_TRY(s->fr_pred.reserve(std::max<uint64_t>(1, n_slots) * sizeof(mgk_hit_t)));
     HIP_TRY(s->fr_counters.reserve(C_N * sizeof(unsigned long long)));
@@ -784,7 +745,6 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     D.rays = s->fr_rays.as<double>();
     D.hits = hits;
     D.queue = s->fr_queue.as<uint32_t>();
-    D.rqueue = s->fr_rqueue.as<uint32_t>();
     D.reason = s->fr_reason.as<uint8_t>();
     D.cand = fb->analytic ? s->fr_cand.as<float>() : nullptr;
     D.pred = s->fr_pred.as<mgk_hit_t>();
@@ -814,12 +774,6 @@ MGK_API e_magik_result_types mgk_render_frame_surrogate(mgk_context_handle ctx, 
     HIP_TRY(hipGetLastError());
     if (D.analytic) {
         hipLaunchKernelGGL(k_sur_mino, dim3(unsigned((n + kDecodeBlock - 1) / kDecodeBlock)), dim3(kDecodeBlock), 0, st, D);
-        HIP_TRY(hipGetLastError());
-    }
-    if (D.analytic && !D.trace_disk) {
-        int n_cu = 0;
-        HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, s->device));
-        hipLaunchKernelGGL(k_sur_resolve, dim3(unsigned(8 * n_cu)), dim3(kDecodeBlock), 0, st, D);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(s->ev[2], st));
