@@ -255,7 +255,17 @@ __host__ __device__ inline MinoResult mino_disk(const MinoRay& ray, const MinoDi
         }
         return 2.0 * W * acc;
     };
-    const double G_half = tail(0.0);
+    // the half period, tail(0) = int_0^{sqrt(u_+)} du / sqrt(U): with u = sqrt(u_+) sin(phi)
+    // it is int_0^{pi/2} dphi / sqrt(A_m cos^2 + (A_m + a^2 u_+) sin^2) = pi / (2 AGM(sqrt(A_m),
+    // sqrt(A_m + a^2 u_+))), the arithmetic-geometric mean converging quadratically (a
+    // few square roots in place of 16 nodes)
+    double ga = sqrt(Am), gb = sqrt(Am + Ap * up);
+    for (int k = 0; k < 12 && fabs(ga - gb) > 1e-15 * ga; ++k) {
+        const double gm = 0.5 * (ga + gb);
+        gb = sqrt(ga * gb);
+        ga = gm;
+    }
+    const double G_half = 3.141592653589793 / (ga + gb);
     const bool toward = (ray.su < 0) == (ray.u > 0.0);   // moving toward u = 0
     const double t_obs = tail(uo);
     const double G1 = toward ? G_half - t_obs : G_half + t_obs;
