@@ -394,10 +394,11 @@ __device__ __forceinline__ void mino_ic(const DecodeParams& P, const IcCamera& C
 // seeded by the network's radius: one they find captured or on the disk is
 // answered; an escape (its direction would be the network's) or an unresolved
 // path is traced under the reason the decode gave it.  A judge path is traced
-// as well, its answer recorded in pred or marked unanswered.  Inside the decode the integrals' registers held it to
-// two waves per SIMD, with spills, and a loop over several paths per thread
-// kept loop-invariant constants in vector registers (246): one path per
-// thread, the kernel takes about 100 (four waves per SIMD).
+// as well, its answer recorded in pred or marked unanswered.  Inside the
+// decode the integrals' registers held it to two waves per SIMD, with spills,
+// and a loop over several paths per thread kept loop-invariant constants in
+// vector registers (246): one path per thread, the kernel takes 91 (five
+// waves per SIMD).
 __global__ void __launch_bounds__(kDecodeBlock) k_sur_mino(DecodeParams P) {
     const IcCamera C = *P.ic;   // uniform, read before any store: scalar loads into SGPRs
     const uint32_t lane = __lane_id();
