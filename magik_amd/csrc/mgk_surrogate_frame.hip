@@ -397,9 +397,11 @@ __device__ __forceinline__ void mino_ic(const DecodeParams& P, const IcCamera& C
 // as well, its answer recorded in pred or marked unanswered.  Inside the
 // decode the integrals' registers held it to two waves per SIMD, with spills,
 // and a loop over several paths per thread kept loop-invariant constants in
-// vector registers (246): one path per thread, the kernel takes 91 (five
-// waves per SIMD).
-__global__ void __launch_bounds__(kDecodeBlock) k_sur_mino(DecodeParams P) {
+// vector registers (246).  One path per thread, it takes 91 registers (five
+// waves per SIMD); held to six (80 registers, a few bytes spilled) it runs
+// 0.3 ms faster per 1080p frame, and eight (64, more spills) gain nothing
+// more.
+__global__ void __launch_bounds__(kDecodeBlock, 6) k_sur_mino(DecodeParams P) {
     const IcCamera C = *P.ic;   // uniform, read before any store: scalar loads into SGPRs
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
